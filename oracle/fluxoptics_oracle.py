@@ -1,0 +1,391 @@
+"""FP64 CPU oracle for the FluxOptics.jl propagation path (ASProp, RSProp, Phase, AS_BPM).
+
+TEST INFRASTRUCTURE, NOT PRODUCT CODE.  Only the tests under ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may import
+this module, and only as the checker (or, for ``cpu_baseline``, as the timed CPU
+restatement).  The product path (``fluxoptics.jl_amd/``) never imports it and fails
+loudly when its HIP library is missing.
+
+This is a plain numpy restatement of the reference algorithm in complex FP64.  The
+reference itself (Julia + FFTW + Zygote) cannot run in this container (no Julia, no
+network), so this restatement is the oracle; every function cites the reference lines
+it follows and is pinned by the checks in ``tests/test_oracle_*.py`` (closed forms,
+brute-force DFTs, mpmath, invariants, finite differences).
+
+Array convention.  A field batch is a numpy array of shape ``(B, ny, nx)`` in C order.
+That is byte-for-byte the reference's column-major ``electric[nx, ny, B]`` (x fastest,
+``src/Fields.jl:118-123``), so element ``(i, j, b)`` sits at offset ``i + nx*(j + ny*b)``.
+A 2-D array ``(ny, nx)`` is a batch of one.
+
+Direction convention.  ``direction=+1`` is ``Forward`` and ``-1`` is ``Backward``
+(``src/optical_components/OpticalComponents.jl:34-99``).
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Sequence
+
+import numpy as np
+
+TWO_PI = 2.0 * math.pi
+
+
+# ----------------------------------------------------------------------------- grids
+
+def spatial_vectors(ns: Sequence[int], ds: Sequence[float], offset=None):
+    """Centred grid vectors x_i = (i - (n-1)/2)*dx - xc.
+
+    Follows ``src/GridUtils.jl:56-60``.  ``ns``/``ds`` are ``(nx, ny)``/``(dx, dy)``.
+    """
+    if offset is None:
+        offset = (0.0,) * len(ns)
+    return tuple((np.arange(n, dtype=np.float64) - (n - 1) / 2.0) * d - c
+                 for n, d, c in zip(ns, ds, offset))
+
+
+def fftfreq(n: int, dx: float) -> np.ndarray:
+    """FFT-order frequencies k/(n*dx), k in [0, n/2) then negative.
+
+    AbstractFFTs ``fftfreq(n, 1/dx)`` as called at
+    ``src/optical_components/freespace_propagators/fourier_kernel.jl:17``.
+    """
+    k = np.arange(n, dtype=np.float64)
+    k[k >= (n + 1) // 2] -= n
+    return k / (n * dx)
+
+
+# ----------------------------------------------------------------------------- FFT
+
+def ft2(u: np.ndarray) -> np.ndarray:
+    """Unnormalised forward 2-D DFT over the two transverse axes (e^{-2*pi*i*jk/n}).
+
+    ``compute_ft!`` with an in-place ``plan_fft!`` over dims (1, 2): ``src/FFTutils.jl:13-23``.
+    """
+    return np.fft.fft2(u, axes=(-2, -1))
+
+
+def ift2(u: np.ndarray) -> np.ndarray:
+    """Inverse 2-D DFT normalised by 1/(nx*ny): ``compute_ift!``, ``src/FFTutils.jl:25-28``."""
+    return np.fft.ifft2(u, axes=(-2, -1))
+
+
+# ----------------------------------------------------------------------------- helpers
+
+def _as_batch(u: np.ndarray) -> np.ndarray:
+    u = np.asarray(u)
+    return u[None] if u.ndim == 2 else u
+
+
+def _lambdas(lams, batch: int) -> np.ndarray:
+    """Per-slice wavelengths.  A scalar applies to every slice (``Fields.jl:28-32``)."""
+    lams = np.atleast_1d(np.asarray(lams, dtype=np.float64))
+    if lams.size == 1:
+        lams = np.repeat(lams, batch)
+    if lams.size != batch:
+        raise ValueError("one wavelength per batch slice expected")
+    return lams
+
+
+def conj_direction(h: np.ndarray, direction: int) -> np.ndarray:
+    """``OpticalComponents.jl:919-925``: the kernel for Forward, its conjugate for Backward."""
+    return h if direction > 0 else np.conj(h)
+
+
+# ----------------------------------------------------------------------------- ASProp
+
+def as_kernel(fx, fy, lam: float, n0: float, z: float):
+    """Angular-spectrum transfer function H(fx, fy), complex FP64.
+
+    ``src/optical_components/freespace_propagators/angular_spectrum.jl:1-12``:
+    with lam' = lam/n0, H = cis(2*pi*z*sqrt(1/lam'^2 - fx^2 - fy^2)) using the principal
+    complex square root, so evanescent components decay as exp(-2*pi*|z|*kappa);
+    for z < 0, H(z) = conj(H(-z)).
+    """
+    if z < 0:
+        return np.conj(as_kernel(fx, fy, lam, n0, -z))
+    lam_eff = lam / n0
+    f2 = complex(1.0 / lam_eff ** 2)
+    arg = np.asarray(f2 - np.asarray(fx) ** 2 - np.asarray(fy) ** 2, dtype=np.complex128)
+    return np.exp(1j * (TWO_PI * z) * np.sqrt(arg))
+
+
+def as_transfer(nx: int, ny: int, dx: float, dy: float, lam: float, n0: float, z: float):
+    """H sampled on the FFT-order frequency grid, shape (ny, nx).
+
+    ``fill_kernel_cache`` broadcasting ``as_kernel`` over ``f_vec = (x=fx, y=fy')``:
+    ``abstract_kernel.jl:15-25``, ``fourier_kernel.jl:17-18``.
+    """
+    fx = fftfreq(nx, dx)[None, :]
+    fy = fftfreq(ny, dy)[:, None]
+    return as_kernel(fx, fy, lam, n0, z)
+
+
+def asprop(u, ds, lams, z: float, n0: float = 1.0, direction: int = 1):
+    """``ASProp`` propagation: FFT2 -> multiply by H (or conj H) per slice -> IFFT2.
+
+    Sequence (ft, ASKernelProp, ift): ``angular_spectrum.jl:193-229``; per-slice kernel
+    multiply ``abstract_kernel.jl:64-83``; Backward = reverse direction
+    ``freespace.jl:61-63`` + ``fourier_operator.jl:65-79``.
+    """
+    u = _as_batch(np.asarray(u, dtype=np.complex128))
+    b, ny, nx = u.shape
+    dx, dy = ds
+    lam_b = _lambdas(lams, b)
+    v = ft2(u)
+    for k in range(b):
+        v[k] *= conj_direction(as_transfer(nx, ny, dx, dy, lam_b[k], n0, z), direction)
+    return ift2(v)
+
+
+# ----------------------------------------------------------------------------- RSProp
+
+def rs_kernel(x, y, lam: float, z: float, nrm_f: float):
+    """Rayleigh-Sommerfeld impulse response h(x, y), complex FP64.
+
+    ``rayleigh_sommerfeld.jl:1-12``: k = 2*pi/lam, r = sqrt(x^2+y^2+z^2),
+    h = nrm_f * (e^{ikr}/r) * (z/r) * (1/r - i*k); for z < 0, h(z) = conj(h(-z)).
+    """
+    if z < 0:
+        return np.conj(rs_kernel(x, y, lam, -z, nrm_f))
+    k = TWO_PI / lam
+    r = np.sqrt(np.asarray(x) ** 2 + np.asarray(y) ** 2 + z * z)
+    return nrm_f * (np.exp(1j * k * r) / r) * (z / r) * (1.0 / r - 1j * k)
+
+
+def rs_offsets(n: int, dx: float, pad: str = "2n-1") -> np.ndarray:
+    """Sample offsets of the padded convolution kernel along one axis.
+
+    ``pad="2n-1"``: ``circshift((1-n):(n-1), n) .* dx`` = [0..n-1, -(n-1)..-1]*dx,
+    exactly as ``convolution_kernel.jl:16``.  ``pad="2n"``: the power-of-two padding
+    [0..n-1, n, -(n-1)..-1]*dx (offset n is never read by a cropped output, and
+    sampling it at +n makes the kernel even, hence its DFT even).
+    """
+    if pad == "2n-1":
+        j = np.concatenate([np.arange(n), np.arange(1 - n, 0)])
+    elif pad == "2n":
+        j = np.concatenate([np.arange(n + 1), np.arange(1 - n, 0)])
+    else:
+        raise ValueError(pad)
+    return j.astype(np.float64) * dx
+
+
+def rs_transfer(nx: int, ny: int, dx: float, dy: float, lam: float, z: float,
+                pad: str = "2n-1"):
+    """FFT2 of the sampled RS kernel on the padded grid (``convolution_kernel.jl:47-55``).
+
+    nrm_f = dx*dy/(2*pi) (``rayleigh_sommerfeld.jl:55``).  The reference rounds h to the
+    field precision before an FP32 FFT; the oracle keeps everything in FP64.
+    """
+    sx = rs_offsets(nx, dx, pad)[None, :]
+    sy = rs_offsets(ny, dy, pad)[:, None]
+    h = rs_kernel(sx, sy, lam, z, dx * dy / TWO_PI)
+    return ft2(h)
+
+
+def pad(u: np.ndarray, shape) -> np.ndarray:
+    """Zero-pad at offset 0 (``pad_crop_operators.jl:1-16``)."""
+    u = _as_batch(u)
+    out = np.zeros(u.shape[:-2] + tuple(shape), dtype=u.dtype)
+    out[..., : u.shape[-2], : u.shape[-1]] = u
+    return out
+
+
+def crop(u: np.ndarray, ny: int, nx: int) -> np.ndarray:
+    """Crop [0, n) at offset 0 (``pad_crop_operators.jl:61-79``)."""
+    return np.ascontiguousarray(u[..., :ny, :nx])
+
+
+def rsprop(u, ds, lams, z: float, direction: int = 1, pad_mode: str = "2n-1"):
+    """``RSProp``: pad -> FFT2 -> multiply by FFT2(h) (or its conjugate) -> IFFT2 -> crop.
+
+    Sequence (pad, ft, RSKernelProp, ift, crop): ``rayleigh_sommerfeld.jl:135-156``;
+    Backward reverses the sequence and conjugates the kernel (``freespace.jl:61-63``,
+    ``pad_crop_operators.jl:338-356``).
+    """
+    u = _as_batch(np.asarray(u, dtype=np.complex128))
+    b, ny, nx = u.shape
+    dx, dy = ds
+    lam_b = _lambdas(lams, b)
+    py, px = (2 * ny - 1, 2 * nx - 1) if pad_mode == "2n-1" else (2 * ny, 2 * nx)
+    v = ft2(pad(u, (py, px)))
+    for k in range(b):
+        v[k] *= conj_direction(rs_transfer(nx, ny, dx, dy, lam_b[k], z, pad_mode), direction)
+    return crop(ift2(v), ny, nx)
+
+
+def rs_valid_distance(nx: int, ny: int, dx: float, dy: float, lam: float) -> float:
+    """Critical distance z_c below which RSProp warns (``rayleigh_sommerfeld.jl:29-36``)."""
+    if dx < lam / 2 and dy < lam / 2:
+        return 0.0
+    zx = 0.0 if dx < lam / 2 else (nx * dx / 2) * math.sqrt(4 * dx * dx / lam ** 2 - 1)
+    zy = 0.0 if dy < lam / 2 else (ny * dy / 2) * math.sqrt(4 * dy * dy / lam ** 2 - 1)
+    return max(zx, zy)
+
+
+# ----------------------------------------------------------------------------- Phase
+
+def phase_apply(u, phi, direction: int = 1):
+    """u * cis(s*phi) with s = sign(direction) (``phasemask.jl:90-98``).
+
+    ``phi`` broadcasts over the batch (2-D mask shared by every slice).
+    """
+    return np.asarray(u, dtype=np.complex128) * np.exp(1j * direction * np.asarray(phi, np.float64))
+
+
+def phase_grad(du, u_saved, direction: int = 1):
+    """Mask gradient s * sum_b Im(du * conj(u_saved)) (``phasemask.jl:115-140``).
+
+    ``du`` is the cotangent after un-applying the mask (``backpropagate_with_gradient!``,
+    ``phasemask.jl:142-150``); ``u_saved`` is the mask input saved by
+    ``propagate_and_save!`` (``phasemask.jl:100-113``).
+    """
+    g = direction * np.imag(_as_batch(du) * np.conj(_as_batch(u_saved)))
+    return g.sum(axis=0)
+
+
+def phase_backward(dv, u_saved, phi, direction: int = 1):
+    """``backpropagate_with_gradient!`` for ``Phase``: returns (du, dphi)."""
+    du = phase_apply(dv, phi, -direction)
+    return du, phase_grad(du, u_saved, direction)
+
+
+# ----------------------------------------------------------------------------- AS_BPM
+
+def bpm_kdz(dz: float, tilts=(0.0, 0.0)) -> float:
+    """kdz = 2*pi*dz / prod(cos(theta)) (``bpm.jl:3-6,70``)."""
+    return TWO_PI * dz / float(np.prod(np.cos(np.asarray(tilts, np.float64))))
+
+
+def _dn_phase(dn_s, kdz, lam_b, s):
+    # apply_dn_slice!: cis(s*kdz/lambdas*dn) with vacuum lambdas (bpm.jl:235-242)
+    return np.exp(1j * (s * kdz / lam_b)[:, None, None] * np.asarray(dn_s, np.float64)[None])
+
+
+def bpm_forward(u, dn, thickness: float, n0: float, lams, ds, direction: int = 1):
+    """``AS_BPM`` split-step forward; returns (v, saved) with saved[s] = slice-s input.
+
+    ``bpm.jl:54-73`` (construction: dz = L/N, P(dz) and P(dz/2) are ASProps with n0),
+    ``bpm.jl:244-260`` (half step, N-1 x [save, phase, full step], save, phase, half step).
+    ``dn`` has shape (N, ny, nx) (the reference's dn[nx, ny, N]).
+    """
+    u = _as_batch(np.asarray(u, dtype=np.complex128)).copy()
+    dn = np.asarray(dn, np.float64)
+    ns = dn.shape[0]
+    if ns < 2:
+        raise ValueError("AS_BPM needs at least 2 slices (bpm.jl:40)")
+    dz = thickness / ns
+    kdz = bpm_kdz(dz)
+    lam_b = _lambdas(lams, u.shape[0])
+    saved = []
+    u = asprop(u, ds, lam_b, dz / 2, n0, direction)
+    for s in range(ns - 1):
+        saved.append(u.copy())
+        u = u * _dn_phase(dn[s], kdz, lam_b, direction)
+        u = asprop(u, ds, lam_b, dz, n0, direction)
+    saved.append(u.copy())
+    u = u * _dn_phase(dn[ns - 1], kdz, lam_b, direction)
+    u = asprop(u, ds, lam_b, dz / 2, n0, direction)
+    return u, saved
+
+
+def bpm_backward(dv, dn, thickness: float, n0: float, lams, ds, saved, direction: int = 1):
+    """``AS_BPM`` adjoint with the dn gradient (``bpm.jl:275-312``); returns (du, ddn).
+
+    ddn[s] = s * kdz/lam * sum_b Im(du_s * conj(saved[s])), du_s being the cotangent
+    after un-applying slice s's phase.
+    """
+    du = _as_batch(np.asarray(dv, dtype=np.complex128)).copy()
+    dn = np.asarray(dn, np.float64)
+    ns = dn.shape[0]
+    dz = thickness / ns
+    kdz = bpm_kdz(dz)
+    lam_b = _lambdas(lams, du.shape[0])
+    ddn = np.zeros_like(dn)
+    coef = (direction * kdz / lam_b)[:, None, None]
+    du = asprop(du, ds, lam_b, dz / 2, n0, -direction)
+    for s in range(ns - 1, -1, -1):
+        du = du * _dn_phase(dn[s], kdz, lam_b, -direction)
+        ddn[s] = (coef * np.imag(du * np.conj(saved[s]))).sum(axis=0)
+        du = asprop(du, ds, lam_b, dz if s > 0 else dz / 2, n0, -direction)
+    return du, ddn
+
+
+# ----------------------------------------------------------------------------- fields, losses
+
+def power(u, ds) -> np.ndarray:
+    """Per-slice power sum|u|^2 * dx*dy (``Fields.jl:843-846``)."""
+    u = _as_batch(u)
+    return (np.abs(u) ** 2).sum(axis=(-2, -1)) * ds[0] * ds[1]
+
+
+def normalize_power(u, ds, target=1.0) -> np.ndarray:
+    """``normalize_power!`` (``Fields.jl:889-892``)."""
+    u = _as_batch(np.asarray(u, dtype=np.complex128))
+    return u * np.sqrt(target / power(u, ds))[:, None, None]
+
+
+def sq_intensity_diff(v, target_intensity):
+    """Sum (|v|^2 - I)^2 and its cotangent 4(|v|^2 - I) v.
+
+    ``src/metrics/squared_intensity_difference.jl:78-90`` (compute_metric and
+    backpropagate_metric with dc = 1).
+    """
+    d = np.abs(v) ** 2 - target_intensity
+    return float((d * d).sum()), 4.0 * d * v
+
+
+def half_sq_field_diff(v, t):
+    """L = 1/2 sum|v - t|^2 and its cotangent v - t (harness loss for the fwd+adjoint unit)."""
+    d = np.asarray(v) - np.asarray(t)
+    return 0.5 * float(np.vdot(d, d).real), d
+
+
+def coupling_efficiency(u, v) -> np.ndarray:
+    """|<u, v>|^2 / (<u,u><v,v>) per slice (``Fields.jl:748-756``)."""
+    u, v = _as_batch(u), _as_batch(v)
+    num = np.abs((np.conj(u) * v).sum(axis=(-2, -1))) ** 2
+    den = (np.abs(u) ** 2).sum(axis=(-2, -1)) * (np.abs(v) ** 2).sum(axis=(-2, -1))
+    return num / den
+
+
+# ----------------------------------------------------------------------------- pipelines
+
+def pipeline_forward(u, stages, ds, lams):
+    """Run a list of stages forward, saving the inputs of every Phase.
+
+    ``OpticalSequence`` forward loop (``optical_sequence.jl:54-60``) with trainable
+    ``Phase`` using ``propagate_and_save!`` (``phasemask.jl:100-113``).  A stage is
+    ``("as", z, n0)``, ``("rs", z, pad_mode)`` or ``("phase", phi)``.
+    """
+    u = _as_batch(np.asarray(u, dtype=np.complex128))
+    saved = []
+    for st in stages:
+        if st[0] == "as":
+            u = asprop(u, ds, lams, st[1], st[2] if len(st) > 2 else 1.0, 1)
+        elif st[0] == "rs":
+            u = rsprop(u, ds, lams, st[1], 1, st[2] if len(st) > 2 else "2n-1")
+        elif st[0] == "phase":
+            saved.append(u.copy())
+            u = phase_apply(u, st[1], 1)
+        else:
+            raise ValueError(st[0])
+    return u, saved
+
+
+def pipeline_backward(dv, stages, ds, lams, saved):
+    """Reverse loop with reversed direction (``optical_sequence.jl:62-66``) and the Phase
+    pullbacks of ``ChainRulesCoreExt.jl:35-65,81-111``; returns (du, [dphi per Phase])."""
+    du = _as_batch(np.asarray(dv, dtype=np.complex128))
+    grads = []
+    k = len(saved)
+    for st in reversed(stages):
+        if st[0] == "as":
+            du = asprop(du, ds, lams, st[1], st[2] if len(st) > 2 else 1.0, -1)
+        elif st[0] == "rs":
+            du = rsprop(du, ds, lams, st[1], -1, st[2] if len(st) > 2 else "2n-1")
+        elif st[0] == "phase":
+            k -= 1
+            du, g = phase_backward(du, saved[k], st[1], 1)
+            grads.append(g)
+    return du, grads[::-1]
