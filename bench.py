@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Headline benchmark: 2048^2 ASProp forward + adjoint propagations per second on one MI355X.
+
+Workload: 8 complex-FP32 fields on a 2048x2048 grid, ds = 0.5 um, lambda = 0.8 um,
+z = 5000 um; each field is a Gaussian (waist 150 um) times exp(i*phi), phi uniform in
+[0, 2*pi) per pixel (seeded).  One step = the forward propagation of the batch plus the
+adjoint propagation of a cotangent batch, both in place and resident in HBM.  One unit
+= one field's forward + adjoint (32*n^2 algorithmic bytes: read and write the field
+twice).
+
+Prints one JSON line.  Only one GPU is supported: --gpus above 1 exits with a message.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fluxoptics.jl_amd"))
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+N, BATCH, DS, LAM, Z, W0 = 2048, 8, 0.5, 0.8, 5000.0, 150.0
+METRIC = "2048² fwd+adjoint propagations/s on 1 MI355X; % HBM roofline; err vs FP64"
+PMC_FILE = os.path.join(ROOT, "profiles", "asprop_2048x2048_b8_traffic.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparator")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="bound on the CPU baseline sample's wall time")
+    return ap.parse_args()
+
+
+def make_inputs():
+    import numpy as np
+    from fluxoptics import inputs
+    g = inputs.gaussian(N, N, DS, DS, W0)
+    u = np.empty((BATCH, N, N), np.complex64)
+    for b in range(BATCH):
+        u[b] = g * np.exp(1j * inputs.uniform(2000 + b, (N, N), 0.0, 2 * math.pi))
+    t = (0.01 * (inputs.normal(77, (BATCH, N, N)) + 1j * inputs.normal(77 + 7919, (BATCH, N, N))))
+    return u, t.astype(np.complex64)
+
+
+def timed(fn, reps, torch):
+    """Average ms of fn() over reps, HIP events on torch's current stream (the stream the
+    library launches on)."""
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def cpu_baseline(u0, dv0, v_dev0, du_dev0, budget_s):
+    """The reference's CPU algorithm (FFT2 -> cached-kernel multiply -> IFFT2, complex
+    FP32, pocketfft threads = all host cores) timed on a bounded sample, plus the error
+    of the device outputs against the FP64 oracle."""
+    import numpy as np
+    from oracle import fluxoptics_oracle as O
+    cores = len(os.sched_getaffinity(0))
+    prop = O.CpuASPropC64(N, N, DS, DS, LAM, Z, 1.0, workers=cores)
+    x = u0[0].copy()
+    prop.forward(x)
+    prop.adjoint(x)  # warm-up (plan creation), excluded like FFTW planning
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while time.perf_counter() < t_end and len(times) < 200:
+        t0 = time.perf_counter()
+        prop.forward(x)
+        prop.adjoint(x)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    v_ref = O.asprop(u0[0].astype(np.complex128), (DS, DS), LAM, Z, 1.0, 1)[0]
+    du_ref = O.asprop(dv0[0].astype(np.complex128), (DS, DS), LAM, Z, 1.0, -1)[0]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return ({"value": 1.0 / med, "unit": "propagations/s", "cores": cores, "kind": "port",
+             "sample": f"1 field of {N}x{N} c64, forward+adjoint, {len(times)} runs (median), "
+                       f"engine scipy.fft pocketfft workers={cores}"},
+            {"field_rel_l2": rel(v_dev0, v_ref), "adjoint_rel_l2": rel(du_dev0, du_ref),
+             "oracle": "FP64 numpy restatement, field 0"})
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the batch would shard across GPUs with no communication, which is not measured here.",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import fluxoptics as fo
+    from fluxoptics import _lib
+    from fluxoptics.components import _ptr
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    u_np, t_np = make_inputs()
+    u = fo.ScalarField(torch.from_numpy(u_np).to(dev), (DS, DS), LAM)
+    prop = fo.ASProp(u, Z)
+    plan = prop.plan(u)
+
+    # correctness sample (device outputs of field 0 before timing)
+    v_dev = prop.propagate(u, fo.Forward)
+    v0 = v_dev.electric[0].cpu().numpy().astype(np.complex128)
+    g = fo.ScalarField(v_dev.electric - torch.from_numpy(t_np).to(dev), (DS, DS), LAM)
+    dv0 = g.electric.cpu().numpy()
+    gd = prop.propagate(g, fo.Backward)
+    du0 = gd.electric[0].cpu().numpy().astype(np.complex128)
+    del v_dev, gd
+
+    def step():
+        prop.propagate_(u, fo.Forward)
+        prop.propagate_(g, fo.Backward)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms = timed(step, args.steps, torch)
+    wall = time.perf_counter() - t0
+    units_per_s = BATCH / (ms * 1e-3)
+    unit_bytes = 32.0 * N * N
+
+    # per-kernel averages (each pass over the whole batch), same stream, HIP events
+    passes = {}
+    for pid, name in ((0, "x_fft"), (1, "y_fft_mul_ifft"), (2, "x_ifft")):
+        fn = lambda pid=pid: _lib.call("fh_propagate_pass", plan, _ptr(u.electric), 1, pid)
+        fn()
+        passes[name] = timed(fn, max(10, args.steps // 2), torch)
+    dom = max(passes, key=passes.get)
+    dom_bytes = 16.0 * N * N * BATCH     # read + write of the batch once
+    achieved = dom_bytes / (passes[dom] * 1e-3)
+    traffic = None
+    if os.path.exists(PMC_FILE):
+        with open(PMC_FILE) as f:
+            traffic = json.load(f).get("kernels", {}).get(dom, {}).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC, "value": units_per_s, "unit": "propagations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"ASProp forward+adjoint, batch {BATCH} x {N}x{N} complex-FP32, "
+                               f"ds={DS}um, lambda={LAM}um, z={Z}um (one step = {BATCH} units)",
+                   "global_batch": BATCH, "grid": N, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dom,
+                     "kernel_ms": passes[dom], "bytes_per_launch": dom_bytes},
+        "path_roofline": {"achieved_GBps": units_per_s * unit_bytes / 1e9,
+                          "frac": units_per_s * unit_bytes / HBM_PEAK,
+                          "bytes_per_unit": unit_bytes},
+        "pass_ms": passes, "wall_s_timed": wall,
+    }
+
+    if not args.no_plain:
+        # plain GPU form: complex FP64, per field, unfused
+        u128 = torch.from_numpy(u_np.astype(np.complex128)).to(dev)
+        g128 = u128.clone()
+
+        def plain_step():
+            prop.propagate_plain_fp64_(u128, LAM, fo.Forward)
+            prop.propagate_plain_fp64_(g128, LAM, fo.Backward)
+
+        plain_step()
+        pms = timed(plain_step, max(3, args.steps // 10), torch)
+        out["plain_fp64"] = {"value": BATCH / (pms * 1e-3), "unit": "propagations/s",
+                             "ms_per_step": pms, "speedup_of_fused": (BATCH / (ms * 1e-3)) / (BATCH / (pms * 1e-3))}
+        del u128, g128
+
+    if not args.no_cpu_baseline:
+        cb, err = cpu_baseline(u_np, dv0, v0, du0, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["error_vs_fp64"] = err
+
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

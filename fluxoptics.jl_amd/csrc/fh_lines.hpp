@@ -1,0 +1,226 @@
+// Stockham line-FFT passes for gfx950.
+//
+// A "line" is one 1-D transform of length n = R0*R1*R2 along either the contiguous
+// axis (x, rows of the column-major field) or the strided axis (y, columns).  A
+// workgroup owns L lines; each line is handled by P = n/V threads that hold V complex
+// values in VGPRs.  Stages exchange data through LDS; the first stage reads global
+// memory and the last stage's registers are stored directly (both at positions
+// j + r*n/R, so consecutive lanes touch consecutive addresses).
+//
+// Forward plan (R0, R1, R2); the inverse runs the reversed radices so that the
+// forward's last stage and the inverse's first stage share a register layout: a
+// fused FFT -> pointwise multiply -> IFFT needs no LDS round trip in the middle.
+#pragma once
+
+#include "fh_dft.hpp"
+
+namespace fh {
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <int A, int B, int C_> struct Radices {
+  static constexpr int R0 = A, R1 = B, R2 = C_;
+  static constexpr int n = A * B * C_;
+  static constexpr int NST = (C_ > 1) ? 3 : ((B > 1) ? 2 : 1);
+  static constexpr int V = cmax(A, cmax(B, C_));
+  static constexpr int P = n / V;
+  static_assert(V % A == 0 && V % B == 0 && V % C_ == 0, "V must be a multiple of every radix");
+  // reversed order (inverse transform)
+  using Rev = Radices<(NST == 3 ? C_ : (NST == 2 ? B : A)), (NST == 3 ? B : (NST == 2 ? A : 1)),
+                      (NST == 3 ? A : 1)>;
+};
+
+// LDS slot of line position pos: one pad slot every 16 complex values spreads
+// power-of-two strides over the 32 banks.
+__device__ __forceinline__ int lds_pos(int pos) { return pos + (pos >> 4); }
+template <int n> struct LdsLine { static constexpr int size = n + n / 16 + 2; };
+
+// twiddle W_n^k (forward) or its conjugate (inverse) from the global table
+template <bool INV, typename C>
+__device__ __forceinline__ C tw_load(const C* __restrict__ tw, int k) {
+  C w = tw[k];
+  if (INV) w.y = -w.y;
+  return w;
+}
+
+// Stockham stage: twiddle + DFT_R on the V/R butterflies held in v.
+template <typename C, int n, int R, int NS, int V, int P, bool INV>
+__device__ __forceinline__ void stage_compute(C* v, int t, const C* __restrict__ tw) {
+#pragma unroll
+  for (int m = 0; m < V / R; ++m) {
+    if (NS > 1) {
+      const int j = t + P * m;
+      const int jm = j % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C w = tw_load<INV>(tw, jm * r * (n / (NS * R)));
+        v[m * R + r] = cmul(v[m * R + r], w);
+      }
+    }
+    dft<R, INV>(&v[m * R]);
+  }
+}
+
+template <typename C, int n, int R, int NS, int V, int P>
+__device__ __forceinline__ void stage_write(C* __restrict__ s, const C* v, int t) {
+#pragma unroll
+  for (int m = 0; m < V / R; ++m) {
+    const int j = t + P * m;
+    const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+    for (int r = 0; r < R; ++r) s[lds_pos(base + r * NS)] = v[m * R + r];
+  }
+}
+
+template <typename C, int n, int R, int V, int P>
+__device__ __forceinline__ void stage_read(C* v, const C* __restrict__ s, int t) {
+#pragma unroll
+  for (int m = 0; m < V / R; ++m) {
+    const int j = t + P * m;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[m * R + r] = s[lds_pos(j + r * (n / R))];
+  }
+}
+
+// Full line transform on registers: v enters holding positions j + r*n/R0 (first
+// stage input) and leaves holding positions j + r*n/Rlast (natural-order output).
+// `s` is this line's LDS slice; the whole workgroup must call this together.
+template <typename C, typename RD, bool INV>
+__device__ __forceinline__ void line_fft(C* v, C* s, int t, const C* __restrict__ tw) {
+  constexpr int n = RD::n, V = RD::V, P = RD::P;
+  stage_compute<C, n, RD::R0, 1, V, P, INV>(v, t, tw);
+  if constexpr (RD::NST >= 2) {
+    __syncthreads();  // earlier readers of s are done
+    stage_write<C, n, RD::R0, 1, V, P>(s, v, t);
+    __syncthreads();
+    stage_read<C, n, RD::R1, V, P>(v, s, t);
+    stage_compute<C, n, RD::R1, RD::R0, V, P, INV>(v, t, tw);
+  }
+  if constexpr (RD::NST >= 3) {
+    __syncthreads();
+    stage_write<C, n, RD::R1, RD::R0, V, P>(s, v, t);
+    __syncthreads();
+    stage_read<C, n, RD::R2, V, P>(v, s, t);
+    stage_compute<C, n, RD::R2, RD::R0 * RD::R1, V, P, INV>(v, t, tw);
+  }
+}
+
+// position held in register (m, r) when the stage with radix R is the first (input) or
+// last (output) stage: j + r*n/R with j = t + P*m
+template <int n, int R, int P>
+__device__ __forceinline__ int reg_pos(int t, int m, int r) { return t + P * m + r * (n / R); }
+
+// ---------------------------------------------------------------- pass descriptors
+
+template <typename C> struct LineIO {
+  const C* in;
+  C* out;
+  long in_f, out_f;   // element stride between fields
+  long in_l, out_l;   // element stride between lines
+  long in_e, out_e;   // element stride between points of a line
+  int nlines;         // lines per field
+  int in_valid;       // points [0, in_valid) are read, the rest are zero
+  int out_count;      // points [0, out_count) are stored
+};
+
+// thread -> (line in workgroup, thread within line); LFAST puts lines on adjacent
+// lanes (strided-axis passes, so a wave reads L adjacent columns of one row).
+template <int L, int P, bool LFAST>
+__device__ __forceinline__ void thread_map(int tid, int& l, int& t) {
+  if (LFAST) { l = tid % L; t = tid / L; } else { t = tid % P; l = tid / P; }
+}
+
+struct NoOp {
+  template <typename C> __device__ __forceinline__ C operator()(C v, int, int, int) const { return v; }
+};
+
+// Forward or inverse line FFT, global -> global, with an elementwise op fused on the
+// time-domain side (prologue on load for a forward transform, epilogue before store
+// for an inverse one).
+template <typename C, typename RDF, int L, bool LFAST, bool INV, typename TimeOp>
+__global__ void __launch_bounds__(L * RDF::P)
+k_line_fft(LineIO<C> io, const C* __restrict__ tw, TimeOp op) {
+  using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
+  constexpr int n = RD::n, V = RD::V, P = RD::P;
+  extern __shared__ unsigned char smem_raw[];
+  C* smem = reinterpret_cast<C*>(smem_raw);
+  int l, t;
+  thread_map<L, P, LFAST>(threadIdx.x, l, t);
+  const int b = blockIdx.y;
+  const int li = blockIdx.x * L + l;
+  const bool live = li < io.nlines;
+  C* s = smem + l * LdsLine<n>::size;
+  const C* src = io.in + b * io.in_f + (long)li * io.in_l;
+  C v[V];
+#pragma unroll
+  for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+    for (int r = 0; r < RD::R0; ++r) {
+      const int pos = reg_pos<n, RD::R0, P>(t, m, r);
+      C x = cmake<C>(0, 0);
+      if (live && pos < io.in_valid) x = src[pos * io.in_e];
+      if (!INV && live) x = op(x, b, li, pos);
+      v[m * RD::R0 + r] = x;
+    }
+  constexpr int RL = (RD::NST == 3) ? RD::R2 : ((RD::NST == 2) ? RD::R1 : RD::R0);
+  line_fft<C, RD, INV>(v, s, t, tw);
+  C* dst = io.out + b * io.out_f + (long)li * io.out_l;
+#pragma unroll
+  for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int pos = reg_pos<n, RL, P>(t, m, r);
+      C x = v[m * RL + r];
+      if (INV && live) x = op(x, b, li, pos);
+      if (live && pos < io.out_count) dst[pos * io.out_e] = x;
+    }
+}
+
+// FFT -> pointwise multiply (freq-domain op, given line index and frequency index)
+// -> IFFT along the lines, global -> global.  Used for the strided (y) axis.
+template <typename C, typename RDF, int L, bool LFAST, typename FreqOp>
+__global__ void __launch_bounds__(L * RDF::P)
+k_line_conv(LineIO<C> io, const C* __restrict__ tw, FreqOp op) {
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  using RI = typename RDF::Rev;
+  extern __shared__ unsigned char smem_raw[];
+  C* smem = reinterpret_cast<C*>(smem_raw);
+  int l, t;
+  thread_map<L, P, LFAST>(threadIdx.x, l, t);
+  const int b = blockIdx.y;
+  const int li = blockIdx.x * L + l;
+  const bool live = li < io.nlines;
+  C* s = smem + l * LdsLine<n>::size;
+  const C* src = io.in + b * io.in_f + (long)li * io.in_l;
+  C v[V];
+#pragma unroll
+  for (int m = 0; m < V / RDF::R0; ++m)
+#pragma unroll
+    for (int r = 0; r < RDF::R0; ++r) {
+      const int pos = reg_pos<n, RDF::R0, P>(t, m, r);
+      C x = cmake<C>(0, 0);
+      if (live && pos < io.in_valid) x = src[pos * io.in_e];
+      v[m * RDF::R0 + r] = x;
+    }
+  line_fft<C, RDF, false>(v, s, t, tw);
+  constexpr int RL = RI::R0;  // last forward radix == first inverse radix
+#pragma unroll
+  for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int pos = reg_pos<n, RL, P>(t, m, r);
+      if (live) v[m * RL + r] = op(v[m * RL + r], b, li, pos);
+    }
+  line_fft<C, RI, true>(v, s, t, tw);
+  constexpr int RE = (RI::NST == 3) ? RI::R2 : ((RI::NST == 2) ? RI::R1 : RI::R0);
+  C* dst = io.out + b * io.out_f + (long)li * io.out_l;
+#pragma unroll
+  for (int m = 0; m < V / RE; ++m)
+#pragma unroll
+    for (int r = 0; r < RE; ++r) {
+      const int pos = reg_pos<n, RE, P>(t, m, r);
+      if (live && pos < io.out_count) dst[pos * io.out_e] = v[m * RE + r];
+    }
+}
+
+}  // namespace fh

@@ -1,0 +1,60 @@
+// Pointwise operators fused into the line passes, and the standalone elementwise
+// kernels (mask multiply, mask gradient, transfer-function tables).
+#pragma once
+
+#include "fh_lines.hpp"
+
+namespace fh {
+
+// ---------------------------------------------------------------- frequency-side ops
+
+// Multiply by a transfer function stored as one quadrant (the function depends on
+// |fx| and |fy| only): entry [min(ky, ny-ky)][min(kx, nx-kx)], row length qx.
+// `tab` holds one quadrant per table index; `tabidx[b]` picks the table of field b.
+// The 1/(nx*ny) of the inverse FFT is folded into the table.
+struct QuadMul {
+  const float2* tab;
+  const int* tabidx;
+  long tabsize;
+  int nx, ny, qx;
+  int conj;  // Backward direction multiplies by the conjugate
+  __device__ __forceinline__ float2 operator()(float2 v, int b, int kx, int ky) const {
+    const int qxi = kx <= nx - kx ? kx : nx - kx;
+    const int qyi = ky <= ny - ky ? ky : ny - ky;
+    const float2 h = tab[tabidx[b] * tabsize + (long)qyi * qx + qxi];
+    return conj ? cmulc(v, h) : cmul(v, h);
+  }
+};
+
+// Plain multiply by a full table (FP64 comparator path); the table is [ky][kx].
+struct FullMulD {
+  const double2* tab;
+  int nx;
+  int conj;
+  __device__ __forceinline__ double2 operator()(double2 v, int, int kx, int ky) const {
+    const double2 h = tab[(long)ky * nx + kx];
+    return conj ? cmulc(v, h) : cmul(v, h);
+  }
+};
+
+// ---------------------------------------------------------------- time-side ops (x lines)
+
+// u * cis(s*phi[y, x]), optionally saving the input first.  The line index is the row y
+// and the line position is x; phi is one nx*ny mask shared by every field.
+struct PhaseTimeOp {
+  const float* phi;
+  float2* saved;   // nullable: [B][ny][nx]
+  long fstride;    // nx*ny
+  int nx;
+  float s;         // +1 Forward, -1 Backward
+  __device__ __forceinline__ float2 operator()(float2 v, int b, int y, int x) const {
+    if (x >= nx) return v;
+    const long i = (long)y * nx + x;
+    if (saved) saved[b * fstride + i] = v;
+    float sn, cs;
+    sincosf(s * phi[i], &sn, &cs);
+    return cmul(v, make_float2(cs, sn));
+  }
+};
+
+}  // namespace fh

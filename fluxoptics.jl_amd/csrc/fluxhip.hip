@@ -1,0 +1,822 @@
+// libfluxhip: the C ABI of include/fluxhip.h and its kernel launches (gfx950).
+//
+// Propagation = x-line FFT pass -> y-line (FFT, transfer-function multiply, IFFT)
+// pass -> x-line IFFT pass, all in place on the caller's c64 array.  Transfer
+// functions are tables computed in FP64 on the device and rounded once to c64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fluxhip.h"
+#include "fh_ops.hpp"
+
+namespace fh {
+
+// ================================================================ errors
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define FH_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) return fail(FH_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// ================================================================ launch configurations
+// (radices, lines per workgroup) per precision, length and axis.  X passes read
+// contiguous lines; Y passes read L adjacent columns per row.
+
+template <typename T, int LOGN, bool Y> struct Cfg;
+#define FH_CFG(T, LOGN, Y, A, B, C_, LL)            \
+  template <> struct Cfg<T, LOGN, Y> {              \
+    using RD = Radices<A, B, C_>;                   \
+    static constexpr int L = LL;                    \
+  };
+// FP32
+FH_CFG(float, 4, false, 4, 4, 1, 64)
+FH_CFG(float, 5, false, 8, 4, 1, 64)
+FH_CFG(float, 6, false, 8, 8, 1, 32)
+FH_CFG(float, 7, false, 16, 8, 1, 32)
+FH_CFG(float, 8, false, 16, 16, 1, 16)
+FH_CFG(float, 9, false, 16, 32, 1, 16)
+FH_CFG(float, 10, false, 32, 32, 1, 8)
+FH_CFG(float, 11, false, 16, 16, 8, 2)
+FH_CFG(float, 12, false, 16, 16, 16, 1)
+FH_CFG(float, 4, true, 4, 4, 1, 16)
+FH_CFG(float, 5, true, 8, 4, 1, 16)
+FH_CFG(float, 6, true, 8, 8, 1, 16)
+FH_CFG(float, 7, true, 16, 8, 1, 16)
+FH_CFG(float, 8, true, 16, 16, 1, 16)
+FH_CFG(float, 9, true, 16, 32, 1, 16)
+FH_CFG(float, 10, true, 32, 32, 1, 8)
+FH_CFG(float, 11, true, 16, 16, 8, 8)
+FH_CFG(float, 12, true, 16, 16, 16, 2)
+// FP64 (tables and the plain comparator)
+FH_CFG(double, 4, false, 4, 4, 1, 64)
+FH_CFG(double, 5, false, 8, 4, 1, 64)
+FH_CFG(double, 6, false, 8, 8, 1, 32)
+FH_CFG(double, 7, false, 16, 8, 1, 16)
+FH_CFG(double, 8, false, 16, 16, 1, 8)
+FH_CFG(double, 9, false, 8, 8, 8, 4)
+FH_CFG(double, 10, false, 16, 8, 8, 4)
+FH_CFG(double, 11, false, 16, 16, 8, 2)
+FH_CFG(double, 12, false, 16, 16, 16, 1)
+FH_CFG(double, 4, true, 4, 4, 1, 16)
+FH_CFG(double, 5, true, 8, 4, 1, 16)
+FH_CFG(double, 6, true, 8, 8, 1, 16)
+FH_CFG(double, 7, true, 16, 8, 1, 16)
+FH_CFG(double, 8, true, 16, 16, 1, 16)
+FH_CFG(double, 9, true, 8, 8, 8, 8)
+FH_CFG(double, 10, true, 16, 8, 8, 8)
+FH_CFG(double, 11, true, 16, 16, 8, 4)
+FH_CFG(double, 12, true, 16, 16, 16, 2)
+#undef FH_CFG
+
+constexpr int kMinLog = 4, kMaxLog = 12;
+
+template <typename C> struct RealOf;
+template <> struct RealOf<float2> { using type = float; };
+template <> struct RealOf<double2> { using type = double; };
+
+template <typename K>
+static hipError_t ensure_lds(K kernel, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <typename C, int LOGN, bool Y, bool INV, typename Op>
+static hipError_t launch_fft_t(const LineIO<C>& io, int batch, const C* tw, Op op, hipStream_t st) {
+  using CF = Cfg<typename RealOf<C>::type, LOGN, Y>;
+  using RD = typename CF::RD;
+  constexpr int L = CF::L;
+  static_assert(RD::n == (1 << LOGN), "radix plan does not match the length");
+  auto kern = k_line_fft<C, RD, L, Y, INV, Op>;
+  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n>::size * sizeof(C) : 0;
+  hipError_t e = ensure_lds(kern, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((io.nlines + L - 1) / L, batch);
+  hipLaunchKernelGGL(kern, grid, dim3(L * RD::P), lds, st, io, tw, op);
+  return hipGetLastError();
+}
+
+template <typename C, int LOGN, typename Op>
+static hipError_t launch_conv_t(const LineIO<C>& io, int batch, const C* tw, Op op, hipStream_t st) {
+  using CF = Cfg<typename RealOf<C>::type, LOGN, true>;
+  using RD = typename CF::RD;
+  constexpr int L = CF::L;
+  auto kern = k_line_conv<C, RD, L, true, Op>;
+  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n>::size * sizeof(C) : 0;
+  hipError_t e = ensure_lds(kern, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((io.nlines + L - 1) / L, batch);
+  hipLaunchKernelGGL(kern, grid, dim3(L * RD::P), lds, st, io, tw, op);
+  return hipGetLastError();
+}
+
+#define FH_LOGN_SWITCH(logn, CALL)                                       \
+  switch (logn) {                                                        \
+    case 4: { constexpr int LG = 4; return CALL; }                       \
+    case 5: { constexpr int LG = 5; return CALL; }                       \
+    case 6: { constexpr int LG = 6; return CALL; }                       \
+    case 7: { constexpr int LG = 7; return CALL; }                       \
+    case 8: { constexpr int LG = 8; return CALL; }                       \
+    case 9: { constexpr int LG = 9; return CALL; }                       \
+    case 10: { constexpr int LG = 10; return CALL; }                     \
+    case 11: { constexpr int LG = 11; return CALL; }                     \
+    case 12: { constexpr int LG = 12; return CALL; }                     \
+    default: return hipErrorInvalidValue;                                \
+  }
+
+template <typename C, bool Y, bool INV, typename Op>
+static hipError_t launch_fft(int logn, const LineIO<C>& io, int batch, const C* tw, Op op,
+                             hipStream_t st) {
+  FH_LOGN_SWITCH(logn, (launch_fft_t<C, LG, Y, INV, Op>(io, batch, tw, op, st)))
+}
+
+template <typename C, typename Op>
+static hipError_t launch_conv(int logn, const LineIO<C>& io, int batch, const C* tw, Op op,
+                              hipStream_t st) {
+  FH_LOGN_SWITCH(logn, (launch_conv_t<C, LG, Op>(io, batch, tw, op, st)))
+}
+
+// ================================================================ table kernels (FP64)
+
+// Angular-spectrum transfer function on one quadrant, computed in FP64 and rounded
+// once to c64 (angular_spectrum.jl:1-12 with double_precision_kernel=true):
+//   a = 1/(lam/n0)^2 - fx^2 - fy^2 (evaluated in this order, no contraction, as the
+//   reference's FP64 broadcast does),
+//   a >= 0: H = cis(2*pi*t), t = z*sqrt(a) in cycles, reduced to t - rint(t) before
+//           sincospi so the phase keeps full FP64 accuracy at |2*pi*t| ~ 1e5 rad;
+//   a <  0: H = exp(-2*pi*|z|*sqrt(-a)) (principal sqrt; the same for z < 0, where
+//           H(z) = conj(H(-z))).
+// The inverse FFT's 1/(nx*ny) is folded in (scale).
+__global__ void k_as_table(float2* __restrict__ tab, int nx, int ny, int qx, int qy, double dx,
+                           double dy, double lam, double n0, double z, double scale) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= qx || j >= qy) return;
+  const double fx = (double)i / ((double)nx * dx);
+  const double fy = (double)j / ((double)ny * dy);
+  const double le = lam / n0;
+  const double f2 = 1.0 / (le * le);
+  const double a = f2 - fx * fx - fy * fy;
+  double re, im;
+  if (a >= 0.0) {
+    const double t = z * sqrt(a);
+    const double r = t - rint(t);
+    sincospi(2.0 * r, &im, &re);
+    re *= scale;
+    im *= scale;
+  } else {
+    re = exp(-2.0 * M_PI * fabs(z) * sqrt(-a)) * scale;
+    im = 0.0;
+  }
+  tab[(long)j * qx + i] = make_float2((float)re, (float)im);
+}
+
+// Full FP64 transfer-function table [ny][nx] for the plain comparator (same formula).
+__global__ void k_as_table_full_d(double2* __restrict__ tab, int nx, int ny, double dx, double dy,
+                                  double lam, double n0, double z, double scale) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx) return;
+  const int ki = i < (nx + 1) / 2 ? i : i - nx;
+  const int kj = j < (ny + 1) / 2 ? j : j - ny;
+  const double fx = (double)ki / ((double)nx * dx);
+  const double fy = (double)kj / ((double)ny * dy);
+  const double le = lam / n0;
+  const double f2 = 1.0 / (le * le);
+  const double a = f2 - fx * fx - fy * fy;
+  double re, im;
+  if (a >= 0.0) {
+    const double t = z * sqrt(a);
+    const double r = t - rint(t);
+    sincospi(2.0 * r, &im, &re);
+  } else {
+    re = exp(-2.0 * M_PI * fabs(z) * sqrt(-a));
+    im = 0.0;
+  }
+  tab[(long)j * nx + i] = make_double2(re * scale, im * scale);
+}
+
+// Rayleigh-Sommerfeld impulse response sampled on the (2nx)x(2ny) padded grid in FP64
+// (rayleigh_sommerfeld.jl:1-12, nrm = dx*dy/(2*pi)).  Offsets are i for i <= nx and
+// i - 2nx above, so the samples are even in both axes (offset +-nx is never read by a
+// cropped output of the linear convolution).  The phase k*r is reduced in cycles:
+// r/lam - rint(r/lam).
+__global__ void k_rs_sample(double2* __restrict__ d, int nx, int ny, double dx, double dy,
+                            double lam, double z, double nrm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int nx2 = 2 * nx;
+  if (i >= nx2) return;
+  const int oi = i <= nx ? i : i - nx2;
+  const int oj = j <= ny ? j : j - 2 * ny;
+  const double x = oi * dx, y = oj * dy;
+  const double az = fabs(z);
+  const double r = sqrt(x * x + y * y + az * az);
+  const double k = 2.0 * M_PI / lam;
+  const double cyc = r / lam;
+  double sn, cs;
+  sincospi(2.0 * (cyc - rint(cyc)), &sn, &cs);
+  // nrm * (e^{ikr}/r) * (z/r) * (1/r - i k)
+  const double g = nrm * az / (r * r);
+  const double ar = 1.0 / r, ai = -k;
+  double re = g * (cs * ar - sn * ai);
+  double im = g * (cs * ai + sn * ar);
+  if (z < 0) im = -im;  // h(z) = conj(h(-z))
+  d[(long)j * nx2 + i] = make_double2(re, im);
+}
+
+// Quadrant [0..nx]x[0..ny] of an FP64 spectrum on the padded grid, scaled and rounded.
+__global__ void k_quad_extract(float2* __restrict__ tab, const double2* __restrict__ d, int nx2,
+                               int qx, int qy, double scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= qx || j >= qy) return;
+  const double2 v = d[(long)j * nx2 + i];
+  tab[(long)j * qx + i] = make_float2((float)(v.x * scale), (float)(v.y * scale));
+}
+
+// ================================================================ elementwise kernels
+
+// u <- u * cis(s*phi), saving u first when saved != nullptr (phasemask.jl:90-113).
+__global__ void k_phase_apply(float2* __restrict__ u, const float* __restrict__ phi,
+                              float2* __restrict__ saved, long npix, int batch, float s) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix;
+       i += (long)gridDim.x * blockDim.x) {
+    float sn, cs;
+    sincosf(s * phi[i], &sn, &cs);
+    const float2 w = make_float2(cs, sn);
+    for (int b = 0; b < batch; ++b) {
+      const long k = b * npix + i;
+      const float2 v = u[k];
+      if (saved) saved[k] = v;
+      u[k] = cmul(v, w);
+    }
+  }
+}
+
+// du <- dv * cis(-s*phi) in place; dphi (+)= s * sum_b Im(du * conj(saved)), summed in
+// field order (phasemask.jl:115-150).
+__global__ void k_phase_backward(float2* __restrict__ dv, const float* __restrict__ phi,
+                                 const float2* __restrict__ saved, float* __restrict__ dphi,
+                                 long npix, int batch, float s, int acc) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix;
+       i += (long)gridDim.x * blockDim.x) {
+    float sn, cs;
+    sincosf(-s * phi[i], &sn, &cs);
+    const float2 w = make_float2(cs, sn);
+    float g = 0.f;
+    for (int b = 0; b < batch; ++b) {
+      const long k = b * npix + i;
+      const float2 du = cmul(dv[k], w);
+      dv[k] = du;
+      if (dphi) g += cmulc(du, saved[k]).y;
+    }
+    if (dphi) dphi[i] = (acc ? dphi[i] : 0.f) + s * g;
+  }
+}
+
+__global__ void k_mul_full_d(double2* __restrict__ u, const double2* __restrict__ tab, long npix,
+                             int conj) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix;
+       i += (long)gridDim.x * blockDim.x) {
+    const double2 h = tab[i];
+    u[i] = conj ? cmulc(u[i], h) : cmul(u[i], h);
+  }
+}
+
+// ================================================================ fused x-IFFT + Phase adjoint + gradient
+// One workgroup owns L rows and loops over the fields of the batch so the mask gradient
+// is summed in registers, in field order, and written once.
+template <typename RDF, int L>
+__global__ void __launch_bounds__(L * RDF::P)
+k_xifft_phase_grad(LineIO<float2> io, int batch, const float2* __restrict__ tw,
+                   const float* __restrict__ phi, const float2* __restrict__ saved,
+                   float* __restrict__ dphi, int nx, float s, int acc) {
+  using RD = typename RDF::Rev;
+  constexpr int n = RD::n, V = RD::V, P = RD::P;
+  constexpr int RL = (RD::NST == 3) ? RD::R2 : ((RD::NST == 2) ? RD::R1 : RD::R0);
+  extern __shared__ unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  int l, t;
+  thread_map<L, P, false>(threadIdx.x, l, t);
+  const int li = blockIdx.x * L + l;
+  const bool live = li < io.nlines;
+  float2* sl = smem + l * LdsLine<n>::size;
+  float g[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) g[k] = 0.f;
+  for (int b = 0; b < batch; ++b) {
+    const float2* src = io.in + b * io.in_f + (long)li * io.in_l;
+    float2 v[V];
+#pragma unroll
+    for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+      for (int r = 0; r < RD::R0; ++r) {
+        const int pos = reg_pos<n, RD::R0, P>(t, m, r);
+        float2 x = make_float2(0.f, 0.f);
+        if (live && pos < io.in_valid) x = src[pos * io.in_e];
+        v[m * RD::R0 + r] = x;
+      }
+    line_fft<float2, RD, true>(v, sl, t, tw);
+    float2* dst = io.out + b * io.out_f + (long)li * io.out_l;
+    const float2* sv = saved + b * io.out_f + (long)li * io.out_l;
+#pragma unroll
+    for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int pos = reg_pos<n, RL, P>(t, m, r);
+        if (live && pos < io.out_count) {
+          float sn, cs;
+          sincosf(-s * phi[(long)li * nx + pos], &sn, &cs);
+          const float2 du = cmul(v[m * RL + r], make_float2(cs, sn));
+          dst[pos * io.out_e] = du;
+          if (dphi) g[m * RL + r] += cmulc(du, sv[pos * io.out_e]).y;
+        }
+      }
+  }
+  if (dphi && live) {
+#pragma unroll
+    for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int pos = reg_pos<n, RL, P>(t, m, r);
+        if (pos < io.out_count) {
+          const long k = (long)li * nx + pos;
+          dphi[k] = (acc ? dphi[k] : 0.f) + s * g[m * RL + r];
+        }
+      }
+  }
+}
+
+template <int LOGN>
+static hipError_t launch_xifft_phase_grad_t(const LineIO<float2>& io, int batch, const float2* tw,
+                                            const float* phi, const float2* saved, float* dphi,
+                                            int nx, float s, int acc, hipStream_t st) {
+  using CF = Cfg<float, LOGN, false>;
+  using RD = typename CF::RD;
+  constexpr int L = CF::L;
+  auto kern = k_xifft_phase_grad<RD, L>;
+  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n>::size * sizeof(float2) : 0;
+  hipError_t e = ensure_lds(kern, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((io.nlines + L - 1) / L, 1);
+  hipLaunchKernelGGL(kern, grid, dim3(L * RD::P), lds, st, io, batch, tw, phi, saved, dphi, nx, s,
+                     acc);
+  return hipGetLastError();
+}
+
+static hipError_t launch_xifft_phase_grad(int logn, const LineIO<float2>& io, int batch,
+                                          const float2* tw, const float* phi, const float2* saved,
+                                          float* dphi, int nx, float s, int acc, hipStream_t st) {
+  FH_LOGN_SWITCH(logn, (launch_xifft_phase_grad_t<LG>(io, batch, tw, phi, saved, dphi, nx, s, acc,
+                                                      st)))
+}
+
+// ================================================================ host objects
+
+static int ilog2_exact(int n) {
+  if (n <= 0 || (n & (n - 1))) return -1;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+}  // namespace fh
+
+using namespace fh;
+
+struct fh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<std::pair<int, int>, void*> twiddles;  // (n, is_double) -> W_n^k table
+  std::mutex mu;
+};
+
+struct fh_plan {
+  fh_ctx* ctx = nullptr;
+  int kind = 0;  // 0 = AS, 1 = RS
+  int nx = 0, ny = 0, batch = 0;
+  int lx = 0, ly = 0;   // log2 of the transform lengths (padded for RS)
+  double dx = 0, dy = 0, z = 0, n0 = 1;
+  std::vector<double> lam_tables;  // distinct wavelengths, table order
+  std::vector<int> field_table;    // table index per field
+  float2* tab = nullptr;           // quadrant tables, one per distinct wavelength
+  long tabsize = 0;
+  int qx = 0, qy = 0;
+  int* d_tabidx = nullptr;
+  float2* scratch = nullptr;       // RS: (2nx) x ny x batch
+  double2* plain_tab = nullptr;    // plain FP64 comparator (AS): full tables
+};
+
+namespace fh {
+
+static int set_device(fh_ctx* c) {
+  FH_HIP(hipSetDevice(c->device));
+  return FH_OK;
+}
+
+// W_n^k = exp(-2*pi*i*k/n), k < n, from long-double sin/cos of the exactly reduced
+// angle, rounded once to the table precision.
+template <typename C>
+static int get_twiddles(fh_ctx* c, int n, const C** out) {
+  const int isd = sizeof(C) == sizeof(double2);
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto key = std::make_pair(n, isd);
+  auto it = c->twiddles.find(key);
+  if (it != c->twiddles.end()) {
+    *out = reinterpret_cast<const C*>(it->second);
+    return FH_OK;
+  }
+  std::vector<C> h(n);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int k = 0; k < n; ++k) {
+    const long double a = two_pi * (long double)k / (long double)n;
+    h[k].x = (decltype(h[k].x))cosl(a);
+    h[k].y = (decltype(h[k].y))(-sinl(a));
+  }
+  void* d = nullptr;
+  if (hipMalloc(&d, n * sizeof(C)) != hipSuccess) return fail(FH_E_NOMEM, "twiddle table: out of device memory");
+  FH_HIP(hipMemcpy(d, h.data(), n * sizeof(C), hipMemcpyHostToDevice));
+  c->twiddles[key] = d;
+  *out = reinterpret_cast<const C*>(d);
+  return FH_OK;
+}
+
+static int check_dir(int direction) {
+  if (direction != 1 && direction != -1) return fail(FH_E_ARG, "direction must be +1 (Forward) or -1 (Backward), got %d", direction);
+  return FH_OK;
+}
+
+static int plan_common(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
+                       const double* lambdas, int nlambdas, fh_plan* p) {
+  if (!ctx) return fail(FH_E_ARG, "null context");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive (nx=%d ny=%d batch=%d)", nx, ny, batch);
+  if (!(dx > 0) || !(dy > 0)) return fail(FH_E_ARG, "sampling must be positive");
+  if (!lambdas || (nlambdas != 1 && nlambdas != batch))
+    return fail(FH_E_ARG, "need 1 or batch (=%d) wavelengths, got %d", batch, nlambdas);
+  for (int k = 0; k < nlambdas; ++k)
+    if (!(lambdas[k] > 0)) return fail(FH_E_ARG, "wavelengths must be positive");
+  p->ctx = ctx;
+  p->nx = nx; p->ny = ny; p->batch = batch; p->dx = dx; p->dy = dy;
+  p->field_table.resize(batch);
+  for (int b = 0; b < batch; ++b) {
+    const double lam = lambdas[nlambdas == 1 ? 0 : b];
+    int idx = -1;
+    for (size_t k = 0; k < p->lam_tables.size(); ++k)
+      if (p->lam_tables[k] == lam) { idx = (int)k; break; }
+    if (idx < 0) { idx = (int)p->lam_tables.size(); p->lam_tables.push_back(lam); }
+    p->field_table[b] = idx;
+  }
+  return FH_OK;
+}
+
+static void plan_free(fh_plan* p) {
+  if (!p) return;
+  if (p->tab) (void)hipFree(p->tab);
+  if (p->d_tabidx) (void)hipFree(p->d_tabidx);
+  if (p->scratch) (void)hipFree(p->scratch);
+  if (p->plain_tab) (void)hipFree(p->plain_tab);
+  delete p;
+}
+
+static int upload_tabidx(fh_plan* p) {
+  if (hipMalloc(&p->d_tabidx, p->batch * sizeof(int)) != hipSuccess) return fail(FH_E_NOMEM, "table index: out of device memory");
+  FH_HIP(hipMemcpy(p->d_tabidx, p->field_table.data(), p->batch * sizeof(int), hipMemcpyHostToDevice));
+  return FH_OK;
+}
+
+// In-place FP64 2-D FFT of a [ny2][nx2] double2 array (table generation).
+static int fft2_d(fh_ctx* c, double2* d, int nx2, int ny2, int lx, int ly) {
+  const double2 *twx = nullptr, *twy = nullptr;
+  int rc;
+  if ((rc = get_twiddles(c, nx2, &twx)) || (rc = get_twiddles(c, ny2, &twy))) return rc;
+  LineIO<double2> io{};
+  io.in = d; io.out = d; io.in_f = io.out_f = (long)nx2 * ny2;
+  io.in_l = io.out_l = nx2; io.in_e = io.out_e = 1;
+  io.nlines = ny2; io.in_valid = io.out_count = nx2;
+  FH_HIP((launch_fft<double2, false, false>(lx, io, 1, twx, NoOp{}, c->stream)));
+  io.in_l = io.out_l = 1; io.in_e = io.out_e = nx2;
+  io.nlines = nx2; io.in_valid = io.out_count = ny2;
+  FH_HIP((launch_fft<double2, true, false>(ly, io, 1, twy, NoOp{}, c->stream)));
+  return FH_OK;
+}
+
+// The passes of one propagation of fields u (or, fused, of Phase then propagation):
+// pass 0 (x FFT) and pass 1 (y FFT * kernel * IFFT); `only` >= 0 runs just that pass.
+template <typename XOp>
+static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1) {
+  fh_ctx* c = p->ctx;
+  const int nxl = 1 << p->lx, nyl = 1 << p->ly;
+  const float2 *twx = nullptr, *twy = nullptr;
+  int rc;
+  if ((rc = get_twiddles(c, nxl, &twx)) || (rc = get_twiddles(c, nyl, &twy))) return rc;
+  float2* mid = p->kind == 0 ? u : p->scratch;   // x-spectrum rows
+  const long midf = (long)nxl * p->ny;
+  QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, nyl, p->qx, direction < 0};
+  LineIO<float2> io{};
+  // x FFT: rows of u (length nx, zero-extended to nxl) -> rows of mid
+  io.in = u; io.out = mid; io.in_f = (long)p->nx * p->ny; io.out_f = midf;
+  io.in_l = p->nx; io.out_l = nxl; io.in_e = io.out_e = 1;
+  io.nlines = p->ny; io.in_valid = p->nx; io.out_count = nxl;
+  if (only < 0 || only == 0)
+    FH_HIP((launch_fft<float2, false, false>(p->lx, io, p->batch, twx, xop, c->stream)));
+  if (only == 0) return FH_OK;
+  // y: FFT, multiply, IFFT on the nxl columns of mid (ny valid rows of nyl)
+  io.in = mid; io.out = mid; io.in_f = io.out_f = midf;
+  io.in_l = io.out_l = 1; io.in_e = io.out_e = nxl;
+  io.nlines = nxl; io.in_valid = p->ny; io.out_count = p->ny;
+  FH_HIP((launch_conv<float2>(p->ly, io, p->batch, twy, q, c->stream)));
+  return FH_OK;
+}
+
+static int run_xifft(fh_plan* p, float2* u) {
+  fh_ctx* c = p->ctx;
+  const int nxl = 1 << p->lx;
+  const float2* twx = nullptr;
+  int rc;
+  if ((rc = get_twiddles(c, nxl, &twx))) return rc;
+  float2* mid = p->kind == 0 ? u : p->scratch;
+  LineIO<float2> io{};
+  io.in = mid; io.out = u; io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
+  io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1;
+  io.nlines = p->ny; io.in_valid = nxl; io.out_count = p->nx;
+  FH_HIP((launch_fft<float2, false, true>(p->lx, io, p->batch, twx, NoOp{}, c->stream)));
+  return FH_OK;
+}
+
+}  // namespace fh
+
+// ================================================================ C ABI
+
+extern "C" {
+
+int fh_version(void) { return 100; }
+
+const char* fh_last_error(void) { return g_err.c_str(); }
+
+int fh_max_length(void) { return 1 << kMaxLog; }
+
+int fh_ctx_create(int device, void* stream, fh_ctx** out) {
+  if (!out) return fail(FH_E_ARG, "null output pointer");
+  int ndev = 0;
+  FH_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(FH_E_ARG, "device %d out of range (%d devices)", device, ndev);
+  fh_ctx* c = new fh_ctx();
+  c->device = device;
+  c->stream = reinterpret_cast<hipStream_t>(stream);
+  *out = c;
+  return FH_OK;
+}
+
+int fh_ctx_set_stream(fh_ctx* ctx, void* stream) {
+  if (!ctx) return fail(FH_E_ARG, "null context");
+  ctx->stream = reinterpret_cast<hipStream_t>(stream);
+  return FH_OK;
+}
+
+int fh_ctx_synchronize(fh_ctx* ctx) {
+  if (!ctx) return fail(FH_E_ARG, "null context");
+  FH_HIP(hipStreamSynchronize(ctx->stream));
+  return FH_OK;
+}
+
+int fh_ctx_destroy(fh_ctx* ctx) {
+  if (!ctx) return FH_OK;
+  (void)hipSetDevice(ctx->device);
+  for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+  delete ctx;
+  return FH_OK;
+}
+
+int fh_asprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
+                     const double* lambdas, int nlambdas, double z, double n0, fh_plan** out) {
+  if (!out) return fail(FH_E_ARG, "null output pointer");
+  *out = nullptr;
+  fh_plan* p = new fh_plan();
+  int rc = plan_common(ctx, nx, ny, batch, dx, dy, lambdas, nlambdas, p);
+  if (rc) { plan_free(p); return rc; }
+  if (!(n0 > 0) || !std::isfinite(z)) { plan_free(p); return fail(FH_E_ARG, "n0 must be positive and z finite"); }
+  p->kind = 0; p->z = z; p->n0 = n0;
+  p->lx = ilog2_exact(nx); p->ly = ilog2_exact(ny);
+  if (p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
+    plan_free(p);
+    return fail(FH_E_UNSUPP, "ASProp needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << kMinLog, 1 << kMaxLog, nx, ny);
+  }
+  if ((rc = set_device(ctx))) { plan_free(p); return rc; }
+  p->qx = nx / 2 + 1; p->qy = ny / 2 + 1; p->tabsize = (long)p->qx * p->qy;
+  const size_t bytes = p->tabsize * p->lam_tables.size() * sizeof(float2);
+  if (hipMalloc(&p->tab, bytes) != hipSuccess) { plan_free(p); return fail(FH_E_NOMEM, "ASProp table: out of device memory"); }
+  const double scale = 1.0 / ((double)nx * (double)ny);
+  for (size_t k = 0; k < p->lam_tables.size(); ++k) {
+    dim3 grid((p->qx + 255) / 256, p->qy);
+    hipLaunchKernelGGL(k_as_table, grid, dim3(256), 0, ctx->stream, p->tab + k * p->tabsize, nx, ny,
+                       p->qx, p->qy, dx, dy, p->lam_tables[k], n0, z, scale);
+    if (hipGetLastError() != hipSuccess) { plan_free(p); return fail(FH_E_HIP, "ASProp table kernel launch failed"); }
+  }
+  if ((rc = upload_tabidx(p))) { plan_free(p); return rc; }
+  *out = p;
+  return FH_OK;
+}
+
+int fh_rsprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
+                     const double* lambdas, int nlambdas, double z, fh_plan** out) {
+  if (!out) return fail(FH_E_ARG, "null output pointer");
+  *out = nullptr;
+  fh_plan* p = new fh_plan();
+  int rc = plan_common(ctx, nx, ny, batch, dx, dy, lambdas, nlambdas, p);
+  if (rc) { plan_free(p); return rc; }
+  if (!std::isfinite(z) || z == 0.0) { plan_free(p); return fail(FH_E_ARG, "RSProp needs a finite, nonzero z"); }
+  p->kind = 1; p->z = z;
+  p->lx = ilog2_exact(2 * nx); p->ly = ilog2_exact(2 * ny);
+  if (p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
+    plan_free(p);
+    return fail(FH_E_UNSUPP, "RSProp needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << (kMinLog - 1), 1 << (kMaxLog - 1), nx, ny);
+  }
+  if ((rc = set_device(ctx))) { plan_free(p); return rc; }
+  const int nx2 = 2 * nx, ny2 = 2 * ny;
+  p->qx = nx + 1; p->qy = ny + 1; p->tabsize = (long)p->qx * p->qy;
+  if (hipMalloc(&p->tab, p->tabsize * p->lam_tables.size() * sizeof(float2)) != hipSuccess) {
+    plan_free(p);
+    return fail(FH_E_NOMEM, "RSProp table: out of device memory");
+  }
+  if (hipMalloc(&p->scratch, (size_t)nx2 * ny * batch * sizeof(float2)) != hipSuccess) {
+    plan_free(p);
+    return fail(FH_E_NOMEM, "RSProp workspace: out of device memory");
+  }
+  double2* d = nullptr;
+  if (hipMalloc(&d, (size_t)nx2 * ny2 * sizeof(double2)) != hipSuccess) {
+    plan_free(p);
+    return fail(FH_E_NOMEM, "RSProp FP64 kernel buffer: out of device memory");
+  }
+  const double scale = 1.0 / ((double)nx2 * (double)ny2);
+  const double nrm = dx * dy / (2.0 * M_PI);
+  for (size_t k = 0; k < p->lam_tables.size() && rc == FH_OK; ++k) {
+    hipLaunchKernelGGL(k_rs_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, d, nx, ny,
+                       dx, dy, p->lam_tables[k], z, nrm);
+    if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "RS sample kernel launch failed"); break; }
+    if ((rc = fft2_d(ctx, d, nx2, ny2, p->lx, p->ly))) break;
+    hipLaunchKernelGGL(k_quad_extract, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
+                       p->tab + k * p->tabsize, d, nx2, p->qx, p->qy, scale);
+    if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "RS table kernel launch failed"); break; }
+  }
+  if (rc == FH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(FH_E_HIP, "RS table generation failed");
+  (void)hipFree(d);
+  if (rc == FH_OK) rc = upload_tabidx(p);
+  if (rc) { plan_free(p); return rc; }
+  *out = p;
+  return FH_OK;
+}
+
+int fh_plan_destroy(fh_plan* plan) {
+  if (!plan) return FH_OK;
+  (void)hipSetDevice(plan->ctx->device);
+  plan_free(plan);
+  return FH_OK;
+}
+
+int fh_propagate(fh_plan* p, void* u, int direction) {
+  if (!p || !u) return fail(FH_E_ARG, "null plan or field");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
+  float2* f = reinterpret_cast<float2*>(u);
+  if ((rc = run_prop(p, f, direction, NoOp{}))) return rc;
+  return run_xifft(p, f);
+}
+
+int fh_propagate_pass(fh_plan* p, void* u, int direction, int pass) {
+  if (!p || !u) return fail(FH_E_ARG, "null plan or field");
+  if (pass < 0 || pass > 2) return fail(FH_E_ARG, "pass must be 0, 1 or 2, got %d", pass);
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
+  float2* f = reinterpret_cast<float2*>(u);
+  if (pass < 2) return run_prop(p, f, direction, NoOp{}, pass);
+  return run_xifft(p, f);
+}
+
+int fh_phase_propagate(fh_plan* p, void* u, const float* phi, void* saved, int direction) {
+  if (!p || !u || !phi) return fail(FH_E_ARG, "null plan, field or mask");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
+  float2* f = reinterpret_cast<float2*>(u);
+  PhaseTimeOp op{phi, reinterpret_cast<float2*>(saved), (long)p->nx * p->ny, p->nx, (float)direction};
+  if ((rc = run_prop(p, f, direction, op))) return rc;
+  return run_xifft(p, f);
+}
+
+int fh_phase_apply(fh_ctx* ctx, void* u, const float* phi, int nx, int ny, int batch, int direction,
+                   void* saved) {
+  if (!ctx || !u || !phi) return fail(FH_E_ARG, "null context, field or mask");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(ctx))) return rc;
+  const long npix = (long)nx * ny;
+  const int blocks = (int)std::min<long>((npix + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_phase_apply, dim3(blocks), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<float2*>(u), phi, reinterpret_cast<float2*>(saved), npix, batch,
+                     (float)direction);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_phase_backward(fh_ctx* ctx, void* dv, const float* phi, const void* saved, float* dphi, int nx,
+                      int ny, int batch, int direction, int accumulate) {
+  if (!ctx || !dv || !phi) return fail(FH_E_ARG, "null context, cotangent or mask");
+  if (dphi && !saved) return fail(FH_E_ARG, "the mask gradient needs the saved mask input");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(ctx))) return rc;
+  const long npix = (long)nx * ny;
+  const int blocks = (int)std::min<long>((npix + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_phase_backward, dim3(blocks), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<float2*>(dv), phi, reinterpret_cast<const float2*>(saved), dphi,
+                     npix, batch, (float)direction, accumulate);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const void* saved, float* dphi,
+                                int direction, int accumulate) {
+  if (!p || !dv || !phi) return fail(FH_E_ARG, "null plan, cotangent or mask");
+  if (dphi && !saved) return fail(FH_E_ARG, "the mask gradient needs the saved mask input");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
+  float2* f = reinterpret_cast<float2*>(dv);
+  // adjoint propagation = the propagator in the reverse direction
+  if ((rc = run_prop(p, f, -direction, NoOp{}))) return rc;
+  const int nxl = 1 << p->lx;
+  const float2* twx = nullptr;
+  if ((rc = get_twiddles(p->ctx, nxl, &twx))) return rc;
+  float2* mid = p->kind == 0 ? f : p->scratch;
+  LineIO<float2> io{};
+  io.in = mid; io.out = f; io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
+  io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1;
+  io.nlines = p->ny; io.in_valid = nxl; io.out_count = p->nx;
+  FH_HIP(launch_xifft_phase_grad(p->lx, io, p->batch, twx, phi, reinterpret_cast<const float2*>(saved),
+                                 dphi, p->nx, (float)direction, accumulate, p->ctx->stream));
+  return FH_OK;
+}
+
+int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {
+  if (!p || !u_c128) return fail(FH_E_ARG, "null plan or field");
+  if (p->kind != 0) return fail(FH_E_UNSUPP, "the plain FP64 comparator covers ASProp only");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
+  fh_ctx* c = p->ctx;
+  const int nx = p->nx, ny = p->ny;
+  const long npix = (long)nx * ny;
+  if (!p->plain_tab) {
+    if (hipMalloc(&p->plain_tab, npix * p->lam_tables.size() * sizeof(double2)) != hipSuccess)
+      return fail(FH_E_NOMEM, "plain FP64 table: out of device memory");
+    for (size_t k = 0; k < p->lam_tables.size(); ++k) {
+      hipLaunchKernelGGL(k_as_table_full_d, dim3((nx + 255) / 256, ny), dim3(256), 0, c->stream,
+                         p->plain_tab + k * npix, nx, ny, p->dx, p->dy, p->lam_tables[k], p->n0, p->z,
+                         1.0 / ((double)nx * ny));
+      FH_HIP(hipGetLastError());
+    }
+  }
+  const double2 *twx = nullptr, *twy = nullptr;
+  if ((rc = get_twiddles(c, nx, &twx)) || (rc = get_twiddles(c, ny, &twy))) return rc;
+  double2* u = reinterpret_cast<double2*>(u_c128);
+  for (int b = 0; b < p->batch; ++b) {
+    double2* f = u + b * npix;
+    LineIO<double2> io{};
+    io.in = f; io.out = f; io.in_f = io.out_f = npix;
+    io.in_l = io.out_l = nx; io.in_e = io.out_e = 1; io.nlines = ny; io.in_valid = io.out_count = nx;
+    LineIO<double2> iy = io;
+    iy.in_l = iy.out_l = 1; iy.in_e = iy.out_e = nx; iy.nlines = nx; iy.in_valid = iy.out_count = ny;
+    FH_HIP((launch_fft<double2, false, false>(p->lx, io, 1, twx, NoOp{}, c->stream)));
+    FH_HIP((launch_fft<double2, true, false>(p->ly, iy, 1, twy, NoOp{}, c->stream)));
+    const int blocks = (int)std::min<long>((npix + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_mul_full_d, dim3(blocks), dim3(256), 0, c->stream, f,
+                       p->plain_tab + p->field_table[b] * npix, npix, direction < 0);
+    FH_HIP(hipGetLastError());
+    FH_HIP((launch_fft<double2, true, true>(p->ly, iy, 1, twy, NoOp{}, c->stream)));
+    FH_HIP((launch_fft<double2, false, true>(p->lx, io, 1, twx, NoOp{}, c->stream)));
+  }
+  return FH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+"""fluxoptics (MI355X): the FluxOptics.jl propagation path on libfluxhip.
+
+Host-side mirror of the reference's interface for ASProp, RSProp, Phase and AS_BPM;
+the arithmetic is the HIP library's (``libfluxhip.so``, C ABI in include/fluxhip.h).
+"""
+
+from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficiency,
+                     rs_valid_distance)
+from .components import (ASProp, Phase, RSProp, phase_propagate_, propagate_phase_backward_)
+
+__all__ = [
+    "ASProp", "Backward", "Direction", "Forward", "Phase", "RSProp", "ScalarField",
+    "coupling_efficiency", "phase_propagate_", "propagate_phase_backward_", "propagate",
+    "propagate_", "backpropagate_", "rs_valid_distance",
+]
+
+
+def propagate_(u, component, direction=Forward):
+    """``propagate!(u, p, direction)``: in place."""
+    return component.propagate_(u, direction)
+
+
+def propagate(u, component, direction=Forward):
+    """``propagate(u, p, direction)``: on a copy."""
+    return component.propagate(u, direction)
+
+
+def backpropagate_(dv, component, direction=Forward):
+    """``backpropagate!(dv, p, direction)``."""
+    return component.backpropagate_(dv, direction)

@@ -1,0 +1,262 @@
+"""Propagators and modulators on the accelerated path: ASProp, RSProp, Phase.
+
+Each class mirrors the reference component's constructor keywords and its
+``propagate!`` / ``backpropagate!`` / ``propagate_and_save!`` /
+``backpropagate_with_gradient!`` contract
+(``src/optical_components/OpticalComponents.jl:726-816``), with Python names
+``propagate_`` (in place), ``propagate`` (copy), ``backpropagate_``,
+``propagate_and_save_`` and ``backpropagate_with_gradient_``.  All arithmetic runs in
+libfluxhip on the field's HIP device.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import warnings
+from typing import Callable, Optional, Union
+
+import numpy as np
+import torch
+
+from . import _lib
+from .fields import Backward, Forward, ScalarField, dir_sign, rs_valid_distance
+
+# --------------------------------------------------------------------------- contexts
+
+_contexts = {}
+
+
+def context(device: torch.device) -> ctypes.c_void_p:
+    """The library context of a HIP device, bound to torch's current stream."""
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    ctx = _contexts.get(idx)
+    if ctx is None:
+        ctx = ctypes.c_void_p()
+        _lib.call("fh_ctx_create", idx, None, ctypes.byref(ctx))
+        _contexts[idx] = ctx
+    stream = torch.cuda.current_stream(idx).cuda_stream
+    _lib.call("fh_ctx_set_stream", ctx, ctypes.c_void_p(stream))
+    return ctx
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def check_field(e: torch.Tensor, what: str = "field") -> None:
+    if not isinstance(e, torch.Tensor) or e.device.type != "cuda":
+        raise TypeError(f"{what} must be a torch tensor on a HIP device (libfluxhip has no CPU path)")
+    if e.dtype != torch.complex64:
+        raise TypeError(f"{what} must be complex64, got {e.dtype}")
+    if not e.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+
+
+def _check_mask(phi: torch.Tensor, ny: int, nx: int, device) -> None:
+    if phi.dtype != torch.float32 or phi.device != device or not phi.is_contiguous():
+        raise TypeError("mask must be a contiguous float32 tensor on the field's device")
+    if tuple(phi.shape) != (ny, nx):
+        raise ValueError(f"mask shape {tuple(phi.shape)} does not match the field ({ny}, {nx})")
+
+
+# --------------------------------------------------------------------------- propagators
+
+class _KernelPropagator:
+    """Shared plan cache of ASProp / RSProp (one plan per device, batch and wavelengths)."""
+
+    def __init__(self, u: ScalarField, ds, z: float):
+        self.ds = tuple(float(d) for d in (ds if ds is not None else u.ds))
+        self.z = float(z)
+        self.nx, self.ny = u.nx, u.ny
+        self._plans = {}
+
+    def _create(self, ctx, batch, lams, out):  # pragma: no cover - abstract
+        raise NotImplementedError
+
+    def plan(self, u: ScalarField) -> ctypes.c_void_p:
+        if (u.nx, u.ny) != (self.nx, self.ny):
+            raise ValueError(f"field grid {(u.nx, u.ny)} does not match the propagator {(self.nx, self.ny)}")
+        check_field(u.electric)
+        lams = tuple(u.lambda_list())
+        key = (u.electric.device.index, u.batch, lams)
+        p = self._plans.get(key)
+        ctx = context(u.electric.device)
+        if p is None:
+            p = ctypes.c_void_p()
+            arr = (ctypes.c_double * len(lams))(*lams)
+            self._create(ctx, u.batch, arr, len(lams), p)
+            self._plans[key] = p
+        return p
+
+    def __del__(self):
+        try:
+            for p in self._plans.values():
+                _lib.load().fh_plan_destroy(p)
+        except Exception:
+            pass
+
+    # -- the component contract
+    def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
+        _lib.call("fh_propagate", self.plan(u), _ptr(u.electric), dir_sign(direction))
+        return u
+
+    def propagate(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self.propagate_(u.copy(), direction)
+
+    def backpropagate_(self, dv: ScalarField, direction=Forward) -> ScalarField:
+        """Adjoint = propagation in the reverse direction (``freespace.jl:61-63``)."""
+        return self.propagate_(dv, Direction_reverse(direction))
+
+    def propagate_plain_fp64_(self, u_c128: torch.Tensor, lambdas, direction=Forward) -> torch.Tensor:
+        """The plain comparator: complex128 per-field, unfused propagation."""
+        if u_c128.dtype != torch.complex128 or u_c128.device.type != "cuda" or not u_c128.is_contiguous():
+            raise TypeError("the plain FP64 path takes a contiguous complex128 HIP tensor")
+        batch = 1 if u_c128.dim() == 2 else u_c128.shape[0]
+        tmp = ScalarField(torch.empty((batch, self.ny, self.nx), dtype=torch.complex64,
+                                      device=u_c128.device), self.ds, lambdas)
+        _lib.call("fh_propagate_plain_fp64", self.plan(tmp), _ptr(u_c128), dir_sign(direction))
+        return u_c128
+
+
+def Direction_reverse(direction):
+    return Backward if dir_sign(direction) > 0 else Forward
+
+
+class ASProp(_KernelPropagator):
+    """Angular-spectrum propagation in a medium of index ``n0``.
+
+    ``ASProp(u, z; use_cache=true, n0=1)`` (``angular_spectrum.jl:193-229``): the
+    transfer function is computed once in FP64 and cached as complex FP32, as the
+    reference does with ``use_cache=true`` (``double_precision_kernel=use_cache``).
+    ``paraxial``, ``filter`` and tilted fields are outside the accelerated path.
+    """
+
+    def __init__(self, u: ScalarField, z: float, ds=None, *, n0: float = 1.0,
+                 use_cache: bool = True, paraxial: bool = False, filter: Optional[Callable] = None,
+                 track_tilts: bool = False, double_precision_kernel: Optional[bool] = None):
+        if paraxial or filter is not None or track_tilts:
+            raise NotImplementedError("paraxial, filtered and tilted ASProp are not on the accelerated path")
+        if double_precision_kernel is False:
+            raise NotImplementedError("transfer functions are always computed in FP64")
+        super().__init__(u, ds, z)
+        self.n0 = float(n0)
+        if not self.n0 > 0:
+            raise ValueError("n0 must be positive")
+
+    def _create(self, ctx, batch, lams, nl, out):
+        _lib.call("fh_asprop_create", ctx, self.nx, self.ny, batch, self.ds[0], self.ds[1], lams, nl,
+                  self.z, self.n0, ctypes.byref(out))
+
+
+class RSProp(_KernelPropagator):
+    """Rayleigh-Sommerfeld propagation (``rayleigh_sommerfeld.jl:128-164``).
+
+    Warns like the reference when |z| is below the critical distance.  Pads to 2n
+    (the reference pads to 2n-1; the cropped result is the same linear convolution).
+    """
+
+    def __init__(self, u: ScalarField, z: float, ds=None, *, use_cache: bool = True,
+                 track_tilts: bool = False, double_precision_kernel: Optional[bool] = None):
+        if track_tilts:
+            raise NotImplementedError("tilted RSProp is not on the accelerated path")
+        super().__init__(u, ds, z)
+        zc = rs_valid_distance(self.nx, self.ny, self.ds[0], self.ds[1], u.lambda_min())
+        if abs(self.z) < zc:
+            warnings.warn(f"RSProp: propagation distance z={self.z} is below critical distance "
+                          f"zc={zc}. Numerical artifacts expected. Consider using ASProp or finer "
+                          "sampling (dx < lambda/2).")
+
+    def _create(self, ctx, batch, lams, nl, out):
+        _lib.call("fh_rsprop_create", ctx, self.nx, self.ny, batch, self.ds[0], self.ds[1], lams, nl,
+                  self.z, ctypes.byref(out))
+
+
+# --------------------------------------------------------------------------- Phase
+
+def _spatial(n: int, d: float) -> np.ndarray:
+    return (np.arange(n, dtype=np.float64) - (n - 1) / 2.0) * d
+
+
+class Phase:
+    """Phase mask u -> u * cis(s*phi) (``phasemask.jl:38-150``).
+
+    ``phi`` is a function ``(x, y) -> value`` evaluated on the centred grid
+    (``function_to_array``) or a (ny, nx) array.  ``trainable=True`` enables the
+    gradient; ``buffered=True`` pre-allocates the saved input and the gradient.
+    """
+
+    def __init__(self, u: ScalarField, phi: Union[Callable, np.ndarray, torch.Tensor, None] = None,
+                 ds=None, *, trainable: bool = False, buffered: bool = False):
+        ds = tuple(ds if ds is not None else u.ds)
+        device = u.electric.device
+        if phi is None:
+            arr = np.zeros((u.ny, u.nx))
+        elif callable(phi):
+            x = _spatial(u.nx, ds[0])[None, :]
+            y = _spatial(u.ny, ds[1])[:, None]
+            arr = np.broadcast_to(np.vectorize(phi)(x, y), (u.ny, u.nx))
+        else:
+            arr = phi
+        self.phi = torch.as_tensor(np.asarray(arr) if not isinstance(arr, torch.Tensor) else arr,
+                                   dtype=torch.float32, device=device).contiguous()
+        _check_mask(self.phi, u.ny, u.nx, device)
+        self.trainable = trainable
+        self.buffered = trainable and buffered
+        self.grad = torch.zeros_like(self.phi) if self.buffered else None
+        self.saved = torch.empty_like(u.electric) if self.buffered else None
+
+    def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
+        check_field(u.electric)
+        _lib.call("fh_phase_apply", context(u.electric.device), _ptr(u.electric), _ptr(self.phi),
+                  u.nx, u.ny, u.batch, dir_sign(direction), None)
+        return u
+
+    def propagate(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self.propagate_(u.copy(), direction)
+
+    def backpropagate_(self, dv: ScalarField, direction=Forward) -> ScalarField:
+        return self.propagate_(dv, Direction_reverse(direction))
+
+    def alloc_saved_buffer(self, u: ScalarField) -> torch.Tensor:
+        return torch.empty_like(u.electric)
+
+    def propagate_and_save_(self, u: ScalarField, saved: Optional[torch.Tensor] = None,
+                            direction=Forward) -> ScalarField:
+        saved = self.saved if saved is None else saved
+        check_field(u.electric)
+        check_field(saved, "saved buffer")
+        _lib.call("fh_phase_apply", context(u.electric.device), _ptr(u.electric), _ptr(self.phi),
+                  u.nx, u.ny, u.batch, dir_sign(direction), _ptr(saved))
+        return u
+
+    def backpropagate_with_gradient_(self, dv: ScalarField, saved: torch.Tensor,
+                                     dphi: Optional[torch.Tensor] = None, direction=Forward):
+        """Returns (du, dphi) with du written over dv (``phasemask.jl:142-150``)."""
+        dphi = self.grad if dphi is None else dphi
+        if dphi is None:
+            dphi = torch.empty_like(self.phi)
+        check_field(dv.electric)
+        check_field(saved, "saved buffer")
+        _lib.call("fh_phase_backward", context(dv.electric.device), _ptr(dv.electric), _ptr(self.phi),
+                  _ptr(saved), _ptr(dphi), dv.nx, dv.ny, dv.batch, dir_sign(direction), 0)
+        return dv, dphi
+
+
+# --------------------------------------------------------------------------- fused pairs
+
+def phase_propagate_(u: ScalarField, phase: Phase, prop: _KernelPropagator,
+                     saved: Optional[torch.Tensor] = None, direction=Forward) -> ScalarField:
+    """Phase then propagator in one pass over the field (mask fused into the first FFT pass)."""
+    _lib.call("fh_phase_propagate", prop.plan(u), _ptr(u.electric), _ptr(phase.phi), _ptr(saved),
+              dir_sign(direction))
+    return u
+
+
+def propagate_phase_backward_(dv: ScalarField, phase: Phase, prop: _KernelPropagator,
+                              saved: Optional[torch.Tensor], dphi: Optional[torch.Tensor],
+                              direction=Forward, accumulate: bool = False) -> ScalarField:
+    """Adjoint of ``phase_propagate_``: propagator adjoint, mask adjoint and mask gradient
+    (summed over the batch in field order) in one pass."""
+    _lib.call("fh_propagate_phase_backward", prop.plan(dv), _ptr(dv.electric), _ptr(phase.phi),
+              _ptr(saved), _ptr(dphi), dir_sign(direction), int(accumulate))
+    return dv

@@ -1,0 +1,122 @@
+"""ScalarField and propagation directions (host-side mirror of the reference's data model).
+
+Memory layout: ``electric`` is a torch tensor of shape ``(B, ny, nx)`` (or ``(ny, nx)``)
+in C order, byte-identical to the reference's column-major ``electric[nx, ny, B]``
+(``src/Fields.jl:118-149``).  Device fields are complex64 on a HIP device.
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Sequence, Union
+
+import numpy as np
+import torch
+
+
+class Direction:
+    """``Forward``/``Backward`` markers with ``sign`` and ``reverse``
+    (``src/optical_components/OpticalComponents.jl:34-99``)."""
+
+    sign = 0
+
+    @classmethod
+    def reverse(cls):
+        return Backward if cls is Forward else Forward
+
+
+class Forward(Direction):
+    sign = 1
+
+
+class Backward(Direction):
+    sign = -1
+
+
+def dir_sign(direction) -> int:
+    if direction in (Forward, Backward):
+        return direction.sign
+    if direction in (1, -1):
+        return int(direction)
+    raise ValueError(f"direction must be Forward or Backward, got {direction!r}")
+
+
+class ScalarField:
+    """Scalar optical field: complex data + sampling ``ds`` + wavelength(s).
+
+    ``lambdas`` is a scalar or one value per field (``src/Fields.jl:28-32``).  A
+    ComplexF32 field in the reference stores lambda as Float32; pass
+    ``float(np.float32(lam))`` to reproduce that exactly.
+    """
+
+    def __init__(self, electric, ds: Sequence[float], lambdas: Union[float, Sequence[float]]):
+        if not isinstance(electric, torch.Tensor):
+            electric = torch.as_tensor(np.asarray(electric))
+        if electric.dim() not in (2, 3):
+            raise ValueError("electric must be (ny, nx) or (B, ny, nx)")
+        self.electric = electric
+        self.ds = (float(ds[0]), float(ds[1]))
+        lam = np.atleast_1d(np.asarray(lambdas, dtype=np.float64))
+        if lam.size not in (1, self.batch):
+            raise ValueError("lambdas must be a scalar or one value per field")
+        if not np.all(lam > 0):
+            raise ValueError("wavelengths must be positive")
+        self.lambdas = float(lam[0]) if lam.size == 1 and np.ndim(lambdas) == 0 else [float(v) for v in lam]
+
+    # -- shape helpers
+    @property
+    def nx(self) -> int:
+        return self.electric.shape[-1]
+
+    @property
+    def ny(self) -> int:
+        return self.electric.shape[-2]
+
+    @property
+    def batch(self) -> int:
+        return 1 if self.electric.dim() == 2 else self.electric.shape[0]
+
+    def lambda_list(self):
+        lam = self.lambdas if isinstance(self.lambdas, list) else [self.lambdas]
+        return lam if len(lam) == self.batch else lam * self.batch
+
+    def lambda_min(self) -> float:
+        return min(self.lambda_list())
+
+    def copy(self) -> "ScalarField":
+        return ScalarField(self.electric.clone(), self.ds, self.lambdas)
+
+    def set_field_data(self, data) -> "ScalarField":
+        """New field over ``data`` with the same grid and wavelengths (``Fields.jl:213-215``)."""
+        return ScalarField(data, self.ds, self.lambdas)
+
+    def power(self) -> torch.Tensor:
+        """Per-field power sum|u|^2 dx dy (``Fields.jl:843-846``)."""
+        return (self.electric.abs() ** 2).sum(dim=(-2, -1)) * (self.ds[0] * self.ds[1])
+
+    def normalize_power_(self, target: float = 1.0) -> "ScalarField":
+        """``normalize_power!`` (``Fields.jl:889-892``)."""
+        p = self.power()
+        self.electric *= torch.sqrt(target / p).to(self.electric.dtype)[..., None, None]
+        return self
+
+    def __repr__(self):
+        return (f"ScalarField(shape={tuple(self.electric.shape)}, dtype={self.electric.dtype}, "
+                f"device={self.electric.device}, ds={self.ds}, lambdas={self.lambdas})")
+
+
+def coupling_efficiency(u: ScalarField, v: ScalarField) -> torch.Tensor:
+    """|<u,v>|^2 / (<u,u><v,v>) per field (``Fields.jl:748-756``)."""
+    a, b = u.electric, v.electric
+    num = (a.conj() * b).sum(dim=(-2, -1)).abs() ** 2
+    den = (a.abs() ** 2).sum(dim=(-2, -1)) * (b.abs() ** 2).sum(dim=(-2, -1))
+    return num / den
+
+
+def rs_valid_distance(nx: int, ny: int, dx: float, dy: float, lam: float) -> float:
+    """Critical RS distance z_c (``rayleigh_sommerfeld.jl:29-36``)."""
+    if dx < lam / 2 and dy < lam / 2:
+        return 0.0
+    zx = 0.0 if dx < lam / 2 else (nx * dx / 2) * math.sqrt(4 * dx * dx / lam ** 2 - 1)
+    zy = 0.0 if dy < lam / 2 else (ny * dy / 2) * math.sqrt(4 * dy * dy / lam ** 2 - 1)
+    return max(zx, zy)

@@ -1,0 +1,114 @@
+/* libfluxhip: MI355X (gfx950) propagation path of FluxOptics.jl behind a plain C ABI.
+ *
+ * Fields are complex FP32 (c64, interleaved re/im) device arrays laid out like the
+ * reference's column-major ScalarField electric[nx, ny, batch]: element (i, j, b) at
+ * offset i + nx*(j + ny*b) (src/Fields.jl:118-149).  Masks are float[ny][nx] (the
+ * reference's phi[nx, ny]), shared by every field of the batch.
+ *
+ * Conventions:
+ *  - every function returns 0 on success and a negative FH_E* code on failure;
+ *    fh_last_error() gives the message of the calling thread's last failure;
+ *  - direction is +1 (Forward) or -1 (Backward), as sign(::Type{<:Direction})
+ *    (src/optical_components/OpticalComponents.jl:34-99);
+ *  - wavelengths, distances and sampling are doubles; a ComplexF32 ScalarField stores
+ *    lambda as Float32 (src/Fields.jl:28-32), so a caller reproducing the reference
+ *    passes (double)(float)lambda;
+ *  - all work is asynchronous on the context's stream; the caller owns field, mask and
+ *    gradient buffers; the library owns plans, tables and workspace.
+ */
+#ifndef FLUXHIP_H
+#define FLUXHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FH_OK 0
+#define FH_E_ARG -1      /* invalid argument (sizes, directions, null pointers) */
+#define FH_E_HIP -2      /* a HIP runtime call failed */
+#define FH_E_NOMEM -3    /* device allocation failed */
+#define FH_E_UNSUPP -4   /* valid but unsupported (e.g. a non power-of-two grid) */
+
+typedef struct fh_ctx fh_ctx;
+typedef struct fh_plan fh_plan;
+typedef struct fh_bpm fh_bpm;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int fh_version(void);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* fh_last_error(void);
+
+/* Largest supported transform length per axis (RS pads to 2n, so RS fields are limited
+ * to half of it). */
+int fh_max_length(void);
+
+/* Context: one HIP device and one stream (NULL = the device's default stream).
+ * Replaces the CUDA extension hook (src/cuda/CUDAExt.jl:10-37). */
+int fh_ctx_create(int device, void* stream, fh_ctx** out);
+int fh_ctx_set_stream(fh_ctx* ctx, void* stream);
+int fh_ctx_synchronize(fh_ctx* ctx);
+int fh_ctx_destroy(fh_ctx* ctx);
+
+/* ASProp: angular-spectrum propagator for fields of shape [nx, ny, batch].
+ * lambdas has nlambdas entries: 1 (shared) or batch (one per field).
+ * Replaces ASProp(u, ds, z; n0) with use_cache=true, double_precision_kernel=true
+ * (src/optical_components/freespace_propagators/angular_spectrum.jl:193-229,1-12):
+ * the transfer function is computed once in FP64 and rounded to c64. */
+int fh_asprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
+                     const double* lambdas, int nlambdas, double z, double n0,
+                     fh_plan** out);
+
+/* RSProp: Rayleigh-Sommerfeld propagator (rayleigh_sommerfeld.jl:128-164).  The
+ * reference pads to (2n-1)^2; this library pads to (2n)^2, which yields the same
+ * cropped linear convolution.  The kernel's FFT is computed in FP64 on the device
+ * and rounded once. */
+int fh_rsprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
+                     const double* lambdas, int nlambdas, double z, fh_plan** out);
+
+int fh_plan_destroy(fh_plan* plan);
+
+/* propagate!(u, p, direction): in place on a c64 [nx, ny, batch] device array
+ * (freespace.jl:29-63; Backward multiplies by the conjugate kernel, which is the
+ * exact adjoint of Forward). */
+int fh_propagate(fh_plan* plan, void* u, int direction);
+
+/* One pass of fh_propagate: pass 0 = x-line FFT, 1 = y-line FFT * kernel * IFFT,
+ * 2 = x-line IFFT.  Running 0, 1, 2 in order equals fh_propagate.  Consecutive
+ * propagators may skip a pass 2 followed by a pass 0 on the same field (the IFFT/FFT
+ * cancellation of merge_rules.jl:104-108); used too for per-kernel timing. */
+int fh_propagate_pass(fh_plan* plan, void* u, int direction, int pass);
+
+/* Phase then propagation fused: u <- P(u * cis(s*phi)); if saved != NULL the mask
+ * input u is stored there first (propagate_and_save!, phasemask.jl:100-113). */
+int fh_phase_propagate(fh_plan* plan, void* u, const float* phi, void* saved, int direction);
+
+/* Phase on its own: u <- u * cis(s*phi), saving the input if saved != NULL
+ * (phasemask.jl:90-113). */
+int fh_phase_apply(fh_ctx* ctx, void* u, const float* phi, int nx, int ny, int batch,
+                   int direction, void* saved);
+
+/* Phase adjoint with the mask gradient (backpropagate_with_gradient!,
+ * phasemask.jl:142-150 and compute_phase_gradient!, :115-140):
+ *   du <- dv * cis(-s*phi)   (in place in dv)
+ *   dphi <- s * sum_b Im(du * conj(saved))   (added to dphi when accumulate != 0). */
+int fh_phase_backward(fh_ctx* ctx, void* dv, const float* phi, const void* saved,
+                      float* dphi, int nx, int ny, int batch, int direction, int accumulate);
+
+/* Adjoint propagation then Phase adjoint with gradient, fused:
+ *   du <- P^H(dv) * cis(-s*phi), dphi (+)= s * sum_b Im(du * conj(saved)). */
+int fh_propagate_phase_backward(fh_plan* plan, void* dv, const float* phi, const void* saved,
+                                float* dphi, int direction, int accumulate);
+
+/* The plain GPU form used as a comparator: the same propagation in complex FP64
+ * (c128 [nx, ny, batch]), per field, unfused (five launches per field: x FFT, y FFT,
+ * multiply by a full FP64 transfer-function table, y IFFT, x IFFT). */
+int fh_propagate_plain_fp64(fh_plan* plan, void* u_c128, int direction);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUXHIP_H */

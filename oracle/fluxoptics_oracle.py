@@ -389,3 +389,35 @@ def pipeline_backward(dv, stages, ds, lams, saved):
             du, g = phase_backward(du, saved[k], st[1], 1)
             grads.append(g)
     return du, grads[::-1]
+
+
+# ----------------------------------------------------------------------------- CPU baseline
+
+class CpuASPropC64:
+    """The reference's CPU ASProp at complex-FP32 precision, for the timed CPU baseline.
+
+    Restates the reference's CPU algorithm: an in-place FFT2 (``FFTutils.jl:13-28``,
+    FFTW there; pocketfft via ``scipy.fft`` with ``workers`` threads here), a multiply by
+    the cached complex-FP32 kernel computed once in FP64 (``abstract_kernel.jl:15-25,
+    39-46``; ``angular_spectrum.jl:94-100``) and the inverse FFT2.  ``forward`` and
+    ``adjoint`` work in place on a (ny, nx) complex64 array.
+    """
+
+    def __init__(self, nx, ny, dx, dy, lam, z, n0=1.0, workers=1):
+        import scipy.fft as sfft
+        self._fft = sfft
+        self.workers = int(workers)
+        self.h = as_transfer(nx, ny, dx, dy, lam, n0, z).astype(np.complex64)
+        self.hc = np.conj(self.h)
+
+    def _apply(self, u, h):
+        f = self._fft.fft2(u, workers=self.workers, overwrite_x=True)
+        np.multiply(f, h, out=f)
+        u[...] = self._fft.ifft2(f, workers=self.workers, overwrite_x=True)
+        return u
+
+    def forward(self, u):
+        return self._apply(u, self.h)
+
+    def adjoint(self, u):
+        return self._apply(u, self.hc)

@@ -1,0 +1,64 @@
+"""C-ABI checks that need no GPU: the library loads, exports every declared symbol,
+and rejects bad arguments before any device work."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+from fluxoptics import _lib
+
+HEADER = os.path.join(ROOT, "include", "fluxhip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), f"{name} declared in fluxhip.h but not exported"
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+    assert set(_lib.SIGNATURES) == set(names)
+
+
+def test_host_only_entry_points():
+    lib = _lib.load()
+    assert lib.fh_version() >= 100
+    assert lib.fh_max_length() == 4096
+    assert lib.fh_last_error() is not None
+
+
+def test_argument_validation_before_device_work():
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    lam = (ctypes.c_double * 1)(0.8)
+    # null context
+    rc = lib.fh_asprop_create(None, 64, 64, 1, 1.0, 1.0, lam, 1, 10.0, 1.0, ctypes.byref(out))
+    assert rc == -1 and b"context" in lib.fh_last_error()
+    rc = lib.fh_rsprop_create(None, 64, 64, 1, 1.0, 1.0, lam, 1, 10.0, ctypes.byref(out))
+    assert rc == -1
+    # null plan / field
+    assert lib.fh_propagate(None, None, 1) == -1
+    assert lib.fh_phase_apply(None, None, None, 4, 4, 1, 1, None) == -1
+    assert lib.fh_propagate_plain_fp64(None, None, 1) == -1
+    assert lib.fh_ctx_create(0, None, None) == -1
+    with pytest.raises(_lib.FluxHipError):
+        _lib.call("fh_propagate", None, None, 1)
+
+
+def test_no_cpu_fallback_in_host_layer():
+    """The host layer refuses CPU tensors instead of computing on the CPU."""
+    import torch
+    import fluxoptics as fo
+    u = fo.ScalarField(torch.zeros((16, 16), dtype=torch.complex64), (1.0, 1.0), 0.8)
+    p = fo.ASProp(u, 10.0)
+    with pytest.raises(TypeError):
+        p.propagate_(u)

@@ -1,0 +1,177 @@
+"""GPU parity of ASProp / RSProp / Phase against the FP64 oracle (through the C ABI).
+
+Tolerance: relative L2 error <= 1e-5 for fields and <= 1e-4 for gradients, the
+bounds of the complex-FP32 path against complex FP64.  Inputs are rounded to complex64
+before the oracle sees them, and both sides use the same wavelength double.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import fluxoptics as fo
+from fluxoptics import inputs
+from oracle import fluxoptics_oracle as O
+from _util import c64, rel_l2, random_field, speckle_batch, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+@pytest.mark.parametrize("nx,ny,batch,ds,lams,z,n0", [
+    (16, 16, 1, (1.0, 1.0), 0.8, 50.0, 1.0),
+    (32, 64, 2, (0.5, 0.7), [0.8, 1.064], 120.0, 1.0),
+    (128, 128, 3, (1.0, 1.0), [0.64, 0.538, 0.455], 1e4, 1.0),
+    (256, 512, 1, (0.5, 0.5), 1.0, -300.0, 1.5),
+    (512, 256, 2, (1.0, 0.5), 0.8, 2000.0, 1.0),
+    (1024, 1024, 2, (1.0, 1.0), 0.64, 1e4, 1.0),
+    (2048, 2048, 2, (0.5, 0.5), 0.8, 5000.0, 1.0),
+    (4096, 4096, 1, (0.5, 0.5), 0.8, 5000.0, 1.0),
+])
+def test_asprop_forward_backward_parity(nx, ny, batch, ds, lams, z, n0):
+    print(f"[asprop parity] nx={nx} ny={ny} batch={batch} z={z}", flush=True)
+    u0 = c64(speckle_batch(batch, nx, ny, ds[0], ds[1], 0.15 * nx * ds[0], 100 + nx))
+    u = fo.ScalarField(to_dev(u0), ds, lams)
+    p = fo.ASProp(u, z, n0=n0)
+    v = p.propagate(u, fo.Forward)
+    ref = O.asprop(u0, ds, lams, z, n0, 1)
+    assert rel_l2(to_host(v.electric), ref) <= FIELD_TOL
+    w = p.propagate(u, fo.Backward)
+    refb = O.asprop(u0, ds, lams, z, n0, -1)
+    assert rel_l2(to_host(w.electric), refb) <= FIELD_TOL
+
+
+def test_asprop_headline_forward_adjoint():
+    """8 x 2048^2, ds=0.5, lam=0.8, z=5000: forward, then the adjoint of v - t."""
+    print("[asprop headline] 8 x 2048^2", flush=True)
+    n, ds, lam, z = 2048, (0.5, 0.5), 0.8, 5000.0
+    u0 = c64(speckle_batch(8, n, n, 0.5, 0.5, 150.0, 2000))
+    t0 = c64(0.01 * random_field((8, n, n), 77))
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    p = fo.ASProp(u, z)
+    p.propagate_(u, fo.Forward)
+    v = to_host(u.electric)
+    vref = O.asprop(u0, ds, lam, z, 1.0, 1)
+    err_v = rel_l2(v, vref)
+    # adjoint input: dL/dv for L = 1/2 |v - t|^2, formed from the device's own v
+    dv0 = c64(v - t0)
+    dv = fo.ScalarField(to_dev(dv0), ds, lam)
+    p.backpropagate_(dv, fo.Forward)
+    du = to_host(dv.electric)
+    duref = O.asprop(dv0, ds, lam, z, 1.0, -1)
+    err_du = rel_l2(du, duref)
+    print(f"rel-L2 field {err_v:.3e}, adjoint {err_du:.3e}")
+    assert err_v <= FIELD_TOL and err_du <= FIELD_TOL
+
+
+@pytest.mark.parametrize("nx,ny,batch,ds,lam,z", [
+    (16, 16, 1, (1.0, 1.0), 1.064, 30.0),
+    (64, 32, 2, (1.0, 1.0), 1.064, 300.0),
+    (256, 256, 1, (1.0, 1.0), 1.064, -1500.0),
+    (1024, 1024, 1, (1.0, 1.0), 1.064, 2000.0),
+    (2048, 2048, 1, (1.0, 1.0), 1.064, 4000.0),
+])
+def test_rsprop_forward_backward_parity(nx, ny, batch, ds, lam, z):
+    print(f"[rsprop parity] nx={nx} ny={ny} batch={batch} z={z}", flush=True)
+    u0 = c64(np.stack([inputs.laguerre_gaussian(nx, ny, ds[0], ds[1], 0.1 * nx, b % 3, b - 1)
+                       for b in range(batch)]))
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        p = fo.RSProp(u, z)
+    v = p.propagate(u, fo.Forward)
+    ref = O.rsprop(u0, ds, lam, z, 1, "2n-1")
+    assert rel_l2(to_host(v.electric), ref) <= FIELD_TOL
+    w = p.propagate(u, fo.Backward)
+    refb = O.rsprop(u0, ds, lam, z, -1, "2n-1")
+    assert rel_l2(to_host(w.electric), refb) <= FIELD_TOL
+
+
+def test_phase_apply_and_gradient():
+    nx, ny, batch = 96, 64, 3
+    u0 = c64(random_field((batch, ny, nx), 5))
+    dv0 = c64(random_field((batch, ny, nx), 6))
+    phi = inputs.uniform(7, (ny, nx), 0, 2 * math.pi).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), (1.0, 1.0), 0.8)
+    ph = fo.Phase(u, phi, trainable=True, buffered=True)
+    ph.propagate_and_save_(u, None, fo.Forward)
+    assert rel_l2(to_host(u.electric), O.phase_apply(u0, phi.astype(np.float64), 1)) <= FIELD_TOL
+    assert np.array_equal(to_host(ph.saved), u0)
+    dv = fo.ScalarField(to_dev(dv0), (1.0, 1.0), 0.8)
+    du, dphi = ph.backpropagate_with_gradient_(dv, ph.saved, None, fo.Forward)
+    du_ref, g_ref = O.phase_backward(dv0, u0, phi.astype(np.float64), 1)
+    assert rel_l2(to_host(du.electric), du_ref) <= FIELD_TOL
+    torch.cuda.synchronize()
+    assert rel_l2(dphi.cpu().numpy(), g_ref) <= GRAD_TOL
+
+
+@pytest.mark.parametrize("kind,n", [("as", 256), ("as", 2048), ("rs", 128)])
+def test_fused_phase_propagate_and_adjoint(kind, n):
+    print(f"[fused phase+{kind}] n={n}", flush=True)
+    batch, ds, lam, z = 3, (1.0, 1.0), [0.64, 0.538, 0.455], 800.0
+    u0 = c64(speckle_batch(batch, n, n, 1.0, 1.0, 0.2 * n, 11))
+    dv0 = c64(random_field((batch, n, n), 12))
+    phi = inputs.uniform(13, (n, n), 0, 2 * math.pi).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    ph = fo.Phase(u, phi, trainable=True, buffered=True)
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        p = fo.ASProp(u, z) if kind == "as" else fo.RSProp(u, z)
+    stages = [("phase", phi.astype(np.float64)), (kind, z)]
+    fo.phase_propagate_(u, ph, p, ph.saved, fo.Forward)
+    vref, saved = O.pipeline_forward(u0, stages, ds, lam)
+    assert rel_l2(to_host(u.electric), vref) <= FIELD_TOL
+    assert np.array_equal(to_host(ph.saved), u0)
+    dv = fo.ScalarField(to_dev(dv0), ds, lam)
+    fo.propagate_phase_backward_(dv, ph, p, ph.saved, ph.grad, fo.Forward)
+    du_ref, (g_ref,) = O.pipeline_backward(dv0, stages, ds, lam, saved)
+    assert rel_l2(to_host(dv.electric), du_ref) <= FIELD_TOL
+    torch.cuda.synchronize()
+    assert rel_l2(ph.grad.cpu().numpy(), g_ref) <= GRAD_TOL
+
+
+def test_plain_fp64_comparator():
+    n, batch, ds, lam, z = 256, 2, (0.5, 0.5), 0.8, 700.0
+    u0 = speckle_batch(batch, n, n, 0.5, 0.5, 40.0, 21)
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    p = fo.ASProp(u, z)
+    d = torch.from_numpy(u0.copy()).cuda()
+    p.propagate_plain_fp64_(d, lam, fo.Forward)
+    torch.cuda.synchronize()
+    assert rel_l2(d.cpu().numpy(), O.asprop(u0, ds, lam, z, 1.0, 1)) <= 1e-12
+    p.propagate_plain_fp64_(d, lam, fo.Backward)
+    torch.cuda.synchronize()
+    assert rel_l2(d.cpu().numpy(), O.asprop(O.asprop(u0, ds, lam, z, 1.0, 1), ds, lam, z, 1.0, -1)) <= 1e-12
+
+
+def test_reference_properties_on_device():
+    """``test/optical_components_test.jl:85-133`` run through the device path."""
+    n = 128
+    u0 = inputs.gaussian(n, n, 1.0, 1.0, 20.0)
+    u = fo.ScalarField(to_dev(u0), (1.0, 1.0), 1.064)
+    p = fo.ASProp(u, 1000.0)
+    back = p.propagate(p.propagate(u, fo.Forward), fo.Backward)
+    assert float(fo.coupling_efficiency(u, back)[0]) > 0.9999
+    r = fo.RSProp(u, 1000.0)
+    back_rs = r.propagate(r.propagate(u, fo.Forward), fo.Backward)
+    assert float(fo.coupling_efficiency(u, back_rs)[0]) > 0.999
+    u1 = fo.ScalarField(to_dev(inputs.gaussian(n, n, 0.5, 0.5, 15.0)), (0.5, 0.5), 1.064)
+    u1.normalize_power_()
+    p1 = fo.ASProp(u1, 500.0)
+    v1 = p1.propagate(u1, fo.Forward)
+    assert abs(float(v1.power()[0]) - float(u1.power()[0])) <= 1e-6 * float(u1.power()[0])
+
+
+def test_error_paths_on_device():
+    u = fo.ScalarField(to_dev(np.zeros((48, 48))), (1.0, 1.0), 0.8)
+    with pytest.raises(fo._lib.FluxHipError) if hasattr(fo, "_lib") else pytest.raises(Exception):
+        fo.ASProp(u, 10.0).propagate_(u)
+    u2 = fo.ScalarField(to_dev(np.zeros((64, 64))), (1.0, 1.0), 0.8)
+    with pytest.raises(ValueError):
+        fo.ASProp(u, 10.0).propagate_(u2)
