@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "fluxoptics", "libfluxhip.so")
 SOURCES = ["fluxhip.hip"]
-HEADERS = ["fh_dft.hpp", "fh_lines.hpp", "fh_ops.hpp"]
+HEADERS = ["fh_dft.hpp", "fh_lines.hpp", "fh_ops.hpp", "fh_bpm.hpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -134,12 +134,26 @@ struct NoOp {
   template <typename C> __device__ __forceinline__ C operator()(C v, int, int, int) const { return v; }
 };
 
+// storage <-> compute conversions (c64 storage may be transformed in FP64)
+template <typename C, typename S> __device__ __forceinline__ C cvt(S v) {
+  C r; r.x = v.x; r.y = v.y; return r;
+}
+
+// Tile order for strided-axis passes: workgroups are dispatched round-robin over the 8
+// XCDs, so neighbouring column tiles (which share 128-byte rows when a tile is 64 bytes
+// wide) are placed 8 blocks apart to land on the same XCD and share its L2.
+__device__ __forceinline__ int xcd_pair_tile(int b, int nb) {
+  if (nb & 15) return b;
+  const int g = b >> 4, i = b & 15;
+  return (g << 4) + ((i & 7) << 1) + (i >> 3);
+}
+
 // Forward or inverse line FFT, global -> global, with an elementwise op fused on the
 // time-domain side (prologue on load for a forward transform, epilogue before store
-// for an inverse one).
-template <typename C, typename RDF, int L, bool LFAST, bool INV, typename TimeOp>
+// for an inverse one).  S is the storage type, C the compute type.
+template <typename S, typename C, typename RDF, int L, bool LFAST, bool INV, typename TimeOp>
 __global__ void __launch_bounds__(L * RDF::P)
-k_line_fft(LineIO<C> io, const C* __restrict__ tw, TimeOp op) {
+k_line_fft(LineIO<S> io, const C* __restrict__ tw, TimeOp op) {
   using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
   constexpr int n = RD::n, V = RD::V, P = RD::P;
   extern __shared__ unsigned char smem_raw[];
@@ -147,10 +161,11 @@ k_line_fft(LineIO<C> io, const C* __restrict__ tw, TimeOp op) {
   int l, t;
   thread_map<L, P, LFAST>(threadIdx.x, l, t);
   const int b = blockIdx.y;
-  const int li = blockIdx.x * L + l;
+  const int tile = LFAST ? xcd_pair_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int li = tile * L + l;
   const bool live = li < io.nlines;
   C* s = smem + l * LdsLine<n>::size;
-  const C* src = io.in + b * io.in_f + (long)li * io.in_l;
+  const S* src = io.in + b * io.in_f + (long)li * io.in_l;
   C v[V];
 #pragma unroll
   for (int m = 0; m < V / RD::R0; ++m)
@@ -158,13 +173,13 @@ k_line_fft(LineIO<C> io, const C* __restrict__ tw, TimeOp op) {
     for (int r = 0; r < RD::R0; ++r) {
       const int pos = reg_pos<n, RD::R0, P>(t, m, r);
       C x = cmake<C>(0, 0);
-      if (live && pos < io.in_valid) x = src[pos * io.in_e];
+      if (live && pos < io.in_valid) x = cvt<C>(src[pos * io.in_e]);
       if (!INV && live) x = op(x, b, li, pos);
       v[m * RD::R0 + r] = x;
     }
   constexpr int RL = (RD::NST == 3) ? RD::R2 : ((RD::NST == 2) ? RD::R1 : RD::R0);
   line_fft<C, RD, INV>(v, s, t, tw);
-  C* dst = io.out + b * io.out_f + (long)li * io.out_l;
+  S* dst = io.out + b * io.out_f + (long)li * io.out_l;
 #pragma unroll
   for (int m = 0; m < V / RL; ++m)
 #pragma unroll
@@ -172,15 +187,15 @@ k_line_fft(LineIO<C> io, const C* __restrict__ tw, TimeOp op) {
       const int pos = reg_pos<n, RL, P>(t, m, r);
       C x = v[m * RL + r];
       if (INV && live) x = op(x, b, li, pos);
-      if (live && pos < io.out_count) dst[pos * io.out_e] = x;
+      if (live && pos < io.out_count) dst[pos * io.out_e] = cvt<S>(x);
     }
 }
 
 // FFT -> pointwise multiply (freq-domain op, given line index and frequency index)
 // -> IFFT along the lines, global -> global.  Used for the strided (y) axis.
-template <typename C, typename RDF, int L, bool LFAST, typename FreqOp>
+template <typename S, typename C, typename RDF, int L, bool LFAST, typename FreqOp>
 __global__ void __launch_bounds__(L * RDF::P)
-k_line_conv(LineIO<C> io, const C* __restrict__ tw, FreqOp op) {
+k_line_conv(LineIO<S> io, const C* __restrict__ tw, FreqOp op) {
   constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
   using RI = typename RDF::Rev;
   extern __shared__ unsigned char smem_raw[];
@@ -188,10 +203,11 @@ k_line_conv(LineIO<C> io, const C* __restrict__ tw, FreqOp op) {
   int l, t;
   thread_map<L, P, LFAST>(threadIdx.x, l, t);
   const int b = blockIdx.y;
-  const int li = blockIdx.x * L + l;
+  const int tile = LFAST ? xcd_pair_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int li = tile * L + l;
   const bool live = li < io.nlines;
   C* s = smem + l * LdsLine<n>::size;
-  const C* src = io.in + b * io.in_f + (long)li * io.in_l;
+  const S* src = io.in + b * io.in_f + (long)li * io.in_l;
   C v[V];
 #pragma unroll
   for (int m = 0; m < V / RDF::R0; ++m)
@@ -199,7 +215,7 @@ k_line_conv(LineIO<C> io, const C* __restrict__ tw, FreqOp op) {
     for (int r = 0; r < RDF::R0; ++r) {
       const int pos = reg_pos<n, RDF::R0, P>(t, m, r);
       C x = cmake<C>(0, 0);
-      if (live && pos < io.in_valid) x = src[pos * io.in_e];
+      if (live && pos < io.in_valid) x = cvt<C>(src[pos * io.in_e]);
       v[m * RDF::R0 + r] = x;
     }
   line_fft<C, RDF, false>(v, s, t, tw);
@@ -213,13 +229,13 @@ k_line_conv(LineIO<C> io, const C* __restrict__ tw, FreqOp op) {
     }
   line_fft<C, RI, true>(v, s, t, tw);
   constexpr int RE = (RI::NST == 3) ? RI::R2 : ((RI::NST == 2) ? RI::R1 : RI::R0);
-  C* dst = io.out + b * io.out_f + (long)li * io.out_l;
+  S* dst = io.out + b * io.out_f + (long)li * io.out_l;
 #pragma unroll
   for (int m = 0; m < V / RE; ++m)
 #pragma unroll
     for (int r = 0; r < RE; ++r) {
       const int pos = reg_pos<n, RE, P>(t, m, r);
-      if (live && pos < io.out_count) dst[pos * io.out_e] = v[m * RE + r];
+      if (live && pos < io.out_count) dst[pos * io.out_e] = cvt<S>(v[m * RE + r]);
     }
 }
 

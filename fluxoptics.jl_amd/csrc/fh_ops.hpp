@@ -11,20 +11,23 @@ namespace fh {
 // Multiply by a transfer function stored as one quadrant (the function depends on
 // |fx| and |fy| only): entry [min(ky, ny-ky)][min(kx, nx-kx)], row length qx.
 // `tab` holds one quadrant per table index; `tabidx[b]` picks the table of field b.
-// The 1/(nx*ny) of the inverse FFT is folded into the table.
-struct QuadMul {
-  const float2* tab;
+// The 1/(nx*ny) of the inverse FFT is folded into the table.  T is the table's complex
+// type (c64, or c128 for the FP64-compute slice steps).
+template <typename T> struct QuadMulT {
+  const T* tab;
   const int* tabidx;
   long tabsize;
   int nx, ny, qx;
   int conj;  // Backward direction multiplies by the conjugate
-  __device__ __forceinline__ float2 operator()(float2 v, int b, int kx, int ky) const {
+  template <typename C>
+  __device__ __forceinline__ C operator()(C v, int b, int kx, int ky) const {
     const int qxi = kx <= nx - kx ? kx : nx - kx;
     const int qyi = ky <= ny - ky ? ky : ny - ky;
-    const float2 h = tab[tabidx[b] * tabsize + (long)qyi * qx + qxi];
+    const C h = cvt<C>(tab[tabidx[b] * tabsize + (long)qyi * qx + qxi]);
     return conj ? cmulc(v, h) : cmul(v, h);
   }
 };
+using QuadMul = QuadMulT<float2>;
 
 // Plain multiply by a full table (FP64 comparator path); the table is [ky][kx].
 struct FullMulD {

@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "../../include/fluxhip.h"
-#include "fh_ops.hpp"
+#include "fh_bpm.hpp"
 
 namespace fh {
 
@@ -102,13 +102,13 @@ static hipError_t ensure_lds(K kernel, size_t bytes) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <typename C, int LOGN, bool Y, bool INV, typename Op>
-static hipError_t launch_fft_t(const LineIO<C>& io, int batch, const C* tw, Op op, hipStream_t st) {
+template <typename S, typename C, int LOGN, bool Y, bool INV, typename Op>
+static hipError_t launch_fft_t(const LineIO<S>& io, int batch, const C* tw, Op op, hipStream_t st) {
   using CF = Cfg<typename RealOf<C>::type, LOGN, Y>;
   using RD = typename CF::RD;
   constexpr int L = CF::L;
   static_assert(RD::n == (1 << LOGN), "radix plan does not match the length");
-  auto kern = k_line_fft<C, RD, L, Y, INV, Op>;
+  auto kern = k_line_fft<S, C, RD, L, Y, INV, Op>;
   const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n>::size * sizeof(C) : 0;
   hipError_t e = ensure_lds(kern, lds);
   if (e != hipSuccess) return e;
@@ -117,12 +117,12 @@ static hipError_t launch_fft_t(const LineIO<C>& io, int batch, const C* tw, Op o
   return hipGetLastError();
 }
 
-template <typename C, int LOGN, typename Op>
-static hipError_t launch_conv_t(const LineIO<C>& io, int batch, const C* tw, Op op, hipStream_t st) {
+template <typename S, typename C, int LOGN, typename Op>
+static hipError_t launch_conv_t(const LineIO<S>& io, int batch, const C* tw, Op op, hipStream_t st) {
   using CF = Cfg<typename RealOf<C>::type, LOGN, true>;
   using RD = typename CF::RD;
   constexpr int L = CF::L;
-  auto kern = k_line_conv<C, RD, L, true, Op>;
+  auto kern = k_line_conv<S, C, RD, L, true, Op>;
   const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n>::size * sizeof(C) : 0;
   hipError_t e = ensure_lds(kern, lds);
   if (e != hipSuccess) return e;
@@ -145,16 +145,16 @@ static hipError_t launch_conv_t(const LineIO<C>& io, int batch, const C* tw, Op 
     default: return hipErrorInvalidValue;                                \
   }
 
-template <typename C, bool Y, bool INV, typename Op>
-static hipError_t launch_fft(int logn, const LineIO<C>& io, int batch, const C* tw, Op op,
+template <typename S, typename C, bool Y, bool INV, typename Op>
+static hipError_t launch_fft(int logn, const LineIO<S>& io, int batch, const C* tw, Op op,
                              hipStream_t st) {
-  FH_LOGN_SWITCH(logn, (launch_fft_t<C, LG, Y, INV, Op>(io, batch, tw, op, st)))
+  FH_LOGN_SWITCH(logn, (launch_fft_t<S, C, LG, Y, INV, Op>(io, batch, tw, op, st)))
 }
 
-template <typename C, typename Op>
-static hipError_t launch_conv(int logn, const LineIO<C>& io, int batch, const C* tw, Op op,
+template <typename S, typename C, typename Op>
+static hipError_t launch_conv(int logn, const LineIO<S>& io, int batch, const C* tw, Op op,
                               hipStream_t st) {
-  FH_LOGN_SWITCH(logn, (launch_conv_t<C, LG, Op>(io, batch, tw, op, st)))
+  FH_LOGN_SWITCH(logn, (launch_conv_t<S, C, LG, Op>(io, batch, tw, op, st)))
 }
 
 // ================================================================ table kernels (FP64)
@@ -517,10 +517,10 @@ static int fft2_d(fh_ctx* c, double2* d, int nx2, int ny2, int lx, int ly) {
   io.in = d; io.out = d; io.in_f = io.out_f = (long)nx2 * ny2;
   io.in_l = io.out_l = nx2; io.in_e = io.out_e = 1;
   io.nlines = ny2; io.in_valid = io.out_count = nx2;
-  FH_HIP((launch_fft<double2, false, false>(lx, io, 1, twx, NoOp{}, c->stream)));
+  FH_HIP((launch_fft<double2, double2, false, false>(lx, io, 1, twx, NoOp{}, c->stream)));
   io.in_l = io.out_l = 1; io.in_e = io.out_e = nx2;
   io.nlines = nx2; io.in_valid = io.out_count = ny2;
-  FH_HIP((launch_fft<double2, true, false>(ly, io, 1, twy, NoOp{}, c->stream)));
+  FH_HIP((launch_fft<double2, double2, true, false>(ly, io, 1, twy, NoOp{}, c->stream)));
   return FH_OK;
 }
 
@@ -542,13 +542,13 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
   io.in_l = p->nx; io.out_l = nxl; io.in_e = io.out_e = 1;
   io.nlines = p->ny; io.in_valid = p->nx; io.out_count = nxl;
   if (only < 0 || only == 0)
-    FH_HIP((launch_fft<float2, false, false>(p->lx, io, p->batch, twx, xop, c->stream)));
+    FH_HIP((launch_fft<float2, float2, false, false>(p->lx, io, p->batch, twx, xop, c->stream)));
   if (only == 0) return FH_OK;
   // y: FFT, multiply, IFFT on the nxl columns of mid (ny valid rows of nyl)
   io.in = mid; io.out = mid; io.in_f = io.out_f = midf;
   io.in_l = io.out_l = 1; io.in_e = io.out_e = nxl;
   io.nlines = nxl; io.in_valid = p->ny; io.out_count = p->ny;
-  FH_HIP((launch_conv<float2>(p->ly, io, p->batch, twy, q, c->stream)));
+  FH_HIP((launch_conv<float2, float2>(p->ly, io, p->batch, twy, q, c->stream)));
   return FH_OK;
 }
 
@@ -563,7 +563,7 @@ static int run_xifft(fh_plan* p, float2* u) {
   io.in = mid; io.out = u; io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
   io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1;
   io.nlines = p->ny; io.in_valid = nxl; io.out_count = p->nx;
-  FH_HIP((launch_fft<float2, false, true>(p->lx, io, p->batch, twx, NoOp{}, c->stream)));
+  FH_HIP((launch_fft<float2, float2, false, true>(p->lx, io, p->batch, twx, NoOp{}, c->stream)));
   return FH_OK;
 }
 
@@ -807,16 +807,316 @@ int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {
     io.in_l = io.out_l = nx; io.in_e = io.out_e = 1; io.nlines = ny; io.in_valid = io.out_count = nx;
     LineIO<double2> iy = io;
     iy.in_l = iy.out_l = 1; iy.in_e = iy.out_e = nx; iy.nlines = nx; iy.in_valid = iy.out_count = ny;
-    FH_HIP((launch_fft<double2, false, false>(p->lx, io, 1, twx, NoOp{}, c->stream)));
-    FH_HIP((launch_fft<double2, true, false>(p->ly, iy, 1, twy, NoOp{}, c->stream)));
+    FH_HIP((launch_fft<double2, double2, false, false>(p->lx, io, 1, twx, NoOp{}, c->stream)));
+    FH_HIP((launch_fft<double2, double2, true, false>(p->ly, iy, 1, twy, NoOp{}, c->stream)));
     const int blocks = (int)std::min<long>((npix + 255) / 256, 8192);
     hipLaunchKernelGGL(k_mul_full_d, dim3(blocks), dim3(256), 0, c->stream, f,
                        p->plain_tab + p->field_table[b] * npix, npix, direction < 0);
     FH_HIP(hipGetLastError());
-    FH_HIP((launch_fft<double2, true, true>(p->ly, iy, 1, twy, NoOp{}, c->stream)));
-    FH_HIP((launch_fft<double2, false, true>(p->lx, io, 1, twx, NoOp{}, c->stream)));
+    FH_HIP((launch_fft<double2, double2, true, true>(p->ly, iy, 1, twy, NoOp{}, c->stream)));
+    FH_HIP((launch_fft<double2, double2, false, true>(p->lx, io, 1, twx, NoOp{}, c->stream)));
   }
   return FH_OK;
+}
+
+}  // extern "C"
+
+// ================================================================ AS_BPM
+
+namespace fh {
+
+// FP64 quadrant of the angular-spectrum transfer function (same formula as k_as_table,
+// kept in FP64 for the slice steps).
+__global__ void k_as_table_d(double2* __restrict__ tab, int nx, int ny, int qx, int qy, double dx,
+                             double dy, double lam, double n0, double z, double scale) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= qx || j >= qy) return;
+  const double fx = (double)i / ((double)nx * dx);
+  const double fy = (double)j / ((double)ny * dy);
+  const double le = lam / n0;
+  const double f2 = 1.0 / (le * le);
+  const double a = f2 - fx * fx - fy * fy;
+  double re, im;
+  if (a >= 0.0) {
+    const double t = z * sqrt(a);
+    const double r = t - rint(t);
+    sincospi(2.0 * r, &im, &re);
+  } else {
+    re = exp(-2.0 * M_PI * fabs(z) * sqrt(-a));
+    im = 0.0;
+  }
+  tab[(long)j * qx + i] = make_double2(re * scale, im * scale);
+}
+
+template <int LOGN>
+static hipError_t launch_bpm_step_t(const LineIO<float2>& io, int batch, const double2* tw,
+                                    const DnPhaseOp& ph, float2* save, float2* ckspec, float* ddn,
+                                    const float2* saved, hipStream_t st) {
+  using CF = Cfg<double, LOGN, false>;
+  using RD = typename CF::RD;
+  constexpr int L = CF::L;
+  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n>::size * sizeof(double2) : 0;
+  const unsigned threads = L * RD::P;
+  if (ddn) {
+    auto kern = k_bpm_xstep_grad<double2, RD, L>;
+    hipError_t e = ensure_lds(kern, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L, 1), dim3(threads), lds, st, io, batch, tw, ph,
+                       saved, ddn);
+  } else {
+    auto kern = k_bpm_xstep<double2, RD, L>;
+    hipError_t e = ensure_lds(kern, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L, batch), dim3(threads), lds, st, io, tw, ph, save,
+                       ckspec);
+  }
+  return hipGetLastError();
+}
+
+static hipError_t launch_bpm_step(int logn, const LineIO<float2>& io, int batch, const double2* tw,
+                                  const DnPhaseOp& ph, float2* save, float2* ckspec, float* ddn,
+                                  const float2* saved, hipStream_t st) {
+  FH_LOGN_SWITCH(logn, (launch_bpm_step_t<LG>(io, batch, tw, ph, save, ckspec, ddn, saved, st)))
+}
+
+}  // namespace fh
+
+struct fh_bpm {
+  fh_ctx* ctx = nullptr;
+  int nx = 0, ny = 0, batch = 0, nslices = 0, every = 1, nck = 0;
+  int lx = 0, ly = 0;
+  double thickness = 0, n0 = 1, dz = 0;
+  std::vector<double> lam_tables;
+  std::vector<int> field_table;
+  double2* tfull = nullptr;  // FP64 quadrant tables of ASProp(dz; n0), one per wavelength
+  double2* thalf = nullptr;  // ... of ASProp(dz/2; n0)
+  int* d_tabidx = nullptr;
+  long tabsize = 0;
+  int qx = 0, qy = 0;
+  double* kl = nullptr;      // [B] kdz / lambda_b (vacuum wavelength)
+  float2* ckpt = nullptr;    // [nck][B][ny][nx] x-spectrum entering slice k, k % every == 0
+  float2* seg = nullptr;     // [every][B][ny][nx] slice inputs of one segment (recompute)
+  float2* work = nullptr;    // [B][ny][nx] recompute work field
+  int saved_dir = 0;         // direction of the last saving forward (0 = none)
+  size_t bytes = 0;
+};
+
+namespace fh {
+
+static LineIO<float2> xrows(fh_bpm* m, float2* f) {
+  LineIO<float2> io{};
+  io.in = f; io.out = f; io.in_f = io.out_f = (long)m->nx * m->ny;
+  io.in_l = io.out_l = m->nx; io.in_e = io.out_e = 1;
+  io.nlines = m->ny; io.in_valid = io.out_count = m->nx;
+  return io;
+}
+
+static void bpm_free(fh_bpm* m) {
+  if (!m) return;
+  for (void* q : {(void*)m->tfull, (void*)m->thalf, (void*)m->d_tabidx, (void*)m->kl, (void*)m->ckpt,
+                  (void*)m->seg, (void*)m->work})
+    if (q) (void)hipFree(q);
+  delete m;
+}
+
+// x FFT of rows, optionally with slice k's phase as prologue
+static int bpm_xfft(fh_bpm* m, float2* f, const float* dn_k, double sgn) {
+  const double2* tw = nullptr;
+  int rc;
+  if ((rc = get_twiddles(m->ctx, m->nx, &tw))) return rc;
+  LineIO<float2> io = xrows(m, f);
+  if (dn_k) {
+    DnPhaseOp ph{dn_k, m->kl, sgn, m->nx};
+    FH_HIP((launch_fft<float2, double2, false, false>(m->lx, io, m->batch, tw, ph, m->ctx->stream)));
+  } else {
+    FH_HIP((launch_fft<float2, double2, false, false>(m->lx, io, m->batch, tw, NoOp{}, m->ctx->stream)));
+  }
+  return FH_OK;
+}
+
+static int bpm_xifft(fh_bpm* m, float2* f) {
+  const double2* tw = nullptr;
+  int rc;
+  if ((rc = get_twiddles(m->ctx, m->nx, &tw))) return rc;
+  FH_HIP((launch_fft<float2, double2, false, true>(m->lx, xrows(m, f), m->batch, tw, NoOp{}, m->ctx->stream)));
+  return FH_OK;
+}
+
+static int bpm_ypass(fh_bpm* m, float2* f, const double2* tab, int direction) {
+  const double2* tw = nullptr;
+  int rc;
+  if ((rc = get_twiddles(m->ctx, m->ny, &tw))) return rc;
+  LineIO<float2> io{};
+  io.in = f; io.out = f; io.in_f = io.out_f = (long)m->nx * m->ny;
+  io.in_l = io.out_l = 1; io.in_e = io.out_e = m->nx;
+  io.nlines = m->nx; io.in_valid = io.out_count = m->ny;
+  QuadMulT<double2> q{tab, m->d_tabidx, m->tabsize, m->nx, m->ny, m->qx, direction < 0};
+  FH_HIP((launch_conv<float2, double2>(m->ly, io, m->batch, tw, q, m->ctx->stream)));
+  return FH_OK;
+}
+
+static int bpm_step(fh_bpm* m, float2* f, const float* dn_k, double sgn, float2* save, float2* ckspec) {
+  const double2* tw = nullptr;
+  int rc;
+  if ((rc = get_twiddles(m->ctx, m->nx, &tw))) return rc;
+  DnPhaseOp ph{dn_k, m->kl, sgn, m->nx};
+  FH_HIP(launch_bpm_step(m->lx, xrows(m, f), m->batch, tw, ph, save, ckspec, nullptr, nullptr, m->ctx->stream));
+  return FH_OK;
+}
+
+static int bpm_step_grad(fh_bpm* m, float2* f, const float* dn_k, double sgn, const float2* saved,
+                         float* ddn_k) {
+  const double2* tw = nullptr;
+  int rc;
+  if ((rc = get_twiddles(m->ctx, m->nx, &tw))) return rc;
+  DnPhaseOp ph{dn_k, m->kl, sgn, m->nx};
+  FH_HIP(launch_bpm_step(m->lx, xrows(m, f), m->batch, tw, ph, nullptr, nullptr, ddn_k, saved, m->ctx->stream));
+  return FH_OK;
+}
+
+// Slice inputs k0 .. k1-1 into seg[0 ..], replayed from the x-spectrum checkpoint of slice
+// k0 with the forward's own kernels, so they equal the forward's values bit for bit.
+static int bpm_recompute(fh_bpm* m, const float* dn, int k0, int k1, double sgn, int direction) {
+  const long npix = (long)m->nx * m->ny;
+  const long fsz = npix * m->batch;
+  FH_HIP(hipMemcpyAsync(m->work, m->ckpt + (long)(k0 / m->every) * fsz, fsz * sizeof(float2),
+                        hipMemcpyDeviceToDevice, m->ctx->stream));
+  int rc;
+  for (int k = k0; k < k1; ++k) {
+    if ((rc = bpm_step(m, m->work, dn + (long)k * npix, sgn, m->seg + (long)(k - k0) * fsz, nullptr))) return rc;
+    if (k + 1 < k1 && (rc = bpm_ypass(m, m->work, m->tfull, direction))) return rc;
+  }
+  return FH_OK;
+}
+
+}  // namespace fh
+
+extern "C" {
+
+int fh_bpm_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy, const double* lambdas,
+                  int nlambdas, double thickness, double n0, int nslices, int checkpoint_every,
+                  fh_bpm** out) {
+  if (!out) return fail(FH_E_ARG, "null output pointer");
+  *out = nullptr;
+  if (!ctx) return fail(FH_E_ARG, "null context");
+  if (nslices < 2) return fail(FH_E_ARG, "AS_BPM needs at least 2 slices, got %d", nslices);
+  if (checkpoint_every < 1) return fail(FH_E_ARG, "checkpoint_every must be >= 1");
+  if (!(thickness > 0) || !std::isfinite(thickness)) return fail(FH_E_ARG, "thickness must be positive");
+  if (!(n0 > 0)) return fail(FH_E_ARG, "n0 must be positive");
+  fh_plan tmp;  // wavelength bookkeeping and argument checks shared with the propagators
+  int rc = plan_common(ctx, nx, ny, batch, dx, dy, lambdas, nlambdas, &tmp);
+  if (rc) return rc;
+  const int lx = ilog2_exact(nx), ly = ilog2_exact(ny);
+  if (lx < kMinLog || lx > kMaxLog || ly < kMinLog || ly > kMaxLog)
+    return fail(FH_E_UNSUPP, "AS_BPM needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << kMinLog, 1 << kMaxLog, nx, ny);
+  if ((rc = set_device(ctx))) return rc;
+  fh_bpm* m = new fh_bpm();
+  m->ctx = ctx; m->nx = nx; m->ny = ny; m->batch = batch; m->nslices = nslices; m->lx = lx; m->ly = ly;
+  m->every = std::min(checkpoint_every, nslices);
+  m->thickness = thickness; m->n0 = n0; m->dz = thickness / nslices;
+  m->lam_tables = tmp.lam_tables; m->field_table = tmp.field_table;
+  m->qx = nx / 2 + 1; m->qy = ny / 2 + 1; m->tabsize = (long)m->qx * m->qy;
+  m->nck = (nslices + m->every - 1) / m->every;
+  const size_t ntab = m->lam_tables.size();
+  const size_t fsz = (size_t)nx * ny * batch * sizeof(float2);
+  const size_t tbytes = m->tabsize * ntab * sizeof(double2);
+  if (hipMalloc(&m->tfull, tbytes) != hipSuccess || hipMalloc(&m->thalf, tbytes) != hipSuccess ||
+      hipMalloc(&m->d_tabidx, batch * sizeof(int)) != hipSuccess ||
+      hipMalloc(&m->kl, batch * sizeof(double)) != hipSuccess ||
+      hipMalloc(&m->ckpt, m->nck * fsz) != hipSuccess || hipMalloc(&m->seg, m->every * fsz) != hipSuccess ||
+      hipMalloc(&m->work, fsz) != hipSuccess) {
+    bpm_free(m);
+    return fail(FH_E_NOMEM, "AS_BPM: out of device memory");
+  }
+  const double scale = 1.0 / ((double)nx * (double)ny);
+  for (size_t k = 0; k < ntab; ++k) {
+    dim3 grid((m->qx + 255) / 256, m->qy);
+    hipLaunchKernelGGL(k_as_table_d, grid, dim3(256), 0, ctx->stream, m->tfull + k * m->tabsize, nx, ny, m->qx,
+                       m->qy, dx, dy, m->lam_tables[k], n0, m->dz, scale);
+    hipLaunchKernelGGL(k_as_table_d, grid, dim3(256), 0, ctx->stream, m->thalf + k * m->tabsize, nx, ny, m->qx,
+                       m->qy, dx, dy, m->lam_tables[k], n0, m->dz / 2, scale);
+  }
+  // kdz = 2*pi*dz / prod(cos(theta)) with on-axis fields (bpm.jl:3-6,70)
+  const double kdz = 2.0 * M_PI * m->dz;
+  std::vector<double> kl(batch);
+  for (int b = 0; b < batch; ++b) kl[b] = kdz / lambdas[nlambdas == 1 ? 0 : b];
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpy(m->kl, kl.data(), batch * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_tabidx, m->field_table.data(), batch * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    bpm_free(m);
+    return fail(FH_E_HIP, "AS_BPM: table set-up failed");
+  }
+  m->bytes = (m->nck + m->every + 1) * fsz + 2 * tbytes;
+  *out = m;
+  return FH_OK;
+}
+
+int fh_bpm_destroy(fh_bpm* m) {
+  if (!m) return FH_OK;
+  (void)hipSetDevice(m->ctx->device);
+  bpm_free(m);
+  return FH_OK;
+}
+
+int fh_bpm_workspace_bytes(fh_bpm* m, size_t* bytes) {
+  if (!m || !bytes) return fail(FH_E_ARG, "null BPM or output");
+  *bytes = m->bytes;
+  return FH_OK;
+}
+
+int fh_bpm_forward(fh_bpm* m, void* u, const float* dn, int direction, int save) {
+  if (!m || !u || !dn) return fail(FH_E_ARG, "null BPM, field or index profile");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(m->ctx))) return rc;
+  float2* f = reinterpret_cast<float2*>(u);
+  const long npix = (long)m->nx * m->ny;
+  const long fsz = npix * m->batch;
+  const double sgn = direction;
+  // P(dz/2); per slice [IFFT_x, phase, FFT_x] + y pass (P(dz), or P(dz/2) after the
+  // last slice); IFFT_x (bpm.jl:244-260)
+  if ((rc = bpm_xfft(m, f, nullptr, sgn))) return rc;
+  if ((rc = bpm_ypass(m, f, m->thalf, direction))) return rc;
+  for (int k = 0; k < m->nslices; ++k) {
+    float2* ck = (save && k % m->every == 0) ? m->ckpt + (long)(k / m->every) * fsz : nullptr;
+    if ((rc = bpm_step(m, f, dn + (long)k * npix, sgn, nullptr, ck))) return rc;
+    if ((rc = bpm_ypass(m, f, k + 1 < m->nslices ? m->tfull : m->thalf, direction))) return rc;
+  }
+  if ((rc = bpm_xifft(m, f))) return rc;
+  m->saved_dir = save ? direction : 0;
+  return FH_OK;
+}
+
+int fh_bpm_backward(fh_bpm* m, void* du, const float* dn, float* ddn, int direction) {
+  if (!m || !du || !dn) return fail(FH_E_ARG, "null BPM, cotangent or index profile");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(m->ctx))) return rc;
+  if (ddn && m->saved_dir != direction)
+    return fail(FH_E_ARG, "the dn gradient needs a saving fh_bpm_forward in the same direction first");
+  float2* f = reinterpret_cast<float2*>(du);
+  const long npix = (long)m->nx * m->ny;
+  const long fsz = npix * m->batch;
+  const double sgn = direction;
+  // reverse loop of bpm.jl:289-312: adjoint of the trailing P(dz/2), then per slice
+  // [IFFT_x, un-apply phase (+ gradient), FFT_x] + adjoint y pass, then IFFT_x
+  if ((rc = bpm_xfft(m, f, nullptr, sgn))) return rc;
+  if ((rc = bpm_ypass(m, f, m->thalf, -direction))) return rc;
+  for (int k = m->nslices - 1; k >= 0; --k) {
+    if (ddn) {
+      const int k0 = (k / m->every) * m->every;
+      if (k == m->nslices - 1 || k % m->every == m->every - 1) {
+        const int k1 = std::min(k0 + m->every, m->nslices);
+        if ((rc = bpm_recompute(m, dn, k0, k1, sgn, direction))) return rc;
+      }
+      if ((rc = bpm_step_grad(m, f, dn + (long)k * npix, sgn, m->seg + (long)(k - k0) * fsz, ddn + (long)k * npix)))
+        return rc;
+    } else {
+      if ((rc = bpm_step(m, f, dn + (long)k * npix, -sgn, nullptr, nullptr))) return rc;
+    }
+    if ((rc = bpm_ypass(m, f, k > 0 ? m->tfull : m->thalf, -direction))) return rc;
+  }
+  return bpm_xifft(m, f);
 }
 
 }  // extern "C"

@@ -6,10 +6,11 @@ the arithmetic is the HIP library's (``libfluxhip.so``, C ABI in include/fluxhip
 
 from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficiency,
                      rs_valid_distance)
-from .components import (ASProp, Phase, RSProp, phase_propagate_, propagate_phase_backward_)
+from .components import (AS_BPM, ASProp, Phase, RSProp, phase_propagate_,
+                         propagate_phase_backward_)
 
 __all__ = [
-    "ASProp", "Backward", "Direction", "Forward", "Phase", "RSProp", "ScalarField",
+    "AS_BPM", "ASProp", "Backward", "Direction", "Forward", "Phase", "RSProp", "ScalarField",
     "coupling_efficiency", "phase_propagate_", "propagate_phase_backward_", "propagate",
     "propagate_", "backpropagate_", "rs_valid_distance",
 ]

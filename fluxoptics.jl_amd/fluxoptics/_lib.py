@@ -44,6 +44,12 @@ SIGNATURES = {
     "fh_propagate_phase_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                             c_int, c_int]),
     "fh_propagate_plain_fp64": (c_int, [c_void_p, c_void_p, c_int]),
+    "fh_bpm_create": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_double_p, c_int,
+                              c_double, c_double, c_int, c_int, ctypes.POINTER(c_void_p)]),
+    "fh_bpm_destroy": (c_int, [c_void_p]),
+    "fh_bpm_workspace_bytes": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
+    "fh_bpm_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int]),
+    "fh_bpm_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
 }
 
 

@@ -260,3 +260,90 @@ def propagate_phase_backward_(dv: ScalarField, phase: Phase, prop: _KernelPropag
     _lib.call("fh_propagate_phase_backward", prop.plan(dv), _ptr(dv.electric), _ptr(phase.phi),
               _ptr(saved), _ptr(dphi), dir_sign(direction), int(accumulate))
     return dv
+
+
+# --------------------------------------------------------------------------- AS_BPM
+
+class AS_BPM:
+    """Split-step beam propagation through a graded-index volume (``bpm.jl:130-142``).
+
+    ``dn`` has shape (nslices, ny, nx) (the reference's ``dn[nx, ny, nslices]``);
+    ``thickness`` is the total length, ``n0`` the background index.  For the gradient
+    the library keeps one checkpoint every ``checkpoint_every`` slices instead of every
+    slice input and recomputes one segment at a time in the adjoint.
+    """
+
+    def __init__(self, u: ScalarField, thickness: float, n0: float, dn, *, trainable: bool = False,
+                 buffered: bool = False, checkpoint_every: int = 20, use_cache: bool = True,
+                 paraxial: bool = False, aperture: Optional[Callable] = None):
+        if paraxial:
+            raise NotImplementedError("paraxial AS_BPM is not on the accelerated path")
+        check_field(u.electric)
+        device = u.electric.device
+        self.dn = torch.as_tensor(np.asarray(dn) if not isinstance(dn, torch.Tensor) else dn,
+                                  dtype=torch.float32, device=device).contiguous()
+        if self.dn.dim() != 3 or tuple(self.dn.shape[1:]) != (u.ny, u.nx):
+            raise ValueError("dn must have shape (nslices, ny, nx)")
+        if self.dn.shape[0] < 2:
+            raise ValueError("AS_BPM needs at least 2 slices (bpm.jl:40)")
+        self.nslices = int(self.dn.shape[0])
+        self.thickness, self.n0 = float(thickness), float(n0)
+        self.nx, self.ny, self.ds = u.nx, u.ny, u.ds
+        self.checkpoint_every = int(checkpoint_every)
+        self.trainable = trainable
+        self.grad = torch.zeros_like(self.dn) if (trainable and buffered) else None
+        self._objs = {}
+
+    def _obj(self, u: ScalarField) -> ctypes.c_void_p:
+        if (u.nx, u.ny) != (self.nx, self.ny):
+            raise ValueError("field grid does not match the BPM volume")
+        check_field(u.electric)
+        lams = tuple(u.lambda_list())
+        key = (u.electric.device.index, u.batch, lams)
+        ctx = context(u.electric.device)
+        m = self._objs.get(key)
+        if m is None:
+            m = ctypes.c_void_p()
+            arr = (ctypes.c_double * len(lams))(*lams)
+            _lib.call("fh_bpm_create", ctx, self.nx, self.ny, u.batch, self.ds[0], self.ds[1], arr,
+                      len(lams), self.thickness, self.n0, self.nslices, self.checkpoint_every,
+                      ctypes.byref(m))
+            self._objs[key] = m
+        return m
+
+    def __del__(self):
+        try:
+            for m in self._objs.values():
+                _lib.load().fh_bpm_destroy(m)
+        except Exception:
+            pass
+
+    def workspace_bytes(self, u: ScalarField) -> int:
+        b = ctypes.c_size_t()
+        _lib.call("fh_bpm_workspace_bytes", self._obj(u), ctypes.byref(b))
+        return int(b.value)
+
+    def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
+        _lib.call("fh_bpm_forward", self._obj(u), _ptr(u.electric), _ptr(self.dn), dir_sign(direction), 0)
+        return u
+
+    def propagate(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self.propagate_(u.copy(), direction)
+
+    def propagate_and_save_(self, u: ScalarField, direction=Forward) -> ScalarField:
+        _lib.call("fh_bpm_forward", self._obj(u), _ptr(u.electric), _ptr(self.dn), dir_sign(direction), 1)
+        return u
+
+    def backpropagate_(self, du: ScalarField, direction=Forward) -> ScalarField:
+        _lib.call("fh_bpm_backward", self._obj(du), _ptr(du.electric), _ptr(self.dn), None, dir_sign(direction))
+        return du
+
+    def backpropagate_with_gradient_(self, du: ScalarField, ddn: Optional[torch.Tensor] = None,
+                                     direction=Forward):
+        """Returns (du, ddn); needs a preceding ``propagate_and_save_`` in the same direction."""
+        ddn = self.grad if ddn is None else ddn
+        if ddn is None:
+            ddn = torch.empty_like(self.dn)
+        _lib.call("fh_bpm_backward", self._obj(du), _ptr(du.electric), _ptr(self.dn), _ptr(ddn),
+                  dir_sign(direction))
+        return du, ddn

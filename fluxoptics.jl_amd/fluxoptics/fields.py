@@ -91,13 +91,14 @@ class ScalarField:
         return ScalarField(data, self.ds, self.lambdas)
 
     def power(self) -> torch.Tensor:
-        """Per-field power sum|u|^2 dx dy (``Fields.jl:843-846``)."""
-        return (self.electric.abs() ** 2).sum(dim=(-2, -1)) * (self.ds[0] * self.ds[1])
+        """Per-field power sum|u|^2 dx dy, shape (B,) (``Fields.jl:843-846``)."""
+        e = self.electric.reshape(self.batch, self.ny, self.nx)
+        return (e.abs() ** 2).sum(dim=(-2, -1)) * (self.ds[0] * self.ds[1])
 
     def normalize_power_(self, target: float = 1.0) -> "ScalarField":
         """``normalize_power!`` (``Fields.jl:889-892``)."""
-        p = self.power()
-        self.electric *= torch.sqrt(target / p).to(self.electric.dtype)[..., None, None]
+        scale = torch.sqrt(target / self.power()).to(self.electric.dtype)
+        self.electric *= scale.reshape((-1, 1, 1) if self.electric.dim() == 3 else (1, 1))
         return self
 
     def __repr__(self):
@@ -106,8 +107,9 @@ class ScalarField:
 
 
 def coupling_efficiency(u: ScalarField, v: ScalarField) -> torch.Tensor:
-    """|<u,v>|^2 / (<u,u><v,v>) per field (``Fields.jl:748-756``)."""
-    a, b = u.electric, v.electric
+    """|<u,v>|^2 / (<u,u><v,v>) per field, shape (B,) (``Fields.jl:748-756``)."""
+    a = u.electric.reshape(u.batch, u.ny, u.nx)
+    b = v.electric.reshape(v.batch, v.ny, v.nx)
     num = (a.conj() * b).sum(dim=(-2, -1)).abs() ** 2
     den = (a.abs() ** 2).sum(dim=(-2, -1)) * (b.abs() ** 2).sum(dim=(-2, -1))
     return num / den

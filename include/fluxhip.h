@@ -107,6 +107,32 @@ int fh_propagate_phase_backward(fh_plan* plan, void* dv, const float* phi, const
  * multiply by a full FP64 transfer-function table, y IFFT, x IFFT). */
 int fh_propagate_plain_fp64(fh_plan* plan, void* u_c128, int direction);
 
+/* AS_BPM: split-step beam propagation through nslices slices of index perturbation dn
+ * (AS_BPM(u, thickness, n0, dn), bpm.jl:130-142 and :8-74): dz = thickness/nslices,
+ * P(dz) and P(dz/2) are ASProps in index n0, the slice phase is
+ * cis(s*kdz/lambda*dn_k) with kdz = 2*pi*dz and the vacuum wavelength.
+ * The reference keeps every slice input for the gradient (bpm.jl:49,227-231); this
+ * library keeps one checkpoint every checkpoint_every slices and recomputes a segment
+ * at a time in the adjoint, bitwise equal to a forward that keeps them all. */
+int fh_bpm_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
+                  const double* lambdas, int nlambdas, double thickness, double n0,
+                  int nslices, int checkpoint_every, fh_bpm** out);
+int fh_bpm_destroy(fh_bpm* bpm);
+
+/* Device memory owned by the BPM object (checkpoints, segment, tables), in bytes. */
+int fh_bpm_workspace_bytes(fh_bpm* bpm, size_t* bytes);
+
+/* propagate!(u, bpm, direction) (bpm.jl:244-260) in place; dn is float
+ * [nslices][ny][nx] (the reference's dn[nx, ny, nslices]).  save != 0 keeps the
+ * checkpoints that fh_bpm_backward needs for the dn gradient
+ * (propagate_and_save!, bpm.jl:262-273). */
+int fh_bpm_forward(fh_bpm* bpm, void* u, const float* dn, int direction, int save);
+
+/* backpropagate!(du, bpm, direction; u_saved, dp) (bpm.jl:289-312): du <- adjoint(du)
+ * in place; if ddn != NULL it receives the dn gradient [nslices][ny][nx], summed over
+ * the batch (requires a saving forward in the same direction with the same dn). */
+int fh_bpm_backward(fh_bpm* bpm, void* du, const float* dn, float* ddn, int direction);
+
 #ifdef __cplusplus
 }
 #endif

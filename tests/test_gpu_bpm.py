@@ -1,0 +1,108 @@
+"""GPU parity of AS_BPM (forward, adjoint, dn gradient, checkpointing) against the oracle."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import fluxoptics as fo
+from fluxoptics import inputs
+from oracle import fluxoptics_oracle as O
+from _util import c64, rel_l2, random_field, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def smooth_dn(ns, n, sigma, peak, seed):
+    """Gaussian-filtered white noise (periodic, FFT filter) scaled to max |dn| = peak."""
+    w = inputs.normal(seed, (ns, n, n))
+    k = np.fft.fftfreq(n)
+    kz = np.fft.fftfreq(ns)
+    g = np.exp(-2 * (math.pi * sigma) ** 2 * (kz[:, None, None] ** 2 + k[None, :, None] ** 2
+                                              + k[None, None, :] ** 2))
+    f = np.real(np.fft.ifftn(np.fft.fftn(w) * g))
+    return (peak * f / np.max(np.abs(f))).astype(np.float32)
+
+
+def tilted_gaussian(n, ds, lam, w0, deg):
+    x = inputs.grid(n, ds)[None, :]
+    return inputs.gaussian(n, n, ds, ds, w0) * np.exp(2j * math.pi * math.sin(math.radians(deg)) * x / lam)
+
+
+def test_bpm_zero_dn_is_asprop():
+    n, ds, lam, n0, L, ns = 128, (0.5, 0.5), 1.0, 1.5, 40.0, 8
+    u0 = c64(random_field((2, n, n), 1))
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    bpm = fo.AS_BPM(u, L, n0, np.zeros((ns, n, n), np.float32))
+    v = bpm.propagate(u, fo.Forward)
+    assert rel_l2(to_host(v.electric), O.asprop(u0, ds, lam, L, n0)) <= 1e-5
+
+
+@pytest.mark.parametrize("n,ns,batch,every", [(64, 6, 2, 1), (64, 6, 2, 4), (256, 12, 1, 5)])
+def test_bpm_forward_adjoint_gradient(n, ns, batch, every):
+    print(f"[bpm parity] n={n} slices={ns} batch={batch} every={every}", flush=True)
+    ds, n0, L = (0.5, 0.5), 1.5, 3.0 * ns
+    lams = [1.0, 0.9][:batch]
+    dn = smooth_dn(ns, n, 2.0, 0.01, 4)
+    u0 = c64(np.stack([tilted_gaussian(n, 0.5, 1.0, 0.15 * n * 0.5, 2.0)] * batch))
+    t0 = c64(0.01 * random_field((batch, n, n), 9))
+    u = fo.ScalarField(to_dev(u0), ds, lams)
+    bpm = fo.AS_BPM(u, L, n0, dn, trainable=True, buffered=True, checkpoint_every=every)
+    bpm.propagate_and_save_(u, fo.Forward)
+    v = to_host(u.electric)
+    vref, saved = O.bpm_forward(u0, dn.astype(np.float64), L, n0, lams, ds)
+    assert rel_l2(v, vref) <= 1e-5
+    dv0 = c64(v - t0)
+    du = fo.ScalarField(to_dev(dv0), ds, lams)
+    bpm.backpropagate_with_gradient_(du, None, fo.Forward)
+    duref, ddnref = O.bpm_backward(dv0, dn.astype(np.float64), L, n0, lams, ds, saved)
+    assert rel_l2(to_host(du.electric), duref) <= 1e-5
+    torch.cuda.synchronize()
+    assert rel_l2(bpm.grad.cpu().numpy(), ddnref) <= 1e-4
+
+
+def test_bpm_checkpointing_matches_full_save():
+    n, ns, ds, lam, n0, L = 128, 10, (0.5, 0.5), 1.0, 1.5, 10.0
+    dn = smooth_dn(ns, n, 2.0, 0.01, 5)
+    u0 = c64(tilted_gaussian(n, 0.5, lam, 15.0, 2.0))
+    dv0 = c64(random_field((n, n), 3))
+    grads = []
+    for every in (1, 3, 10):
+        u = fo.ScalarField(to_dev(u0), ds, lam)
+        bpm = fo.AS_BPM(u, L, n0, dn, trainable=True, buffered=True, checkpoint_every=every)
+        bpm.propagate_and_save_(u, fo.Forward)
+        du = fo.ScalarField(to_dev(dv0), ds, lam)
+        bpm.backpropagate_with_gradient_(du, None, fo.Forward)
+        torch.cuda.synchronize()
+        grads.append((bpm.grad.cpu().numpy(), to_host(du.electric)))
+    for g, d in grads[1:]:
+        assert np.array_equal(g, grads[0][0])
+        assert np.array_equal(d, grads[0][1])
+
+
+def test_bpm_config4_graded_index_volume():
+    """512^2, ds=0.5, lam=1, n_bulk=1.5, 400 slices of dz=1, checkpoint every 20."""
+    print("[bpm config4] 512^2 x 400 slices", flush=True)
+    n, ns, ds, lam, n0 = 512, 400, (0.5, 0.5), 1.0, 1.5
+    dn = smooth_dn(ns, n, 3.0, 0.01, 4)
+    u0 = c64(tilted_gaussian(n, 0.5, lam, 80.0, 2.0))
+    t0 = c64(0.01 * random_field((n, n), 11))
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    bpm = fo.AS_BPM(u, float(ns), n0, dn, trainable=True, buffered=True, checkpoint_every=20)
+    torch.cuda.reset_peak_memory_stats()
+    bpm.propagate_and_save_(u, fo.Forward)
+    v = to_host(u.electric)
+    vref, saved = O.bpm_forward(u0, dn.astype(np.float64), float(ns), n0, lam, ds)
+    ev = rel_l2(v, vref)
+    dv0 = c64(v - t0)
+    du = fo.ScalarField(to_dev(dv0), ds, lam)
+    bpm.backpropagate_with_gradient_(du, None, fo.Forward)
+    duref, ddnref = O.bpm_backward(dv0, dn.astype(np.float64), float(ns), n0, lam, ds, saved)
+    edu = rel_l2(to_host(du.electric), duref)
+    torch.cuda.synchronize()
+    eg = rel_l2(bpm.grad.cpu().numpy(), ddnref)
+    ws = bpm.workspace_bytes(u)
+    print(f"config4: field {ev:.2e} adjoint {edu:.2e} ddn {eg:.2e}; library workspace {ws / 2**20:.1f} MiB "
+          f"(a full save of every slice input is {ns * n * n * 8 / 2**20:.0f} MiB)")
+    assert ev <= 1e-5 and edu <= 1e-5 and eg <= 1e-4
