@@ -1,0 +1,274 @@
+// Fast c64 line passes: the same Stockham transforms as fh_lines.hpp with the address
+// arithmetic moved out of the vector units.
+//
+//  - global memory goes through buffer instructions with one descriptor per field; a
+//    register holds the thread's base offset and every other offset is a compile-time or
+//    workgroup-uniform constant (SGPR soffset), so a load costs no VALU address math.
+//    Offsets past the field's end read 0 and drop stores (the descriptor's bound).
+//  - LDS positions are linear in the register index, so each exchange computes one base
+//    per thread and uses immediate offsets;
+//  - stage twiddles come from per-plan tables laid out [j mod NS][r-1], one base per
+//    butterfly plus immediate offsets;
+//  - lines shorter than the transform (zero-padded RS input, cropped RS output) are
+//    exactly half of it, so the valid/zero decision is a compile-time predicate.
+#pragma once
+
+#include "fh_lines.hpp"
+
+namespace fh {
+
+constexpr int kBufWord3 = 0x00020000;  // raw buffer, gfx9-family data format
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, long bytes) {
+  const long lim = bytes < 0x7fffffffL ? bytes : 0x7fffffffL;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)lim, kBufWord3);
+}
+
+__device__ __forceinline__ float2 buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, int voff, int soff, float2 v) {
+  using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, voff, soff, 0);
+}
+
+// Pass geometry of the fast kernels (offsets in elements; fields are < 2 GiB).
+struct FastIO {
+  const float2* in;
+  float2* out;
+  long in_f, out_f;      // elements between fields
+  int in_l, out_l;       // elements between lines (x passes) or 1 (y passes)
+  int in_e, out_e;       // 1 (x passes) or elements between rows (y passes)
+  int in_blk, out_blk;   // 4x4-blocked layout (elem_addr)
+  int nlines;
+  long in_bytes, out_bytes;  // bytes of one field (descriptor bound)
+};
+
+// Offset of (line li, position pos) and the per-position stride K such that
+// offset(li, t + C) = offset(li, t) + C*K whenever C is a multiple of 4.
+template <bool LFAST>
+__device__ __forceinline__ int fast_off(int li, int pos, int lstride, int estride, int blk) {
+  return (int)elem_addr<LFAST>(li, pos, lstride, estride, blk);
+}
+template <bool LFAST>
+__device__ __forceinline__ int fast_k(int lstride, int estride, int blk) {
+  return LFAST ? estride : (blk ? 4 : 1);
+}
+
+// ---------------------------------------------------------------- LDS with linear offsets
+
+// Stage writes go to positions base_m + r*NS with base_m = base + m*P*R.  With R and NS
+// dividing 16 or being multiples of 16 and P*R a multiple of 16, (base mod 16) +
+// (offset mod 16) < 16 for every thread, so lds_pos(base + offset) =
+// lds_pos(base) + offset + offset/16 and every slot is an immediate offset.
+template <typename C, int n, int R, int NS, int V, int P>
+__device__ __forceinline__ void fast_write(C* __restrict__ s, const C* v, int t) {
+  static_assert(P % NS == 0, "write stage needs P a multiple of NS");
+  static_assert(NS % 16 == 0 || 16 % NS == 0, "NS must divide or be a multiple of 16");
+  static_assert(R % 16 == 0 || 16 % R == 0, "R must divide or be a multiple of 16");
+  static_assert((P * R) % 16 == 0, "P*R must be a multiple of 16");
+  const int base = (t / NS) * NS * R + (t % NS);
+  const int lb = lds_pos(base);
+#pragma unroll
+  for (int m = 0; m < V / R; ++m)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      // base_m = base + m*P*R (P*R a multiple of 16), element offset r*NS
+      const int c = m * P * R + r * NS;
+      s[lb + c + (c >> 4)] = v[m * R + r];
+    }
+}
+
+template <typename C, int n, int R, int V, int P>
+__device__ __forceinline__ void fast_read(C* v, const C* __restrict__ s, int t) {
+  if constexpr (P % 16 == 0 && (n / R) % 16 == 0) {
+    const int lb = lds_pos(t);
+#pragma unroll
+    for (int m = 0; m < V / R; ++m)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int c = P * m + r * (n / R);
+        v[m * R + r] = s[lb + c + (c >> 4)];
+      }
+  } else {
+#pragma unroll
+    for (int m = 0; m < V / R; ++m)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[m * R + r] = s[lds_pos(t + P * m + r * (n / R))];
+  }
+}
+
+// twiddles of a stage: table [jm][r-1], jm = j mod NS, j = t + P*m
+template <typename C, int n, int R, int NS, int V, int P>
+__device__ __forceinline__ void fast_twiddle(C* v, int t, const C* __restrict__ tab) {
+  if constexpr (NS > 1) {
+#pragma unroll
+    for (int m = 0; m < V / R; ++m) {
+      const int jm = (t + P * m) & (NS - 1);
+      const C* w = tab + jm * (R - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[m * R + r] = cmul(v[m * R + r], w[r - 1]);
+    }
+  }
+}
+
+template <typename C, int R, int V>
+__device__ __forceinline__ void fast_dfts(C* v) {
+#pragma unroll
+  for (int m = 0; m < V / R; ++m) dft<R, false>(&v[m * R]);
+}
+template <typename C, int R, int V>
+__device__ __forceinline__ void fast_idfts(C* v) {
+#pragma unroll
+  for (int m = 0; m < V / R; ++m) dft<R, true>(&v[m * R]);
+}
+
+// Stage tables of a radix plan, concatenated: stage 1 (NS = R0, R = R1) then stage 2
+// (NS = R0*R1, R = R2).  Inverse plans get their own (conjugated) tables.
+template <typename RD> struct FastTw {
+  static constexpr int off1 = 0;
+  static constexpr int off2 = RD::R0 * (RD::R1 - 1);
+  static constexpr int size = off2 + (RD::NST == 3 ? RD::R0 * RD::R1 * (RD::R2 - 1) : 0);
+};
+
+// Full transform in registers (forward or inverse according to INV; the tables
+// already carry the sign).
+template <typename C, typename RD, bool INV>
+__device__ __forceinline__ void fast_fft(C* v, C* s, int t, const C* __restrict__ tab) {
+  constexpr int n = RD::n, V = RD::V, P = RD::P;
+  if (INV) fast_idfts<C, RD::R0, V>(v); else fast_dfts<C, RD::R0, V>(v);
+  if constexpr (RD::NST >= 2) {
+    __syncthreads();
+    fast_write<C, n, RD::R0, 1, V, P>(s, v, t);
+    __syncthreads();
+    fast_read<C, n, RD::R1, V, P>(v, s, t);
+    fast_twiddle<C, n, RD::R1, RD::R0, V, P>(v, t, tab + FastTw<RD>::off1);
+    if (INV) fast_idfts<C, RD::R1, V>(v); else fast_dfts<C, RD::R1, V>(v);
+  }
+  if constexpr (RD::NST >= 3) {
+    __syncthreads();
+    fast_write<C, n, RD::R1, RD::R0, V, P>(s, v, t);
+    __syncthreads();
+    fast_read<C, n, RD::R2, V, P>(v, s, t);
+    fast_twiddle<C, n, RD::R2, RD::R0 * RD::R1, V, P>(v, t, tab + FastTw<RD>::off2);
+    if (INV) fast_idfts<C, RD::R2, V>(v); else fast_dfts<C, RD::R2, V>(v);
+  }
+}
+
+template <typename RD> struct LastRadix {
+  static constexpr int value = (RD::NST == 3) ? RD::R2 : ((RD::NST == 2) ? RD::R1 : RD::R0);
+};
+
+// ---------------------------------------------------------------- kernels
+
+// Line FFT (forward or inverse), c64 global -> global, with a time-side op (prologue of a
+// forward transform, epilogue of an inverse one).  IN_HALF: only the first n/2 points of
+// each line are read (the rest are zero); OUT_HALF: only the first n/2 are stored.
+template <typename RDF, int L, bool LFAST, bool INV, bool IN_HALF, bool OUT_HALF, typename TimeOp>
+__global__ void __launch_bounds__(L * RDF::P)
+k_fast_fft(FastIO io, const float2* __restrict__ tab, TimeOp op) {
+  using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
+  constexpr int n = RD::n, V = RD::V, P = RD::P;
+  constexpr int RL = LastRadix<RD>::value;
+  extern __shared__ unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  int l, t;
+  thread_map<L, P, LFAST>(threadIdx.x, l, t);
+  const int b = blockIdx.y;
+  const int li = blockIdx.x * L + l;
+  const bool live = li < io.nlines;
+  float2* s = smem + l * LdsLine<n>::size;
+  const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
+  const int vin = fast_off<LFAST>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
+  const int kin = fast_k<LFAST>(io.in_l, io.in_e, io.in_blk) * 8;
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+    for (int r = 0; r < RD::R0; ++r) {
+      const int c = P * m + r * (n / RD::R0);
+      float2 x = make_float2(0.f, 0.f);
+      if (!IN_HALF || c < n / 2) x = buf_load(rin, vin, c * kin);
+      v[m * RD::R0 + r] = x;
+    }
+  if (!INV && live) {
+#pragma unroll
+    for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+      for (int r = 0; r < RD::R0; ++r)
+        v[m * RD::R0 + r] = op(v[m * RD::R0 + r], b, li, t + P * m + r * (n / RD::R0));
+  }
+  fast_fft<float2, RD, INV>(v, s, t, tab);
+  const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
+  const int vout = fast_off<LFAST>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
+  const int kout = fast_k<LFAST>(io.out_l, io.out_e, io.out_blk) * 8;
+  if (live) {
+#pragma unroll
+    for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int c = P * m + r * (n / RL);
+        float2 x = v[m * RL + r];
+        if (INV) x = op(x, b, li, t + c);
+        if (!OUT_HALF || c < n / 2) buf_store(rout, vout, c * kout, x);
+      }
+  }
+}
+
+// FFT -> pointwise multiply -> IFFT along the lines, c64 global -> global.
+template <typename RDF, int L, bool LFAST, bool IN_HALF, bool OUT_HALF, typename FreqOp>
+__global__ void __launch_bounds__(L * RDF::P)
+k_fast_conv(FastIO io, const float2* __restrict__ tabf, const float2* __restrict__ tabi, FreqOp op, int G) {
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  using RI = typename RDF::Rev;
+  constexpr int RL = RI::R0;
+  constexpr int RE = LastRadix<RI>::value;
+  extern __shared__ unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  int l, t;
+  thread_map<L, P, LFAST>(threadIdx.x, l, t);
+  const int b = blockIdx.y;
+  const int tile = LFAST ? xcd_group_tile(blockIdx.x, gridDim.x, G) : (int)blockIdx.x;
+  const int li = tile * L + l;
+  const bool live = li < io.nlines;
+  float2* s = smem + l * LdsLine<n>::size;
+  const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
+  const int vin = fast_off<LFAST>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
+  const int kin = fast_k<LFAST>(io.in_l, io.in_e, io.in_blk) * 8;
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V / RDF::R0; ++m)
+#pragma unroll
+    for (int r = 0; r < RDF::R0; ++r) {
+      const int c = P * m + r * (n / RDF::R0);
+      float2 x = make_float2(0.f, 0.f);
+      if (!IN_HALF || c < n / 2) x = buf_load(rin, vin, c * kin);
+      v[m * RDF::R0 + r] = x;
+    }
+  fast_fft<float2, RDF, false>(v, s, t, tabf);
+  if (live) {
+#pragma unroll
+    for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int c = P * m + r * (n / RL);
+        v[m * RL + r] = op(v[m * RL + r], b, li, t + c);
+      }
+  }
+  fast_fft<float2, RI, true>(v, s, t, tabi);
+  const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
+  const int vout = fast_off<LFAST>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
+  const int kout = fast_k<LFAST>(io.out_l, io.out_e, io.out_blk) * 8;
+  if (live) {
+#pragma unroll
+    for (int m = 0; m < V / RE; ++m)
+#pragma unroll
+      for (int r = 0; r < RE; ++r) {
+        const int c = P * m + r * (n / RE);
+        if (!OUT_HALF || c < n / 2) buf_store(rout, vout, c * kout, v[m * RE + r]);
+      }
+  }
+}
+
+}  // namespace fh

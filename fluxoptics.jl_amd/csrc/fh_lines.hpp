@@ -121,7 +121,21 @@ template <typename C> struct LineIO {
   int nlines;         // lines per field
   int in_valid;       // points [0, in_valid) are read, the rest are zero
   int out_count;      // points [0, out_count) are stored
+  int in_blk, out_blk;  // 1: 4x4-blocked spectrum layout (see elem_addr)
 };
+
+// Element (x, y) of a field with rows of `rowlen` elements.  Natural layout: y*rowlen + x.
+// Blocked layout (the intermediate spectrum between passes): 4x4 blocks of 128 bytes,
+// block (y/4, x/4) at ((y/4)*(rowlen/4) + x/4)*16, element (y%4)*4 + x%4 inside.  Rows
+// 4g..4g+3 occupy the same address range in both layouts, so an x pass that owns whole
+// 4-row groups can convert in place, and a y pass over 4 columns reads whole lines.
+template <bool LFAST>
+__device__ __forceinline__ long elem_addr(int li, int pos, long lstride, long estride, int blk) {
+  if (!blk) return (long)li * lstride + (long)pos * estride;
+  const int x = LFAST ? li : pos, y = LFAST ? pos : li;
+  const long rowlen = LFAST ? estride : lstride;
+  return ((((long)(y >> 2)) * (rowlen >> 2) + (x >> 2)) << 4) + ((y & 3) << 2) + (x & 3);
+}
 
 // thread -> (line in workgroup, thread within line); LFAST puts lines on adjacent
 // lanes (strided-axis passes, so a wave reads L adjacent columns of one row).
@@ -139,13 +153,15 @@ template <typename C, typename S> __device__ __forceinline__ C cvt(S v) {
   C r; r.x = v.x; r.y = v.y; return r;
 }
 
-// Tile order for strided-axis passes: workgroups are dispatched round-robin over the 8
-// XCDs, so neighbouring column tiles (which share 128-byte rows when a tile is 64 bytes
-// wide) are placed 8 blocks apart to land on the same XCD and share its L2.
-__device__ __forceinline__ int xcd_pair_tile(int b, int nb) {
-  if (nb & 15) return b;
-  const int g = b >> 4, i = b & 15;
-  return (g << 4) + ((i & 7) << 1) + (i >> 3);
+// Tile order for strided-axis passes.  Workgroups are dispatched round-robin over the
+// 8 XCDs; G neighbouring column tiles share each 128-byte row segment (G = 128 B / tile
+// width), so they are placed 8 blocks apart to run on the same XCD at about the same
+// time and share its L2 instead of each fetching the line from HBM.
+__device__ __forceinline__ int xcd_group_tile(int b, int nb, int G) {
+  const int span = 8 * G;
+  if (G <= 1 || nb % span) return b;
+  const int i = b % span;
+  return (b - i) + (i & 7) * G + (i >> 3);
 }
 
 // Forward or inverse line FFT, global -> global, with an elementwise op fused on the
@@ -153,7 +169,7 @@ __device__ __forceinline__ int xcd_pair_tile(int b, int nb) {
 // for an inverse one).  S is the storage type, C the compute type.
 template <typename S, typename C, typename RDF, int L, bool LFAST, bool INV, typename TimeOp>
 __global__ void __launch_bounds__(L * RDF::P)
-k_line_fft(LineIO<S> io, const C* __restrict__ tw, TimeOp op) {
+k_line_fft(LineIO<S> io, const C* __restrict__ tw, TimeOp op, int G) {
   using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
   constexpr int n = RD::n, V = RD::V, P = RD::P;
   extern __shared__ unsigned char smem_raw[];
@@ -161,11 +177,11 @@ k_line_fft(LineIO<S> io, const C* __restrict__ tw, TimeOp op) {
   int l, t;
   thread_map<L, P, LFAST>(threadIdx.x, l, t);
   const int b = blockIdx.y;
-  const int tile = LFAST ? xcd_pair_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int tile = LFAST ? xcd_group_tile(blockIdx.x, gridDim.x, G) : (int)blockIdx.x;
   const int li = tile * L + l;
   const bool live = li < io.nlines;
   C* s = smem + l * LdsLine<n>::size;
-  const S* src = io.in + b * io.in_f + (long)li * io.in_l;
+  const S* src = io.in + b * io.in_f;
   C v[V];
 #pragma unroll
   for (int m = 0; m < V / RD::R0; ++m)
@@ -173,13 +189,13 @@ k_line_fft(LineIO<S> io, const C* __restrict__ tw, TimeOp op) {
     for (int r = 0; r < RD::R0; ++r) {
       const int pos = reg_pos<n, RD::R0, P>(t, m, r);
       C x = cmake<C>(0, 0);
-      if (live && pos < io.in_valid) x = cvt<C>(src[pos * io.in_e]);
+      if (live && pos < io.in_valid) x = cvt<C>(src[elem_addr<LFAST>(li, pos, io.in_l, io.in_e, io.in_blk)]);
       if (!INV && live) x = op(x, b, li, pos);
       v[m * RD::R0 + r] = x;
     }
   constexpr int RL = (RD::NST == 3) ? RD::R2 : ((RD::NST == 2) ? RD::R1 : RD::R0);
   line_fft<C, RD, INV>(v, s, t, tw);
-  S* dst = io.out + b * io.out_f + (long)li * io.out_l;
+  S* dst = io.out + b * io.out_f;
 #pragma unroll
   for (int m = 0; m < V / RL; ++m)
 #pragma unroll
@@ -187,7 +203,7 @@ k_line_fft(LineIO<S> io, const C* __restrict__ tw, TimeOp op) {
       const int pos = reg_pos<n, RL, P>(t, m, r);
       C x = v[m * RL + r];
       if (INV && live) x = op(x, b, li, pos);
-      if (live && pos < io.out_count) dst[pos * io.out_e] = cvt<S>(x);
+      if (live && pos < io.out_count) dst[elem_addr<LFAST>(li, pos, io.out_l, io.out_e, io.out_blk)] = cvt<S>(x);
     }
 }
 
@@ -195,7 +211,7 @@ k_line_fft(LineIO<S> io, const C* __restrict__ tw, TimeOp op) {
 // -> IFFT along the lines, global -> global.  Used for the strided (y) axis.
 template <typename S, typename C, typename RDF, int L, bool LFAST, typename FreqOp>
 __global__ void __launch_bounds__(L * RDF::P)
-k_line_conv(LineIO<S> io, const C* __restrict__ tw, FreqOp op) {
+k_line_conv(LineIO<S> io, const C* __restrict__ tw, FreqOp op, int G) {
   constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
   using RI = typename RDF::Rev;
   extern __shared__ unsigned char smem_raw[];
@@ -203,11 +219,11 @@ k_line_conv(LineIO<S> io, const C* __restrict__ tw, FreqOp op) {
   int l, t;
   thread_map<L, P, LFAST>(threadIdx.x, l, t);
   const int b = blockIdx.y;
-  const int tile = LFAST ? xcd_pair_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int tile = LFAST ? xcd_group_tile(blockIdx.x, gridDim.x, G) : (int)blockIdx.x;
   const int li = tile * L + l;
   const bool live = li < io.nlines;
   C* s = smem + l * LdsLine<n>::size;
-  const S* src = io.in + b * io.in_f + (long)li * io.in_l;
+  const S* src = io.in + b * io.in_f;
   C v[V];
 #pragma unroll
   for (int m = 0; m < V / RDF::R0; ++m)
@@ -215,7 +231,7 @@ k_line_conv(LineIO<S> io, const C* __restrict__ tw, FreqOp op) {
     for (int r = 0; r < RDF::R0; ++r) {
       const int pos = reg_pos<n, RDF::R0, P>(t, m, r);
       C x = cmake<C>(0, 0);
-      if (live && pos < io.in_valid) x = cvt<C>(src[pos * io.in_e]);
+      if (live && pos < io.in_valid) x = cvt<C>(src[elem_addr<LFAST>(li, pos, io.in_l, io.in_e, io.in_blk)]);
       v[m * RDF::R0 + r] = x;
     }
   line_fft<C, RDF, false>(v, s, t, tw);
@@ -229,13 +245,14 @@ k_line_conv(LineIO<S> io, const C* __restrict__ tw, FreqOp op) {
     }
   line_fft<C, RI, true>(v, s, t, tw);
   constexpr int RE = (RI::NST == 3) ? RI::R2 : ((RI::NST == 2) ? RI::R1 : RI::R0);
-  S* dst = io.out + b * io.out_f + (long)li * io.out_l;
+  S* dst = io.out + b * io.out_f;
 #pragma unroll
   for (int m = 0; m < V / RE; ++m)
 #pragma unroll
     for (int r = 0; r < RE; ++r) {
       const int pos = reg_pos<n, RE, P>(t, m, r);
-      if (live && pos < io.out_count) dst[pos * io.out_e] = cvt<S>(v[m * RE + r]);
+      if (live && pos < io.out_count)
+        dst[elem_addr<LFAST>(li, pos, io.out_l, io.out_e, io.out_blk)] = cvt<S>(v[m * RE + r]);
     }
 }
 

@@ -16,6 +16,7 @@
 
 #include "../../include/fluxhip.h"
 #include "fh_bpm.hpp"
+#include "fh_fast.hpp"
 
 namespace fh {
 
@@ -57,8 +58,8 @@ FH_CFG(float, 7, false, 16, 8, 1, 32)
 FH_CFG(float, 8, false, 16, 16, 1, 16)
 FH_CFG(float, 9, false, 16, 32, 1, 16)
 FH_CFG(float, 10, false, 32, 32, 1, 8)
-FH_CFG(float, 11, false, 16, 16, 8, 2)
-FH_CFG(float, 12, false, 16, 16, 16, 1)
+FH_CFG(float, 11, false, 16, 16, 8, 4)
+FH_CFG(float, 12, false, 16, 16, 16, 4)
 FH_CFG(float, 4, true, 4, 4, 1, 16)
 FH_CFG(float, 5, true, 8, 4, 1, 16)
 FH_CFG(float, 6, true, 8, 8, 1, 16)
@@ -66,8 +67,8 @@ FH_CFG(float, 7, true, 16, 8, 1, 16)
 FH_CFG(float, 8, true, 16, 16, 1, 16)
 FH_CFG(float, 9, true, 16, 32, 1, 16)
 FH_CFG(float, 10, true, 32, 32, 1, 8)
-FH_CFG(float, 11, true, 16, 16, 8, 8)
-FH_CFG(float, 12, true, 16, 16, 16, 2)
+FH_CFG(float, 11, true, 16, 16, 8, 4)
+FH_CFG(float, 12, true, 16, 16, 16, 4)
 // FP64 (tables and the plain comparator)
 FH_CFG(double, 4, false, 4, 4, 1, 64)
 FH_CFG(double, 5, false, 8, 4, 1, 64)
@@ -113,7 +114,8 @@ static hipError_t launch_fft_t(const LineIO<S>& io, int batch, const C* tw, Op o
   hipError_t e = ensure_lds(kern, lds);
   if (e != hipSuccess) return e;
   dim3 grid((io.nlines + L - 1) / L, batch);
-  hipLaunchKernelGGL(kern, grid, dim3(L * RD::P), lds, st, io, tw, op);
+  const int G = (Y && !io.in_blk) ? std::max(1, (int)(128 / (L * sizeof(S)))) : 1;
+  hipLaunchKernelGGL(kern, grid, dim3(L * RD::P), lds, st, io, tw, op, G);
   return hipGetLastError();
 }
 
@@ -127,7 +129,8 @@ static hipError_t launch_conv_t(const LineIO<S>& io, int batch, const C* tw, Op 
   hipError_t e = ensure_lds(kern, lds);
   if (e != hipSuccess) return e;
   dim3 grid((io.nlines + L - 1) / L, batch);
-  hipLaunchKernelGGL(kern, grid, dim3(L * RD::P), lds, st, io, tw, op);
+  const int G = io.in_blk ? 1 : std::max(1, (int)(128 / (L * sizeof(S))));
+  hipLaunchKernelGGL(kern, grid, dim3(L * RD::P), lds, st, io, tw, op, G);
   return hipGetLastError();
 }
 
@@ -329,7 +332,7 @@ k_xifft_phase_grad(LineIO<float2> io, int batch, const float2* __restrict__ tw,
 #pragma unroll
   for (int k = 0; k < V; ++k) g[k] = 0.f;
   for (int b = 0; b < batch; ++b) {
-    const float2* src = io.in + b * io.in_f + (long)li * io.in_l;
+    const float2* src = io.in + b * io.in_f;
     float2 v[V];
 #pragma unroll
     for (int m = 0; m < V / RD::R0; ++m)
@@ -337,7 +340,7 @@ k_xifft_phase_grad(LineIO<float2> io, int batch, const float2* __restrict__ tw,
       for (int r = 0; r < RD::R0; ++r) {
         const int pos = reg_pos<n, RD::R0, P>(t, m, r);
         float2 x = make_float2(0.f, 0.f);
-        if (live && pos < io.in_valid) x = src[pos * io.in_e];
+        if (live && pos < io.in_valid) x = src[elem_addr<false>(li, pos, io.in_l, io.in_e, io.in_blk)];
         v[m * RD::R0 + r] = x;
       }
     line_fft<float2, RD, true>(v, sl, t, tw);
@@ -412,6 +415,7 @@ struct fh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::map<std::pair<int, int>, void*> twiddles;  // (n, is_double) -> W_n^k table
+  std::map<std::vector<int>, void*> stage_tw;      // (n, R0, R1, R2, inverse) -> stage tables
   std::mutex mu;
 };
 
@@ -429,6 +433,7 @@ struct fh_plan {
   int* d_tabidx = nullptr;
   float2* scratch = nullptr;       // RS: (2nx) x ny x batch
   double2* plain_tab = nullptr;    // plain FP64 comparator (AS): full tables
+  int conv_var = -1;               // y-pass variant (-1 = built-in default)
 };
 
 namespace fh {
@@ -463,6 +468,158 @@ static int get_twiddles(fh_ctx* c, int n, const C** out) {
   c->twiddles[key] = d;
   *out = reinterpret_cast<const C*>(d);
   return FH_OK;
+}
+
+// Stage twiddle tables of the fast passes for radix plan (R0, R1, R2) of length n:
+// stage 1 [jm < R0][r-1 < R1-1] = W_n^{s*jm*r*n/(R0*R1)}, then stage 2
+// [jm < R0*R1][r-1 < R2-1] = W_n^{s*jm*r}, s = -1 forward / +1 inverse, each from a
+// long-double sin/cos of the exactly reduced angle, rounded once.
+static int get_stage_tw(fh_ctx* c, int n, int R0, int R1, int R2, bool inv, const float2** out) {
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::vector<int> key{n, R0, R1, R2, inv ? 1 : 0};
+  auto it = c->stage_tw.find(key);
+  if (it != c->stage_tw.end()) {
+    *out = reinterpret_cast<const float2*>(it->second);
+    return FH_OK;
+  }
+  std::vector<float2> h;
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  auto push = [&](long k) {
+    k %= n;
+    const long double a = two_pi * (long double)k / (long double)n;
+    h.push_back(make_float2((float)cosl(a), (float)(inv ? sinl(a) : -sinl(a))));
+  };
+  if (R1 > 1)
+    for (int jm = 0; jm < R0; ++jm)
+      for (int r = 1; r < R1; ++r) push((long)jm * r * (n / (R0 * R1)));
+  if (R2 > 1)
+    for (int jm = 0; jm < R0 * R1; ++jm)
+      for (int r = 1; r < R2; ++r) push((long)jm * r);
+  if (h.empty()) h.push_back(make_float2(1.f, 0.f));
+  void* d = nullptr;
+  if (hipMalloc(&d, h.size() * sizeof(float2)) != hipSuccess) return fail(FH_E_NOMEM, "stage twiddles: out of device memory");
+  FH_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
+  c->stage_tw[key] = d;
+  *out = reinterpret_cast<const float2*>(d);
+  return FH_OK;
+}
+
+// ---------------------------------------------------------------- fast c64 launches
+
+template <int LOGN, bool INV, bool IH, bool OH, typename Op>
+static int fast_x_t(fh_ctx* c, const FastIO& io, int batch, Op op) {
+  using CF = Cfg<float, LOGN, false>;
+  using RDF = typename CF::RD;
+  using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
+  constexpr int L = CF::L;
+  const float2* tab = nullptr;
+  int rc = get_stage_tw(c, RD::n, RD::R0, RD::R1, RD::R2, INV, &tab);
+  if (rc) return rc;
+  auto kern = k_fast_fft<RDF, L, false, INV, IH, OH, Op>;
+  const size_t lds = (size_t)L * LdsLine<RD::n>::size * sizeof(float2);
+  FH_HIP(ensure_lds(kern, lds));
+  hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L, batch), dim3(L * RDF::P), lds, c->stream, io, tab, op);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+#define FH_LOGN_SWITCH_INT(logn, CALL)                                   \
+  switch (logn) {                                                        \
+    case 4: { constexpr int LG = 4; return CALL; }                       \
+    case 5: { constexpr int LG = 5; return CALL; }                       \
+    case 6: { constexpr int LG = 6; return CALL; }                       \
+    case 7: { constexpr int LG = 7; return CALL; }                       \
+    case 8: { constexpr int LG = 8; return CALL; }                       \
+    case 9: { constexpr int LG = 9; return CALL; }                       \
+    case 10: { constexpr int LG = 10; return CALL; }                     \
+    case 11: { constexpr int LG = 11; return CALL; }                     \
+    case 12: { constexpr int LG = 12; return CALL; }                     \
+    default: return fail(FH_E_UNSUPP, "transform length 2^%d unsupported", logn); \
+  }
+
+// forward x FFT; half_in: rows hold n/2 points (zero-padded RS input)
+template <typename Op>
+static int fast_xfwd(fh_ctx* c, int logn, bool half_in, const FastIO& io, int batch, Op op) {
+  if (half_in) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, true, false, Op>(c, io, batch, op))) }
+  FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, false, false, Op>(c, io, batch, op)))
+}
+
+// inverse x FFT; half_out: only n/2 points are stored (RS crop)
+static int fast_xinv(fh_ctx* c, int logn, bool half_out, const FastIO& io, int batch) {
+  if (half_out) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, true, NoOp>(c, io, batch, NoOp{}))) }
+  FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, false, NoOp>(c, io, batch, NoOp{})))
+}
+
+// y-pass variants (radices, columns per workgroup) for the autotuner
+using FastConvLaunch = int (*)(fh_ctx*, const FastIO&, int, QuadMul, bool);
+struct FastConvVariant {
+  const char* name;
+  FastConvLaunch fn;
+};
+
+template <int A, int B, int C_, int L>
+static int fast_conv_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool half) {
+  using RDF = Radices<A, B, C_>;
+  using RI = typename RDF::Rev;
+  const float2 *tf = nullptr, *ti = nullptr;
+  int rc;
+  if ((rc = get_stage_tw(c, RDF::n, RDF::R0, RDF::R1, RDF::R2, false, &tf)) ||
+      (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
+    return rc;
+  const size_t lds = (size_t)L * LdsLine<RDF::n>::size * sizeof(float2);
+  const int G = io.in_blk ? 1 : std::max(1, 16 / L);
+  dim3 grid((io.nlines + L - 1) / L, batch);
+  if (half) {
+    auto kern = k_fast_conv<RDF, L, true, true, true, QuadMul>;
+    FH_HIP(ensure_lds(kern, lds));
+    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, tf, ti, op, G);
+  } else {
+    auto kern = k_fast_conv<RDF, L, true, false, false, QuadMul>;
+    FH_HIP(ensure_lds(kern, lds));
+    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, tf, ti, op, G);
+  }
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+static const FastConvVariant kFast4[] = {{"4x4 L16", fast_conv_v<4, 4, 1, 16>}};
+static const FastConvVariant kFast5[] = {{"8x4 L16", fast_conv_v<8, 4, 1, 16>}};
+static const FastConvVariant kFast6[] = {{"8x8 L16", fast_conv_v<8, 8, 1, 16>}};
+static const FastConvVariant kFast7[] = {{"16x8 L16", fast_conv_v<16, 8, 1, 16>}};
+static const FastConvVariant kFast8[] = {{"16x16 L16", fast_conv_v<16, 16, 1, 16>},
+                                         {"16x16 L8", fast_conv_v<16, 16, 1, 8>}};
+// First entry = default (fastest measured on MI355X for batched 2^LOGN lines).
+static const FastConvVariant kFast9[] = {{"8x8x8 L4", fast_conv_v<8, 8, 8, 4>},
+                                         {"8x8x8 L8", fast_conv_v<8, 8, 8, 8>},
+                                         {"16x32 L16", fast_conv_v<16, 32, 1, 16>},
+                                         {"16x32 L8", fast_conv_v<16, 32, 1, 8>}};
+static const FastConvVariant kFast10[] = {{"32x32 L8", fast_conv_v<32, 32, 1, 8>},
+                                          {"32x32 L4", fast_conv_v<32, 32, 1, 4>},
+                                          {"16x8x8 L8", fast_conv_v<16, 8, 8, 8>},
+                                          {"16x8x8 L4", fast_conv_v<16, 8, 8, 4>}};
+static const FastConvVariant kFast11[] = {{"32x8x8 L8", fast_conv_v<32, 8, 8, 8>},
+                                          {"16x16x8 L8", fast_conv_v<16, 16, 8, 8>},
+                                          {"32x8x8 L4", fast_conv_v<32, 8, 8, 4>},
+                                          {"8x16x16 L4", fast_conv_v<8, 16, 16, 4>},
+                                          {"16x16x8 L4", fast_conv_v<16, 16, 8, 4>}};
+static const FastConvVariant kFast12[] = {{"32x16x8 L4", fast_conv_v<32, 16, 8, 4>},
+                                          {"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
+                                          {"32x16x8 L2", fast_conv_v<32, 16, 8, 2>},
+                                          {"16x16x16 L2", fast_conv_v<16, 16, 16, 2>}};
+
+static const FastConvVariant* fast_conv_variants(int logn, int* count) {
+  switch (logn) {
+    case 4: *count = 1; return kFast4;
+    case 5: *count = 1; return kFast5;
+    case 6: *count = 1; return kFast6;
+    case 7: *count = 1; return kFast7;
+    case 8: *count = 2; return kFast8;
+    case 9: *count = 4; return kFast9;
+    case 10: *count = 4; return kFast10;
+    case 11: *count = 5; return kFast11;
+    case 12: *count = 4; return kFast12;
+    default: *count = 0; return nullptr;
+  }
 }
 
 static int check_dir(int direction) {
@@ -526,45 +683,44 @@ static int fft2_d(fh_ctx* c, double2* d, int nx2, int ny2, int lx, int ly) {
 
 // The passes of one propagation of fields u (or, fused, of Phase then propagation):
 // pass 0 (x FFT) and pass 1 (y FFT * kernel * IFFT); `only` >= 0 runs just that pass.
+// Between passes the spectrum sits in the 4x4-blocked layout (elem_addr); for RS it sits
+// in the (2nx) x ny workspace and only its first ny rows are ever stored.
 template <typename XOp>
 static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1) {
   fh_ctx* c = p->ctx;
-  const int nxl = 1 << p->lx, nyl = 1 << p->ly;
-  const float2 *twx = nullptr, *twy = nullptr;
-  int rc;
-  if ((rc = get_twiddles(c, nxl, &twx)) || (rc = get_twiddles(c, nyl, &twy))) return rc;
-  float2* mid = p->kind == 0 ? u : p->scratch;   // x-spectrum rows
+  const int nxl = 1 << p->lx;
+  const bool rs = p->kind == 1;
+  float2* mid = rs ? p->scratch : u;
   const long midf = (long)nxl * p->ny;
-  QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, nyl, p->qx, direction < 0};
-  LineIO<float2> io{};
-  // x FFT: rows of u (length nx, zero-extended to nxl) -> rows of mid
-  io.in = u; io.out = mid; io.in_f = (long)p->nx * p->ny; io.out_f = midf;
-  io.in_l = p->nx; io.out_l = nxl; io.in_e = io.out_e = 1;
-  io.nlines = p->ny; io.in_valid = p->nx; io.out_count = nxl;
-  if (only < 0 || only == 0)
-    FH_HIP((launch_fft<float2, float2, false, false>(p->lx, io, p->batch, twx, xop, c->stream)));
+  int rc;
+  FastIO io{};
+  if (only < 0 || only == 0) {
+    io.in = u; io.out = mid; io.in_f = (long)p->nx * p->ny; io.out_f = midf;
+    io.in_l = p->nx; io.out_l = nxl; io.in_e = io.out_e = 1; io.in_blk = 0; io.out_blk = 1;
+    io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = midf * 8;
+    if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop))) return rc;
+  }
   if (only == 0) return FH_OK;
-  // y: FFT, multiply, IFFT on the nxl columns of mid (ny valid rows of nyl)
+  QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, 1 << p->ly, p->qx, direction < 0};
   io.in = mid; io.out = mid; io.in_f = io.out_f = midf;
-  io.in_l = io.out_l = 1; io.in_e = io.out_e = nxl;
-  io.nlines = nxl; io.in_valid = p->ny; io.out_count = p->ny;
-  FH_HIP((launch_conv<float2, float2>(p->ly, io, p->batch, twy, q, c->stream)));
-  return FH_OK;
+  io.in_l = io.out_l = 1; io.in_e = io.out_e = nxl; io.in_blk = io.out_blk = 1;
+  io.nlines = nxl; io.in_bytes = io.out_bytes = midf * 8;
+  int nvar = 0;
+  const FastConvVariant* vars = fast_conv_variants(p->ly, &nvar);
+  if (!vars) return fail(FH_E_UNSUPP, "y length 2^%d unsupported", p->ly);
+  const int v = (p->conv_var >= 0 && p->conv_var < nvar) ? p->conv_var : 0;
+  return vars[v].fn(c, io, p->batch, q, rs);
 }
 
 static int run_xifft(fh_plan* p, float2* u) {
-  fh_ctx* c = p->ctx;
   const int nxl = 1 << p->lx;
-  const float2* twx = nullptr;
-  int rc;
-  if ((rc = get_twiddles(c, nxl, &twx))) return rc;
-  float2* mid = p->kind == 0 ? u : p->scratch;
-  LineIO<float2> io{};
+  const bool rs = p->kind == 1;
+  float2* mid = rs ? p->scratch : u;
+  FastIO io{};
   io.in = mid; io.out = u; io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
-  io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1;
-  io.nlines = p->ny; io.in_valid = nxl; io.out_count = p->nx;
-  FH_HIP((launch_fft<float2, float2, false, true>(p->lx, io, p->batch, twx, NoOp{}, c->stream)));
-  return FH_OK;
+  io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1; io.in_blk = 1; io.out_blk = 0;
+  io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
+  return fast_xinv(p->ctx, p->lx, rs, io, p->batch);
 }
 
 }  // namespace fh
@@ -607,6 +763,7 @@ int fh_ctx_destroy(fh_ctx* ctx) {
   if (!ctx) return FH_OK;
   (void)hipSetDevice(ctx->device);
   for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+  for (auto& kv : ctx->stage_tw) (void)hipFree(kv.second);
   delete ctx;
   return FH_OK;
 }
@@ -716,6 +873,69 @@ int fh_propagate_pass(fh_plan* p, void* u, int direction, int pass) {
   return run_xifft(p, f);
 }
 
+int fh_plan_variants(fh_plan* p, int pass, int* count, int* current) {
+  if (!p || !count || !current) return fail(FH_E_ARG, "null plan or output");
+  if (pass < 0 || pass > 2) return fail(FH_E_ARG, "pass must be 0, 1 or 2, got %d", pass);
+  int n = 0;
+  if (pass == 1) fast_conv_variants(p->ly, &n);
+  *count = n > 0 ? n : 1;
+  *current = (pass == 1 && n > 0) ? (p->conv_var >= 0 ? p->conv_var : -1) : 0;
+  return FH_OK;
+}
+
+int fh_plan_set_variant(fh_plan* p, int pass, int variant) {
+  if (!p) return fail(FH_E_ARG, "null plan");
+  if (pass < 0 || pass > 2) return fail(FH_E_ARG, "pass must be 0, 1 or 2, got %d", pass);
+  int n = 0;
+  if (pass == 1) fast_conv_variants(p->ly, &n);
+  if (variant < -1 || variant >= std::max(n, 1)) return fail(FH_E_ARG, "variant %d out of range (%d)", variant, n);
+  if (pass == 1 && n > 0) p->conv_var = variant;
+  return FH_OK;
+}
+
+int fh_autotune(fh_plan* p, void* u, int reps, char* report, int report_len) {
+  if (!p || !u) return fail(FH_E_ARG, "null plan or field");
+  if (reps < 1) return fail(FH_E_ARG, "reps must be >= 1");
+  int rc;
+  if ((rc = set_device(p->ctx))) return rc;
+  int n = 0;
+  const FastConvVariant* vars = fast_conv_variants(p->ly, &n);
+  std::string js = "{\"pass\": 1, \"variants\": [";
+  int best = -1;
+  float best_ms = 0.f;
+  float2* f = reinterpret_cast<float2*>(u);
+  hipEvent_t e0, e1;
+  FH_HIP(hipEventCreate(&e0));
+  FH_HIP(hipEventCreate(&e1));
+  for (int v = 0; v < n; ++v) {
+    p->conv_var = v;
+    if ((rc = run_prop(p, f, 1, NoOp{}, 1))) { p->conv_var = -1; return rc; }  // warm-up
+    FH_HIP(hipEventRecord(e0, p->ctx->stream));
+    for (int r = 0; r < reps; ++r)
+      if ((rc = run_prop(p, f, 1, NoOp{}, 1))) { p->conv_var = -1; return rc; }
+    FH_HIP(hipEventRecord(e1, p->ctx->stream));
+    FH_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    ms /= reps;
+    char buf[160];
+    snprintf(buf, sizeof buf, "%s{\"index\": %d, \"name\": \"%s\", \"ms\": %.6f}", v ? ", " : "", v, vars[v].name, ms);
+    js += buf;
+    if (best < 0 || ms < best_ms) { best = v; best_ms = ms; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->conv_var = best;
+  char buf[96];
+  snprintf(buf, sizeof buf, "], \"best\": %d}", best);
+  js += buf;
+  if (report && report_len > 0) {
+    strncpy(report, js.c_str(), report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return FH_OK;
+}
+
 int fh_phase_propagate(fh_plan* p, void* u, const float* phi, void* saved, int direction) {
   if (!p || !u || !phi) return fail(FH_E_ARG, "null plan, field or mask");
   int rc;
@@ -774,6 +994,7 @@ int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const vo
   io.in = mid; io.out = f; io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
   io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1;
   io.nlines = p->ny; io.in_valid = nxl; io.out_count = p->nx;
+  io.in_blk = 1; io.out_blk = 0;
   FH_HIP(launch_xifft_phase_grad(p->lx, io, p->batch, twx, phi, reinterpret_cast<const float2*>(saved),
                                  dphi, p->nx, (float)direction, accumulate, p->ctx->stream));
   return FH_OK;

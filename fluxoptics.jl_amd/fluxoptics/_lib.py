@@ -37,6 +37,9 @@ SIGNATURES = {
     "fh_plan_destroy": (c_int, [c_void_p]),
     "fh_propagate": (c_int, [c_void_p, c_void_p, c_int]),
     "fh_propagate_pass": (c_int, [c_void_p, c_void_p, c_int, c_int]),
+    "fh_plan_variants": (c_int, [c_void_p, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "fh_plan_set_variant": (c_int, [c_void_p, c_int, c_int]),
+    "fh_autotune": (c_int, [c_void_p, c_void_p, c_int, c_char_p, c_int]),
     "fh_phase_propagate": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "fh_phase_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
     "fh_phase_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
