@@ -272,3 +272,101 @@ k_fast_conv(FastIO io, const float2* __restrict__ tabf, const float2* __restrict
 }
 
 }  // namespace fh
+
+namespace fh {
+
+// Quadrant transfer-function tables for the batched y pass (column-major, see QuadMulT).
+struct QuadTab {
+  const float2* tab;
+  const int* tabidx;
+  long tabsize;  // entries per table
+  int nx;        // transform length along x (columns of the spectrum)
+  int qcol;      // entries per table column (n/2 + 1)
+  int conj;
+};
+
+// One field of the batched y pass: FFT, transfer-function multiply, IFFT, store.
+template <typename RDF, int L, bool OUT_HALF>
+__device__ __forceinline__ void yconv_field(float2* v, float2* s, int t, int li, bool live, int b,
+                                            const FastIO& io, const float2* __restrict__ tabf,
+                                            const float2* __restrict__ tabi, const QuadTab& qt,
+                                            int vout, int kout, int qoff_d, int qoff_m) {
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  using RI = typename RDF::Rev;
+  constexpr int RL = RI::R0;
+  constexpr int RE = LastRadix<RI>::value;
+  fast_fft<float2, RDF, false>(v, s, t, tabf);
+  // H(kx, ky): ky = t + c; c < n/2 reads column entry t + c, c >= n/2 the mirror n - t - c,
+  // written as (P - t) + (n - c - P) so both offsets stay non-negative.
+  const auto rt = make_rsrc(qt.tab + (long)qt.tabidx[b] * qt.tabsize, qt.tabsize * 8);
+#pragma unroll
+  for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int c = P * m + r * (n / RL);
+      const float2 h = (c < n / 2) ? buf_load(rt, qoff_d, c * 8) : buf_load(rt, qoff_m, (n - c - P) * 8);
+      v[m * RL + r] = qt.conj ? cmulc(v[m * RL + r], h) : cmul(v[m * RL + r], h);
+    }
+  fast_fft<float2, RI, true>(v, s, t, tabi);
+  if (live) {
+    const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
+#pragma unroll
+    for (int m = 0; m < V / RE; ++m)
+#pragma unroll
+      for (int r = 0; r < RE; ++r) {
+        const int c = P * m + r * (n / RE);
+        if (!OUT_HALF || c < n / 2) buf_store(rout, vout, c * kout, v[m * RE + r]);
+      }
+  }
+}
+
+template <int V, int R0, int n, int P, bool IN_HALF>
+__device__ __forceinline__ void yconv_load(float2* v, const FastIO& io, int b, int vin, int kin) {
+  const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
+#pragma unroll
+  for (int m = 0; m < V / R0; ++m)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int c = P * m + r * (n / R0);
+      v[m * R0 + r] = (!IN_HALF || c < n / 2) ? buf_load(rin, vin, c * kin) : make_float2(0.f, 0.f);
+    }
+}
+
+// Batched y pass: a workgroup owns L spectrum columns and walks the fields of the batch
+// (load, FFT, multiply, IFFT, store).  The transfer-function columns it needs are read
+// from HBM for the first field and from L2 for the others; two or more resident
+// workgroups per CU overlap one's memory phase with another's arithmetic.
+// LANES_T: adjacent lanes take adjacent positions of one column (conflict-free LDS
+// exchanges; the four columns of a 128-byte block row are read by sibling waves and meet
+// in L2) instead of adjacent columns of one row.
+template <typename RDF, int L, bool IN_HALF, bool OUT_HALF, bool LANES_T>
+__global__ void __launch_bounds__(L * RDF::P)
+k_fast_yconv(FastIO io, int batch, const float2* __restrict__ tabf, const float2* __restrict__ tabi,
+             QuadTab qt) {
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  extern __shared__ unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  int l, t;
+  thread_map<L, P, !LANES_T>(threadIdx.x, l, t);
+  const int li = blockIdx.x * L + l;
+  const bool live = li < io.nlines;
+  float2* s = smem + l * LdsLine<n>::size;
+  const int vin = fast_off<true>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
+  const int kin = fast_k<true>(io.in_l, io.in_e, io.in_blk) * 8;
+  const int vout = fast_off<true>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
+  const int kout = fast_k<true>(io.out_l, io.out_e, io.out_blk) * 8;
+  const int lic = live ? li : 0;
+  const int qxi = lic <= qt.nx - lic ? lic : qt.nx - lic;
+  const int qoff_d = (qxi * qt.qcol + t) * 8;
+  const int qoff_m = (qxi * qt.qcol + P - t) * 8;
+  for (int b = 0; b < batch; ++b) {
+    // compiler-only barrier: keeps the per-field twiddle loads inside the loop (hoisting
+    // them would hold every twiddle in registers and halve the resident workgroups)
+    asm volatile("" ::: "memory");
+    float2 v[V];
+    yconv_load<V, RDF::R0, n, P, IN_HALF>(v, io, b, vin, kin);
+    yconv_field<RDF, L, OUT_HALF>(v, s, t, li, live, b, io, tabf, tabi, qt, vout, kout, qoff_d, qoff_m);
+  }
+}
+
+}  // namespace fh

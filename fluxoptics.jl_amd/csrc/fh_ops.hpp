@@ -9,7 +9,8 @@ namespace fh {
 // ---------------------------------------------------------------- frequency-side ops
 
 // Multiply by a transfer function stored as one quadrant (the function depends on
-// |fx| and |fy| only): entry [min(ky, ny-ky)][min(kx, nx-kx)], row length qx.
+// |fx| and |fy| only), column-major: entry (qx, qy) = (min(kx, nx-kx), min(ky, ny-ky)) at
+// qx*qcol + qy, so the values one column of the y pass needs are contiguous.
 // `tab` holds one quadrant per table index; `tabidx[b]` picks the table of field b.
 // The 1/(nx*ny) of the inverse FFT is folded into the table.  T is the table's complex
 // type (c64, or c128 for the FP64-compute slice steps).
@@ -17,13 +18,13 @@ template <typename T> struct QuadMulT {
   const T* tab;
   const int* tabidx;
   long tabsize;
-  int nx, ny, qx;
+  int nx, ny, qcol;
   int conj;  // Backward direction multiplies by the conjugate
   template <typename C>
   __device__ __forceinline__ C operator()(C v, int b, int kx, int ky) const {
     const int qxi = kx <= nx - kx ? kx : nx - kx;
     const int qyi = ky <= ny - ky ? ky : ny - ky;
-    const C h = cvt<C>(tab[tabidx[b] * tabsize + (long)qyi * qx + qxi]);
+    const C h = cvt<C>(tab[tabidx[b] * tabsize + (long)qxi * qcol + qyi]);
     return conj ? cmulc(v, h) : cmul(v, h);
   }
 };

@@ -193,7 +193,7 @@ __global__ void k_as_table(float2* __restrict__ tab, int nx, int ny, int qx, int
     re = exp(-2.0 * M_PI * fabs(z) * sqrt(-a)) * scale;
     im = 0.0;
   }
-  tab[(long)j * qx + i] = make_float2((float)re, (float)im);
+  tab[(long)i * qy + j] = make_float2((float)re, (float)im);
 }
 
 // Full FP64 transfer-function table [ny][nx] for the plain comparator (same formula).
@@ -258,7 +258,7 @@ __global__ void k_quad_extract(float2* __restrict__ tab, const double2* __restri
   const int j = blockIdx.y;
   if (i >= qx || j >= qy) return;
   const double2 v = d[(long)j * nx2 + i];
-  tab[(long)j * qx + i] = make_float2((float)(v.x * scale), (float)(v.y * scale));
+  tab[(long)i * qy + j] = make_float2((float)(v.x * scale), (float)(v.y * scale));
 }
 
 // ================================================================ elementwise kernels
@@ -557,7 +557,7 @@ struct FastConvVariant {
   FastConvLaunch fn;
 };
 
-template <int A, int B, int C_, int L>
+template <int A, int B, int C_, int L, bool LT = false>
 static int fast_conv_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool half) {
   using RDF = Radices<A, B, C_>;
   using RI = typename RDF::Rev;
@@ -567,16 +567,16 @@ static int fast_conv_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool 
       (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
     return rc;
   const size_t lds = (size_t)L * LdsLine<RDF::n>::size * sizeof(float2);
-  const int G = io.in_blk ? 1 : std::max(1, 16 / L);
-  dim3 grid((io.nlines + L - 1) / L, batch);
+  QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj};
+  dim3 grid((io.nlines + L - 1) / L);
   if (half) {
-    auto kern = k_fast_conv<RDF, L, true, true, true, QuadMul>;
+    auto kern = k_fast_yconv<RDF, L, true, true, LT>;
     FH_HIP(ensure_lds(kern, lds));
-    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, tf, ti, op, G);
+    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, batch, tf, ti, qt);
   } else {
-    auto kern = k_fast_conv<RDF, L, true, false, false, QuadMul>;
+    auto kern = k_fast_yconv<RDF, L, false, false, LT>;
     FH_HIP(ensure_lds(kern, lds));
-    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, tf, ti, op, G);
+    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, batch, tf, ti, qt);
   }
   FH_HIP(hipGetLastError());
   return FH_OK;
@@ -591,20 +591,16 @@ static const FastConvVariant kFast8[] = {{"16x16 L16", fast_conv_v<16, 16, 1, 16
 // First entry = default (fastest measured on MI355X for batched 2^LOGN lines).
 static const FastConvVariant kFast9[] = {{"8x8x8 L4", fast_conv_v<8, 8, 8, 4>},
                                          {"8x8x8 L8", fast_conv_v<8, 8, 8, 8>},
-                                         {"16x32 L16", fast_conv_v<16, 32, 1, 16>},
                                          {"16x32 L8", fast_conv_v<16, 32, 1, 8>}};
-static const FastConvVariant kFast10[] = {{"32x32 L8", fast_conv_v<32, 32, 1, 8>},
-                                          {"32x32 L4", fast_conv_v<32, 32, 1, 4>},
-                                          {"16x8x8 L8", fast_conv_v<16, 8, 8, 8>},
-                                          {"16x8x8 L4", fast_conv_v<16, 8, 8, 4>}};
-static const FastConvVariant kFast11[] = {{"32x8x8 L8", fast_conv_v<32, 8, 8, 8>},
-                                          {"16x16x8 L8", fast_conv_v<16, 16, 8, 8>},
-                                          {"32x8x8 L4", fast_conv_v<32, 8, 8, 4>},
+static const FastConvVariant kFast10[] = {{"16x8x8 L4", fast_conv_v<16, 8, 8, 4>},
+                                          {"32x32 L8", fast_conv_v<32, 32, 1, 8>},
+                                          {"16x8x8 L8", fast_conv_v<16, 8, 8, 8>}};
+static const FastConvVariant kFast11[] = {{"16x16x8 L4", fast_conv_v<16, 16, 8, 4>},
+                                          {"32x8x8 L8", fast_conv_v<32, 8, 8, 8>},
                                           {"8x16x16 L4", fast_conv_v<8, 16, 16, 4>},
-                                          {"16x16x8 L4", fast_conv_v<16, 16, 8, 4>}};
-static const FastConvVariant kFast12[] = {{"32x16x8 L4", fast_conv_v<32, 16, 8, 4>},
-                                          {"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
-                                          {"32x16x8 L2", fast_conv_v<32, 16, 8, 2>},
+                                          {"32x8x8 L4", fast_conv_v<32, 8, 8, 4>}};
+static const FastConvVariant kFast12[] = {{"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
+                                          {"32x16x8 L4", fast_conv_v<32, 16, 8, 4>},
                                           {"16x16x16 L2", fast_conv_v<16, 16, 16, 2>}};
 
 static const FastConvVariant* fast_conv_variants(int logn, int* count) {
@@ -614,10 +610,10 @@ static const FastConvVariant* fast_conv_variants(int logn, int* count) {
     case 6: *count = 1; return kFast6;
     case 7: *count = 1; return kFast7;
     case 8: *count = 2; return kFast8;
-    case 9: *count = 4; return kFast9;
-    case 10: *count = 4; return kFast10;
-    case 11: *count = 5; return kFast11;
-    case 12: *count = 4; return kFast12;
+    case 9: *count = 3; return kFast9;
+    case 10: *count = 3; return kFast10;
+    case 11: *count = 4; return kFast11;
+    case 12: *count = 3; return kFast12;
     default: *count = 0; return nullptr;
   }
 }
@@ -701,7 +697,7 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
     if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop))) return rc;
   }
   if (only == 0) return FH_OK;
-  QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, 1 << p->ly, p->qx, direction < 0};
+  QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, 1 << p->ly, p->qy, direction < 0};
   io.in = mid; io.out = mid; io.in_f = io.out_f = midf;
   io.in_l = io.out_l = 1; io.in_e = io.out_e = nxl; io.in_blk = io.out_blk = 1;
   io.nlines = nxl; io.in_bytes = io.out_bytes = midf * 8;
@@ -1068,7 +1064,7 @@ __global__ void k_as_table_d(double2* __restrict__ tab, int nx, int ny, int qx, 
     re = exp(-2.0 * M_PI * fabs(z) * sqrt(-a));
     im = 0.0;
   }
-  tab[(long)j * qx + i] = make_double2(re * scale, im * scale);
+  tab[(long)i * qy + j] = make_double2(re * scale, im * scale);
 }
 
 template <int LOGN>
@@ -1173,7 +1169,7 @@ static int bpm_ypass(fh_bpm* m, float2* f, const double2* tab, int direction) {
   io.in = f; io.out = f; io.in_f = io.out_f = (long)m->nx * m->ny;
   io.in_l = io.out_l = 1; io.in_e = io.out_e = m->nx;
   io.nlines = m->nx; io.in_valid = io.out_count = m->ny;
-  QuadMulT<double2> q{tab, m->d_tabidx, m->tabsize, m->nx, m->ny, m->qx, direction < 0};
+  QuadMulT<double2> q{tab, m->d_tabidx, m->tabsize, m->nx, m->ny, m->qy, direction < 0};
   FH_HIP((launch_conv<float2, double2>(m->ly, io, m->batch, tw, q, m->ctx->stream)));
   return FH_OK;
 }
