@@ -26,7 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "fluxoptics.jl_amd"))
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 N, BATCH, DS, LAM, Z, W0 = 2048, 8, 0.5, 0.8, 5000.0, 150.0
 METRIC = "2048² fwd+adjoint propagations/s on 1 MI355X; % HBM roofline; err vs FP64"
-PMC_FILE = os.path.join(ROOT, "profiles", "asprop_2048x2048_b8_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "asprop_2048x2048_b8_traffic.txt")
 
 
 def parse():
@@ -38,7 +38,32 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparator")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample's wall time")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 512 MiB device copy (the reference kernel that calibrates "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE into bytes) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    import torch
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.complex64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copied 512 MiB x5")
+
+
+def read_traffic(pass_name):
+    """HBM bytes per launch of a pass from the committed rocprofv3 summary (or None)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and parts[0] == "traffic" and parts[1] == pass_name:
+                return int(parts[2])
+    return None
 
 
 def make_inputs():
@@ -97,6 +122,9 @@ def cpu_baseline(u0, dv0, v_dev0, du_dev0, budget_s):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the batch would shard across GPUs with no communication, which is not measured here.",
@@ -146,10 +174,7 @@ def main():
     dom = max(passes, key=passes.get)
     dom_bytes = 16.0 * N * N * BATCH     # read + write of the batch once
     achieved = dom_bytes / (passes[dom] * 1e-3)
-    traffic = None
-    if os.path.exists(PMC_FILE):
-        with open(PMC_FILE) as f:
-            traffic = json.load(f).get("kernels", {}).get(dom, {}).get("hbm_bytes_per_launch")
+    traffic = read_traffic(dom)
 
     out = {
         "metric": METRIC, "value": units_per_s, "unit": "propagations/s", "n_gpus": 1,
