@@ -283,6 +283,7 @@ struct QuadTab {
   int nx;        // transform length along x (columns of the spectrum)
   int qcol;      // entries per table column (n/2 + 1)
   int conj;
+  int one_tab;   // every field uses table tabidx[0]
 };
 
 // One field of the batched y pass: FFT, transfer-function multiply, IFFT, store.
@@ -332,8 +333,8 @@ __device__ __forceinline__ void yconv_load(float2* v, const FastIO& io, int b, i
     }
 }
 
-// Batched y pass: a workgroup owns L spectrum columns and walks the fields of the batch
-// (load, FFT, multiply, IFFT, store).  The transfer-function columns it needs are read
+// Batched y pass: a workgroup owns L spectrum columns and walks `fpw` fields of the batch
+// (load, FFT, multiply, IFFT, store); the grid's y dimension covers the batch in steps of fpw.  The transfer-function columns it needs are read
 // from HBM for the first field and from L2 for the others; two or more resident
 // workgroups per CU overlap one's memory phase with another's arithmetic.
 // LANES_T: adjacent lanes take adjacent positions of one column (conflict-free LDS
@@ -341,8 +342,8 @@ __device__ __forceinline__ void yconv_load(float2* v, const FastIO& io, int b, i
 // in L2) instead of adjacent columns of one row.
 template <typename RDF, int L, bool IN_HALF, bool OUT_HALF, bool LANES_T>
 __global__ void __launch_bounds__(L * RDF::P)
-k_fast_yconv(FastIO io, int batch, const float2* __restrict__ tabf, const float2* __restrict__ tabi,
-             QuadTab qt) {
+k_fast_yconv(FastIO io, int batch, int fpw, const float2* __restrict__ tabf,
+             const float2* __restrict__ tabi, QuadTab qt) {
   constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
   extern __shared__ unsigned char smem_raw[];
   float2* smem = reinterpret_cast<float2*>(smem_raw);
@@ -359,13 +360,192 @@ k_fast_yconv(FastIO io, int batch, const float2* __restrict__ tabf, const float2
   const int qxi = lic <= qt.nx - lic ? lic : qt.nx - lic;
   const int qoff_d = (qxi * qt.qcol + t) * 8;
   const int qoff_m = (qxi * qt.qcol + P - t) * 8;
-  for (int b = 0; b < batch; ++b) {
+  const int b0 = blockIdx.y * fpw;
+  const int b1 = b0 + fpw < batch ? b0 + fpw : batch;
+  for (int b = b0; b < b1; ++b) {
     // compiler-only barrier: keeps the per-field twiddle loads inside the loop (hoisting
     // them would hold every twiddle in registers and halve the resident workgroups)
     asm volatile("" ::: "memory");
     float2 v[V];
     yconv_load<V, RDF::R0, n, P, IN_HALF>(v, io, b, vin, kin);
     yconv_field<RDF, L, OUT_HALF>(v, s, t, li, live, b, io, tabf, tabi, qt, vout, kout, qoff_d, qoff_m);
+  }
+}
+
+}  // namespace fh
+
+namespace fh {
+
+// ---------------------------------------------------------------- software-pipelined y pass
+// Stage twiddles from base powers: for a butterfly with j mod NS = jm the twiddles are
+// w^r, r < R, with w = W_n^{jm*n/(NS*R)}; the table entries r = 1, 2, 4, 8, 16 are loaded once
+// per workgroup (they depend on the thread only) and w^r is built with one complex product per
+// r from w^(r - lowbit(r)) * w^lowbit(r) (at most popcount(r)-1 products: <= 3 roundings).
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+
+template <typename C, int R, int NS, int V, int P>
+struct TwBases {
+  static constexpr int NB = ilog2c(R);   // bases per butterfly
+  static constexpr int NBF = V / R;      // butterflies per thread
+  C b[NBF][NB > 0 ? NB : 1];
+  __device__ __forceinline__ void load(const C* __restrict__ tab, int t) {
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int m = 0; m < NBF; ++m) {
+        const int jm = (t + P * m) & (NS - 1);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) b[m][k] = tab[jm * (R - 1) + ((1 << k) - 1)];
+      }
+    }
+  }
+  __device__ __forceinline__ void apply(C* v) const {
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int m = 0; m < NBF; ++m) {
+        C w[R];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const int low = r & (-r);
+          const int k = ilog2c(low);
+          w[r] = (r == low) ? b[m][k] : cmul(w[r - low], b[m][k]);
+          v[m * R + r] = cmul(v[m * R + r], w[r]);
+        }
+      }
+    }
+  }
+};
+
+template <typename RD> struct PlanBases {
+  using C = float2;
+  static constexpr int n = RD::n, V = RD::V, P = RD::P;
+  TwBases<C, RD::R1, RD::R0, V, P> s1;
+  TwBases<C, RD::R2, RD::R0 * RD::R1, V, P> s2;
+  __device__ __forceinline__ void load(const C* __restrict__ tab, int t) {
+    if constexpr (RD::NST >= 2) s1.load(tab + FastTw<RD>::off1, t);
+    if constexpr (RD::NST >= 3) s2.load(tab + FastTw<RD>::off2, t);
+  }
+};
+
+template <typename RD, bool INV>
+__device__ __forceinline__ void fast_fft_b(float2* v, float2* s, int t, const PlanBases<RD>& pb) {
+  using C = float2;
+  constexpr int n = RD::n, V = RD::V, P = RD::P;
+  if (INV) fast_idfts<C, RD::R0, V>(v); else fast_dfts<C, RD::R0, V>(v);
+  if constexpr (RD::NST >= 2) {
+    __syncthreads();
+    fast_write<C, n, RD::R0, 1, V, P>(s, v, t);
+    __syncthreads();
+    fast_read<C, n, RD::R1, V, P>(v, s, t);
+    pb.s1.apply(v);
+    if (INV) fast_idfts<C, RD::R1, V>(v); else fast_dfts<C, RD::R1, V>(v);
+  }
+  if constexpr (RD::NST >= 3) {
+    __syncthreads();
+    fast_write<C, n, RD::R1, RD::R0, V, P>(s, v, t);
+    __syncthreads();
+    fast_read<C, n, RD::R2, V, P>(v, s, t);
+    pb.s2.apply(v);
+    if (INV) fast_idfts<C, RD::R2, V>(v); else fast_dfts<C, RD::R2, V>(v);
+  }
+}
+
+template <typename RDF>
+__device__ __forceinline__ void ypipe_loadh(float2* h, const QuadTab& qt, int b, int qoff_d, int qoff_m) {
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  constexpr int RL = RDF::Rev::R0;
+  const auto rt = make_rsrc(qt.tab + (long)qt.tabidx[b] * qt.tabsize, qt.tabsize * 8);
+#pragma unroll
+  for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int c = P * m + r * (n / RL);
+      h[m * RL + r] = (c < n / 2) ? buf_load(rt, qoff_d, c * 8) : buf_load(rt, qoff_m, (n - c - P) * 8);
+    }
+}
+
+// One field: FFT with cur's registers, multiply by h, IFFT, store; the caller has already
+// issued the loads of the next field (data and transfer function) into other registers.
+template <typename RDF, bool OUT_HALF>
+__device__ __forceinline__ void ypipe_field(float2* v, const float2* h, float2* s, int t, bool live, int b,
+                                            const FastIO& io, const PlanBases<RDF>& pf,
+                                            const PlanBases<typename RDF::Rev>& pi, int conj, int vout,
+                                            int kout) {
+  using RI = typename RDF::Rev;
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  constexpr int RL = RI::R0;
+  constexpr int RE = LastRadix<RI>::value;
+  fast_fft_b<RDF, false>(v, s, t, pf);
+#pragma unroll
+  for (int k = 0; k < V; ++k) v[k] = conj ? cmulc(v[k], h[k]) : cmul(v[k], h[k]);
+  fast_fft_b<RI, true>(v, s, t, pi);
+  if (live) {
+    const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
+#pragma unroll
+    for (int m = 0; m < V / RE; ++m)
+#pragma unroll
+      for (int r = 0; r < RE; ++r) {
+        const int c = P * m + r * (n / RE);
+        if (!OUT_HALF || c < n / 2) buf_store(rout, vout, c * kout, v[m * RE + r]);
+      }
+  }
+}
+
+// Software-pipelined batched y pass (one resident workgroup per CU, up to 256 VGPRs): while
+// field b is transformed, the column data of field b+1 is already in flight into a second
+// register set (ping-pong over two unrolled iterations).  ONE_TAB: every field uses the same
+// transfer function (one wavelength), so its values are loaded once and kept in registers;
+// otherwise they are prefetched with the data.
+template <typename RDF, int L, bool IN_HALF, bool OUT_HALF, bool ONE_TAB>
+__global__ void __launch_bounds__(L * RDF::P)
+k_fast_ypipe(FastIO io, int batch, const float2* __restrict__ tabf, const float2* __restrict__ tabi,
+             QuadTab qt) {
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  using RI = typename RDF::Rev;
+  extern __shared__ unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  int l, t;
+  thread_map<L, P, true>(threadIdx.x, l, t);
+  const int li = blockIdx.x * L + l;
+  const bool live = li < io.nlines;
+  float2* s = smem + l * LdsLine<n>::size;
+  const int vin = fast_off<true>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
+  const int kin = fast_k<true>(io.in_l, io.in_e, io.in_blk) * 8;
+  const int vout = fast_off<true>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
+  const int kout = fast_k<true>(io.out_l, io.out_e, io.out_blk) * 8;
+  const int lic = live ? li : 0;
+  const int qxi = lic <= qt.nx - lic ? lic : qt.nx - lic;
+  const int qoff_d = (qxi * qt.qcol + t) * 8;
+  const int qoff_m = (qxi * qt.qcol + P - t) * 8;
+  PlanBases<RDF> pf;
+  PlanBases<RI> pi;
+  pf.load(tabf, t);
+  pi.load(tabi, t);
+  float2 va[V], vb[V], ha[V];
+  yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, 0, vin, kin);
+  ypipe_loadh<RDF>(ha, qt, 0, qoff_d, qoff_m);
+  if constexpr (ONE_TAB) {
+    for (int b = 0; b < batch; b += 2) {
+      if (b + 1 < batch) yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b + 1, vin, kin);
+      ypipe_field<RDF, OUT_HALF>(va, ha, s, t, live, b, io, pf, pi, qt.conj, vout, kout);
+      if (b + 1 >= batch) break;
+      if (b + 2 < batch) yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b + 2, vin, kin);
+      ypipe_field<RDF, OUT_HALF>(vb, ha, s, t, live, b + 1, io, pf, pi, qt.conj, vout, kout);
+    }
+  } else {
+    float2 hb[V];
+    for (int b = 0; b < batch; b += 2) {
+      if (b + 1 < batch) {
+        yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b + 1, vin, kin);
+        ypipe_loadh<RDF>(hb, qt, b + 1, qoff_d, qoff_m);
+      }
+      ypipe_field<RDF, OUT_HALF>(va, ha, s, t, live, b, io, pf, pi, qt.conj, vout, kout);
+      if (b + 1 >= batch) break;
+      if (b + 2 < batch) {
+        yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b + 2, vin, kin);
+        ypipe_loadh<RDF>(ha, qt, b + 2, qoff_d, qoff_m);
+      }
+      ypipe_field<RDF, OUT_HALF>(vb, hb, s, t, live, b + 1, io, pf, pi, qt.conj, vout, kout);
+    }
   }
 }
 

@@ -33,7 +33,10 @@ template <int A, int B, int C_> struct Radices {
 // LDS slot of line position pos: one pad slot every 16 complex values spreads
 // power-of-two strides over the 32 banks.
 __device__ __forceinline__ int lds_pos(int pos) { return pos + (pos >> 4); }
-template <int n> struct LdsLine { static constexpr int size = n + n / 16 + 2; };
+// Lines are 8 slots (16 dwords) past a multiple of 16 slots apart, so when adjacent lanes
+// work on adjacent lines (strided-axis passes) their banks differ by 16: at most two lanes
+// of a half-wave meet in a bank.
+template <int n> struct LdsLine { static constexpr int size = ((n + n / 16 + 15) / 16) * 16 + 8; };
 
 // twiddle W_n^k (forward) or its conjugate (inverse) from the global table
 template <bool INV, typename C>

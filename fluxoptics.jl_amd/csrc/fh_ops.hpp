@@ -19,7 +19,8 @@ template <typename T> struct QuadMulT {
   const int* tabidx;
   long tabsize;
   int nx, ny, qcol;
-  int conj;  // Backward direction multiplies by the conjugate
+  int conj;     // Backward direction multiplies by the conjugate
+  int one_tab;  // all fields share one table
   template <typename C>
   __device__ __forceinline__ C operator()(C v, int b, int kx, int ky) const {
     const int qxi = kx <= nx - kx ? kx : nx - kx;

@@ -557,7 +557,7 @@ struct FastConvVariant {
   FastConvLaunch fn;
 };
 
-template <int A, int B, int C_, int L, bool LT = false>
+template <int A, int B, int C_, int L, int FPW = 1 << 20, bool LT = false>
 static int fast_conv_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool half) {
   using RDF = Radices<A, B, C_>;
   using RI = typename RDF::Rev;
@@ -567,16 +567,17 @@ static int fast_conv_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool 
       (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
     return rc;
   const size_t lds = (size_t)L * LdsLine<RDF::n>::size * sizeof(float2);
-  QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj};
-  dim3 grid((io.nlines + L - 1) / L);
+  QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab};
+  const int fpw = FPW < batch ? FPW : batch;
+  dim3 grid((io.nlines + L - 1) / L, (batch + fpw - 1) / fpw);
   if (half) {
     auto kern = k_fast_yconv<RDF, L, true, true, LT>;
     FH_HIP(ensure_lds(kern, lds));
-    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, batch, tf, ti, qt);
+    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, batch, fpw, tf, ti, qt);
   } else {
     auto kern = k_fast_yconv<RDF, L, false, false, LT>;
     FH_HIP(ensure_lds(kern, lds));
-    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, batch, tf, ti, qt);
+    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, batch, fpw, tf, ti, qt);
   }
   FH_HIP(hipGetLastError());
   return FH_OK;
@@ -588,20 +589,54 @@ static const FastConvVariant kFast6[] = {{"8x8 L16", fast_conv_v<8, 8, 1, 16>}};
 static const FastConvVariant kFast7[] = {{"16x8 L16", fast_conv_v<16, 8, 1, 16>}};
 static const FastConvVariant kFast8[] = {{"16x16 L16", fast_conv_v<16, 16, 1, 16>},
                                          {"16x16 L8", fast_conv_v<16, 16, 1, 8>}};
-// First entry = default (fastest measured on MI355X for batched 2^LOGN lines).
-static const FastConvVariant kFast9[] = {{"8x8x8 L4", fast_conv_v<8, 8, 8, 4>},
-                                         {"8x8x8 L8", fast_conv_v<8, 8, 8, 8>},
+template <int A, int B, int C_, int L>
+static int fast_pipe_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool half) {
+  using RDF = Radices<A, B, C_>;
+  using RI = typename RDF::Rev;
+  const float2 *tf = nullptr, *ti = nullptr;
+  int rc;
+  if ((rc = get_stage_tw(c, RDF::n, RDF::R0, RDF::R1, RDF::R2, false, &tf)) ||
+      (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
+    return rc;
+  const size_t lds = (size_t)L * LdsLine<RDF::n>::size * sizeof(float2);
+  QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab};
+  dim3 grid((io.nlines + L - 1) / L);
+  auto launch = [&](auto kern) -> int {
+    FH_HIP(ensure_lds(kern, lds));
+    hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, batch, tf, ti, qt);
+    return FH_OK;
+  };
+  if (half) rc = qt.one_tab ? launch(k_fast_ypipe<RDF, L, true, true, true>) : launch(k_fast_ypipe<RDF, L, true, true, false>);
+  else rc = qt.one_tab ? launch(k_fast_ypipe<RDF, L, false, false, true>) : launch(k_fast_ypipe<RDF, L, false, false, false>);
+  if (rc) return rc;
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+// Variant lists per length; the defaults (first entry; for batches that share one transfer
+// function the pipelined kernel when listed second) are the fastest measured on MI355X.
+static const FastConvVariant kFast9[] = {{"8x8x8 L4 f1", fast_conv_v<8, 8, 8, 4, 1>},
+                                         {"8x8x8 L4", fast_conv_v<8, 8, 8, 4>},
                                          {"16x32 L8", fast_conv_v<16, 32, 1, 8>}};
-static const FastConvVariant kFast10[] = {{"16x8x8 L4", fast_conv_v<16, 8, 8, 4>},
-                                          {"32x32 L8", fast_conv_v<32, 32, 1, 8>},
-                                          {"16x8x8 L8", fast_conv_v<16, 8, 8, 8>}};
-static const FastConvVariant kFast11[] = {{"16x16x8 L4", fast_conv_v<16, 16, 8, 4>},
-                                          {"32x8x8 L8", fast_conv_v<32, 8, 8, 8>},
-                                          {"8x16x16 L4", fast_conv_v<8, 16, 16, 4>},
-                                          {"32x8x8 L4", fast_conv_v<32, 8, 8, 4>}};
-static const FastConvVariant kFast12[] = {{"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
-                                          {"32x16x8 L4", fast_conv_v<32, 16, 8, 4>},
-                                          {"16x16x16 L2", fast_conv_v<16, 16, 16, 2>}};
+static const FastConvVariant kFast10[] = {{"16x8x8 L4 f1", fast_conv_v<16, 8, 8, 4, 1>},
+                                          {"pipe 16x8x8 L4", fast_pipe_v<16, 8, 8, 4>},
+                                          {"16x8x8 L4", fast_conv_v<16, 8, 8, 4>},
+                                          {"32x32 L8 f1", fast_conv_v<32, 32, 1, 8, 1>}};
+static const FastConvVariant kFast11[] = {{"8x16x16 L4 f1", fast_conv_v<8, 16, 16, 4, 1>},
+                                          {"pipe 8x16x16 L4", fast_pipe_v<8, 16, 16, 4>},
+                                          {"pipe 16x16x8 L4", fast_pipe_v<16, 16, 8, 4>},
+                                          {"16x16x8 L4 f1", fast_conv_v<16, 16, 8, 4, 1>},
+                                          {"16x16x8 L4", fast_conv_v<16, 16, 8, 4>},
+                                          {"16x16x8 L4 f2", fast_conv_v<16, 16, 8, 4, 2>}};
+static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16, 16, 4, 1>},
+                                          {"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
+                                          {"16x16x16 L2 f1", fast_conv_v<16, 16, 16, 2, 1>}};
+
+// default variant index for a y length and whether the batch shares one transfer function
+static int default_conv_variant(int logn, bool one_tab) {
+  if (one_tab && logn == 11) return 1;
+  return 0;
+}
 
 static const FastConvVariant* fast_conv_variants(int logn, int* count) {
   switch (logn) {
@@ -611,8 +646,8 @@ static const FastConvVariant* fast_conv_variants(int logn, int* count) {
     case 7: *count = 1; return kFast7;
     case 8: *count = 2; return kFast8;
     case 9: *count = 3; return kFast9;
-    case 10: *count = 3; return kFast10;
-    case 11: *count = 4; return kFast11;
+    case 10: *count = 4; return kFast10;
+    case 11: *count = 6; return kFast11;
     case 12: *count = 3; return kFast12;
     default: *count = 0; return nullptr;
   }
@@ -697,14 +732,15 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
     if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop))) return rc;
   }
   if (only == 0) return FH_OK;
-  QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, 1 << p->ly, p->qy, direction < 0};
+  QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, 1 << p->ly, p->qy, direction < 0, p->lam_tables.size() == 1};
   io.in = mid; io.out = mid; io.in_f = io.out_f = midf;
   io.in_l = io.out_l = 1; io.in_e = io.out_e = nxl; io.in_blk = io.out_blk = 1;
   io.nlines = nxl; io.in_bytes = io.out_bytes = midf * 8;
   int nvar = 0;
   const FastConvVariant* vars = fast_conv_variants(p->ly, &nvar);
   if (!vars) return fail(FH_E_UNSUPP, "y length 2^%d unsupported", p->ly);
-  const int v = (p->conv_var >= 0 && p->conv_var < nvar) ? p->conv_var : 0;
+  const int v = (p->conv_var >= 0 && p->conv_var < nvar) ? p->conv_var
+                                                          : default_conv_variant(p->ly, q.one_tab);
   return vars[v].fn(c, io, p->batch, q, rs);
 }
 
@@ -1169,7 +1205,7 @@ static int bpm_ypass(fh_bpm* m, float2* f, const double2* tab, int direction) {
   io.in = f; io.out = f; io.in_f = io.out_f = (long)m->nx * m->ny;
   io.in_l = io.out_l = 1; io.in_e = io.out_e = m->nx;
   io.nlines = m->nx; io.in_valid = io.out_count = m->ny;
-  QuadMulT<double2> q{tab, m->d_tabidx, m->tabsize, m->nx, m->ny, m->qy, direction < 0};
+  QuadMulT<double2> q{tab, m->d_tabidx, m->tabsize, m->nx, m->ny, m->qy, direction < 0, m->lam_tables.size() == 1};
   FH_HIP((launch_conv<float2, double2>(m->ly, io, m->batch, tw, q, m->ctx->stream)));
   return FH_OK;
 }
