@@ -284,6 +284,8 @@ struct QuadTab {
   int qcol;      // entries per table column (n/2 + 1)
   int conj;
   int one_tab;   // every field uses table tabidx[0]
+  int full;      // tables hold every column (no x mirror) and both column halves
+  int moff;      // full tables: offset of the mirrored column half (n/2), else 0
 };
 
 // One field of the batched y pass: FFT, transfer-function multiply, IFFT, store.
@@ -297,8 +299,9 @@ __device__ __forceinline__ void yconv_field(float2* v, float2* s, int t, int li,
   constexpr int RL = RI::R0;
   constexpr int RE = LastRadix<RI>::value;
   fast_fft<float2, RDF, false>(v, s, t, tabf);
-  // H(kx, ky): ky = t + c; c < n/2 reads column entry t + c, c >= n/2 the mirror n - t - c,
-  // written as (P - t) + (n - c - P) so both offsets stay non-negative.
+  // H(kx, ky): ky = t + c; c < n/2 reads column entry t + c, c >= n/2 entry moff + n - t - c
+  // (the mirror for even tables, the second half for full ones), written as
+  // moff + (P - t) + (n - c - P) so both offsets stay non-negative.
   const auto rt = make_rsrc(qt.tab + (long)qt.tabidx[b] * qt.tabsize, qt.tabsize * 8);
 #pragma unroll
   for (int m = 0; m < V / RL; ++m)
@@ -357,9 +360,9 @@ k_fast_yconv(FastIO io, int batch, int fpw, const float2* __restrict__ tabf,
   const int vout = fast_off<true>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
   const int kout = fast_k<true>(io.out_l, io.out_e, io.out_blk) * 8;
   const int lic = live ? li : 0;
-  const int qxi = lic <= qt.nx - lic ? lic : qt.nx - lic;
+  const int qxi = qt.full ? lic : (lic <= qt.nx - lic ? lic : qt.nx - lic);
   const int qoff_d = (qxi * qt.qcol + t) * 8;
-  const int qoff_m = (qxi * qt.qcol + P - t) * 8;
+  const int qoff_m = (qxi * qt.qcol + qt.moff + P - t) * 8;
   const int b0 = blockIdx.y * fpw;
   const int b1 = b0 + fpw < batch ? b0 + fpw : batch;
   for (int b = b0; b < b1; ++b) {
@@ -513,9 +516,9 @@ k_fast_ypipe(FastIO io, int batch, const float2* __restrict__ tabf, const float2
   const int vout = fast_off<true>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
   const int kout = fast_k<true>(io.out_l, io.out_e, io.out_blk) * 8;
   const int lic = live ? li : 0;
-  const int qxi = lic <= qt.nx - lic ? lic : qt.nx - lic;
+  const int qxi = qt.full ? lic : (lic <= qt.nx - lic ? lic : qt.nx - lic);
   const int qoff_d = (qxi * qt.qcol + t) * 8;
-  const int qoff_m = (qxi * qt.qcol + P - t) * 8;
+  const int qoff_m = (qxi * qt.qcol + qt.moff + P - t) * 8;
   PlanBases<RDF> pf;
   PlanBases<RI> pi;
   pf.load(tabf, t);

@@ -21,10 +21,11 @@ template <typename T> struct QuadMulT {
   int nx, ny, qcol;
   int conj;     // Backward direction multiplies by the conjugate
   int one_tab;  // all fields share one table
+  int full = 0; // full tables: every column kx; column entries ky < n/2 at ky, the rest at n/2 + n - ky
   template <typename C>
   __device__ __forceinline__ C operator()(C v, int b, int kx, int ky) const {
-    const int qxi = kx <= nx - kx ? kx : nx - kx;
-    const int qyi = ky <= ny - ky ? ky : ny - ky;
+    const int qxi = full ? kx : (kx <= nx - kx ? kx : nx - kx);
+    const int qyi = full ? (ky < ny / 2 ? ky : ny / 2 + ny - ky) : (ky <= ny - ky ? ky : ny - ky);
     const C h = cvt<C>(tab[tabidx[b] * tabsize + (long)qxi * qcol + qyi]);
     return conj ? cmulc(v, h) : cmul(v, h);
   }

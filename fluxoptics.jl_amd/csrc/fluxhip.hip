@@ -162,38 +162,78 @@ static hipError_t launch_conv(int logn, const LineIO<S>& io, int batch, const C*
 
 // ================================================================ table kernels (FP64)
 
-// Angular-spectrum transfer function on one quadrant, computed in FP64 and rounded
-// once to c64 (angular_spectrum.jl:1-12 with double_precision_kernel=true):
-//   a = 1/(lam/n0)^2 - fx^2 - fy^2 (evaluated in this order, no contraction, as the
-//   reference's FP64 broadcast does),
-//   a >= 0: H = cis(2*pi*t), t = z*sqrt(a) in cycles, reduced to t - rint(t) before
-//           sincospi so the phase keeps full FP64 accuracy at |2*pi*t| ~ 1e5 rad;
-//   a <  0: H = exp(-2*pi*|z|*sqrt(-a)) (principal sqrt; the same for z < 0, where
-//           H(z) = conj(H(-z))).
+// Transfer-function table layouts (column-major, one column per kx so the y pass reads a
+// contiguous run):
+//   even tables (the function depends on |fx|, |fy| only): row i = |kx| in [0, nx/2],
+//     entry j = |ky| in [0, ny/2];
+//   full tables (tilted fields): row i = kx in [0, nx), entry j <= ny/2 holds ky = j and
+//     entry j > ny/2 holds ky = 3*ny/2 - j (the upper half in reverse).
+// Signed frequency index of an FFT bin: k < n/2 ? k : k - n (fftfreq order).
+__device__ inline void tab_freq_index(int i, int j, int nx, int ny, int full, int& kx, int& ky) {
+  if (!full) {
+    kx = i;
+    ky = j;
+    return;
+  }
+  kx = i < nx / 2 ? i : i - nx;
+  const int ku = j <= ny / 2 ? j : 3 * ny / 2 - j;
+  ky = ku < ny / 2 ? ku : ku - ny;
+}
+
+// Angular-spectrum transfer functions, computed in FP64 and rounded once to c64
+// (double_precision_kernel=true).  lam' = lam/n0, f0 = sin(theta)/lam'.
+//   mode 0, on-axis (angular_spectrum.jl:1-12): a = 1/lam'^2 - fx^2 - fy^2 (evaluated in
+//     this order, no contraction, as the reference's FP64 broadcast does);
+//     a >= 0: H = cis(2*pi*t), t = z*sqrt(a) in cycles, reduced to t - rint(t) before
+//     sincospi so the phase keeps full FP64 accuracy at |2*pi*t| ~ 1e5 rad;
+//     a < 0: H = exp(-2*pi*|z|*sqrt(-a)) (principal sqrt; the same for z < 0, where
+//     H(z) = conj(H(-z))).
+//   mode 1, paraxial (:27-35): t = z/lam' - lam'*z*((fx+f0x)^2 + (fy+f0y)^2)/2
+//     - [tracked] (f0x*fx + f0y*fy)/pi, for either sign of z.
+//   mode 2, tilted (:45-62): a = 1/lam'^2 - (fx+f0x)^2 - (fy+f0y)^2, off = [tracked]
+//     (f0x*fx + f0y*fy)*lam'; a >= 0: t = z*(sqrt(a) + off); a < 0: amplitude
+//     exp(-2*pi*|z|*sqrt(-a)) and t = z*off (conj(H(-z)) for z < 0 gives the same).
 // The inverse FFT's 1/(nx*ny) is folded in (scale).
-__global__ void k_as_table(float2* __restrict__ tab, int nx, int ny, int qx, int qy, double dx,
-                           double dy, double lam, double n0, double z, double scale) {
+__global__ void k_as_table(float2* __restrict__ tab, int nx, int ny, int rows, int qcol, int full,
+                           double dx, double dy, double lam, double n0, double z, double tx,
+                           double ty, int mode, int track, double scale) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
-  if (i >= qx || j >= qy) return;
-  const double fx = (double)i / ((double)nx * dx);
-  const double fy = (double)j / ((double)ny * dy);
+  if (i >= rows || j >= qcol) return;
+  int kx, ky;
+  tab_freq_index(i, j, nx, ny, full, kx, ky);
+  const double fx = (double)kx / ((double)nx * dx);
+  const double fy = (double)ky / ((double)ny * dy);
   const double le = lam / n0;
-  const double f2 = 1.0 / (le * le);
-  const double a = f2 - fx * fx - fy * fy;
-  double re, im;
-  if (a >= 0.0) {
-    const double t = z * sqrt(a);
-    const double r = t - rint(t);
-    sincospi(2.0 * r, &im, &re);
-    re *= scale;
-    im *= scale;
+  double amp = 1.0, t = 0.0;
+  if (mode == 0) {
+    const double f2 = 1.0 / (le * le);
+    const double a = f2 - fx * fx - fy * fy;
+    if (a >= 0.0) t = z * sqrt(a);
+    else amp = exp(-2.0 * M_PI * fabs(z) * sqrt(-a));
   } else {
-    re = exp(-2.0 * M_PI * fabs(z) * sqrt(-a)) * scale;
-    im = 0.0;
+    const double f0x = sin(tx) / le, f0y = sin(ty) / le;
+    const double gx = fx + f0x, gy = fy + f0y;
+    if (mode == 1) {
+      t = z / le - 0.5 * le * z * (gx * gx + gy * gy);
+      if (track) t -= (f0x * fx + f0y * fy) / M_PI;
+    } else {
+      const double f2 = 1.0 / (le * le);
+      const double a = f2 - gx * gx - gy * gy;
+      const double off = track ? (f0x * fx + f0y * fy) * le : 0.0;
+      if (a >= 0.0) t = z * (sqrt(a) + off);
+      else {
+        amp = exp(-2.0 * M_PI * fabs(z) * sqrt(-a));
+        t = z * off;
+      }
+    }
   }
-  tab[(long)i * qy + j] = make_float2((float)re, (float)im);
+  const double r = t - rint(t);
+  double sn, cs;
+  sincospi(2.0 * r, &sn, &cs);
+  const double re = amp * cs, im = amp * sn;
+  tab[(long)i * qcol + j] = make_float2((float)(re * scale), (float)(im * scale));
 }
 
 // Full FP64 transfer-function table [ny][nx] for the plain comparator (same formula).
@@ -228,7 +268,7 @@ __global__ void k_as_table_full_d(double2* __restrict__ tab, int nx, int ny, dou
 // cropped output of the linear convolution).  The phase k*r is reduced in cycles:
 // r/lam - rint(r/lam).
 __global__ void k_rs_sample(double2* __restrict__ d, int nx, int ny, double dx, double dy,
-                            double lam, double z, double nrm) {
+                            double lam, double z, double nrm, double tx, double ty, int tilt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   const int nx2 = 2 * nx;
@@ -247,18 +287,30 @@ __global__ void k_rs_sample(double2* __restrict__ d, int nx, int ny, double dx, 
   const double ar = 1.0 / r, ai = -k;
   double re = g * (cs * ar - sn * ai);
   double im = g * (cs * ai + sn * ar);
+  if (tilt) {
+    // tilted field, tilts not tracked (rayleigh_sommerfeld.jl:14-22): times
+    // cis(-2*pi*(x*f0x + y*f0y)), f0 = sin(theta)/lam, reduced in cycles
+    const double c = x * (sin(tx) / lam) + y * (sin(ty) / lam);
+    double ls, lc;
+    sincospi(-2.0 * (c - rint(c)), &ls, &lc);
+    const double r2 = re * lc - im * ls;
+    im = re * ls + im * lc;
+    re = r2;
+  }
   if (z < 0) im = -im;  // h(z) = conj(h(-z))
   d[(long)j * nx2 + i] = make_double2(re, im);
 }
 
-// Quadrant [0..nx]x[0..ny] of an FP64 spectrum on the padded grid, scaled and rounded.
-__global__ void k_quad_extract(float2* __restrict__ tab, const double2* __restrict__ d, int nx2,
-                               int qx, int qy, double scale) {
+// Table of an FP64 spectrum [ny2][nx2] on the padded grid, scaled and rounded: the even
+// quadrant [0..nx2/2]x[0..ny2/2] or the full layout (see tab_freq_index).
+__global__ void k_tab_extract(float2* __restrict__ tab, const double2* __restrict__ d, int nx2, int ny2,
+                              int rows, int qcol, int full, double scale) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
-  if (i >= qx || j >= qy) return;
-  const double2 v = d[(long)j * nx2 + i];
-  tab[(long)i * qy + j] = make_float2((float)(v.x * scale), (float)(v.y * scale));
+  if (i >= rows || j >= qcol) return;
+  const int ky = (full && j > ny2 / 2) ? 3 * ny2 / 2 - j : j;
+  const double2 v = d[(long)ky * nx2 + i];
+  tab[(long)i * qcol + j] = make_float2((float)(v.x * scale), (float)(v.y * scale));
 }
 
 // ================================================================ elementwise kernels
@@ -299,6 +351,76 @@ __global__ void k_phase_backward(float2* __restrict__ dv, const float* __restric
     }
     if (dphi) dphi[i] = (acc ? dphi[i] : 0.f) + s * g;
   }
+}
+
+// ================================================================ metrics
+// SquaredIntensityDifference (squared_intensity_difference.jl:78-90): d = |u_b|^2 - I_b per
+// field ("full dimensions") or d = sum_b |u_b|^2 - I ("spatial only"); loss = sum d^2 and the
+// pullback du_b = 4 dc d u_b.  d and d^2 are formed in FP64 from the FP32 data; the sum is a
+// fixed-shape tree (block partials, then one ordered reduction), so repeated calls agree bit
+// for bit.
+constexpr int kMetricThreads = 256;
+constexpr int kMetricBlocks = 256;  // partials per output slot
+
+__device__ inline double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+  return s;  // valid on thread 0
+}
+
+__global__ void __launch_bounds__(kMetricThreads)
+k_sqint_partial(const float2* __restrict__ u, const float* __restrict__ I, long npix, int batch,
+                int spatial, const double* __restrict__ dc, float2* __restrict__ du,
+                double* __restrict__ part) {
+  __shared__ double red[kMetricThreads / 64];
+  const int slot = blockIdx.y;
+  const double g = dc ? 4.0 * dc[slot] : 4.0;
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)kMetricThreads + threadIdx.x; i < npix;
+       i += (long)gridDim.x * kMetricThreads) {
+    if (spatial) {
+      double s = 0.0;
+      for (int b = 0; b < batch; ++b) {
+        const float2 v = u[b * npix + i];
+        s += (double)v.x * v.x + (double)v.y * v.y;
+      }
+      const double d = s - (double)I[i];
+      acc += d * d;
+      if (du) {
+        const float f = (float)(g * d);
+        for (int b = 0; b < batch; ++b) {
+          const float2 v = u[b * npix + i];
+          du[b * npix + i] = make_float2(f * v.x, f * v.y);
+        }
+      }
+    } else {
+      const long k = slot * npix + i;
+      const float2 v = u[k];
+      const double d = (double)v.x * v.x + (double)v.y * v.y - (double)I[k];
+      acc += d * d;
+      if (du) {
+        const float f = (float)(g * d);
+        du[k] = make_float2(f * v.x, f * v.y);
+      }
+    }
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) part[slot * (long)gridDim.x + blockIdx.x] = s;
+}
+
+// loss[slot] = ordered sum of the slot's partials (one workgroup per slot)
+__global__ void __launch_bounds__(kMetricThreads)
+k_sum_partials(const double* __restrict__ part, int nper, double* __restrict__ out) {
+  __shared__ double red[kMetricThreads / 64];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nper; i += kMetricThreads) acc += part[blockIdx.x * (long)nper + i];
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 __global__ void k_mul_full_d(double2* __restrict__ u, const double2* __restrict__ tab, long npix,
@@ -417,6 +539,8 @@ struct fh_ctx {
   std::map<std::pair<int, int>, void*> twiddles;  // (n, is_double) -> W_n^k table
   std::map<std::vector<int>, void*> stage_tw;      // (n, R0, R1, R2, inverse) -> stage tables
   std::mutex mu;
+  double* metric_part = nullptr;  // metric block partials
+  size_t metric_part_n = 0;
 };
 
 struct fh_plan {
@@ -425,9 +549,14 @@ struct fh_plan {
   int nx = 0, ny = 0, batch = 0;
   int lx = 0, ly = 0;   // log2 of the transform lengths (padded for RS)
   double dx = 0, dy = 0, z = 0, n0 = 1;
-  std::vector<double> lam_tables;  // distinct wavelengths, table order
+  std::vector<double> lam_tables;  // per table: wavelength ...
+  std::vector<double> tab_z;       // ... distance ...
+  std::vector<double> tab_tx, tab_ty;  // ... and tilt angles (radians)
   std::vector<int> field_table;    // table index per field
-  float2* tab = nullptr;           // quadrant tables, one per distinct wavelength
+  int mode = 0;                    // AS kernel: 0 on-axis, 1 paraxial, 2 tilted
+  int track = 0;                   // tilts tracked
+  int full = 0;                    // full (non-even) table layout
+  float2* tab = nullptr;           // transfer-function tables, one per distinct key
   long tabsize = 0;
   int qx = 0, qy = 0;
   int* d_tabidx = nullptr;
@@ -567,7 +696,8 @@ static int fast_conv_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool 
       (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
     return rc;
   const size_t lds = (size_t)L * LdsLine<RDF::n>::size * sizeof(float2);
-  QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab};
+  QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab, op.full,
+             op.full ? op.ny / 2 : 0};
   const int fpw = FPW < batch ? FPW : batch;
   dim3 grid((io.nlines + L - 1) / L, (batch + fpw - 1) / fpw);
   if (half) {
@@ -599,7 +729,8 @@ static int fast_pipe_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool 
       (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
     return rc;
   const size_t lds = (size_t)L * LdsLine<RDF::n>::size * sizeof(float2);
-  QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab};
+  QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab, op.full,
+             op.full ? op.ny / 2 : 0};
   dim3 grid((io.nlines + L - 1) / L);
   auto launch = [&](auto kern) -> int {
     FH_HIP(ensure_lds(kern, lds));
@@ -658,8 +789,11 @@ static int check_dir(int direction) {
   return FH_OK;
 }
 
+// Validates the common plan parameters and assigns one table per distinct
+// (wavelength, z, tilt x, tilt y) of the batch.  zs / tilts may be null (z = 0, no tilt).
 static int plan_common(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
-                       const double* lambdas, int nlambdas, fh_plan* p) {
+                       const double* lambdas, int nlambdas, fh_plan* p, const double* zs = nullptr,
+                       int nz = 0, const double* tx = nullptr, const double* ty = nullptr, int nt = 0) {
   if (!ctx) return fail(FH_E_ARG, "null context");
   if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive (nx=%d ny=%d batch=%d)", nx, ny, batch);
   if (!(dx > 0) || !(dy > 0)) return fail(FH_E_ARG, "sampling must be positive");
@@ -667,18 +801,43 @@ static int plan_common(fh_ctx* ctx, int nx, int ny, int batch, double dx, double
     return fail(FH_E_ARG, "need 1 or batch (=%d) wavelengths, got %d", batch, nlambdas);
   for (int k = 0; k < nlambdas; ++k)
     if (!(lambdas[k] > 0)) return fail(FH_E_ARG, "wavelengths must be positive");
+  if (zs && nz != 1 && nz != batch) return fail(FH_E_ARG, "need 1 or batch (=%d) distances, got %d", batch, nz);
+  for (int k = 0; zs && k < nz; ++k)
+    if (!std::isfinite(zs[k])) return fail(FH_E_ARG, "distances must be finite");
+  if (nt != 0 && ((nt != 1 && nt != batch) || !tx || !ty))
+    return fail(FH_E_ARG, "need 0, 1 or batch (=%d) tilt pairs, got %d", batch, nt);
+  for (int k = 0; k < nt; ++k)
+    if (!std::isfinite(tx[k]) || !std::isfinite(ty[k]) || fabs(tx[k]) >= M_PI / 2 || fabs(ty[k]) >= M_PI / 2)
+      return fail(FH_E_ARG, "tilt angles must be finite radians in (-pi/2, pi/2)");
   p->ctx = ctx;
   p->nx = nx; p->ny = ny; p->batch = batch; p->dx = dx; p->dy = dy;
   p->field_table.resize(batch);
   for (int b = 0; b < batch; ++b) {
     const double lam = lambdas[nlambdas == 1 ? 0 : b];
+    const double z = zs ? zs[nz == 1 ? 0 : b] : 0.0;
+    const double ax = nt ? tx[nt == 1 ? 0 : b] : 0.0, ay = nt ? ty[nt == 1 ? 0 : b] : 0.0;
     int idx = -1;
     for (size_t k = 0; k < p->lam_tables.size(); ++k)
-      if (p->lam_tables[k] == lam) { idx = (int)k; break; }
-    if (idx < 0) { idx = (int)p->lam_tables.size(); p->lam_tables.push_back(lam); }
+      if (p->lam_tables[k] == lam && p->tab_z[k] == z && p->tab_tx[k] == ax && p->tab_ty[k] == ay) {
+        idx = (int)k;
+        break;
+      }
+    if (idx < 0) {
+      idx = (int)p->lam_tables.size();
+      p->lam_tables.push_back(lam);
+      p->tab_z.push_back(z);
+      p->tab_tx.push_back(ax);
+      p->tab_ty.push_back(ay);
+    }
     p->field_table[b] = idx;
   }
   return FH_OK;
+}
+
+static bool plan_tilted(const fh_plan* p) {
+  for (size_t k = 0; k < p->tab_tx.size(); ++k)
+    if (p->tab_tx[k] != 0.0 || p->tab_ty[k] != 0.0) return true;
+  return false;
 }
 
 static void plan_free(fh_plan* p) {
@@ -732,7 +891,8 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
     if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop))) return rc;
   }
   if (only == 0) return FH_OK;
-  QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, 1 << p->ly, p->qy, direction < 0, p->lam_tables.size() == 1};
+  QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, 1 << p->ly, p->qy, direction < 0,
+            p->lam_tables.size() == 1, p->full};
   io.in = mid; io.out = mid; io.in_f = io.out_f = midf;
   io.in_l = io.out_l = 1; io.in_e = io.out_e = nxl; io.in_blk = io.out_blk = 1;
   io.nlines = nxl; io.in_bytes = io.out_bytes = midf * 8;
@@ -796,33 +956,45 @@ int fh_ctx_destroy(fh_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
   for (auto& kv : ctx->stage_tw) (void)hipFree(kv.second);
+  if (ctx->metric_part) (void)hipFree(ctx->metric_part);
   delete ctx;
   return FH_OK;
 }
 
-int fh_asprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
-                     const double* lambdas, int nlambdas, double z, double n0, fh_plan** out) {
-  if (!out) return fail(FH_E_ARG, "null output pointer");
+int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
+  if (!out || !d) return fail(FH_E_ARG, "null descriptor or output pointer");
   *out = nullptr;
+  if (d->flags & ~(FH_PROP_PARAXIAL | FH_PROP_TRACK_TILTS)) return fail(FH_E_ARG, "unknown flags 0x%x", d->flags);
+  if (!d->z) return fail(FH_E_ARG, "null distances");
   fh_plan* p = new fh_plan();
-  int rc = plan_common(ctx, nx, ny, batch, dx, dy, lambdas, nlambdas, p);
+  int rc = plan_common(ctx, d->nx, d->ny, d->batch, d->dx, d->dy, d->lambdas, d->nlambdas, p, d->z, d->nz,
+                       d->tilts_x, d->tilts_y, d->ntilts);
   if (rc) { plan_free(p); return rc; }
-  if (!(n0 > 0) || !std::isfinite(z)) { plan_free(p); return fail(FH_E_ARG, "n0 must be positive and z finite"); }
-  p->kind = 0; p->z = z; p->n0 = n0;
+  const int nx = d->nx, ny = d->ny;
+  if (!(d->n0 > 0)) { plan_free(p); return fail(FH_E_ARG, "n0 must be positive"); }
+  p->kind = 0; p->z = p->tab_z[0]; p->n0 = d->n0;
   p->lx = ilog2_exact(nx); p->ly = ilog2_exact(ny);
   if (p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
     plan_free(p);
     return fail(FH_E_UNSUPP, "ASProp needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << kMinLog, 1 << kMaxLog, nx, ny);
   }
   if ((rc = set_device(ctx))) { plan_free(p); return rc; }
-  p->qx = nx / 2 + 1; p->qy = ny / 2 + 1; p->tabsize = (long)p->qx * p->qy;
+  // kernel choice of angular_spectrum.jl:126-139; tilted tables are not even in fx, fy
+  const bool tilted = plan_tilted(p);
+  p->mode = (d->flags & FH_PROP_PARAXIAL) ? 1 : (tilted ? 2 : 0);
+  p->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
+  p->full = tilted ? 1 : 0;
+  p->qx = p->full ? nx : nx / 2 + 1;
+  p->qy = p->full ? ny + 1 : ny / 2 + 1;
+  p->tabsize = (long)p->qx * p->qy;
   const size_t bytes = p->tabsize * p->lam_tables.size() * sizeof(float2);
   if (hipMalloc(&p->tab, bytes) != hipSuccess) { plan_free(p); return fail(FH_E_NOMEM, "ASProp table: out of device memory"); }
   const double scale = 1.0 / ((double)nx * (double)ny);
   for (size_t k = 0; k < p->lam_tables.size(); ++k) {
     dim3 grid((p->qx + 255) / 256, p->qy);
     hipLaunchKernelGGL(k_as_table, grid, dim3(256), 0, ctx->stream, p->tab + k * p->tabsize, nx, ny,
-                       p->qx, p->qy, dx, dy, p->lam_tables[k], n0, z, scale);
+                       p->qx, p->qy, p->full, d->dx, d->dy, p->lam_tables[k], d->n0, p->tab_z[k],
+                       p->tab_tx[k], p->tab_ty[k], p->mode, p->track, scale);
     if (hipGetLastError() != hipSuccess) { plan_free(p); return fail(FH_E_HIP, "ASProp table kernel launch failed"); }
   }
   if ((rc = upload_tabidx(p))) { plan_free(p); return rc; }
@@ -830,15 +1002,27 @@ int fh_asprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double d
   return FH_OK;
 }
 
-int fh_rsprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
-                     const double* lambdas, int nlambdas, double z, fh_plan** out) {
-  if (!out) return fail(FH_E_ARG, "null output pointer");
+int fh_asprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
+                     const double* lambdas, int nlambdas, double z, double n0, fh_plan** out) {
+  fh_prop_desc d{};
+  d.nx = nx; d.ny = ny; d.batch = batch; d.dx = dx; d.dy = dy;
+  d.lambdas = lambdas; d.nlambdas = nlambdas; d.z = &z; d.nz = 1; d.n0 = n0;
+  return fh_asprop_create_ex(ctx, &d, out);
+}
+
+int fh_rsprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
+  if (!out || !d) return fail(FH_E_ARG, "null descriptor or output pointer");
   *out = nullptr;
+  if (d->flags & ~FH_PROP_TRACK_TILTS) return fail(FH_E_ARG, "RSProp takes FH_PROP_TRACK_TILTS only (flags 0x%x)", d->flags);
+  if (!d->z) return fail(FH_E_ARG, "null distances");
   fh_plan* p = new fh_plan();
-  int rc = plan_common(ctx, nx, ny, batch, dx, dy, lambdas, nlambdas, p);
+  int rc = plan_common(ctx, d->nx, d->ny, d->batch, d->dx, d->dy, d->lambdas, d->nlambdas, p, d->z, d->nz,
+                       d->tilts_x, d->tilts_y, d->ntilts);
   if (rc) { plan_free(p); return rc; }
-  if (!std::isfinite(z) || z == 0.0) { plan_free(p); return fail(FH_E_ARG, "RSProp needs a finite, nonzero z"); }
-  p->kind = 1; p->z = z;
+  for (double z : p->tab_z)
+    if (z == 0.0) { plan_free(p); return fail(FH_E_ARG, "RSProp needs a finite, nonzero z"); }
+  const int nx = d->nx, ny = d->ny;
+  p->kind = 1; p->z = p->tab_z[0];
   p->lx = ilog2_exact(2 * nx); p->ly = ilog2_exact(2 * ny);
   if (p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
     plan_free(p);
@@ -846,37 +1030,53 @@ int fh_rsprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double d
   }
   if ((rc = set_device(ctx))) { plan_free(p); return rc; }
   const int nx2 = 2 * nx, ny2 = 2 * ny;
-  p->qx = nx + 1; p->qy = ny + 1; p->tabsize = (long)p->qx * p->qy;
+  // tilted kernels (rayleigh_sommerfeld.jl:90-96) carry a linear phase unless tracked: odd spectra
+  const bool tilted = plan_tilted(p);
+  p->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
+  p->mode = tilted ? 2 : 0;
+  p->full = (tilted && !p->track) ? 1 : 0;
+  p->qx = p->full ? nx2 : nx + 1;
+  p->qy = p->full ? ny2 + 1 : ny + 1;
+  p->tabsize = (long)p->qx * p->qy;
   if (hipMalloc(&p->tab, p->tabsize * p->lam_tables.size() * sizeof(float2)) != hipSuccess) {
     plan_free(p);
     return fail(FH_E_NOMEM, "RSProp table: out of device memory");
   }
-  if (hipMalloc(&p->scratch, (size_t)nx2 * ny * batch * sizeof(float2)) != hipSuccess) {
+  if (hipMalloc(&p->scratch, (size_t)nx2 * ny * d->batch * sizeof(float2)) != hipSuccess) {
     plan_free(p);
     return fail(FH_E_NOMEM, "RSProp workspace: out of device memory");
   }
-  double2* d = nullptr;
-  if (hipMalloc(&d, (size_t)nx2 * ny2 * sizeof(double2)) != hipSuccess) {
+  double2* dd = nullptr;
+  if (hipMalloc(&dd, (size_t)nx2 * ny2 * sizeof(double2)) != hipSuccess) {
     plan_free(p);
     return fail(FH_E_NOMEM, "RSProp FP64 kernel buffer: out of device memory");
   }
   const double scale = 1.0 / ((double)nx2 * (double)ny2);
-  const double nrm = dx * dy / (2.0 * M_PI);
+  const double nrm = d->dx * d->dy / (2.0 * M_PI);
   for (size_t k = 0; k < p->lam_tables.size() && rc == FH_OK; ++k) {
-    hipLaunchKernelGGL(k_rs_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, d, nx, ny,
-                       dx, dy, p->lam_tables[k], z, nrm);
+    const int tilt = (p->tab_tx[k] != 0.0 || p->tab_ty[k] != 0.0) && !p->track;
+    hipLaunchKernelGGL(k_rs_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, dd, nx, ny,
+                       d->dx, d->dy, p->lam_tables[k], p->tab_z[k], nrm, p->tab_tx[k], p->tab_ty[k], tilt);
     if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "RS sample kernel launch failed"); break; }
-    if ((rc = fft2_d(ctx, d, nx2, ny2, p->lx, p->ly))) break;
-    hipLaunchKernelGGL(k_quad_extract, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
-                       p->tab + k * p->tabsize, d, nx2, p->qx, p->qy, scale);
+    if ((rc = fft2_d(ctx, dd, nx2, ny2, p->lx, p->ly))) break;
+    hipLaunchKernelGGL(k_tab_extract, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
+                       p->tab + k * p->tabsize, dd, nx2, ny2, p->qx, p->qy, p->full, scale);
     if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "RS table kernel launch failed"); break; }
   }
   if (rc == FH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(FH_E_HIP, "RS table generation failed");
-  (void)hipFree(d);
+  (void)hipFree(dd);
   if (rc == FH_OK) rc = upload_tabidx(p);
   if (rc) { plan_free(p); return rc; }
   *out = p;
   return FH_OK;
+}
+
+int fh_rsprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
+                     const double* lambdas, int nlambdas, double z, fh_plan** out) {
+  fh_prop_desc d{};
+  d.nx = nx; d.ny = ny; d.batch = batch; d.dx = dx; d.dy = dy;
+  d.lambdas = lambdas; d.nlambdas = nlambdas; d.z = &z; d.nz = 1;
+  return fh_rsprop_create_ex(ctx, &d, out);
 }
 
 int fh_plan_destroy(fh_plan* plan) {
@@ -1009,6 +1209,37 @@ int fh_phase_backward(fh_ctx* ctx, void* dv, const float* phi, const void* saved
   return FH_OK;
 }
 
+int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx, int ny, int batch,
+                         int target_batch, const double* dc, double* loss, void* du) {
+  if (!ctx || !u || !target || !loss) return fail(FH_E_ARG, "null context, field, target or loss");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  if (target_batch != batch && target_batch != 1)
+    return fail(FH_E_ARG, "the target holds %d intensities; expected %d (per field) or 1 (spatial only)",
+                target_batch, batch);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const int spatial = target_batch == 1 && batch > 1;
+  const int nout = spatial ? 1 : batch;
+  const size_t need = (size_t)nout * kMetricBlocks;
+  if (ctx->metric_part_n < need) {
+    if (ctx->metric_part) FH_HIP(hipFree(ctx->metric_part));
+    ctx->metric_part = nullptr;
+    ctx->metric_part_n = 0;
+    if (hipMalloc(&ctx->metric_part, need * sizeof(double)) != hipSuccess)
+      return fail(FH_E_NOMEM, "metric partials: out of device memory");
+    ctx->metric_part_n = need;
+  }
+  const long npix = (long)nx * ny;
+  hipLaunchKernelGGL(k_sqint_partial, dim3(kMetricBlocks, nout), dim3(kMetricThreads), 0, ctx->stream,
+                     reinterpret_cast<const float2*>(u), target, npix, batch, spatial, dc,
+                     reinterpret_cast<float2*>(du), ctx->metric_part);
+  FH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_partials, dim3(nout), dim3(kMetricThreads), 0, ctx->stream,
+                     (const double*)ctx->metric_part, kMetricBlocks, loss);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
 int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const void* saved, float* dphi,
                                 int direction, int accumulate) {
   if (!p || !dv || !phi) return fail(FH_E_ARG, "null plan, cotangent or mask");
@@ -1034,7 +1265,7 @@ int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const vo
 
 int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {
   if (!p || !u_c128) return fail(FH_E_ARG, "null plan or field");
-  if (p->kind != 0) return fail(FH_E_UNSUPP, "the plain FP64 comparator covers ASProp only");
+  if (p->kind != 0 || p->mode != 0) return fail(FH_E_UNSUPP, "the plain FP64 comparator covers on-axis ASProp only");
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   fh_ctx* c = p->ctx;
@@ -1045,7 +1276,7 @@ int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {
       return fail(FH_E_NOMEM, "plain FP64 table: out of device memory");
     for (size_t k = 0; k < p->lam_tables.size(); ++k) {
       hipLaunchKernelGGL(k_as_table_full_d, dim3((nx + 255) / 256, ny), dim3(256), 0, c->stream,
-                         p->plain_tab + k * npix, nx, ny, p->dx, p->dy, p->lam_tables[k], p->n0, p->z,
+                         p->plain_tab + k * npix, nx, ny, p->dx, p->dy, p->lam_tables[k], p->n0, p->tab_z[k],
                          1.0 / ((double)nx * ny));
       FH_HIP(hipGetLastError());
     }

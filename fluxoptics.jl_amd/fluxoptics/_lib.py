@@ -21,6 +21,18 @@ c_int, c_double, c_void_p, c_char_p = ctypes.c_int, ctypes.c_double, ctypes.c_vo
 c_float_p = ctypes.POINTER(ctypes.c_float)
 c_double_p = ctypes.POINTER(ctypes.c_double)
 
+FH_PROP_PARAXIAL = 1
+FH_PROP_TRACK_TILTS = 2
+
+
+class PropDesc(ctypes.Structure):
+    """``fh_prop_desc`` of include/fluxhip.h."""
+    _fields_ = [("nx", c_int), ("ny", c_int), ("batch", c_int), ("dx", c_double), ("dy", c_double),
+                ("lambdas", c_double_p), ("nlambdas", c_int), ("z", c_double_p), ("nz", c_int),
+                ("tilts_x", c_double_p), ("tilts_y", c_double_p), ("ntilts", c_int), ("n0", c_double),
+                ("flags", c_int)]
+
+
 # name -> (restype, argtypes); mirrors include/fluxhip.h one to one
 SIGNATURES = {
     "fh_version": (c_int, []),
@@ -34,6 +46,8 @@ SIGNATURES = {
                                  c_int, c_double, c_double, ctypes.POINTER(c_void_p)]),
     "fh_rsprop_create": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_double_p,
                                  c_int, c_double, ctypes.POINTER(c_void_p)]),
+    "fh_asprop_create_ex": (c_int, [c_void_p, ctypes.POINTER(PropDesc), ctypes.POINTER(c_void_p)]),
+    "fh_rsprop_create_ex": (c_int, [c_void_p, ctypes.POINTER(PropDesc), ctypes.POINTER(c_void_p)]),
     "fh_plan_destroy": (c_int, [c_void_p]),
     "fh_propagate": (c_int, [c_void_p, c_void_p, c_int]),
     "fh_propagate_pass": (c_int, [c_void_p, c_void_p, c_int, c_int]),
@@ -47,6 +61,8 @@ SIGNATURES = {
     "fh_propagate_phase_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                             c_int, c_int]),
     "fh_propagate_plain_fp64": (c_int, [c_void_p, c_void_p, c_int]),
+    "fh_sq_intensity_diff": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p,
+                                     c_void_p, c_void_p]),
     "fh_bpm_create": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_double_p, c_int,
                               c_double, c_double, c_int, c_int, ctypes.POINTER(c_void_p)]),
     "fh_bpm_destroy": (c_int, [c_void_p]),

@@ -59,32 +59,72 @@ def _check_mask(phi: torch.Tensor, ny: int, nx: int, device) -> None:
         raise ValueError(f"mask shape {tuple(phi.shape)} does not match the field ({ny}, {nx})")
 
 
+# --------------------------------------------------------------------------- pipe
+
+class _PipeComponent:
+    """``c1 | c2`` builds an ``OpticalSystem`` (the reference's ``c1 |> c2``,
+    ``optical_system.jl:150-180``); ``c | {"merge_components": True}`` passes keywords."""
+
+    trainable = False
+
+    def __or__(self, other):
+        from .system import OpticalSystem, _components_of
+        if isinstance(other, dict):
+            return OpticalSystem(None, self, **other)
+        return OpticalSystem(None, self, *_components_of(other))
+
+
 # --------------------------------------------------------------------------- propagators
 
-class _KernelPropagator:
+class _KernelPropagator(_PipeComponent):
     """Shared plan cache of ASProp / RSProp (one plan per device, batch and wavelengths)."""
 
-    def __init__(self, u: ScalarField, ds, z: float):
+    def __init__(self, u: ScalarField, ds, z):
         self.ds = tuple(float(d) for d in (ds if ds is not None else u.ds))
-        self.z = float(z)
+        # one distance, or one per field of the batch (an extension for uneven batches)
+        self.z = float(z) if np.ndim(z) == 0 else [float(v) for v in np.asarray(z).ravel()]
         self.nx, self.ny = u.nx, u.ny
         self._plans = {}
 
-    def _create(self, ctx, batch, lams, out):  # pragma: no cover - abstract
-        raise NotImplementedError
+    track_tilts = False
+    _create_ex = ""
+
+    def _desc(self, batch, lams, tilts, flags) -> "_lib.PropDesc":
+        arr = lambda v: (ctypes.c_double * len(v))(*v)
+        d = _lib.PropDesc()
+        d.nx, d.ny, d.batch = self.nx, self.ny, batch
+        d.dx, d.dy = self.ds
+        zs = self.z if isinstance(self.z, list) else [self.z]
+        if len(zs) not in (1, batch):
+            raise ValueError(f"{len(zs)} distances for a batch of {batch} fields")
+        d._keep = (arr(lams), arr(zs), arr(tilts[0]), arr(tilts[1]))
+        d.lambdas = ctypes.cast(d._keep[0], _lib.c_double_p)
+        d.nlambdas = len(lams)
+        d.z = ctypes.cast(d._keep[1], _lib.c_double_p)
+        d.nz = len(zs)
+        d.tilts_x = ctypes.cast(d._keep[2], _lib.c_double_p)
+        d.tilts_y = ctypes.cast(d._keep[3], _lib.c_double_p)
+        d.ntilts = batch
+        d.n0 = getattr(self, "n0", 1.0)
+        d.flags = flags
+        return d
+
+    def _flags(self) -> int:
+        return _lib.FH_PROP_TRACK_TILTS if self.track_tilts else 0
 
     def plan(self, u: ScalarField) -> ctypes.c_void_p:
         if (u.nx, u.ny) != (self.nx, self.ny):
             raise ValueError(f"field grid {(u.nx, u.ny)} does not match the propagator {(self.nx, self.ny)}")
         check_field(u.electric)
         lams = tuple(u.lambda_list())
-        key = (u.electric.device.index, u.batch, lams)
+        tilts = tuple(tuple(t) for t in u.tilt_lists())
+        key = (u.electric.device.index, u.batch, lams, tilts)
         p = self._plans.get(key)
         ctx = context(u.electric.device)
         if p is None:
             p = ctypes.c_void_p()
-            arr = (ctypes.c_double * len(lams))(*lams)
-            self._create(ctx, u.batch, arr, len(lams), p)
+            desc = self._desc(u.batch, lams, tilts, self._flags())
+            _lib.call(self._create_ex, ctx, ctypes.byref(desc), ctypes.byref(p))
             self._plans[key] = p
         return p
 
@@ -128,24 +168,40 @@ class ASProp(_KernelPropagator):
     ``ASProp(u, z; use_cache=true, n0=1)`` (``angular_spectrum.jl:193-229``): the
     transfer function is computed once in FP64 and cached as complex FP32, as the
     reference does with ``use_cache=true`` (``double_precision_kernel=use_cache``).
-    ``paraxial``, ``filter`` and tilted fields are outside the accelerated path.
+    The kernel follows the reference's choice (``angular_spectrum.jl:126-139``): paraxial
+    when ``paraxial=true``, else on-axis for untilted fields, else tilted (with the field's
+    tilts, ``track_tilts`` adding the walk-off phase).  ``filter`` (a Julia function of the
+    frequencies) is not on the accelerated path.
     """
+
+    _create_ex = "fh_asprop_create_ex"
 
     def __init__(self, u: ScalarField, z: float, ds=None, *, n0: float = 1.0,
                  use_cache: bool = True, paraxial: bool = False, filter: Optional[Callable] = None,
                  track_tilts: bool = False, double_precision_kernel: Optional[bool] = None):
-        if paraxial or filter is not None or track_tilts:
-            raise NotImplementedError("paraxial, filtered and tilted ASProp are not on the accelerated path")
+        if filter is not None:
+            raise NotImplementedError("a frequency filter is not on the accelerated path")
         if double_precision_kernel is False:
             raise NotImplementedError("transfer functions are always computed in FP64")
         super().__init__(u, ds, z)
         self.n0 = float(n0)
+        self.paraxial = bool(paraxial)
+        self.track_tilts = bool(track_tilts)
         if not self.n0 > 0:
             raise ValueError("n0 must be positive")
 
-    def _create(self, ctx, batch, lams, nl, out):
-        _lib.call("fh_asprop_create", ctx, self.nx, self.ny, batch, self.ds[0], self.ds[1], lams, nl,
-                  self.z, self.n0, ctypes.byref(out))
+    def _flags(self) -> int:
+        return super()._flags() | (_lib.FH_PROP_PARAXIAL if self.paraxial else 0)
+
+    @classmethod
+    def _from_params(cls, nx: int, ny: int, ds, z: float, n0: float) -> "ASProp":
+        """An ASProp for an (ny, nx) grid without a template field (used by merging)."""
+        self = cls.__new__(cls)
+        _KernelPropagator.__init__(self, _GridOnly(nx, ny, ds), ds, z)
+        self.n0 = float(n0)
+        self.paraxial = False
+        self.track_tilts = False
+        return self
 
 
 class RSProp(_KernelPropagator):
@@ -153,31 +209,37 @@ class RSProp(_KernelPropagator):
 
     Warns like the reference when |z| is below the critical distance.  Pads to 2n
     (the reference pads to 2n-1; the cropped result is the same linear convolution).
+    Tilted fields use the tilted kernel (``rayleigh_sommerfeld.jl:14-27,90-96``).
     """
+
+    _create_ex = "fh_rsprop_create_ex"
 
     def __init__(self, u: ScalarField, z: float, ds=None, *, use_cache: bool = True,
                  track_tilts: bool = False, double_precision_kernel: Optional[bool] = None):
-        if track_tilts:
-            raise NotImplementedError("tilted RSProp is not on the accelerated path")
         super().__init__(u, ds, z)
+        self.track_tilts = bool(track_tilts)
         zc = rs_valid_distance(self.nx, self.ny, self.ds[0], self.ds[1], u.lambda_min())
-        if abs(self.z) < zc:
-            warnings.warn(f"RSProp: propagation distance z={self.z} is below critical distance "
+        zmin = min(abs(v) for v in (self.z if isinstance(self.z, list) else [self.z]))
+        if zmin < zc:
+            warnings.warn(f"RSProp: propagation distance z={zmin} is below critical distance "
                           f"zc={zc}. Numerical artifacts expected. Consider using ASProp or finer "
                           "sampling (dx < lambda/2).")
 
-    def _create(self, ctx, batch, lams, nl, out):
-        _lib.call("fh_rsprop_create", ctx, self.nx, self.ny, batch, self.ds[0], self.ds[1], lams, nl,
-                  self.z, ctypes.byref(out))
-
 
 # --------------------------------------------------------------------------- Phase
+
+class _GridOnly:
+    """Grid description standing in for a template field."""
+
+    def __init__(self, nx, ny, ds):
+        self.nx, self.ny, self.ds = int(nx), int(ny), tuple(float(d) for d in ds)
+
 
 def _spatial(n: int, d: float) -> np.ndarray:
     return (np.arange(n, dtype=np.float64) - (n - 1) / 2.0) * d
 
 
-class Phase:
+class Phase(_PipeComponent):
     """Phase mask u -> u * cis(s*phi) (``phasemask.jl:38-150``).
 
     ``phi`` is a function ``(x, y) -> value`` evaluated on the centred grid
@@ -205,8 +267,20 @@ class Phase:
         self.grad = torch.zeros_like(self.phi) if self.buffered else None
         self.saved = torch.empty_like(u.electric) if self.buffered else None
 
+    @classmethod
+    def _from_tensor(cls, phi: torch.Tensor, trainable: bool = False) -> "Phase":
+        """A static Phase over an existing (ny, nx) float32 device mask (used by merging)."""
+        self = cls.__new__(cls)
+        self.phi = phi.to(torch.float32).contiguous()
+        self.trainable = trainable
+        self.buffered = False
+        self.grad = None
+        self.saved = None
+        return self
+
     def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
         check_field(u.electric)
+        _check_mask(self.phi, u.ny, u.nx, u.electric.device)
         _lib.call("fh_phase_apply", context(u.electric.device), _ptr(u.electric), _ptr(self.phi),
                   u.nx, u.ny, u.batch, dir_sign(direction), None)
         return u
@@ -230,15 +304,17 @@ class Phase:
         return u
 
     def backpropagate_with_gradient_(self, dv: ScalarField, saved: torch.Tensor,
-                                     dphi: Optional[torch.Tensor] = None, direction=Forward):
-        """Returns (du, dphi) with du written over dv (``phasemask.jl:142-150``)."""
+                                     dphi: Optional[torch.Tensor] = None, direction=Forward,
+                                     accumulate: bool = False):
+        """Returns (du, dphi) with du written over dv (``phasemask.jl:142-150``);
+        ``accumulate`` adds to dphi instead of overwriting it."""
         dphi = self.grad if dphi is None else dphi
         if dphi is None:
             dphi = torch.empty_like(self.phi)
         check_field(dv.electric)
         check_field(saved, "saved buffer")
         _lib.call("fh_phase_backward", context(dv.electric.device), _ptr(dv.electric), _ptr(self.phi),
-                  _ptr(saved), _ptr(dphi), dv.nx, dv.ny, dv.batch, dir_sign(direction), 0)
+                  _ptr(saved), _ptr(dphi), dv.nx, dv.ny, dv.batch, dir_sign(direction), int(accumulate))
         return dv, dphi
 
 
@@ -264,7 +340,7 @@ def propagate_phase_backward_(dv: ScalarField, phase: Phase, prop: _KernelPropag
 
 # --------------------------------------------------------------------------- AS_BPM
 
-class AS_BPM:
+class AS_BPM(_PipeComponent):
     """Split-step beam propagation through a graded-index volume (``bpm.jl:130-142``).
 
     ``dn`` has shape (nslices, ny, nx) (the reference's ``dn[nx, ny, nslices]``);

@@ -49,7 +49,8 @@ class ScalarField:
     ``float(np.float32(lam))`` to reproduce that exactly.
     """
 
-    def __init__(self, electric, ds: Sequence[float], lambdas: Union[float, Sequence[float]]):
+    def __init__(self, electric, ds: Sequence[float], lambdas: Union[float, Sequence[float]],
+                 tilts=(0.0, 0.0)):
         if not isinstance(electric, torch.Tensor):
             electric = torch.as_tensor(np.asarray(electric))
         if electric.dim() not in (2, 3):
@@ -62,6 +63,15 @@ class ScalarField:
         if not np.all(lam > 0):
             raise ValueError("wavelengths must be positive")
         self.lambdas = float(lam[0]) if lam.size == 1 and np.ndim(lambdas) == 0 else [float(v) for v in lam]
+        if len(tilts) != 2:
+            raise ValueError("tilts must be (theta_x, theta_y)")
+        tl = []
+        for t in tilts:
+            a = np.atleast_1d(np.asarray(t, dtype=np.float64))
+            if a.size not in (1, self.batch):
+                raise ValueError("each tilt must be a scalar or one value per field")
+            tl.append(float(a[0]) if np.ndim(t) == 0 else [float(v) for v in a])
+        self.tilts = tuple(tl)
 
     # -- shape helpers
     @property
@@ -83,12 +93,28 @@ class ScalarField:
     def lambda_min(self) -> float:
         return min(self.lambda_list())
 
+    def tilt_lists(self):
+        """(theta_x per field, theta_y per field) in radians (``Fields.jl:168-180``)."""
+        out = []
+        for t in self.tilts:
+            lst = t if isinstance(t, list) else [t]
+            out.append(lst if len(lst) == self.batch else lst * self.batch)
+        return tuple(out)
+
+    def is_on_axis(self) -> bool:
+        return all(v == 0.0 for lst in self.tilt_lists() for v in lst)
+
     def copy(self) -> "ScalarField":
-        return ScalarField(self.electric.clone(), self.ds, self.lambdas)
+        return ScalarField(self.electric.clone(), self.ds, self.lambdas, self.tilts)
 
     def set_field_data(self, data) -> "ScalarField":
-        """New field over ``data`` with the same grid and wavelengths (``Fields.jl:213-215``)."""
-        return ScalarField(data, self.ds, self.lambdas)
+        """New field over ``data`` with the same grid, wavelengths and tilts
+        (``Fields.jl:213-215``)."""
+        return ScalarField(data, self.ds, self.lambdas, self.tilts)
+
+    def set_field_tilts(self, tilts) -> "ScalarField":
+        """New field over the same data with other tilts (``Fields.jl:220-230``)."""
+        return ScalarField(self.electric, self.ds, self.lambdas, tilts)
 
     def power(self) -> torch.Tensor:
         """Per-field power sum|u|^2 dx dy, shape (B,) (``Fields.jl:843-846``)."""
@@ -103,7 +129,8 @@ class ScalarField:
 
     def __repr__(self):
         return (f"ScalarField(shape={tuple(self.electric.shape)}, dtype={self.electric.dtype}, "
-                f"device={self.electric.device}, ds={self.ds}, lambdas={self.lambdas})")
+                f"device={self.electric.device}, ds={self.ds}, lambdas={self.lambdas}, "
+                f"tilts={self.tilts})")
 
 
 def coupling_efficiency(u: ScalarField, v: ScalarField) -> torch.Tensor:

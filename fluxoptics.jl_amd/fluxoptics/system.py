@@ -1,0 +1,274 @@
+"""Sources, sequences, systems and the gradient driver.
+
+Mirrors ``src/optical_components/sources/scalar_source.jl`` (``ScalarSource``),
+``src/optical_components/system/optical_sequence.jl`` (``OpticalSequence``: forward loop,
+reverse loop with the reversed direction, :54-66), ``optical_system.jl`` (``OpticalSystem``,
+the pipe ``|>`` written ``|`` here, ``system()`` returning ``(out, probes)``, :126-214) and
+``merge_rules.jl`` (static Phase + Phase adds the masks; consecutive Fourier operators of
+opposite directions cancel, which for two on-axis angular-spectrum propagators in one medium
+leaves one propagator over the summed distance).
+
+The reference differentiates a system with Zygote through the component rrules
+(``ChainRulesCoreExt.jl:21-111``): the forward runs ``propagate_and_save!`` on trainable
+components and the pullback runs ``backpropagate_with_gradient!`` in reverse order.
+``withgradient`` does exactly that sequence with the device kernels, and fuses each
+``Phase`` that precedes a propagator into the propagator's first pass (forward) and last
+pass (adjoint + mask gradient).
+"""
+
+from __future__ import annotations
+
+from typing import Callable, Dict, List, Optional, Sequence
+
+import torch
+
+from .components import AS_BPM, ASProp, Phase, RSProp, _KernelPropagator, check_field
+from .components import phase_propagate_, propagate_phase_backward_
+from .fields import Backward, Forward, ScalarField, dir_sign
+
+
+class ScalarSource:
+    """``ScalarSource(u; trainable=false, buffered=false)``: reproduces a stored copy of ``u``."""
+
+    def __init__(self, u: ScalarField, *, trainable: bool = False, buffered: bool = False):
+        check_field(u.electric)
+        self.u0 = u.copy()
+        self.trainable = trainable
+        self.grad = torch.empty_like(u.electric) if (trainable and buffered) else None
+
+    def propagate(self) -> ScalarField:
+        return self.u0.copy()
+
+    def backpropagate_with_gradient(self, dv: ScalarField) -> torch.Tensor:
+        g = self.grad if self.grad is not None else torch.empty_like(dv.electric)
+        g.copy_(dv.electric)
+        return g
+
+    def __or__(self, other):
+        return OpticalSystem(self, *_components_of(other))
+
+
+def _components_of(x) -> list:
+    if isinstance(x, OpticalSystem):
+        if x.source is not None:
+            raise ValueError("cannot pipe into a system that has a source")
+        return list(x.components)
+    if isinstance(x, OpticalSequence):
+        return list(x.optical_components)
+    return [x]
+
+
+def _is_static_phase(c) -> bool:
+    return isinstance(c, Phase) and not c.trainable
+
+
+def _merge_pair(a, b):
+    """``merge_rules.jl``: returns the merged component or None."""
+    if _is_static_phase(a) and _is_static_phase(b) and a.phi.shape == b.phi.shape:
+        return Phase._from_tensor(a.phi + b.phi)
+    if (type(a) is ASProp and type(b) is ASProp and a.n0 == b.n0 and a.ds == b.ds
+            and (a.nx, a.ny) == (b.nx, b.ny) and not a.paraxial and not b.paraxial
+            and not a.track_tilts and not b.track_tilts
+            and not isinstance(a.z, list) and not isinstance(b.z, list)
+            and (a.z >= 0) == (b.z >= 0)):
+        # H(z1) H(z2) = H(z1 + z2) for distances of one sign (evanescent terms decay with |z|)
+        return ASProp._from_params(a.nx, a.ny, a.ds, a.z + b.z, a.n0)
+    return None
+
+
+def merge(components: Sequence) -> list:
+    out: list = []
+    for c in components:
+        while out:
+            m = _merge_pair(out[-1], c)
+            if m is None:
+                break
+            out.pop()
+            c = m
+        out.append(c)
+    return out
+
+
+class OpticalSequence:
+    """``OpticalSequence(components...)``: forward loop / reverse loop with the reversed
+    direction (``optical_sequence.jl:54-66``)."""
+
+    def __init__(self, *components):
+        self.optical_components = tuple(components)
+
+    def __len__(self):
+        return len(self.optical_components)
+
+    @property
+    def trainable(self) -> bool:
+        return any(getattr(c, "trainable", False) for c in self.optical_components)
+
+    def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
+        comps = self.optical_components if dir_sign(direction) > 0 else self.optical_components[::-1]
+        for c in comps:
+            u = c.propagate_(u, direction)
+        return u
+
+    def propagate(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self.propagate_(u.copy(), direction)
+
+    def backpropagate_(self, dv: ScalarField, direction=Forward) -> ScalarField:
+        return self.propagate_(dv, Backward if dir_sign(direction) > 0 else Forward)
+
+
+class SystemOutput:
+    def __init__(self, out: ScalarField, probes: Optional[Dict] = None):
+        self.out = out
+        self.probes = probes or {}
+
+
+class OpticalSystem:
+    """``OpticalSystem(source, components...; merge_components=false, inplace=false)``.
+
+    ``source | c1 | c2`` builds it; ``system | {"merge_components": True}`` rebuilds it with
+    keywords; ``system()`` (or ``system(x)`` without a source) returns ``.out``.
+    """
+
+    def __init__(self, source, *components, inplace: bool = False, direction=Forward,
+                 merge_components: bool = False):
+        if source is not None and not isinstance(source, ScalarSource):
+            components = (source,) + components
+            source = None
+        self.source = source
+        self.input_components = tuple(components)
+        self.components = tuple(merge(components)) if merge_components else tuple(components)
+        self.inplace = inplace
+        self.direction = direction
+        self.merge_components = merge_components
+
+    def __len__(self):
+        return len(self.components)
+
+    def __or__(self, other):
+        if isinstance(other, dict):
+            kw = dict(inplace=self.inplace, direction=self.direction,
+                      merge_components=self.merge_components)
+            kw.update(other)
+            return OpticalSystem(self.source, *self.input_components, **kw)
+        return OpticalSystem(self.source, *self.input_components, *_components_of(other),
+                             inplace=self.inplace, direction=self.direction,
+                             merge_components=self.merge_components)
+
+    def get_source(self):
+        return self.source
+
+    def get_components(self):
+        return self.components
+
+    # -- forward
+    def _start(self, x: Optional[ScalarField]) -> ScalarField:
+        if (x is None) == (self.source is None):
+            raise ValueError("give an input field exactly when the system has no source")
+        if x is None:
+            return self.source.propagate()
+        return x if self.inplace else x.copy()
+
+    def __call__(self, x: Optional[ScalarField] = None) -> SystemOutput:
+        u = self._start(x)
+        _run_forward(list(self.components), u, self.direction, save=False)
+        return SystemOutput(u)
+
+    # -- gradient
+    def withgradient(self, loss, x: Optional[ScalarField] = None):
+        """Loss value and gradients of every trainable component.
+
+        ``loss`` is a metric with ``value_and_pullback`` (e.g. ``SquaredIntensityDifference``);
+        the objective is the sum of its values.  Returns ``(value, grads)`` where ``value``
+        is a float64 device scalar and ``grads`` maps each trainable component (and a
+        trainable source) to its gradient tensor.
+        """
+        u = self._start(x)
+        comps = list(self.components)
+        tape = _run_forward(comps, u, self.direction, save=True)
+        losses, (du,) = loss.value_and_pullback((u,), inplace=True)
+        value = sum(l.sum() for l in losses)
+        grads: Dict = {}
+        du = _run_backward(comps, tape, du, self.direction, grads)
+        if self.source is not None and self.source.trainable:
+            grads[self.source] = self.source.backpropagate_with_gradient(du)
+        return value, grads
+
+
+# --------------------------------------------------------------------------- executor
+
+def _is_prop(c) -> bool:
+    return isinstance(c, _KernelPropagator)
+
+
+def _run_forward(comps: List, u: ScalarField, direction, save: bool) -> list:
+    """Run the components in order; returns the tape of (kind, components, saved buffer).
+    A Phase followed by a propagator runs fused with it."""
+    tape = []
+    seq = comps if dir_sign(direction) > 0 else comps[::-1]
+    i = 0
+    while i < len(seq):
+        c = seq[i]
+        if isinstance(c, Phase) and i + 1 < len(seq) and _is_prop(seq[i + 1]):
+            p = seq[i + 1]
+            saved = None
+            if save and c.trainable:
+                saved = c.saved if (c.saved is not None and not any(t[2] is c.saved for t in tape)) \
+                    else torch.empty_like(u.electric)
+            phase_propagate_(u, c, p, saved, direction)
+            tape.append(("phase+prop", (c, p), saved))
+            i += 2
+            continue
+        if save and isinstance(c, Phase) and c.trainable:
+            saved = c.saved if (c.saved is not None and not any(t[2] is c.saved for t in tape)) \
+                else torch.empty_like(u.electric)
+            c.propagate_and_save_(u, saved, direction)
+            tape.append(("phase", (c,), saved))
+        elif save and isinstance(c, AS_BPM) and c.trainable:
+            if any(t[0] == "bpm" and t[1][0] is c for t in tape):
+                raise NotImplementedError("a trainable AS_BPM may appear once in a differentiated system "
+                                          "(it keeps one set of checkpoints)")
+            c.propagate_and_save_(u, direction)
+            tape.append(("bpm", (c,), None))
+        else:
+            c.propagate_(u, direction)
+            tape.append(("plain", (c,), None))
+        i += 1
+    return tape
+
+
+def _grad_buffer(grads: Dict, c, like: torch.Tensor):
+    """(buffer, accumulate) for a trainable component's gradient."""
+    if c in grads:
+        return grads[c], True
+    g = c.grad if getattr(c, "grad", None) is not None else torch.empty_like(like)
+    grads[c] = g
+    return g, False
+
+
+def _run_backward(comps: List, tape: list, du: ScalarField, direction, grads: Dict) -> ScalarField:
+    for kind, cs, saved in reversed(tape):
+        if kind == "phase+prop":
+            ph, p = cs
+            if ph.trainable:
+                g, acc = _grad_buffer(grads, ph, ph.phi)
+                propagate_phase_backward_(du, ph, p, saved, g, direction, acc)
+            else:
+                propagate_phase_backward_(du, ph, p, None, None, direction)
+        elif kind == "phase":
+            (ph,) = cs
+            g, acc = _grad_buffer(grads, ph, ph.phi)
+            ph.backpropagate_with_gradient_(du, saved, g, direction, accumulate=acc)
+        elif kind == "bpm":
+            (m,) = cs
+            if m in grads:
+                tmp = torch.empty_like(m.dn)
+                m.backpropagate_with_gradient_(du, tmp, direction)
+                grads[m] += tmp
+            else:
+                g = m.grad if m.grad is not None else torch.empty_like(m.dn)
+                m.backpropagate_with_gradient_(du, g, direction)
+                grads[m] = g
+        else:
+            (c,) = cs
+            c.backpropagate_(du, direction)
+    return du

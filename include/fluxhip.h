@@ -68,6 +68,37 @@ int fh_asprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double d
 int fh_rsprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
                      const double* lambdas, int nlambdas, double z, fh_plan** out);
 
+/* Plan descriptor for fields with per-field distances and tilts, and paraxial kernels.
+ * lambdas: 1 or batch values; z: 1 or batch values (one transfer function per distinct
+ * (lambda, z, tilt) in the batch); tilts_x/tilts_y: 0, 1 or batch angles in radians, the
+ * field's Fourier offset f0 = sin(theta)/lambda' (Fields.jl:34-37,168-180; a ComplexF32
+ * field stores them as Float32, pass Float64(Float32(theta)) for parity).  n0 is used by
+ * ASProp only. */
+#define FH_PROP_PARAXIAL 1    /* ASProp(...; paraxial=true): as_paraxial_kernel */
+#define FH_PROP_TRACK_TILTS 2 /* track_tilts=true */
+typedef struct fh_prop_desc {
+  int nx, ny, batch;
+  double dx, dy;
+  const double* lambdas;
+  int nlambdas;
+  const double* z;
+  int nz;
+  const double* tilts_x;
+  const double* tilts_y;
+  int ntilts;
+  double n0;
+  int flags;
+} fh_prop_desc;
+
+/* ASProp with the reference's kernel choice (angular_spectrum.jl:126-139): paraxial
+ * (as_paraxial_kernel, :27-35) when FH_PROP_PARAXIAL is set, else on-axis (as_kernel) when
+ * every tilt is zero, else tilted (tilted_as_kernel, :45-62), with per-field z. */
+int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, fh_plan** out);
+
+/* RSProp with per-field z and tilted fields (rs_tilted_kernel, rayleigh_sommerfeld.jl:14-27,
+ * selected at :90-96 when a tilt is nonzero).  flags: FH_PROP_TRACK_TILTS only. */
+int fh_rsprop_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, fh_plan** out);
+
 int fh_plan_destroy(fh_plan* plan);
 
 /* propagate!(u, p, direction): in place on a c64 [nx, ny, batch] device array
@@ -115,6 +146,16 @@ int fh_propagate_phase_backward(fh_plan* plan, void* dv, const float* phi, const
  * (c128 [nx, ny, batch]), per field, unfused (five launches per field: x FFT, y FFT,
  * multiply by a full FP64 transfer-function table, y IFFT, x IFFT). */
 int fh_propagate_plain_fp64(fh_plan* plan, void* u_c128, int direction);
+
+/* SquaredIntensityDifference (squared_intensity_difference.jl:78-90, compute_metric and
+ * backpropagate_metric).  target holds target_batch FP32 intensity planes [nx, ny]:
+ *   target_batch == batch ("full dimensions"): loss[b] = sum (|u_b|^2 - I_b)^2, b < batch;
+ *   target_batch == 1 < batch ("spatial only"): loss[0] = sum (sum_b |u_b|^2 - I)^2.
+ * loss is a device array of doubles (batch or 1 values), summed in FP64 in a fixed order.
+ * When du is not NULL the pullback du_b = 4 dc[slot] d u_b is written there (du may equal u);
+ * dc is a device array with one cotangent per loss value, or NULL for ones. */
+int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx, int ny, int batch,
+                         int target_batch, const double* dc, double* loss, void* du);
 
 /* AS_BPM: split-step beam propagation through nslices slices of index perturbation dn
  * (AS_BPM(u, thickness, n0, dn), bpm.jl:130-142 and :8-74): dz = thickness/nslices,

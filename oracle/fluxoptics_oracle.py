@@ -110,31 +110,80 @@ def as_kernel(fx, fy, lam: float, n0: float, z: float):
     return np.exp(1j * (TWO_PI * z) * np.sqrt(arg))
 
 
-def as_transfer(nx: int, ny: int, dx: float, dy: float, lam: float, n0: float, z: float):
+def as_paraxial_kernel(fx, fy, lam: float, tx: float, ty: float, track: bool, n0: float, z: float):
+    """Paraxial (Fresnel) transfer function, complex FP64.
+
+    ``angular_spectrum.jl:27-35``: lam' = lam/n0, f0 = sin(theta)/lam',
+    H = cis(-pi*lam'*z*((fx+f0x)^2 + (fy+f0y)^2) - offset + 2*pi*z/lam') with
+    offset = 2*(f0x*fx + f0y*fy) when tilts are tracked, else 0 (for either sign of z).
+    """
+    le = lam / n0
+    f0x, f0y = math.sin(tx) / le, math.sin(ty) / le
+    fx, fy = np.asarray(fx, dtype=np.float64), np.asarray(fy, dtype=np.float64)
+    off = 2.0 * (f0x * fx + f0y * fy) if track else 0.0
+    return np.exp(1j * (-math.pi * le * z * ((fx + f0x) ** 2 + (fy + f0y) ** 2) - off + TWO_PI * z / le))
+
+
+def tilted_as_kernel(fx, fy, lam: float, tx: float, ty: float, track: bool, n0: float, z: float):
+    """Angular-spectrum transfer function of a tilted field, complex FP64.
+
+    ``angular_spectrum.jl:45-62``: f0 = sin(theta)/lam', H evaluated at (fx+f0x, fy+f0y),
+    plus the phase 2*pi*z*(f0x*fx + f0y*fy)*lam' when tilts are tracked; z < 0 gives
+    conj(H(-z)).
+    """
+    if z < 0:
+        return np.conj(tilted_as_kernel(fx, fy, lam, tx, ty, track, n0, -z))
+    le = lam / n0
+    f0x, f0y = math.sin(tx) / le, math.sin(ty) / le
+    fx, fy = np.asarray(fx, dtype=np.float64), np.asarray(fy, dtype=np.float64)
+    off = (f0x * fx + f0y * fy) * le if track else 0.0
+    arg = np.asarray(complex(1.0 / le ** 2) - (fx + f0x) ** 2 - (fy + f0y) ** 2, dtype=np.complex128)
+    return np.exp(1j * (TWO_PI * z) * (np.sqrt(arg) + off))
+
+
+def as_transfer(nx: int, ny: int, dx: float, dy: float, lam: float, n0: float, z: float,
+                tilts=(0.0, 0.0), paraxial: bool = False, track: bool = False):
     """H sampled on the FFT-order frequency grid, shape (ny, nx).
 
     ``fill_kernel_cache`` broadcasting ``as_kernel`` over ``f_vec = (x=fx, y=fy')``:
-    ``abstract_kernel.jl:15-25``, ``fourier_kernel.jl:17-18``.
+    ``abstract_kernel.jl:15-25``, ``fourier_kernel.jl:17-18``; the kernel choice of
+    ``angular_spectrum.jl:126-139`` (paraxial, else on-axis when both tilts are zero, else
+    tilted).
     """
     fx = fftfreq(nx, dx)[None, :]
     fy = fftfreq(ny, dy)[:, None]
-    return as_kernel(fx, fy, lam, n0, z)
+    if paraxial:
+        return as_paraxial_kernel(fx, fy, lam, tilts[0], tilts[1], track, n0, z)
+    if tilts[0] == 0 and tilts[1] == 0:
+        return as_kernel(fx, fy, lam, n0, z)
+    return tilted_as_kernel(fx, fy, lam, tilts[0], tilts[1], track, n0, z)
 
 
-def asprop(u, ds, lams, z: float, n0: float = 1.0, direction: int = 1):
+def _per_field(v, b: int):
+    a = np.atleast_1d(np.asarray(v, dtype=np.float64))
+    return np.broadcast_to(a, (b,)) if a.size == 1 else a
+
+
+def asprop(u, ds, lams, z, n0: float = 1.0, direction: int = 1, tilts=None, paraxial: bool = False,
+           track: bool = False):
     """``ASProp`` propagation: FFT2 -> multiply by H (or conj H) per slice -> IFFT2.
 
     Sequence (ft, ASKernelProp, ift): ``angular_spectrum.jl:193-229``; per-slice kernel
     multiply ``abstract_kernel.jl:64-83``; Backward = reverse direction
-    ``freespace.jl:61-63`` + ``fourier_operator.jl:65-79``.
+    ``freespace.jl:61-63`` + ``fourier_operator.jl:65-79``.  ``z`` and each tilt angle
+    may be one value or one per field; ``tilts`` = (theta_x, theta_y) in radians.
     """
     u = _as_batch(np.asarray(u, dtype=np.complex128))
     b, ny, nx = u.shape
     dx, dy = ds
     lam_b = _lambdas(lams, b)
+    z_b = _per_field(z, b)
+    tx_b = _per_field(0.0 if tilts is None else tilts[0], b)
+    ty_b = _per_field(0.0 if tilts is None else tilts[1], b)
     v = ft2(u)
     for k in range(b):
-        v[k] *= conj_direction(as_transfer(nx, ny, dx, dy, lam_b[k], n0, z), direction)
+        h = as_transfer(nx, ny, dx, dy, lam_b[k], n0, z_b[k], (tx_b[k], ty_b[k]), paraxial, track)
+        v[k] *= conj_direction(h, direction)
     return ift2(v)
 
 
@@ -151,6 +200,19 @@ def rs_kernel(x, y, lam: float, z: float, nrm_f: float):
     k = TWO_PI / lam
     r = np.sqrt(np.asarray(x) ** 2 + np.asarray(y) ** 2 + z * z)
     return nrm_f * (np.exp(1j * k * r) / r) * (z / r) * (1.0 / r - 1j * k)
+
+
+def rs_tilted_kernel(x, y, lam: float, tx: float, ty: float, track: bool, z: float, nrm_f: float):
+    """RS impulse response of a tilted field (``rayleigh_sommerfeld.jl:14-27``): h times
+    cis(-2*pi*(x*f0x + y*f0y)), f0 = sin(theta)/lam, unless tilts are tracked; z < 0 gives
+    conj(h(-z))."""
+    if z < 0:
+        return np.conj(rs_tilted_kernel(x, y, lam, tx, ty, track, -z, nrm_f))
+    h = rs_kernel(x, y, lam, z, nrm_f)
+    if track:
+        return h
+    f0x, f0y = math.sin(tx) / lam, math.sin(ty) / lam
+    return h * np.exp(-1j * TWO_PI * (np.asarray(x) * f0x + np.asarray(y) * f0y))
 
 
 def rs_offsets(n: int, dx: float, pad: str = "2n-1") -> np.ndarray:
@@ -171,15 +233,19 @@ def rs_offsets(n: int, dx: float, pad: str = "2n-1") -> np.ndarray:
 
 
 def rs_transfer(nx: int, ny: int, dx: float, dy: float, lam: float, z: float,
-                pad: str = "2n-1"):
+                pad: str = "2n-1", tilts=(0.0, 0.0), track: bool = False):
     """FFT2 of the sampled RS kernel on the padded grid (``convolution_kernel.jl:47-55``).
 
     nrm_f = dx*dy/(2*pi) (``rayleigh_sommerfeld.jl:55``).  The reference rounds h to the
-    field precision before an FP32 FFT; the oracle keeps everything in FP64.
+    field precision before an FP32 FFT; the oracle keeps everything in FP64.  Nonzero
+    tilts select the tilted kernel (``rayleigh_sommerfeld.jl:90-96``).
     """
     sx = rs_offsets(nx, dx, pad)[None, :]
     sy = rs_offsets(ny, dy, pad)[:, None]
-    h = rs_kernel(sx, sy, lam, z, dx * dy / TWO_PI)
+    if tilts[0] == 0 and tilts[1] == 0:
+        h = rs_kernel(sx, sy, lam, z, dx * dy / TWO_PI)
+    else:
+        h = rs_tilted_kernel(sx, sy, lam, tilts[0], tilts[1], track, z, dx * dy / TWO_PI)
     return ft2(h)
 
 
@@ -196,21 +262,26 @@ def crop(u: np.ndarray, ny: int, nx: int) -> np.ndarray:
     return np.ascontiguousarray(u[..., :ny, :nx])
 
 
-def rsprop(u, ds, lams, z: float, direction: int = 1, pad_mode: str = "2n-1"):
+def rsprop(u, ds, lams, z, direction: int = 1, pad_mode: str = "2n-1", tilts=None,
+           track: bool = False):
     """``RSProp``: pad -> FFT2 -> multiply by FFT2(h) (or its conjugate) -> IFFT2 -> crop.
 
     Sequence (pad, ft, RSKernelProp, ift, crop): ``rayleigh_sommerfeld.jl:135-156``;
     Backward reverses the sequence and conjugates the kernel (``freespace.jl:61-63``,
-    ``pad_crop_operators.jl:338-356``).
+    ``pad_crop_operators.jl:338-356``).  ``z`` and the tilts may be per field.
     """
     u = _as_batch(np.asarray(u, dtype=np.complex128))
     b, ny, nx = u.shape
     dx, dy = ds
     lam_b = _lambdas(lams, b)
+    z_b = _per_field(z, b)
+    tx_b = _per_field(0.0 if tilts is None else tilts[0], b)
+    ty_b = _per_field(0.0 if tilts is None else tilts[1], b)
     py, px = (2 * ny - 1, 2 * nx - 1) if pad_mode == "2n-1" else (2 * ny, 2 * nx)
     v = ft2(pad(u, (py, px)))
     for k in range(b):
-        v[k] *= conj_direction(rs_transfer(nx, ny, dx, dy, lam_b[k], z, pad_mode), direction)
+        h = rs_transfer(nx, ny, dx, dy, lam_b[k], z_b[k], pad_mode, (tx_b[k], ty_b[k]), track)
+        v[k] *= conj_direction(h, direction)
     return crop(ift2(v), ny, nx)
 
 
@@ -325,14 +396,33 @@ def normalize_power(u, ds, target=1.0) -> np.ndarray:
     return u * np.sqrt(target / power(u, ds))[:, None, None]
 
 
-def sq_intensity_diff(v, target_intensity):
-    """Sum (|v|^2 - I)^2 and its cotangent 4(|v|^2 - I) v.
+def sq_intensity_diff_values(v, target_intensity, dc=None):
+    """Per-slot values and cotangent of ``SquaredIntensityDifference``.
 
-    ``src/metrics/squared_intensity_difference.jl:78-90`` (compute_metric and
-    backpropagate_metric with dc = 1).
+    ``src/metrics/squared_intensity_difference.jl:60-90``: a target with the field's
+    full dims gives one value per field, sum (|v_b|^2 - I_b)^2; a target with the spatial
+    dims only (and more than one field) gives one value, sum (sum_b |v_b|^2 - I)^2.  The
+    cotangent is 4 dc d v_b with d the difference of the field's slot.
     """
-    d = np.abs(v) ** 2 - target_intensity
-    return float((d * d).sum()), 4.0 * d * v
+    v = _as_batch(np.asarray(v, dtype=np.complex128))
+    t = np.asarray(target_intensity, dtype=np.float64)
+    spatial = t.ndim == 2 and v.shape[0] > 1
+    if spatial:
+        d = (np.abs(v) ** 2).sum(axis=0) - t
+        vals = np.array([(d * d).sum()])
+        g = 4.0 * (1.0 if dc is None else float(np.ravel(dc)[0]))
+        return vals, g * d[None] * v
+    d = np.abs(v) ** 2 - t.reshape(v.shape)
+    vals = (d * d).sum(axis=(-2, -1))
+    g = 4.0 * (np.ones(v.shape[0]) if dc is None else np.asarray(dc, dtype=np.float64).reshape(-1))
+    return vals, g[:, None, None] * d * v
+
+
+def sq_intensity_diff(v, target_intensity):
+    """Objective sum of the ``SquaredIntensityDifference`` values and its cotangent
+    (``backpropagate_metric`` with dc = 1)."""
+    vals, dv = sq_intensity_diff_values(v, target_intensity)
+    return float(vals.sum()), dv.reshape(np.shape(v))
 
 
 def half_sq_field_diff(v, t):

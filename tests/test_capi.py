@@ -62,3 +62,22 @@ def test_no_cpu_fallback_in_host_layer():
     p = fo.ASProp(u, 10.0)
     with pytest.raises(TypeError):
         p.propagate_(u)
+
+
+def test_prop_desc_layout_matches_header(tmp_path):
+    """The ctypes mirror of fh_prop_desc has the C compiler's size and field offsets."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no host C compiler")
+    names = [f[0] for f in _lib.PropDesc._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "fluxhip.h"\nint main(void) {\n'
+                   '  printf("%zu", sizeof(fh_prop_desc));\n'
+                   + "".join(f'  printf(" %zu", offsetof(fh_prop_desc, {n}));\n' for n in names)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(_lib.PropDesc)] + [getattr(_lib.PropDesc, n).offset for n in names]
+    assert got == want

@@ -1,0 +1,151 @@
+"""GPU parity: the SquaredIntensityDifference kernel, and whole systems differentiated on the
+device (the beam-splitter pipeline of RSProps and Phases, and the three-wavelength Phase/ASProp
+cascade) against the FP64 oracle.  Tolerances: 1e-5 fields and losses, 1e-4 gradients."""
+
+import math
+import warnings
+
+import numpy as np
+import pytest
+import torch
+
+import fluxoptics as fo
+from fluxoptics import inputs
+from oracle import fluxoptics_oracle as O
+
+from _util import c64, rel_l2, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL, GRAD_TOL = 1e-5, 1e-4
+
+
+def test_sqint_reference_doctest_values_on_device():
+    """``squared_intensity_difference.jl:19-45`` doctests through the device kernel."""
+    u = fo.ScalarField(to_dev(np.ones((2, 4, 4))), (1.0, 1.0), 1.064)
+    m = fo.SquaredIntensityDifference((u, np.zeros((2, 4, 4))))
+    (vals,) = m(u)
+    assert vals.cpu().tolist() == [16.0, 16.0]
+    u3 = fo.ScalarField(to_dev(np.ones((3, 4, 4))), (1.0, 1.0), 1.064)
+    m3 = fo.SquaredIntensityDifference((u3, 3 * np.ones((4, 4))))
+    (vals,) = m3(u3)
+    assert vals.cpu().tolist() == [0.0]
+
+
+@pytest.mark.parametrize("spatial", [False, True])
+def test_sqint_values_and_pullback(spatial):
+    b, ny, nx = 3, 192, 256
+    print(f"[sqint] spatial={spatial}", flush=True)
+    v0 = c64(inputs.normal(5, (b, ny, nx)) + 1j * inputs.normal(6, (b, ny, nx)))
+    t = inputs.uniform(7, (ny, nx) if spatial else (b, ny, nx), 0.0, 3.0).astype(np.float32)
+    dc = np.array([0.7]) if spatial else np.array([0.5, 1.0, 2.0])
+    u = fo.ScalarField(to_dev(v0), (1.0, 1.0), 0.8)
+    m = fo.SquaredIntensityDifference((u, t))
+    vals_ref, dv_ref = O.sq_intensity_diff_values(v0, t.astype(np.float64), dc)
+    (vals,) = m(u)
+    assert np.allclose(vals.cpu().numpy(), vals_ref, rtol=1e-12, atol=0)
+    (du,) = m.backpropagate_metric([u], [torch.from_numpy(dc)])
+    assert rel_l2(to_host(du.electric), dv_ref) <= FIELD_TOL
+    # repeated calls give identical sums (fixed reduction order)
+    (vals2,) = m(u)
+    assert torch.equal(vals, vals2)
+
+
+def _two_gaussians(n, ds, w0, shift):
+    t = inputs.gaussian(n, n, ds, ds, w0, -shift, 0.0) + inputs.gaussian(n, n, ds, ds, w0, shift, 0.0)
+    return inputs.normalize_power(t, ds, ds)
+
+
+@pytest.mark.parametrize("masks", ["zeros", "random"])
+def test_beam_splitter_pipeline_gradient(masks):
+    """Gaussian source (waist 30 um) on 512x512, ds 1 um, lambda 1.064 um, power-normalised;
+    RSProp(1500) -> Phase -> RSProp(2000) -> Phase -> RSProp(1500); loss sum of squared
+    intensity differences against two Gaussians at +-60 um; gradients of both masks.
+    The oracle pads to 2n-1 as the reference does; the device pads to 2n."""
+    n, ds, lam = 512, 1.0, 1.064
+    print(f"[beam splitter] masks={masks}", flush=True)
+    u0 = c64(inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 30.0), ds, ds))
+    target_i = (np.abs(_two_gaussians(n, ds, 30.0, 60.0)) ** 2).astype(np.float32)
+    if masks == "zeros":
+        phi1 = phi2 = np.zeros((n, n), np.float32)
+    else:
+        phi1 = inputs.uniform(41, (n, n), 0, 2 * math.pi).astype(np.float32)
+        phi2 = inputs.uniform(42, (n, n), 0, 2 * math.pi).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), (ds, ds), lam)
+    doe1 = fo.Phase(u, phi1, trainable=True, buffered=True)
+    doe2 = fo.Phase(u, phi2, trainable=True, buffered=True)
+    prop1, prop2 = fo.RSProp(u, 1500.0), fo.RSProp(u, 2000.0)
+    system = fo.ScalarSource(u) | prop1 | doe1 | prop2 | doe2 | prop1
+    metric = fo.SquaredIntensityDifference((u, target_i))
+
+    stages = [("rs", 1500.0), ("phase", phi1.astype(np.float64)), ("rs", 2000.0),
+              ("phase", phi2.astype(np.float64)), ("rs", 1500.0)]
+    v_ref, saved = O.pipeline_forward(u0, stages, (ds, ds), lam)
+    l_ref, dv_ref = O.sq_intensity_diff(v_ref, target_i.astype(np.float64))
+    _, (g1_ref, g2_ref) = O.pipeline_backward(dv_ref, stages, (ds, ds), lam, saved)
+
+    out = system().out
+    assert rel_l2(to_host(out.electric), v_ref) <= FIELD_TOL
+    value, grads = system.withgradient(metric)
+    assert abs(float(value) - l_ref) <= FIELD_TOL * abs(l_ref)
+    torch.cuda.synchronize()
+    assert set(grads) == {doe1, doe2}
+    assert grads[doe1] is doe1.grad and grads[doe2] is doe2.grad
+    assert rel_l2(doe1.grad.cpu().numpy(), g1_ref) <= GRAD_TOL
+    assert rel_l2(doe2.grad.cpu().numpy(), g2_ref) <= GRAD_TOL
+
+
+def test_rgb_cascade_gradients():
+    """1024x1024, ds 1 um, lambda in {0.640, 0.538, 0.455} um batched as one field of three
+    slices, Gaussian source (waist 100 um); Phase -> ASProp(1e4) -> Phase -> ASProp(1e4) ->
+    Phase with masks U[0, 2pi) (seeds 31..33); full-dims intensity loss against three
+    Gaussians at (-200, 0), (0, 0), (200, 0); all three mask gradients (summed over lambda)."""
+    n, ds, lams, z = 1024, 1.0, [0.640, 0.538, 0.455], 1e4
+    print("[rgb cascade] 1024^2 x 3", flush=True)
+    g = inputs.gaussian(n, n, ds, ds, 100.0)
+    u0 = c64(inputs.normalize_power(np.stack([g, g, g]), ds, ds))
+    tgt = np.stack([np.abs(inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 100.0, c, 0.0), ds, ds)) ** 2
+                    for c in (-200.0, 0.0, 200.0)]).astype(np.float32)
+    phis = [inputs.uniform(31 + k, (n, n), 0, 2 * math.pi).astype(np.float32) for k in range(3)]
+    u = fo.ScalarField(to_dev(u0), (ds, ds), lams)
+    does = [fo.Phase(u, p, trainable=True, buffered=True) for p in phis]
+    prop = fo.ASProp(u, z)
+    system = fo.ScalarSource(u) | does[0] | prop | does[1] | prop | does[2]
+    metric = fo.SquaredIntensityDifference((u, tgt))
+
+    stages = [("phase", phis[0].astype(np.float64)), ("as", z), ("phase", phis[1].astype(np.float64)),
+              ("as", z), ("phase", phis[2].astype(np.float64))]
+    v_ref, saved = O.pipeline_forward(u0, stages, (ds, ds), lams)
+    vals_ref, dv_ref = O.sq_intensity_diff_values(v_ref, tgt.astype(np.float64))
+    _, g_ref = O.pipeline_backward(dv_ref, stages, (ds, ds), lams, saved)
+
+    value, grads = system.withgradient(metric)
+    assert abs(float(value) - vals_ref.sum()) <= FIELD_TOL * vals_ref.sum()
+    torch.cuda.synchronize()
+    for k in range(2):
+        assert rel_l2(does[k].grad.cpu().numpy(), g_ref[k]) <= GRAD_TOL, k
+    # the last mask does not change any intensity: its gradient is zero up to rounding
+    # (the oracle's is ~1e-27), so it is checked against the scale of the others
+    scale = np.linalg.norm(g_ref[0])
+    assert np.linalg.norm(g_ref[2]) <= 1e-12 * scale
+    assert np.linalg.norm(does[2].grad.cpu().numpy()) <= GRAD_TOL * scale
+
+
+def test_merged_system_matches_unmerged():
+    """``merge_components=true``: Phase+Phase and ASProp+ASProp merge without changing the
+    output (beyond FP32 rounding)."""
+    n, ds, lam = 256, 1.0, 0.8
+    u0 = c64(inputs.gaussian(n, n, ds, ds, 30.0) * np.exp(1j * inputs.uniform(9, (n, n), 0, 1)))
+    u = fo.ScalarField(to_dev(u0), (ds, ds), lam)
+    a = fo.Phase(u, inputs.uniform(10, (n, n), 0, 1).astype(np.float32))
+    b = fo.Phase(u, inputs.uniform(11, (n, n), 0, 1).astype(np.float32))
+    p, q = fo.ASProp(u, 300.0), fo.ASProp(u, 700.0)
+    s = a | b | p | q
+    sm = s | {"merge_components": True}
+    assert len(sm.components) == 2
+    out = to_host(s(u).out.electric)
+    outm = to_host(sm(u).out.electric)
+    assert rel_l2(outm, out) <= 2 * FIELD_TOL
+    ref = O.asprop(O.phase_apply(u0, (a.phi + b.phi).cpu().numpy().astype(np.float64), 1), (ds, ds), lam,
+                   1000.0, 1.0, 1)
+    assert rel_l2(outm, ref) <= FIELD_TOL

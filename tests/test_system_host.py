@@ -1,0 +1,81 @@
+"""CPU tests: the oracle's SquaredIntensityDifference against the reference's own doctest
+values and finite differences, and the host-side system logic (pipe, merge rules)."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import fluxoptics as fo
+from fluxoptics import inputs
+from oracle import fluxoptics_oracle as O
+
+
+def test_sqint_reference_doctest_values():
+    """``src/metrics/squared_intensity_difference.jl:19-45``: ones(4,4,2) against zeros gives
+    16 per field; ones(4,4,3) against 3*ones(4,4) (spatial only) gives 0."""
+    vals, _ = O.sq_intensity_diff_values(np.ones((2, 4, 4), complex), np.zeros((2, 4, 4)))
+    assert np.array_equal(vals, [16.0, 16.0])
+    vals, dv = O.sq_intensity_diff_values(np.ones((3, 4, 4), complex), 3 * np.ones((4, 4)))
+    assert np.array_equal(vals, [0.0]) and np.all(dv == 0)
+
+
+@pytest.mark.parametrize("spatial", [False, True])
+def test_sqint_cotangent_by_finite_differences(spatial):
+    """The cotangent dL/dRe + i dL/dIm of the summed metric, by central differences."""
+    b, n = 3, 6
+    v = inputs.normal(1, (b, n, n)) + 1j * inputs.normal(2, (b, n, n))
+    t = inputs.uniform(3, (n, n) if spatial else (b, n, n), 0.0, 2.0)
+    dc = np.array([0.7]) if spatial else np.array([0.5, 1.0, 2.0])
+    _, dv = O.sq_intensity_diff_values(v, t, dc)
+    f = lambda w: float((dc * O.sq_intensity_diff_values(w, t)[0]).sum())
+    h = 1e-6
+    for k, (bb, i, j) in enumerate([(0, 1, 2), (1, 4, 4), (2, 0, 5), (1, 3, 0)]):
+        e = np.zeros_like(v)
+        e[bb, i, j] = h
+        g_re = (f(v + e) - f(v - e)) / (2 * h)
+        g_im = (f(v + 1j * e) - f(v - 1j * e)) / (2 * h)
+        assert abs(g_re - dv[bb, i, j].real) <= 1e-6 * (1 + abs(g_re))
+        assert abs(g_im - dv[bb, i, j].imag) <= 1e-6 * (1 + abs(g_im))
+
+
+def _cpu_field(n=16, batch=1):
+    return fo.ScalarField(torch.zeros((batch, n, n), dtype=torch.complex64), (1.0, 1.0), 0.8)
+
+
+def test_merge_rules():
+    """``merge_rules.jl``: static Phase + Phase adds masks, trainable ones stay, two
+    on-axis ASProps in one medium become one over the summed distance, RSProps stay."""
+    u = _cpu_field()
+    a = np.arange(256, dtype=np.float32).reshape(16, 16)
+    p1, p2 = fo.Phase(u, a), fo.Phase(u, 2 * a)
+    m = fo.merge([p1, p2])
+    assert len(m) == 1 and torch.equal(m[0].phi, torch.from_numpy(3 * a))
+    pt = fo.Phase(u, a, trainable=True)
+    assert len(fo.merge([p1, pt])) == 2
+    m = fo.merge([fo.ASProp(u, 100.0), fo.ASProp(u, 250.0), fo.ASProp(u, 1.0, n0=1.5)])
+    assert len(m) == 2 and m[0].z == 350.0 and m[1].n0 == 1.5
+    with pytest.warns(UserWarning):
+        r = fo.RSProp(u, 1.0)
+    assert len(fo.merge([r, r])) == 2
+
+
+def test_pipe_builds_systems():
+    u = _cpu_field()
+    src = fo.ScalarSource.__new__(fo.ScalarSource)  # a source without device data
+    src.trainable = False
+    p, q = fo.ASProp(u, 10.0), fo.ASProp(u, 20.0)
+    ph = fo.Phase(u, None)
+    s = src | p | ph | q
+    assert s.source is src and s.components == (p, ph, q)
+    s2 = s | {"merge_components": True}
+    assert s2.merge_components and len(s2.components) == 3
+    s3 = src | p | q | {"merge_components": True}
+    assert len(s3.components) == 1 and s3.components[0].z == 30.0
+    chain = p | ph
+    assert chain.source is None and chain.components == (p, ph)
+    with pytest.raises(ValueError):
+        s(u)  # a system with a source takes no input field
+    with pytest.raises(ValueError):
+        chain()  # a system without a source needs one
