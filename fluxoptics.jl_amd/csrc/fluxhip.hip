@@ -5,6 +5,7 @@
 // functions are tables computed in FP64 on the device and rounded once to c64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1601,6 +1602,177 @@ int fh_bpm_backward(fh_bpm* m, void* du, const float* dn, float* ddn, int direct
     if ((rc = bpm_ypass(m, f, k > 0 ? m->tfull : m->thalf, -direction))) return rc;
   }
   return bpm_xifft(m, f);
+}
+
+}  // extern "C"
+
+// ================================================================ propagation groups
+// A group runs several plans (fields of different sizes, wavelengths and distances) as one
+// hipGraph: the plans' launch chains are captured on parallel streams (a fork/join), so
+// small fields fill the GPU next to large ones and the whole batch costs one graph launch.
+// Graphs are cached per (direction, field pointers).
+
+struct fh_group {
+  fh_ctx* ctx = nullptr;
+  std::vector<fh_plan*> plans;
+  hipStream_t cap = nullptr;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> ev;  // ev[0]: fork; ev[1 + s]: join of side stream s
+  struct Graph {
+    int direction;
+    std::vector<void*> fields;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+};
+
+namespace fh {
+
+static size_t plan_field_bytes(const fh_plan* p) { return (size_t)p->nx * p->ny * p->batch * sizeof(float2); }
+
+static void group_free(fh_group* g) {
+  if (!g) return;
+  for (auto& gr : g->graphs) {
+    (void)hipGraphExecDestroy(gr.exec);
+    (void)hipGraphDestroy(gr.graph);
+  }
+  for (auto e : g->ev) (void)hipEventDestroy(e);
+  for (auto s : g->side) (void)hipStreamDestroy(s);
+  if (g->cap) (void)hipStreamDestroy(g->cap);
+  delete g;
+}
+
+// Capture every plan's propagation of fields[i] (plan i on side stream i mod nside).
+static int group_capture(fh_group* g, void* const* fields, int direction, fh_group::Graph& out) {
+  fh_ctx* c = g->ctx;
+  hipStream_t user = c->stream;
+  int rc = FH_OK;
+  FH_HIP(hipStreamBeginCapture(g->cap, hipStreamCaptureModeRelaxed));
+  hipError_t e = hipEventRecord(g->ev[0], g->cap);
+  for (size_t s = 0; e == hipSuccess && s < g->side.size(); ++s) e = hipStreamWaitEvent(g->side[s], g->ev[0], 0);
+  for (size_t i = 0; e == hipSuccess && rc == FH_OK && i < g->plans.size(); ++i) {
+    c->stream = g->side[i % g->side.size()];
+    float2* f = reinterpret_cast<float2*>(fields[i]);
+    rc = run_prop(g->plans[i], f, direction, NoOp{});
+    if (rc == FH_OK) rc = run_xifft(g->plans[i], f);
+  }
+  c->stream = user;
+  for (size_t s = 0; e == hipSuccess && s < g->side.size(); ++s) {
+    e = hipEventRecord(g->ev[1 + s], g->side[s]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(g->cap, g->ev[1 + s], 0);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ee = hipStreamEndCapture(g->cap, &graph);
+  if (rc != FH_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess || ee != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(FH_E_HIP, "graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : ee));
+  }
+  hipGraphExec_t exec = nullptr;
+  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return fail(FH_E_HIP, "graph instantiation failed");
+  }
+  out.direction = direction;
+  out.fields.assign(fields, fields + g->plans.size());
+  out.graph = graph;
+  out.exec = exec;
+  return FH_OK;
+}
+
+}  // namespace fh
+
+extern "C" {
+
+int fh_group_create(fh_ctx* ctx, int nplans, fh_plan* const* plans, int autotune_reps, fh_group** out) {
+  if (!out) return fail(FH_E_ARG, "null output pointer");
+  *out = nullptr;
+  if (!ctx || !plans || nplans < 1) return fail(FH_E_ARG, "need a context and at least one plan");
+  if (autotune_reps < 0) return fail(FH_E_ARG, "autotune_reps must be >= 0");
+  for (int i = 0; i < nplans; ++i) {
+    if (!plans[i]) return fail(FH_E_ARG, "plan %d is null", i);
+    if (plans[i]->ctx != ctx) return fail(FH_E_ARG, "plan %d belongs to another context", i);
+  }
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  fh_group* g = new fh_group();
+  g->ctx = ctx;
+  g->plans.assign(plans, plans + nplans);
+  const int nside = nplans < 4 ? nplans : 4;
+  // streams and events at the default priority
+  if (hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking) != hipSuccess) { group_free(g); return fail(FH_E_HIP, "stream creation failed"); }
+  for (int s = 0; s < nside; ++s) {
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) { group_free(g); return fail(FH_E_HIP, "stream creation failed"); }
+    g->side.push_back(st);
+  }
+  for (int k = 0; k < 1 + nside; ++k) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { group_free(g); return fail(FH_E_HIP, "event creation failed"); }
+    g->ev.push_back(e);
+  }
+  // One eager propagation per plan on a scratch field fills every lazily built table
+  // (twiddles, stage tables, LDS attributes) so that capture records kernels only; with
+  // autotune_reps > 0 each plan's y pass is also tuned on that scratch field.
+  size_t big = 0;
+  for (auto* p : g->plans) big = std::max(big, plan_field_bytes(p));
+  void* tmp = nullptr;
+  if (hipMalloc(&tmp, big) != hipSuccess) { group_free(g); return fail(FH_E_NOMEM, "group warm-up field: out of device memory"); }
+  rc = FH_OK;
+  if (hipMemsetAsync(tmp, 0, big, ctx->stream) != hipSuccess) rc = fail(FH_E_HIP, "warm-up clear failed");
+  for (auto* p : g->plans) {
+    if (rc == FH_OK && autotune_reps > 0) rc = fh_autotune(p, tmp, autotune_reps, nullptr, 0);
+    if (rc == FH_OK) rc = fh_propagate(p, tmp, 1);
+    if (rc == FH_OK) rc = fh_propagate(p, tmp, -1);
+  }
+  if (rc == FH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(FH_E_HIP, "group warm-up failed");
+  (void)hipFree(tmp);
+  if (rc) { group_free(g); return rc; }
+  *out = g;
+  return FH_OK;
+}
+
+int fh_group_propagate(fh_group* g, void* const* fields, int direction) {
+  if (!g || !fields) return fail(FH_E_ARG, "null group or field list");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(g->ctx))) return rc;
+  for (size_t i = 0; i < g->plans.size(); ++i)
+    if (!fields[i]) return fail(FH_E_ARG, "field %zu is null", i);
+  fh_group::Graph* gr = nullptr;
+  for (auto& x : g->graphs)
+    if (x.direction == direction && std::equal(x.fields.begin(), x.fields.end(), fields)) { gr = &x; break; }
+  if (!gr) {
+    if (g->graphs.size() >= 8) {  // bounded cache: drop the oldest graph once it has finished
+      FH_HIP(hipStreamSynchronize(g->ctx->stream));
+      (void)hipGraphExecDestroy(g->graphs.front().exec);
+      (void)hipGraphDestroy(g->graphs.front().graph);
+      g->graphs.erase(g->graphs.begin());
+    }
+    fh_group::Graph ng{};
+    if ((rc = group_capture(g, fields, direction, ng))) return rc;
+    g->graphs.push_back(ng);
+    gr = &g->graphs.back();
+  }
+  FH_HIP(hipGraphLaunch(gr->exec, g->ctx->stream));
+  return FH_OK;
+}
+
+int fh_group_graph_count(fh_group* g, int* count) {
+  if (!g || !count) return fail(FH_E_ARG, "null group or output");
+  *count = (int)g->graphs.size();
+  return FH_OK;
+}
+
+int fh_group_destroy(fh_group* g) {
+  if (!g) return FH_OK;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  group_free(g);
+  return FH_OK;
 }
 
 }  // extern "C"

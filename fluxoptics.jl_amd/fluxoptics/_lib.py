@@ -48,6 +48,10 @@ SIGNATURES = {
                                  c_int, c_double, ctypes.POINTER(c_void_p)]),
     "fh_asprop_create_ex": (c_int, [c_void_p, ctypes.POINTER(PropDesc), ctypes.POINTER(c_void_p)]),
     "fh_rsprop_create_ex": (c_int, [c_void_p, ctypes.POINTER(PropDesc), ctypes.POINTER(c_void_p)]),
+    "fh_group_create": (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_void_p)]),
+    "fh_group_propagate": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_int]),
+    "fh_group_graph_count": (c_int, [c_void_p, ctypes.POINTER(c_int)]),
+    "fh_group_destroy": (c_int, [c_void_p]),
     "fh_plan_destroy": (c_int, [c_void_p]),
     "fh_propagate": (c_int, [c_void_p, c_void_p, c_int]),
     "fh_propagate_pass": (c_int, [c_void_p, c_void_p, c_int, c_int]),

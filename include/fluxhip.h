@@ -34,6 +34,7 @@ extern "C" {
 typedef struct fh_ctx fh_ctx;
 typedef struct fh_plan fh_plan;
 typedef struct fh_bpm fh_bpm;
+typedef struct fh_group fh_group;
 
 /* Library version (major*10000 + minor*100 + patch). */
 int fh_version(void);
@@ -182,6 +183,18 @@ int fh_bpm_forward(fh_bpm* bpm, void* u, const float* dn, int direction, int sav
  * in place; if ddn != NULL it receives the dn gradient [nslices][ny][nx], summed over
  * the batch (requires a saving forward in the same direction with the same dn). */
 int fh_bpm_backward(fh_bpm* bpm, void* du, const float* dn, float* ddn, int direction);
+
+/* Propagation groups (no counterpart in the reference, which launches per field): many
+ * plans (fields of uneven sizes, wavelengths, distances) run as one hipGraph whose plan
+ * chains are captured on parallel streams.  fh_group_create warms every plan up on a
+ * scratch field (and, with autotune_reps > 0, tunes each plan's y pass there); all plans
+ * must share ctx.  fh_group_propagate(g, fields, direction) propagates fields[i] with
+ * plans[i] in place, launching the graph on the context's stream; graphs are captured on
+ * first use and cached per (direction, field pointers), at most 8. */
+int fh_group_create(fh_ctx* ctx, int nplans, fh_plan* const* plans, int autotune_reps, fh_group** out);
+int fh_group_propagate(fh_group* group, void* const* fields, int direction);
+int fh_group_graph_count(fh_group* group, int* count);
+int fh_group_destroy(fh_group* group);
 
 #ifdef __cplusplus
 }
