@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparator")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample's wall time")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the secondary workloads (beam splitter, RGB cascade, AS_BPM volume, "
+                         "uneven batch)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 512 MiB device copy (the reference kernel that calibrates "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE into bytes) and exit")
@@ -118,6 +121,132 @@ def cpu_baseline(u0, dv0, v_dev0, du_dev0, budget_s):
                        f"engine scipy.fft pocketfft workers={cores}"},
             {"field_rel_l2": rel(v_dev0, v_ref), "adjoint_rel_l2": rel(du_dev0, du_ref),
              "oracle": "FP64 numpy restatement, field 0"})
+
+
+def _timed_host(fn, reps, torch):
+    """Average ms of fn() over reps (host-driven sequences; synchronised on both sides)."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def workload_beam_splitter(torch, fo, inputs):
+    """512x512 Gaussian (w 30 um), ds 1, lambda 1.064: RSProp(1500) -> Phase -> RSProp(2000)
+    -> Phase -> RSProp(1500), intensity loss against two Gaussians at +-60 um, both mask
+    gradients.  One iteration = value + gradients."""
+    import numpy as np
+    n, ds, lam = 512, 1.0, 1.064
+    g0 = inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 30.0), ds, ds)
+    t = inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 30.0, -60.0, 0.0)
+                               + inputs.gaussian(n, n, ds, ds, 30.0, 60.0, 0.0), ds, ds)
+    u = fo.ScalarField(torch.from_numpy(g0.astype(np.complex64)).cuda(), (ds, ds), lam)
+    d1 = fo.Phase(u, np.zeros((n, n), np.float32), trainable=True, buffered=True)
+    d2 = fo.Phase(u, np.zeros((n, n), np.float32), trainable=True, buffered=True)
+    p1, p2 = fo.RSProp(u, 1500.0), fo.RSProp(u, 2000.0)
+    system = fo.ScalarSource(u) | p1 | d1 | p2 | d2 | p1
+    metric = fo.SquaredIntensityDifference((u, (np.abs(t) ** 2).astype(np.float32)))
+    system.withgradient(metric)
+    ms = _timed_host(lambda: system.withgradient(metric), 20, torch)
+    return {"ms_per_iter": ms, "iters_per_s": 1e3 / ms,
+            "what": "512^2 RS->Phase->RS->Phase->RS, loss + 2 mask gradients, RS padded to 1024^2"}
+
+
+def workload_rgb_cascade(torch, fo, inputs):
+    """1024x1024 x 3 wavelengths, Phase -> ASProp(1e4) -> Phase -> ASProp(1e4) -> Phase,
+    intensity loss, three mask gradients."""
+    import numpy as np
+    n, ds, lams, z = 1024, 1.0, [0.640, 0.538, 0.455], 1e4
+    g = inputs.gaussian(n, n, ds, ds, 100.0)
+    u0 = inputs.normalize_power(np.stack([g, g, g]), ds, ds).astype(np.complex64)
+    tgt = np.stack([np.abs(inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 100.0, c, 0.0), ds, ds)) ** 2
+                    for c in (-200.0, 0.0, 200.0)]).astype(np.float32)
+    u = fo.ScalarField(torch.from_numpy(u0).cuda(), (ds, ds), lams)
+    does = [fo.Phase(u, inputs.uniform(31 + k, (n, n), 0, 2 * math.pi).astype(np.float32),
+                     trainable=True, buffered=True) for k in range(3)]
+    prop = fo.ASProp(u, z)
+    system = fo.ScalarSource(u) | does[0] | prop | does[1] | prop | does[2]
+    metric = fo.SquaredIntensityDifference((u, tgt))
+    system.withgradient(metric)
+    ms = _timed_host(lambda: system.withgradient(metric), 20, torch)
+    return {"ms_per_iter": ms, "iters_per_s": 1e3 / ms,
+            "what": "1024^2 x 3 lambda Phase/ASProp cascade, loss + 3 mask gradients"}
+
+
+def workload_bpm(torch, fo, inputs):
+    """AS_BPM 512x512, ds 0.5, lambda 1.0, n 1.5, 400 slices of 1 um, checkpoint every 20:
+    forward with checkpoints + adjoint with the dn gradient, and the peak device footprint."""
+    import numpy as np
+    n, ns, ds, lam = 512, 400, 0.5, 1.0
+    dn = (0.01 * inputs.uniform(4, (ns, n, n), -1.0, 1.0)).astype(np.float32)
+    x = inputs.grid(n, ds)[None, :]
+    u0 = (inputs.gaussian(n, n, ds, ds, 80.0) * np.exp(2j * math.pi * math.sin(math.radians(2.0)) * x / lam))
+    u = fo.ScalarField(torch.from_numpy(u0.astype(np.complex64)).cuda(), (ds, ds), lam)
+    bpm = fo.AS_BPM(u, 400.0, 1.5, torch.from_numpy(dn).cuda(), trainable=True, buffered=True,
+                    checkpoint_every=20)
+    w = u.copy()
+
+    def it():
+        w.electric.copy_(u.electric)
+        bpm.propagate_and_save_(w)
+        bpm.backpropagate_with_gradient_(w)
+
+    it()
+    ms = _timed_host(it, 3, torch)
+    ws = bpm.workspace_bytes(u)
+    caller = 2 * dn.nbytes + 2 * u0.size * 8
+    return {"ms_per_iter": ms, "slices_per_s": 2 * ns * 1e3 / ms,
+            "library_workspace_bytes": ws, "peak_device_bytes": ws + caller,
+            "reference_saved_stack_bytes": ns * n * n * 8,
+            "what": "512^2 x 400 slices, checkpoint every 20, forward + dn adjoint "
+                    "(dn synthetic uniform noise of peak 0.01)"}
+
+
+def workload_uneven_batch(torch, fo, inputs):
+    """64 fields, 16 each of 256^2, 512^2, 1024^2, 2048^2, random LG modes, ds 1, lambda
+    1.064, RSProp (2n padding) with z ~ U[500, 5000]; forward + adjoint of all fields as one
+    graph per direction, against 64 separate per-field launches."""
+    import warnings
+    import numpy as np
+    sizes, per, ds, lam = (256, 512, 1024, 2048), 16, 1.0, 1.064
+    pairs, singles = [], []
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        for k, n in enumerate(sizes):
+            ps = inputs.uniform(500 + 10 * k, (per,), 0, 4).astype(int)
+            ls = inputs.uniform(501 + 10 * k, (per,), -4, 5).astype(int)
+            zs = list(inputs.uniform(505 + 10 * k, (per,), 500.0, 5000.0))
+            u0 = np.stack([inputs.laguerre_gaussian(n, n, ds, ds, 0.1 * n * ds, int(p), int(l))
+                           for p, l in zip(ps, ls)]).astype(np.complex64)
+            u = fo.ScalarField(torch.from_numpy(u0).cuda(), (ds, ds), lam)
+            pairs.append((fo.RSProp(u, zs), u))
+            for b in range(per):
+                ub = fo.ScalarField(u.electric[b].clone(), (ds, ds), lam)
+                singles.append((fo.RSProp(ub, zs[b]), ub))
+    g = fo.PropagationGroup(pairs, autotune_reps=3)
+
+    def graph_step():
+        g.propagate_(fo.Forward)
+        g.propagate_(fo.Backward)
+
+    def separate_step():
+        for p, ub in singles:
+            p.propagate_(ub, fo.Forward)
+        for p, ub in singles:
+            p.propagate_(ub, fo.Backward)
+
+    graph_step()
+    separate_step()
+    gms = _timed_host(graph_step, 10, torch)
+    sms = _timed_host(separate_step, 5, torch)
+    alg = sum(32.0 * n * n * per for n in sizes)
+    return {"graph_ms_per_step": gms, "separate_ms_per_step": sms, "speedup": sms / gms,
+            "units_per_s": len(singles) * 1e3 / gms, "algorithmic_bytes_per_step": alg,
+            "hbm_roofline_frac": alg / (gms * 1e-3) / HBM_PEAK,
+            "what": "64 RS fields (16 each 256^2..2048^2, 2n padding, z~U[500,5000]), fwd+adj as 2 graph "
+                    "launches vs 128 separate per-field propagations"}
 
 
 def main():
@@ -206,6 +335,16 @@ def main():
         out["plain_fp64"] = {"value": BATCH / (pms * 1e-3), "unit": "propagations/s",
                              "ms_per_step": pms, "speedup_of_fused": (BATCH / (ms * 1e-3)) / (BATCH / (pms * 1e-3))}
         del u128, g128
+
+    if not args.no_workloads:
+        from fluxoptics import inputs as inp
+        out["workloads"] = {
+            "beam_splitter_512": workload_beam_splitter(torch, fo, inp),
+            "rgb_cascade_1024x3": workload_rgb_cascade(torch, fo, inp),
+            "as_bpm_512x400": workload_bpm(torch, fo, inp),
+            "uneven_batch_64": workload_uneven_batch(torch, fo, inp),
+        }
+        torch.cuda.empty_cache()
 
     if not args.no_cpu_baseline:
         cb, err = cpu_baseline(u_np, dv0, v0, du0, args.cpu_seconds)

@@ -61,7 +61,7 @@ k_bpm_xstep(LineIO<float2> io, const C* __restrict__ tw, DnPhaseOp ph, float2* _
   const int b = blockIdx.y;
   const int li = blockIdx.x * L + l;
   const bool live = li < io.nlines;
-  C* s = smem + l * LdsLine<n>::size;
+  C* s = smem + l * LdsLine<n, L, sizeof(C)>::size;
   const long roff = b * io.out_f + (long)li * io.out_l;
   float2* row = io.out + roff;
   C v[V];
@@ -119,7 +119,7 @@ k_bpm_xstep_grad(LineIO<float2> io, int batch, const C* __restrict__ tw, DnPhase
   thread_map<L, P, false>(threadIdx.x, l, t);
   const int li = blockIdx.x * L + l;
   const bool live = li < io.nlines;
-  C* s = smem + l * LdsLine<n>::size;
+  C* s = smem + l * LdsLine<n, L, sizeof(C)>::size;
   const int nx = ph.nx;
   R g[V];
 #pragma unroll

@@ -165,7 +165,11 @@ template <typename RD> struct LastRadix {
 // Line FFT (forward or inverse), c64 global -> global, with a time-side op (prologue of a
 // forward transform, epilogue of an inverse one).  IN_HALF: only the first n/2 points of
 // each line are read (the rest are zero); OUT_HALF: only the first n/2 are stored.
-template <typename RDF, int L, bool LFAST, bool INV, bool IN_HALF, bool OUT_HALF, typename TimeOp>
+// LANES: the workgroup's L lines sit on adjacent lanes (a wave instruction then covers whole
+// 128-byte lines of the 4x4-blocked layout, and 4 rows x 128 bytes of the natural one);
+// otherwise a wave covers 64 consecutive positions of one line.
+template <typename RDF, int L, bool LFAST, bool INV, bool IN_HALF, bool OUT_HALF, typename TimeOp,
+          bool LANES = LFAST>
 __global__ void __launch_bounds__(L * RDF::P)
 k_fast_fft(FastIO io, const float2* __restrict__ tab, TimeOp op) {
   using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
@@ -174,11 +178,11 @@ k_fast_fft(FastIO io, const float2* __restrict__ tab, TimeOp op) {
   extern __shared__ unsigned char smem_raw[];
   float2* smem = reinterpret_cast<float2*>(smem_raw);
   int l, t;
-  thread_map<L, P, LFAST>(threadIdx.x, l, t);
+  thread_map<L, P, LANES>(threadIdx.x, l, t);
   const int b = blockIdx.y;
   const int li = blockIdx.x * L + l;
   const bool live = li < io.nlines;
-  float2* s = smem + l * LdsLine<n>::size;
+  float2* s = smem + l * LdsLine<n, L>::size;
   const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
   const int vin = fast_off<LFAST>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
   const int kin = fast_k<LFAST>(io.in_l, io.in_e, io.in_blk) * 8;
@@ -232,7 +236,7 @@ k_fast_conv(FastIO io, const float2* __restrict__ tabf, const float2* __restrict
   const int tile = LFAST ? xcd_group_tile(blockIdx.x, gridDim.x, G) : (int)blockIdx.x;
   const int li = tile * L + l;
   const bool live = li < io.nlines;
-  float2* s = smem + l * LdsLine<n>::size;
+  float2* s = smem + l * LdsLine<n, L>::size;
   const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
   const int vin = fast_off<LFAST>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
   const int kin = fast_k<LFAST>(io.in_l, io.in_e, io.in_blk) * 8;
@@ -354,7 +358,7 @@ k_fast_yconv(FastIO io, int batch, int fpw, const float2* __restrict__ tabf,
   thread_map<L, P, !LANES_T>(threadIdx.x, l, t);
   const int li = blockIdx.x * L + l;
   const bool live = li < io.nlines;
-  float2* s = smem + l * LdsLine<n>::size;
+  float2* s = smem + l * LdsLine<n, L>::size;
   const int vin = fast_off<true>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
   const int kin = fast_k<true>(io.in_l, io.in_e, io.in_blk) * 8;
   const int vout = fast_off<true>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
@@ -508,9 +512,10 @@ k_fast_ypipe(FastIO io, int batch, const float2* __restrict__ tabf, const float2
   float2* smem = reinterpret_cast<float2*>(smem_raw);
   int l, t;
   thread_map<L, P, true>(threadIdx.x, l, t);
-  const int li = blockIdx.x * L + l;
+  // fewer than 4 columns per workgroup: the tiles sharing a 128-byte line run on one XCD
+  const int li = (L < 4 ? xcd_group_tile(blockIdx.x, gridDim.x, 4 / L) : (int)blockIdx.x) * L + l;
   const bool live = li < io.nlines;
-  float2* s = smem + l * LdsLine<n>::size;
+  float2* s = smem + l * LdsLine<n, L>::size;
   const int vin = fast_off<true>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
   const int kin = fast_k<true>(io.in_l, io.in_e, io.in_blk) * 8;
   const int vout = fast_off<true>(li, t, io.out_l, io.out_e, io.out_blk) * 8;

@@ -33,10 +33,16 @@ template <int A, int B, int C_> struct Radices {
 // LDS slot of line position pos: one pad slot every 16 complex values spreads
 // power-of-two strides over the 32 banks.
 __device__ __forceinline__ int lds_pos(int pos) { return pos + (pos >> 4); }
-// Lines are 8 slots (16 dwords) past a multiple of 16 slots apart, so when adjacent lanes
-// work on adjacent lines (strided-axis passes) their banks differ by 16: at most two lanes
-// of a half-wave meet in a bank.
-template <int n> struct LdsLine { static constexpr int size = ((n + n / 16 + 15) / 16) * 16 + 8; };
+// LDS line stride.  In strided-axis passes the lanes of a wave cover L lines x (64/L)
+// positions with the line index fastest.  LDS serves 16 lanes of 8-byte accesses per cycle
+// over 32 four-byte banks, and a stage's positions of consecutive t are 8 or 16 slots
+// apart, so lines are spaced 16/L slots past a multiple of 16: the L lines of a
+// 16-lane group then fall in distinct banks.  For 16-byte elements the lines stay 8 slots
+// apart.
+template <int n, int L = 4, int ESZ = 8> struct LdsLine {
+  static constexpr int pad = ESZ == 8 ? (L >= 16 ? 1 : 16 / L) : 8;
+  static constexpr int size = ((n + n / 16 + 15) / 16) * 16 + pad;
+};
 
 // twiddle W_n^k (forward) or its conjugate (inverse) from the global table
 template <bool INV, typename C>
@@ -183,7 +189,7 @@ k_line_fft(LineIO<S> io, const C* __restrict__ tw, TimeOp op, int G) {
   const int tile = LFAST ? xcd_group_tile(blockIdx.x, gridDim.x, G) : (int)blockIdx.x;
   const int li = tile * L + l;
   const bool live = li < io.nlines;
-  C* s = smem + l * LdsLine<n>::size;
+  C* s = smem + l * LdsLine<n, L, sizeof(C)>::size;
   const S* src = io.in + b * io.in_f;
   C v[V];
 #pragma unroll
@@ -225,7 +231,7 @@ k_line_conv(LineIO<S> io, const C* __restrict__ tw, FreqOp op, int G) {
   const int tile = LFAST ? xcd_group_tile(blockIdx.x, gridDim.x, G) : (int)blockIdx.x;
   const int li = tile * L + l;
   const bool live = li < io.nlines;
-  C* s = smem + l * LdsLine<n>::size;
+  C* s = smem + l * LdsLine<n, L, sizeof(C)>::size;
   const S* src = io.in + b * io.in_f;
   C v[V];
 #pragma unroll

@@ -111,7 +111,7 @@ static hipError_t launch_fft_t(const LineIO<S>& io, int batch, const C* tw, Op o
   constexpr int L = CF::L;
   static_assert(RD::n == (1 << LOGN), "radix plan does not match the length");
   auto kern = k_line_fft<S, C, RD, L, Y, INV, Op>;
-  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n>::size * sizeof(C) : 0;
+  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n, L, sizeof(C)>::size * sizeof(C) : 0;
   hipError_t e = ensure_lds(kern, lds);
   if (e != hipSuccess) return e;
   dim3 grid((io.nlines + L - 1) / L, batch);
@@ -126,7 +126,7 @@ static hipError_t launch_conv_t(const LineIO<S>& io, int batch, const C* tw, Op 
   using RD = typename CF::RD;
   constexpr int L = CF::L;
   auto kern = k_line_conv<S, C, RD, L, true, Op>;
-  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n>::size * sizeof(C) : 0;
+  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n, L, sizeof(C)>::size * sizeof(C) : 0;
   hipError_t e = ensure_lds(kern, lds);
   if (e != hipSuccess) return e;
   dim3 grid((io.nlines + L - 1) / L, batch);
@@ -450,7 +450,7 @@ k_xifft_phase_grad(LineIO<float2> io, int batch, const float2* __restrict__ tw,
   thread_map<L, P, false>(threadIdx.x, l, t);
   const int li = blockIdx.x * L + l;
   const bool live = li < io.nlines;
-  float2* sl = smem + l * LdsLine<n>::size;
+  float2* sl = smem + l * LdsLine<n, L>::size;
   float g[V];
 #pragma unroll
   for (int k = 0; k < V; ++k) g[k] = 0.f;
@@ -505,7 +505,7 @@ static hipError_t launch_xifft_phase_grad_t(const LineIO<float2>& io, int batch,
   using RD = typename CF::RD;
   constexpr int L = CF::L;
   auto kern = k_xifft_phase_grad<RD, L>;
-  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n>::size * sizeof(float2) : 0;
+  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n, L>::size * sizeof(float2) : 0;
   hipError_t e = ensure_lds(kern, lds);
   if (e != hipSuccess) return e;
   dim3 grid((io.nlines + L - 1) / L, 1);
@@ -564,6 +564,7 @@ struct fh_plan {
   float2* scratch = nullptr;       // RS: (2nx) x ny x batch
   double2* plain_tab = nullptr;    // plain FP64 comparator (AS): full tables
   int conv_var = -1;               // y-pass variant (-1 = built-in default)
+  int xf_var = -1, xi_var = -1;    // x-pass variants (forward, inverse)
 };
 
 namespace fh {
@@ -637,7 +638,7 @@ static int get_stage_tw(fh_ctx* c, int n, int R0, int R1, int R2, bool inv, cons
 // ---------------------------------------------------------------- fast c64 launches
 
 template <int LOGN, bool INV, bool IH, bool OH, typename Op>
-static int fast_x_t(fh_ctx* c, const FastIO& io, int batch, Op op) {
+static int fast_x_t(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
   using CF = Cfg<float, LOGN, false>;
   using RDF = typename CF::RD;
   using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
@@ -645,8 +646,8 @@ static int fast_x_t(fh_ctx* c, const FastIO& io, int batch, Op op) {
   const float2* tab = nullptr;
   int rc = get_stage_tw(c, RD::n, RD::R0, RD::R1, RD::R2, INV, &tab);
   if (rc) return rc;
-  auto kern = k_fast_fft<RDF, L, false, INV, IH, OH, Op>;
-  const size_t lds = (size_t)L * LdsLine<RD::n>::size * sizeof(float2);
+  auto kern = lanes ? k_fast_fft<RDF, L, false, INV, IH, OH, Op, true> : k_fast_fft<RDF, L, false, INV, IH, OH, Op, false>;
+  const size_t lds = (size_t)L * LdsLine<RD::n, L>::size * sizeof(float2);
   FH_HIP(ensure_lds(kern, lds));
   hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L, batch), dim3(L * RDF::P), lds, c->stream, io, tab, op);
   FH_HIP(hipGetLastError());
@@ -669,16 +670,21 @@ static int fast_x_t(fh_ctx* c, const FastIO& io, int batch, Op op) {
 
 // forward x FFT; half_in: rows hold n/2 points (zero-padded RS input)
 template <typename Op>
-static int fast_xfwd(fh_ctx* c, int logn, bool half_in, const FastIO& io, int batch, Op op) {
-  if (half_in) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, true, false, Op>(c, io, batch, op))) }
-  FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, false, false, Op>(c, io, batch, op)))
+static int fast_xfwd(fh_ctx* c, int logn, bool half_in, const FastIO& io, int batch, Op op, bool lanes) {
+  if (half_in) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, true, false, Op>(c, io, batch, op, lanes))) }
+  FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, false, false, Op>(c, io, batch, op, lanes)))
 }
 
 // inverse x FFT; half_out: only n/2 points are stored (RS crop)
-static int fast_xinv(fh_ctx* c, int logn, bool half_out, const FastIO& io, int batch) {
-  if (half_out) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, true, NoOp>(c, io, batch, NoOp{}))) }
-  FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, false, NoOp>(c, io, batch, NoOp{})))
+static int fast_xinv(fh_ctx* c, int logn, bool half_out, const FastIO& io, int batch, bool lanes) {
+  if (half_out) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, true, NoOp>(c, io, batch, NoOp{}, lanes))) }
+  FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, false, NoOp>(c, io, batch, NoOp{}, lanes)))
 }
+
+// x-pass variants: 0 = lanes over positions of a line, 1 = lanes over the workgroup's lines
+static const char* const kXVariants[] = {"lanes over x", "lanes over rows"};
+constexpr int kNumXVariants = 2;
+static int default_x_variant(int logn) { (void)logn; return 0; }
 
 // y-pass variants (radices, columns per workgroup) for the autotuner
 using FastConvLaunch = int (*)(fh_ctx*, const FastIO&, int, QuadMul, bool);
@@ -696,7 +702,7 @@ static int fast_conv_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool 
   if ((rc = get_stage_tw(c, RDF::n, RDF::R0, RDF::R1, RDF::R2, false, &tf)) ||
       (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
     return rc;
-  const size_t lds = (size_t)L * LdsLine<RDF::n>::size * sizeof(float2);
+  const size_t lds = (size_t)L * LdsLine<RDF::n, L>::size * sizeof(float2);
   QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab, op.full,
              op.full ? op.ny / 2 : 0};
   const int fpw = FPW < batch ? FPW : batch;
@@ -729,7 +735,7 @@ static int fast_pipe_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool 
   if ((rc = get_stage_tw(c, RDF::n, RDF::R0, RDF::R1, RDF::R2, false, &tf)) ||
       (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
     return rc;
-  const size_t lds = (size_t)L * LdsLine<RDF::n>::size * sizeof(float2);
+  const size_t lds = (size_t)L * LdsLine<RDF::n, L>::size * sizeof(float2);
   QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab, op.full,
              op.full ? op.ny / 2 : 0};
   dim3 grid((io.nlines + L - 1) / L);
@@ -745,8 +751,7 @@ static int fast_pipe_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool 
   return FH_OK;
 }
 
-// Variant lists per length; the defaults (first entry; for batches that share one transfer
-// function the pipelined kernel when listed second) are the fastest measured on MI355X.
+// Variant lists per length; default_conv_variant picks the fastest measured on MI355X.
 static const FastConvVariant kFast9[] = {{"8x8x8 L4 f1", fast_conv_v<8, 8, 8, 4, 1>},
                                          {"8x8x8 L4", fast_conv_v<8, 8, 8, 4>},
                                          {"16x32 L8", fast_conv_v<16, 32, 1, 8>}};
@@ -759,14 +764,17 @@ static const FastConvVariant kFast11[] = {{"8x16x16 L4 f1", fast_conv_v<8, 16, 1
                                           {"pipe 16x16x8 L4", fast_pipe_v<16, 16, 8, 4>},
                                           {"16x16x8 L4 f1", fast_conv_v<16, 16, 8, 4, 1>},
                                           {"16x16x8 L4", fast_conv_v<16, 16, 8, 4>},
-                                          {"16x16x8 L4 f2", fast_conv_v<16, 16, 8, 4, 2>}};
+                                          {"16x16x8 L4 f2", fast_conv_v<16, 16, 8, 4, 2>},
+                                          {"pipe 16x16x8 L2", fast_pipe_v<16, 16, 8, 2>},
+                                          {"pipe 8x16x16 L2", fast_pipe_v<8, 16, 16, 2>}};
 static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16, 16, 4, 1>},
                                           {"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
                                           {"16x16x16 L2 f1", fast_conv_v<16, 16, 16, 2, 1>}};
 
 // default variant index for a y length and whether the batch shares one transfer function
 static int default_conv_variant(int logn, bool one_tab) {
-  if (one_tab && logn == 11) return 1;
+  if (one_tab && logn == 11) return 2;  // pipe 16x16x8 L4
+  if (logn == 12) return 1;             // 16x16x16 L4
   return 0;
 }
 
@@ -779,7 +787,7 @@ static const FastConvVariant* fast_conv_variants(int logn, int* count) {
     case 8: *count = 2; return kFast8;
     case 9: *count = 3; return kFast9;
     case 10: *count = 4; return kFast10;
-    case 11: *count = 6; return kFast11;
+    case 11: *count = 8; return kFast11;
     case 12: *count = 3; return kFast12;
     default: *count = 0; return nullptr;
   }
@@ -889,7 +897,8 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
     io.in = u; io.out = mid; io.in_f = (long)p->nx * p->ny; io.out_f = midf;
     io.in_l = p->nx; io.out_l = nxl; io.in_e = io.out_e = 1; io.in_blk = 0; io.out_blk = 1;
     io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = midf * 8;
-    if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop))) return rc;
+    const int xv = p->xf_var >= 0 ? p->xf_var : default_x_variant(p->lx);
+    if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop, xv == 1))) return rc;
   }
   if (only == 0) return FH_OK;
   QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, 1 << p->ly, p->qy, direction < 0,
@@ -913,7 +922,8 @@ static int run_xifft(fh_plan* p, float2* u) {
   io.in = mid; io.out = u; io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
   io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1; io.in_blk = 1; io.out_blk = 0;
   io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
-  return fast_xinv(p->ctx, p->lx, rs, io, p->batch);
+  const int xv = p->xi_var >= 0 ? p->xi_var : default_x_variant(p->lx);
+  return fast_xinv(p->ctx, p->lx, rs, io, p->batch, xv == 1);
 }
 
 }  // namespace fh
@@ -1109,20 +1119,22 @@ int fh_propagate_pass(fh_plan* p, void* u, int direction, int pass) {
 int fh_plan_variants(fh_plan* p, int pass, int* count, int* current) {
   if (!p || !count || !current) return fail(FH_E_ARG, "null plan or output");
   if (pass < 0 || pass > 2) return fail(FH_E_ARG, "pass must be 0, 1 or 2, got %d", pass);
-  int n = 0;
+  int n = kNumXVariants;
   if (pass == 1) fast_conv_variants(p->ly, &n);
   *count = n > 0 ? n : 1;
-  *current = (pass == 1 && n > 0) ? (p->conv_var >= 0 ? p->conv_var : -1) : 0;
+  *current = pass == 1 ? p->conv_var : (pass == 0 ? p->xf_var : p->xi_var);
   return FH_OK;
 }
 
 int fh_plan_set_variant(fh_plan* p, int pass, int variant) {
   if (!p) return fail(FH_E_ARG, "null plan");
   if (pass < 0 || pass > 2) return fail(FH_E_ARG, "pass must be 0, 1 or 2, got %d", pass);
-  int n = 0;
+  int n = kNumXVariants;
   if (pass == 1) fast_conv_variants(p->ly, &n);
   if (variant < -1 || variant >= std::max(n, 1)) return fail(FH_E_ARG, "variant %d out of range (%d)", variant, n);
-  if (pass == 1 && n > 0) p->conv_var = variant;
+  if (pass == 1) p->conv_var = variant;
+  else if (pass == 0) p->xf_var = variant;
+  else p->xi_var = variant;
   return FH_OK;
 }
 
@@ -1131,37 +1143,45 @@ int fh_autotune(fh_plan* p, void* u, int reps, char* report, int report_len) {
   if (reps < 1) return fail(FH_E_ARG, "reps must be >= 1");
   int rc;
   if ((rc = set_device(p->ctx))) return rc;
-  int n = 0;
-  const FastConvVariant* vars = fast_conv_variants(p->ly, &n);
-  std::string js = "{\"pass\": 1, \"variants\": [";
-  int best = -1;
-  float best_ms = 0.f;
   float2* f = reinterpret_cast<float2*>(u);
   hipEvent_t e0, e1;
   FH_HIP(hipEventCreate(&e0));
   FH_HIP(hipEventCreate(&e1));
-  for (int v = 0; v < n; ++v) {
-    p->conv_var = v;
-    if ((rc = run_prop(p, f, 1, NoOp{}, 1))) { p->conv_var = -1; return rc; }  // warm-up
-    FH_HIP(hipEventRecord(e0, p->ctx->stream));
-    for (int r = 0; r < reps; ++r)
-      if ((rc = run_prop(p, f, 1, NoOp{}, 1))) { p->conv_var = -1; return rc; }
-    FH_HIP(hipEventRecord(e1, p->ctx->stream));
-    FH_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    FH_HIP(hipEventElapsedTime(&ms, e0, e1));
-    ms /= reps;
-    char buf[160];
-    snprintf(buf, sizeof buf, "%s{\"index\": %d, \"name\": \"%s\", \"ms\": %.6f}", v ? ", " : "", v, vars[v].name, ms);
+  std::string js = "{";
+  static const char* const kPassName[] = {"x_fft", "y_fft_mul_ifft", "x_ifft"};
+  for (int pass = 0; pass < 3 && rc == FH_OK; ++pass) {
+    int n = kNumXVariants;
+    const FastConvVariant* vars = pass == 1 ? fast_conv_variants(p->ly, &n) : nullptr;
+    int* sel = pass == 0 ? &p->xf_var : (pass == 1 ? &p->conv_var : &p->xi_var);
+    auto run = [&]() { return pass == 2 ? run_xifft(p, f) : run_prop(p, f, 1, NoOp{}, pass); };
+    char buf[200];
+    snprintf(buf, sizeof buf, "%s\"%s\": {\"variants\": [", pass ? ", " : "", kPassName[pass]);
     js += buf;
-    if (best < 0 || ms < best_ms) { best = v; best_ms = ms; }
+    int best = -1;
+    float best_ms = 0.f;
+    for (int v = 0; v < n && rc == FH_OK; ++v) {
+      *sel = v;
+      if ((rc = run())) break;  // warm-up
+      if (hipEventRecord(e0, p->ctx->stream) != hipSuccess) { rc = fail(FH_E_HIP, "event record failed"); break; }
+      for (int r = 0; r < reps && rc == FH_OK; ++r) rc = run();
+      if (rc) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, p->ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail(FH_E_HIP, "event timing failed"); break; }
+      ms /= reps;
+      snprintf(buf, sizeof buf, "%s{\"index\": %d, \"name\": \"%s\", \"ms\": %.6f}", v ? ", " : "", v,
+               pass == 1 ? vars[v].name : kXVariants[v], ms);
+      js += buf;
+      if (best < 0 || ms < best_ms) { best = v; best_ms = ms; }
+    }
+    *sel = rc ? -1 : best;
+    snprintf(buf, sizeof buf, "], \"best\": %d}", best);
+    js += buf;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  p->conv_var = best;
-  char buf[96];
-  snprintf(buf, sizeof buf, "], \"best\": %d}", best);
-  js += buf;
+  if (rc) return rc;
+  js += "}";
   if (report && report_len > 0) {
     strncpy(report, js.c_str(), report_len - 1);
     report[report_len - 1] = 0;
@@ -1342,7 +1362,7 @@ static hipError_t launch_bpm_step_t(const LineIO<float2>& io, int batch, const d
   using CF = Cfg<double, LOGN, false>;
   using RD = typename CF::RD;
   constexpr int L = CF::L;
-  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n>::size * sizeof(double2) : 0;
+  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n, L, sizeof(double2)>::size * sizeof(double2) : 0;
   const unsigned threads = L * RD::P;
   if (ddn) {
     auto kern = k_bpm_xstep_grad<double2, RD, L>;

@@ -113,13 +113,14 @@ int fh_propagate(fh_plan* plan, void* u, int direction);
  * cancellation of merge_rules.jl:104-108); used too for per-kernel timing. */
 int fh_propagate_pass(fh_plan* plan, void* u, int direction, int pass);
 
-/* Launch variants of a pass (radix split, lines per workgroup).  count receives the
- * number of variants, current the selected one (-1 = built-in default). */
+/* Launch variants of a pass (pass 1: radix split and lines per workgroup; passes 0 and 2:
+ * lane mapping).  count receives the number of variants, current the selected one
+ * (-1 = built-in default). */
 int fh_plan_variants(fh_plan* plan, int pass, int* count, int* current);
 int fh_plan_set_variant(fh_plan* plan, int pass, int variant);
 
-/* Autotuner: times every variant of the plan's y pass on the device array u (its contents
- * are overwritten), keeps the fastest and writes a JSON summary into report. */
+/* Autotuner: times every variant of each of the plan's three passes on the device array u
+ * (its contents are overwritten), keeps the fastest and writes a JSON summary into report. */
 int fh_autotune(fh_plan* plan, void* u, int reps, char* report, int report_len);
 
 /* Phase then propagation fused: u <- P(u * cis(s*phi)); if saved != NULL the mask
