@@ -44,49 +44,60 @@ dir_sign(::Type{Backward}) = Cint(-1)
 # lambdas as the reference stores them (Float32 for ComplexF32 fields, Fields.jl:28-32), promoted
 lambdas_f64(u::ScalarField) = Float64.(collect(vec(u.lambdas.collection)))
 
-# plan cache keyed by (propagator object, batch, lambdas)
+# fh_prop_desc of include/fluxhip.h (C layout: Julia's struct layout rules match the C ABI)
+struct PropDesc
+    nx::Cint; ny::Cint; batch::Cint
+    dx::Cdouble; dy::Cdouble
+    lambdas::Ptr{Cdouble}; nlambdas::Cint
+    z::Ptr{Cdouble}; nz::Cint
+    tilts_x::Ptr{Cdouble}; tilts_y::Ptr{Cdouble}; ntilts::Cint
+    n0::Cdouble
+    flags::Cint
+end
+const FH_PROP_PARAXIAL = Cint(1)
+const FH_PROP_TRACK_TILTS = Cint(2)
+
+# tilts as the reference stores them (one collection per axis, Fields.jl:34-37), promoted
+tilts_f64(u::ScalarField, batch) = map(c -> (v = Float64.(collect(vec(c))); length(v) == 1 ? fill(v[1], batch) : v),
+                                       u.tilts.collection)
+
+# plan cache keyed by (propagator object, batch, lambdas, tilts)
 const PLANS = IdDict{Any, Dict{Any, PlanPtr}}()
 
-function asprop_plan(p, u::ScalarField{<:ROCArray{ComplexF32}}, z::Float64, n0::Float64)
+function prop_plan(p, u::ScalarField{<:ROCArray{ComplexF32}}, z::Float64, n0::Float64, flags::Cint, fn::Symbol)
     nx, ny = size(u.electric, 1), size(u.electric, 2)
     batch = div(length(u.electric), nx * ny)
     lams = lambdas_f64(u)
+    tx, ty = tilts_f64(u, batch)
     cache = get!(PLANS, p) do; Dict{Any, PlanPtr}() end
-    get!(cache, (batch, lams)) do
+    get!(cache, (batch, lams, tx, ty)) do
+        zs = [z]
         r = Ref{PlanPtr}(C_NULL)
-        check(ccall((:fh_asprop_create, libfluxhip), Cint,
-                    (CtxPtr, Cint, Cint, Cint, Cdouble, Cdouble, Ptr{Cdouble}, Cint, Cdouble, Cdouble, Ptr{PlanPtr}),
-                    context(), nx, ny, batch, u.ds[1], u.ds[2], lams, length(lams), z, n0, r), :fh_asprop_create)
-        r[]
-    end
-end
-
-function rsprop_plan(p, u::ScalarField{<:ROCArray{ComplexF32}}, z::Float64)
-    nx, ny = size(u.electric, 1), size(u.electric, 2)
-    batch = div(length(u.electric), nx * ny)
-    lams = lambdas_f64(u)
-    cache = get!(PLANS, p) do; Dict{Any, PlanPtr}() end
-    get!(cache, (batch, lams)) do
-        r = Ref{PlanPtr}(C_NULL)
-        check(ccall((:fh_rsprop_create, libfluxhip), Cint,
-                    (CtxPtr, Cint, Cint, Cint, Cdouble, Cdouble, Ptr{Cdouble}, Cint, Cdouble, Ptr{PlanPtr}),
-                    context(), nx, ny, batch, u.ds[1], u.ds[2], lams, length(lams), z, r), :fh_rsprop_create)
+        GC.@preserve lams zs tx ty begin
+            d = Ref(PropDesc(nx, ny, batch, u.ds[1], u.ds[2], pointer(lams), length(lams), pointer(zs), 1,
+                             pointer(tx), pointer(ty), batch, n0, flags))
+            check(ccall((fn, libfluxhip), Cint, (CtxPtr, Ptr{PropDesc}, Ptr{PlanPtr}), context(), d, r), fn)
+        end
         r[]
     end
 end
 
 # ASProp / RSProp are AbstractSequences (ft, kernel, ift[, pad/crop]); the whole sequence runs
-# as one libfluxhip propagation.  z and n0 come from the kernel component (angular_spectrum.jl:78-102).
+# as one libfluxhip propagation.  z, n0 and the kernel options come from the kernel component
+# (angular_spectrum.jl:78-102, rayleigh_sommerfeld.jl:38-60).
 function propagate!(u::ScalarField{<:ROCArray{ComplexF32}}, p::ASProp, direction::Type{<:Direction})
     k = p.optical_components[2]
-    plan = asprop_plan(p, u, Float64(k.z), Float64(k.n0))
+    k.filter === nothing || error("libfluxhip: ASProp filters are not on the accelerated path")
+    flags = (k.is_paraxial ? FH_PROP_PARAXIAL : Cint(0)) | (k.track_tilts ? FH_PROP_TRACK_TILTS : Cint(0))
+    plan = prop_plan(p, u, Float64(k.z), Float64(k.n0), flags, :fh_asprop_create_ex)
     check(ccall((:fh_propagate, libfluxhip), Cint, (PlanPtr, Ptr{Cvoid}, Cint), plan, pointer(u.electric), dir_sign(direction)), :fh_propagate)
     u
 end
 
 function propagate!(u::ScalarField{<:ROCArray{ComplexF32}}, p::RSProp, direction::Type{<:Direction})
     k = p.optical_components[3]
-    plan = rsprop_plan(p, u, Float64(k.z))
+    flags = k.track_tilts ? FH_PROP_TRACK_TILTS : Cint(0)
+    plan = prop_plan(p, u, Float64(k.z), 1.0, flags, :fh_rsprop_create_ex)
     check(ccall((:fh_propagate, libfluxhip), Cint, (PlanPtr, Ptr{Cvoid}, Cint), plan, pointer(u.electric), dir_sign(direction)), :fh_propagate)
     u
 end
@@ -122,6 +133,22 @@ function backpropagate_with_gradient!(∂v::ScalarField{<:ROCArray{ComplexF32}},
                 context(), pointer(∂v.electric), pointer(p.ϕ), pointer(u_saved), pointer(∂p.ϕ),
                 nx, ny, batch, dir_sign(direction), 0), :fh_phase_backward)
     (∂v, ∂p)
+end
+
+# SquaredIntensityDifference (squared_intensity_difference.jl:78-90) on the device: the values
+# (FP64, one per field or one for a spatial-only target) and, in the pullback, 4 dc d u.
+function sq_intensity_diff!(loss::ROCArray{Float64}, du::Union{Nothing, ROCArray{ComplexF32}},
+                            u::ScalarField{<:ROCArray{ComplexF32}}, target::ROCArray{Float32},
+                            dc::Union{Nothing, ROCArray{Float64}} = nothing)
+    nx, ny = size(u.electric, 1), size(u.electric, 2)
+    batch = div(length(u.electric), nx * ny)
+    tb = div(length(target), nx * ny)
+    check(ccall((:fh_sq_intensity_diff, libfluxhip), Cint,
+                (CtxPtr, Ptr{Cvoid}, Ptr{Cfloat}, Cint, Cint, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cvoid}),
+                context(), pointer(u.electric), pointer(target), nx, ny, batch, tb,
+                dc === nothing ? C_NULL : pointer(dc), pointer(loss), du === nothing ? C_NULL : pointer(du)),
+          :fh_sq_intensity_diff)
+    loss
 end
 
 # The rrules of ChainRulesCoreExt.jl:21-111 call propagate!/backpropagate! and
