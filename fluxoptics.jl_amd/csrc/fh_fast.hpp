@@ -24,6 +24,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, lo
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)lim, kBufWord3);
 }
 
+// s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left at their maxima (gfx9
+// encoding: vmcnt[3:0] in bits 3:0, expcnt in 6:4, lgkmcnt in 11:8, vmcnt[5:4] in 15:14)
+constexpr int kWaitVmcnt0 = 0x0F70;
+
 __device__ __forceinline__ float2 buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
   return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
@@ -485,6 +489,11 @@ __device__ __forceinline__ void ypipe_field(float2* v, const float2* h, float2* 
 #pragma unroll
   for (int k = 0; k < V; ++k) v[k] = conj ? cmulc(v[k], h[k]) : cmul(v[k], h[k]);
   fast_fft_b<RI, true>(v, s, t, pi);
+  // Loads and stores share the vector-memory counter, and a wait for a load with stores
+  // still pending drains everything.  Wait here, before this field's stores, while every
+  // outstanding operation is old (the next field's prefetch, the previous field's stores):
+  // the next field then starts with its data known present and nothing to drain.
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   if (live) {
     const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
 #pragma unroll
@@ -531,27 +540,28 @@ k_fast_ypipe(FastIO io, int batch, const float2* __restrict__ tabf, const float2
   float2 va[V], vb[V], ha[V];
   yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, 0, vin, kin);
   ypipe_loadh<RDF>(ha, qt, 0, qoff_d, qoff_m);
+  // The prefetches are unconditional (past the last field they re-read field batch-1, which
+  // is harmless): a load under a branch makes the wait-count pass drain every outstanding
+  // load at the join, which would serialise the prefetch with the arithmetic.
   if constexpr (ONE_TAB) {
     for (int b = 0; b < batch; b += 2) {
-      if (b + 1 < batch) yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b + 1, vin, kin);
+      yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b + 1 < batch ? b + 1 : batch - 1, vin, kin);
       ypipe_field<RDF, OUT_HALF>(va, ha, s, t, live, b, io, pf, pi, qt.conj, vout, kout);
       if (b + 1 >= batch) break;
-      if (b + 2 < batch) yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b + 2, vin, kin);
+      yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b + 2 < batch ? b + 2 : batch - 1, vin, kin);
       ypipe_field<RDF, OUT_HALF>(vb, ha, s, t, live, b + 1, io, pf, pi, qt.conj, vout, kout);
     }
   } else {
     float2 hb[V];
     for (int b = 0; b < batch; b += 2) {
-      if (b + 1 < batch) {
-        yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b + 1, vin, kin);
-        ypipe_loadh<RDF>(hb, qt, b + 1, qoff_d, qoff_m);
-      }
+      const int b1 = b + 1 < batch ? b + 1 : batch - 1;
+      yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b1, vin, kin);
+      ypipe_loadh<RDF>(hb, qt, b1, qoff_d, qoff_m);
       ypipe_field<RDF, OUT_HALF>(va, ha, s, t, live, b, io, pf, pi, qt.conj, vout, kout);
       if (b + 1 >= batch) break;
-      if (b + 2 < batch) {
-        yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b + 2, vin, kin);
-        ypipe_loadh<RDF>(ha, qt, b + 2, qoff_d, qoff_m);
-      }
+      const int b2 = b + 2 < batch ? b + 2 : batch - 1;
+      yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b2, vin, kin);
+      ypipe_loadh<RDF>(ha, qt, b2, qoff_d, qoff_m);
       ypipe_field<RDF, OUT_HALF>(vb, hb, s, t, live, b + 1, io, pf, pi, qt.conj, vout, kout);
     }
   }

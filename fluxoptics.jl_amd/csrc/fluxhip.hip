@@ -120,11 +120,17 @@ static hipError_t launch_fft_t(const LineIO<S>& io, int batch, const C* tw, Op o
   return hipGetLastError();
 }
 
-template <typename S, typename C, int LOGN, typename Op>
-static hipError_t launch_conv_t(const LineIO<S>& io, int batch, const C* tw, Op op, hipStream_t st) {
-  using CF = Cfg<typename RealOf<C>::type, LOGN, true>;
-  using RD = typename CF::RD;
-  constexpr int L = CF::L;
+// Small problems (e.g. one 512^2 field per AS_BPM slice) give too few workgroups to cover the
+// 256 CUs at the configured lines per workgroup; launches then halve L until there are at
+// least kFillWGs workgroups (or L = 1).
+constexpr long kFillWGs = 512;
+
+template <typename S, typename C, typename RD, int L, typename Op>
+static hipError_t launch_conv_l(const LineIO<S>& io, int batch, const C* tw, Op op, hipStream_t st) {
+  if constexpr (L > 1) {
+    if ((long)((io.nlines + L - 1) / L) * batch < kFillWGs)
+      return launch_conv_l<S, C, RD, L / 2, Op>(io, batch, tw, op, st);
+  }
   auto kern = k_line_conv<S, C, RD, L, true, Op>;
   const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n, L, sizeof(C)>::size * sizeof(C) : 0;
   hipError_t e = ensure_lds(kern, lds);
@@ -133,6 +139,12 @@ static hipError_t launch_conv_t(const LineIO<S>& io, int batch, const C* tw, Op 
   const int G = io.in_blk ? 1 : std::max(1, (int)(128 / (L * sizeof(S))));
   hipLaunchKernelGGL(kern, grid, dim3(L * RD::P), lds, st, io, tw, op, G);
   return hipGetLastError();
+}
+
+template <typename S, typename C, int LOGN, typename Op>
+static hipError_t launch_conv_t(const LineIO<S>& io, int batch, const C* tw, Op op, hipStream_t st) {
+  using CF = Cfg<typename RealOf<C>::type, LOGN, true>;
+  return launch_conv_l<S, C, typename CF::RD, CF::L, Op>(io, batch, tw, op, st);
 }
 
 #define FH_LOGN_SWITCH(logn, CALL)                                       \
@@ -1355,13 +1367,15 @@ __global__ void k_as_table_d(double2* __restrict__ tab, int nx, int ny, int qx, 
   tab[(long)i * qy + j] = make_double2(re * scale, im * scale);
 }
 
-template <int LOGN>
-static hipError_t launch_bpm_step_t(const LineIO<float2>& io, int batch, const double2* tw,
+template <typename RD, int L>
+static hipError_t launch_bpm_step_l(const LineIO<float2>& io, int batch, const double2* tw,
                                     const DnPhaseOp& ph, float2* save, float2* ckspec, float* ddn,
                                     const float2* saved, hipStream_t st) {
-  using CF = Cfg<double, LOGN, false>;
-  using RD = typename CF::RD;
-  constexpr int L = CF::L;
+  if constexpr (L > 1) {
+    // the gradient kernel loops over the batch inside a workgroup
+    const long wgs = (long)((io.nlines + L - 1) / L) * (ddn ? 1 : batch);
+    if (wgs < kFillWGs) return launch_bpm_step_l<RD, L / 2>(io, batch, tw, ph, save, ckspec, ddn, saved, st);
+  }
   const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n, L, sizeof(double2)>::size * sizeof(double2) : 0;
   const unsigned threads = L * RD::P;
   if (ddn) {
@@ -1378,6 +1392,14 @@ static hipError_t launch_bpm_step_t(const LineIO<float2>& io, int batch, const d
                        ckspec);
   }
   return hipGetLastError();
+}
+
+template <int LOGN>
+static hipError_t launch_bpm_step_t(const LineIO<float2>& io, int batch, const double2* tw,
+                                    const DnPhaseOp& ph, float2* save, float2* ckspec, float* ddn,
+                                    const float2* saved, hipStream_t st) {
+  using CF = Cfg<double, LOGN, false>;
+  return launch_bpm_step_l<typename CF::RD, CF::L>(io, batch, tw, ph, save, ckspec, ddn, saved, st);
 }
 
 static hipError_t launch_bpm_step(int logn, const LineIO<float2>& io, int batch, const double2* tw,
