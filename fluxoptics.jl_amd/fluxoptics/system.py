@@ -182,16 +182,52 @@ class OpticalSystem:
         is a float64 device scalar and ``grads`` maps each trainable component (and a
         trainable source) to its gradient tensor.
         """
+        losses, grads = self._withgradient_launches(loss, x)
+        return _total(losses), grads
+
+    def _withgradient_launches(self, loss, x):
+        """The device work of ``withgradient``: per-pair loss tensors and the gradients."""
         u = self._start(x)
         comps = list(self.components)
         tape = _run_forward(comps, u, self.direction, save=True)
         losses, (du,) = loss.value_and_pullback((u,), inplace=True)
-        value = sum(l.sum() for l in losses)
         grads: Dict = {}
         du = _run_backward(comps, tape, du, self.direction, grads)
         if self.source is not None and self.source.trainable:
             grads[self.source] = self.source.backpropagate_with_gradient(du)
-        return value, grads
+        return losses, grads
+
+    def capture_withgradient(self, loss, x: Optional[ScalarField] = None) -> "CapturedGradient":
+        """Record the whole ``withgradient`` launch sequence (forward with saves, metric and
+        pullback, adjoint with gradients) once as a HIP graph; ``replay()`` then reruns it with
+        one graph launch and returns ``(value, grads)`` in the same buffers every time.
+
+        The inputs are those of the capture: the source's field (or ``x``'s buffer), the
+        masks and volumes of the components and the metric's targets; change them in place
+        between replays.  One eager call runs first so that every plan, table and workspace
+        exists before capture.
+        """
+        self._withgradient_launches(loss, x)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            losses, grads = self._withgradient_launches(loss, x)
+        return CapturedGradient(graph, losses, grads)
+
+
+def _total(losses):
+    return sum(l.sum() for l in losses)
+
+
+class CapturedGradient:
+    """A recorded ``withgradient`` (see ``OpticalSystem.capture_withgradient``)."""
+
+    def __init__(self, graph, losses, grads):
+        self.graph, self.losses, self.grads = graph, losses, grads
+
+    def replay(self):
+        self.graph.replay()
+        return _total(self.losses), self.grads
 
 
 # --------------------------------------------------------------------------- executor

@@ -149,3 +149,31 @@ def test_merged_system_matches_unmerged():
     ref = O.asprop(O.phase_apply(u0, (a.phi + b.phi).cpu().numpy().astype(np.float64), 1), (ds, ds), lam,
                    1000.0, 1.0, 1)
     assert rel_l2(outm, ref) <= FIELD_TOL
+
+
+def test_captured_withgradient_replays_match_eager():
+    """A recorded withgradient replays bit for bit like the eager calls, with eager work and
+    synchronisations between replays and the masks changed in place between them."""
+    n, ds, lam = 256, 1.0, 1.064
+    print("[captured withgradient] 256^2 RS/Phase system", flush=True)
+    u0 = c64(inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 20.0), ds, ds))
+    tgt = (np.abs(_two_gaussians(n, ds, 20.0, 30.0)) ** 2).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), (ds, ds), lam)
+    doe1 = fo.Phase(u, inputs.uniform(61, (n, n), 0, 1).astype(np.float32), trainable=True, buffered=True)
+    doe2 = fo.Phase(u, inputs.uniform(62, (n, n), 0, 1).astype(np.float32), trainable=True, buffered=True)
+    p1, p2 = fo.RSProp(u, 900.0), fo.ASProp(u, 700.0)
+    system = fo.ScalarSource(u) | p1 | doe1 | p2 | doe2 | p1
+    metric = fo.SquaredIntensityDifference((u, tgt))
+    step = system.capture_withgradient(metric)
+    for rep in range(3):
+        with torch.no_grad():
+            doe1.phi.add_(0.05 * rep)
+        _ = torch.randn(4096, device="cuda").sum()
+        torch.cuda.synchronize()
+        value, grads = step.replay()
+        torch.cuda.synchronize()
+        g1, g2, v = grads[doe1].clone(), grads[doe2].clone(), float(value)
+        ev, eg = system.withgradient(metric)
+        torch.cuda.synchronize()
+        assert v == float(ev), rep
+        assert torch.equal(g1, eg[doe1]) and torch.equal(g2, eg[doe2]), rep
