@@ -360,7 +360,8 @@ k_fast_yconv(FastIO io, int batch, int fpw, const float2* __restrict__ tabf,
   float2* smem = reinterpret_cast<float2*>(smem_raw);
   int l, t;
   thread_map<L, P, !LANES_T>(threadIdx.x, l, t);
-  const int li = blockIdx.x * L + l;
+  // fewer than 4 columns per workgroup: the tiles sharing a 128-byte line run on one XCD
+  const int li = (L < 4 ? xcd_group_tile(blockIdx.x, gridDim.x, 4 / L) : (int)blockIdx.x) * L + l;
   const bool live = li < io.nlines;
   float2* s = smem + l * LdsLine<n, L>::size;
   const int vin = fast_off<true>(li, t, io.in_l, io.in_e, io.in_blk) * 8;

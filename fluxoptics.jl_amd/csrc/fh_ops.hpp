@@ -63,4 +63,15 @@ struct PhaseTimeOp {
   }
 };
 
+// Multiply by a real constant (the 1/(nx*ny) of an inverse 2-D FFT, in the last pass).
+struct ScaleOp {
+  float s;
+  template <typename C>
+  __device__ __forceinline__ C operator()(C v, int, int, int) const {
+    v.x *= s;
+    v.y *= s;
+    return v;
+  }
+};
+
 }  // namespace fh

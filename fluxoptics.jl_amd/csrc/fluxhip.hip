@@ -509,13 +509,18 @@ k_xifft_phase_grad(LineIO<float2> io, int batch, const float2* __restrict__ tw,
   }
 }
 
-template <int LOGN>
-static hipError_t launch_xifft_phase_grad_t(const LineIO<float2>& io, int batch, const float2* tw,
+// one workgroup per L rows, looping over the batch; L halves (down to one full wave) while
+// there would be fewer than kFillWGs workgroups, but stays a multiple of 4: the pass reads
+// the blocked spectrum and writes natural rows in place, so a workgroup must own whole 4-row
+// groups
+template <typename RD, int L>
+static hipError_t launch_xifft_phase_grad_l(const LineIO<float2>& io, int batch, const float2* tw,
                                             const float* phi, const float2* saved, float* dphi,
                                             int nx, float s, int acc, hipStream_t st) {
-  using CF = Cfg<float, LOGN, false>;
-  using RD = typename CF::RD;
-  constexpr int L = CF::L;
+  if constexpr (L >= 8 && (L / 2) * RD::P >= 64) {
+    if ((long)((io.nlines + L - 1) / L) < kFillWGs)
+      return launch_xifft_phase_grad_l<RD, L / 2>(io, batch, tw, phi, saved, dphi, nx, s, acc, st);
+  }
   auto kern = k_xifft_phase_grad<RD, L>;
   const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n, L>::size * sizeof(float2) : 0;
   hipError_t e = ensure_lds(kern, lds);
@@ -524,6 +529,14 @@ static hipError_t launch_xifft_phase_grad_t(const LineIO<float2>& io, int batch,
   hipLaunchKernelGGL(kern, grid, dim3(L * RD::P), lds, st, io, batch, tw, phi, saved, dphi, nx, s,
                      acc);
   return hipGetLastError();
+}
+
+template <int LOGN>
+static hipError_t launch_xifft_phase_grad_t(const LineIO<float2>& io, int batch, const float2* tw,
+                                            const float* phi, const float2* saved, float* dphi,
+                                            int nx, float s, int acc, hipStream_t st) {
+  using CF = Cfg<float, LOGN, false>;
+  return launch_xifft_phase_grad_l<typename CF::RD, CF::L>(io, batch, tw, phi, saved, dphi, nx, s, acc, st);
 }
 
 static hipError_t launch_xifft_phase_grad(int logn, const LineIO<float2>& io, int batch,
@@ -649,12 +662,26 @@ static int get_stage_tw(fh_ctx* c, int n, int R0, int R1, int R2, bool inv, cons
 
 // ---------------------------------------------------------------- fast c64 launches
 
+template <int LOGN, bool INV, bool IH, bool OH, typename Op, int L>
+static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes);
+
 template <int LOGN, bool INV, bool IH, bool OH, typename Op>
 static int fast_x_t(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
-  using CF = Cfg<float, LOGN, false>;
-  using RDF = typename CF::RD;
+  return fast_x_l<LOGN, INV, IH, OH, Op, Cfg<float, LOGN, false>::L>(c, io, batch, op, lanes);
+}
+
+// Lines per workgroup halve (down to one full wave per workgroup) while a launch would have
+// fewer than kFillWGs workgroups: single small fields otherwise leave most CUs idle.  An x
+// pass converts between the natural and the 4x4-blocked layout in place, which is only safe
+// when every workgroup owns whole 4-row groups, so L stays a multiple of 4.
+template <int LOGN, bool INV, bool IH, bool OH, typename Op, int L>
+static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
+  using RDF = typename Cfg<float, LOGN, false>::RD;
   using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
-  constexpr int L = CF::L;
+  if constexpr (L >= 8 && (L / 2) * RDF::P >= 64) {
+    if ((long)((io.nlines + L - 1) / L) * batch < kFillWGs)
+      return fast_x_l<LOGN, INV, IH, OH, Op, L / 2>(c, io, batch, op, lanes);
+  }
   const float2* tab = nullptr;
   int rc = get_stage_tw(c, RD::n, RD::R0, RD::R1, RD::R2, INV, &tab);
   if (rc) return rc;
@@ -709,6 +736,11 @@ template <int A, int B, int C_, int L, int FPW = 1 << 20, bool LT = false>
 static int fast_conv_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool half) {
   using RDF = Radices<A, B, C_>;
   using RI = typename RDF::Rev;
+  if constexpr (L > 1 && (L / 2) * RDF::P >= 64 && !LT) {
+    const int fpw0 = FPW < batch ? FPW : batch;
+    if ((long)((io.nlines + L - 1) / L) * ((batch + fpw0 - 1) / fpw0) < kFillWGs)
+      return fast_conv_v<A, B, C_, L / 2, FPW, LT>(c, io, batch, op, half);
+  }
   const float2 *tf = nullptr, *ti = nullptr;
   int rc;
   if ((rc = get_stage_tw(c, RDF::n, RDF::R0, RDF::R1, RDF::R2, false, &tf)) ||
@@ -1239,6 +1271,40 @@ int fh_phase_backward(fh_ctx* ctx, void* dv, const float* phi, const void* saved
                      reinterpret_cast<float2*>(dv), phi, reinterpret_cast<const float2*>(saved), dphi,
                      npix, batch, (float)direction, accumulate);
   FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double scale) {
+  if (!ctx || !u) return fail(FH_E_ARG, "null context or field");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  if (sign != 1 && sign != -1) return fail(FH_E_ARG, "sign must be -1 (forward) or +1 (inverse)");
+  if (!std::isfinite(scale)) return fail(FH_E_ARG, "scale must be finite");
+  const int lx = ilog2_exact(nx), ly = ilog2_exact(ny);
+  if (lx < kMinLog || lx > kMaxLog || ly < kMinLog || ly > kMaxLog)
+    return fail(FH_E_UNSUPP, "FFT2 needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << kMinLog, 1 << kMaxLog, nx, ny);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const float2 *twx = nullptr, *twy = nullptr;
+  if ((rc = get_twiddles(ctx, nx, &twx)) || (rc = get_twiddles(ctx, ny, &twy))) return rc;
+  float2* f = reinterpret_cast<float2*>(u);
+  const long npix = (long)nx * ny;
+  LineIO<float2> io{};
+  io.in = f; io.out = f; io.in_f = io.out_f = npix;
+  io.in_l = io.out_l = nx; io.in_e = io.out_e = 1; io.nlines = ny; io.in_valid = io.out_count = nx;
+  LineIO<float2> iy = io;
+  iy.in_l = iy.out_l = 1; iy.in_e = iy.out_e = nx; iy.nlines = nx; iy.in_valid = iy.out_count = ny;
+  const ScaleOp sc{(float)scale};
+  // the scale rides on the first pass of a forward transform (prologue) and on the last
+  // pass of an inverse one (epilogue)
+  if (sign < 0) {
+    if (scale == 1.0) FH_HIP((launch_fft<float2, float2, false, false>(lx, io, batch, twx, NoOp{}, ctx->stream)));
+    else FH_HIP((launch_fft<float2, float2, false, false>(lx, io, batch, twx, sc, ctx->stream)));
+    FH_HIP((launch_fft<float2, float2, true, false>(ly, iy, batch, twy, NoOp{}, ctx->stream)));
+  } else {
+    FH_HIP((launch_fft<float2, float2, false, true>(lx, io, batch, twx, NoOp{}, ctx->stream)));
+    if (scale == 1.0) FH_HIP((launch_fft<float2, float2, true, true>(ly, iy, batch, twy, NoOp{}, ctx->stream)));
+    else FH_HIP((launch_fft<float2, float2, true, true>(ly, iy, batch, twy, sc, ctx->stream)));
+  }
   return FH_OK;
 }
 

@@ -6,14 +6,15 @@ the arithmetic is the HIP library's (``libfluxhip.so``, C ABI in include/fluxhip
 
 from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficiency,
                      rs_valid_distance)
-from .components import (AS_BPM, ASProp, Phase, RSProp, phase_propagate_,
-                         propagate_phase_backward_)
+from .components import (AS_BPM, ASProp, FourierOperator, FourierWrapper, Phase, RSProp,
+                         phase_propagate_, propagate_phase_backward_)
 from .group import PropagationGroup
 from .metrics import SquaredIntensityDifference
 from .system import OpticalSequence, OpticalSystem, ScalarSource, merge
 
 __all__ = [
-    "AS_BPM", "ASProp", "Backward", "Direction", "Forward", "Phase", "RSProp", "ScalarField",
+    "AS_BPM", "ASProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
+    "RSProp", "ScalarField",
     "OpticalSequence", "OpticalSystem", "PropagationGroup", "ScalarSource", "SquaredIntensityDifference", "merge",
     "coupling_efficiency", "phase_propagate_", "propagate_phase_backward_", "propagate",
     "propagate_", "backpropagate_", "rs_valid_distance",

@@ -226,6 +226,51 @@ class RSProp(_KernelPropagator):
                           "sampling (dx < lambda/2).")
 
 
+# --------------------------------------------------------------------------- Fourier operators
+
+class FourierOperator(_PipeComponent):
+    """``FourierOperator(u, direct)`` (``fourier_operator.jl:37-79``): FFT2 of the field when
+    ``direct`` (IFFT2 otherwise); the Backward direction swaps them.  Unnormalised forward,
+    1/(nx*ny) inverse (``FFTutils.jl:13-28``)."""
+
+    def __init__(self, u: ScalarField, direct: bool):
+        self.nx, self.ny = u.nx, u.ny
+        self.direct = bool(direct)
+
+    def _transform(self, direction):
+        """(sign, scale) of the transform applied in ``direction``."""
+        fft = self.direct if dir_sign(direction) > 0 else not self.direct
+        return (-1, 1.0) if fft else (1, 1.0 / (self.nx * self.ny))
+
+    def _run(self, u: ScalarField, sign: int, scale: float) -> ScalarField:
+        check_field(u.electric)
+        if (u.nx, u.ny) != (self.nx, self.ny):
+            raise ValueError("field grid does not match the Fourier operator")
+        _lib.call("fh_fft2", context(u.electric.device), _ptr(u.electric), u.nx, u.ny, u.batch, sign,
+                  float(scale))
+        return u
+
+    def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self._run(u, *self._transform(direction))
+
+    def propagate(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self.propagate_(u.copy(), direction)
+
+    def backpropagate_(self, dv: ScalarField, direction=Forward) -> ScalarField:
+        """The exact adjoint, as Zygote's pullback through the FFT plan gives it
+        (``OpticalComponents.jl:690-696``): the FFT's adjoint is the unnormalised inverse and
+        the normalised IFFT's is FFT/(nx*ny)."""
+        sign, scale = self._transform(direction)
+        return self._run(dv, -sign, scale)
+
+
+def FourierWrapper(u: ScalarField, *components):
+    """``FourierWrapper(u, components...)`` (``fourier_wrapper.jl:1-5,34-39``): the sequence
+    FFT -> components -> IFFT."""
+    from .system import OpticalSequence
+    return OpticalSequence(FourierOperator(u, True), *components, FourierOperator(u, False))
+
+
 # --------------------------------------------------------------------------- Phase
 
 class _GridOnly:

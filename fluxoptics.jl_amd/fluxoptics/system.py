@@ -22,7 +22,8 @@ from typing import Callable, Dict, List, Optional, Sequence
 
 import torch
 
-from .components import AS_BPM, ASProp, Phase, RSProp, _KernelPropagator, check_field
+from .components import (AS_BPM, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
+                         check_field)
 from .components import phase_propagate_, propagate_phase_backward_
 from .fields import Backward, Forward, ScalarField, dir_sign
 
@@ -63,7 +64,11 @@ def _is_static_phase(c) -> bool:
 
 
 def _merge_pair(a, b):
-    """``merge_rules.jl``: returns the merged component or None."""
+    """``merge_rules.jl``: returns the merged component, ``_CANCEL`` when the pair is the
+    identity (FFT followed by IFFT), or None."""
+    if (isinstance(a, FourierOperator) and isinstance(b, FourierOperator) and a.direct != b.direct
+            and (a.nx, a.ny) == (b.nx, b.ny)):
+        return _CANCEL
     if _is_static_phase(a) and _is_static_phase(b) and a.phi.shape == b.phi.shape:
         return Phase._from_tensor(a.phi + b.phi)
     if (type(a) is ASProp and type(b) is ASProp and a.n0 == b.n0 and a.ds == b.ds
@@ -76,6 +81,9 @@ def _merge_pair(a, b):
     return None
 
 
+_CANCEL = object()
+
+
 def merge(components: Sequence) -> list:
     out: list = []
     for c in components:
@@ -85,7 +93,10 @@ def merge(components: Sequence) -> list:
                 break
             out.pop()
             c = m
-        out.append(c)
+            if c is _CANCEL:
+                break
+        if c is not _CANCEL:
+            out.append(c)
     return out
 
 
@@ -135,6 +146,10 @@ class OpticalSystem:
             components = (source,) + components
             source = None
         self.source = source
+        flat = []
+        for c in components:  # sequences (e.g. FourierWrapper) run as their components
+            flat.extend(c.optical_components if isinstance(c, OpticalSequence) else [c])
+        components = tuple(flat)
         self.input_components = tuple(components)
         self.components = tuple(merge(components)) if merge_components else tuple(components)
         self.inplace = inplace

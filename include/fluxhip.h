@@ -149,6 +149,12 @@ int fh_propagate_phase_backward(fh_plan* plan, void* dv, const float* phi, const
  * multiply by a full FP64 transfer-function table, y IFFT, x IFFT). */
 int fh_propagate_plain_fp64(fh_plan* plan, void* u_c128, int direction);
 
+/* FourierOperator (fourier_operator.jl:65-79, FFTutils.jl:13-28): in-place 2-D DFT of a c64
+ * [nx, ny, batch] device array, u <- scale * sum u e^{sign 2 pi i jk/n}.  The reference's FFT
+ * is sign = -1, scale = 1; its IFFT is sign = +1, scale = 1/(nx*ny); the adjoint of
+ * (sign, scale) is (-sign, scale). */
+int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double scale);
+
 /* SquaredIntensityDifference (squared_intensity_difference.jl:78-90, compute_metric and
  * backpropagate_metric).  target holds target_batch FP32 intensity planes [nx, ny]:
  *   target_batch == batch ("full dimensions"): loss[b] = sum (|u_b|^2 - I_b)^2, b < batch;

@@ -29,6 +29,11 @@ GRAD_TOL = 1e-4
     (256, 512, 1, (0.5, 0.5), 1.0, -300.0, 1.5),
     (512, 256, 2, (1.0, 0.5), 0.8, 2000.0, 1.0),
     (1024, 1024, 2, (1.0, 1.0), 0.64, 1e4, 1.0),
+    # single small fields: the launches use fewer lines per workgroup
+    (1024, 1024, 1, (1.0, 1.0), 0.8, 3000.0, 1.0),
+    (512, 512, 1, (1.0, 1.0), 1.064, 800.0, 1.0),
+    (64, 64, 1, (1.0, 1.0), 0.8, 40.0, 1.0),
+    (2048, 1024, 1, (0.5, 1.0), 0.8, 2000.0, 1.0),
     (2048, 2048, 2, (0.5, 0.5), 0.8, 5000.0, 1.0),
     (4096, 4096, 1, (0.5, 0.5), 0.8, 5000.0, 1.0),
 ])
@@ -110,7 +115,7 @@ def test_phase_apply_and_gradient():
     assert rel_l2(dphi.cpu().numpy(), g_ref) <= GRAD_TOL
 
 
-@pytest.mark.parametrize("kind,n", [("as", 256), ("as", 2048), ("rs", 128)])
+@pytest.mark.parametrize("kind,n", [("as", 256), ("as", 1024), ("as", 2048), ("rs", 128), ("rs", 512)])
 def test_fused_phase_propagate_and_adjoint(kind, n):
     print(f"[fused phase+{kind}] n={n}", flush=True)
     batch, ds, lam, z = 3, (1.0, 1.0), [0.64, 0.538, 0.455], 800.0

@@ -79,3 +79,20 @@ def test_pipe_builds_systems():
         s(u)  # a system with a source takes no input field
     with pytest.raises(ValueError):
         chain()  # a system without a source needs one
+
+
+def test_fourier_operator_merge_and_wrapper():
+    """``merge_rules.jl``: FourierOperators of opposite direction cancel; FourierWrapper is
+    the sequence FFT -> components -> IFFT and a system runs it flattened."""
+    u = _cpu_field()
+    ft, ift = fo.FourierOperator(u, True), fo.FourierOperator(u, False)
+    assert fo.merge([ft, ift]) == [] and fo.merge([ift, ft]) == []
+    assert fo.merge([ft, ft]) == [ft, ft]
+    ph = fo.Phase(u, None)
+    w = fo.FourierWrapper(u, ph)
+    assert isinstance(w, fo.OpticalSequence) and len(w) == 3 and w.optical_components[1] is ph
+    s = fo.OpticalSystem(None, fo.ASProp(u, 1.0), w)
+    assert len(s.components) == 4
+    p2 = fo.ASProp(u, 2.0)
+    sm = fo.OpticalSystem(None, w, fo.FourierWrapper(u, p2), merge_components=True)
+    assert [type(c).__name__ for c in sm.components] == ["FourierOperator", "Phase", "ASProp", "FourierOperator"]
