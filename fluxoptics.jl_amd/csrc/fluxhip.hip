@@ -807,10 +807,7 @@ static const FastConvVariant kFast11[] = {{"8x16x16 L4 f1", fast_conv_v<8, 16, 1
                                           {"pipe 8x16x16 L4", fast_pipe_v<8, 16, 16, 4>},
                                           {"pipe 16x16x8 L4", fast_pipe_v<16, 16, 8, 4>},
                                           {"16x16x8 L4 f1", fast_conv_v<16, 16, 8, 4, 1>},
-                                          {"16x16x8 L4", fast_conv_v<16, 16, 8, 4>},
-                                          {"16x16x8 L4 f2", fast_conv_v<16, 16, 8, 4, 2>},
-                                          {"pipe 16x16x8 L2", fast_pipe_v<16, 16, 8, 2>},
-                                          {"pipe 8x16x16 L2", fast_pipe_v<8, 16, 16, 2>}};
+                                          {"16x16x8 L4 f2", fast_conv_v<16, 16, 8, 4, 2>}};
 static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16, 16, 4, 1>},
                                           {"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
                                           {"16x16x16 L2 f1", fast_conv_v<16, 16, 16, 2, 1>}};
@@ -831,7 +828,7 @@ static const FastConvVariant* fast_conv_variants(int logn, int* count) {
     case 8: *count = 2; return kFast8;
     case 9: *count = 3; return kFast9;
     case 10: *count = 4; return kFast10;
-    case 11: *count = 8; return kFast11;
+    case 11: *count = 5; return kFast11;
     case 12: *count = 3; return kFast12;
     default: *count = 0; return nullptr;
   }
