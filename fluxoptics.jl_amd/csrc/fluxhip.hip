@@ -436,6 +436,68 @@ k_sum_partials(const double* __restrict__ part, int nper, double* __restrict__ o
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// DotProduct / PowerCoupling (metrics/dot_product.jl:60-95, power_coupling.jl:55-64), one
+// value per mode: c_b = ds * sum u_b conj(v_b) in FP64 (block partials, then an ordered sum).
+__global__ void __launch_bounds__(kMetricThreads)
+k_dot_partial(const float2* __restrict__ u, const float2* __restrict__ v, long npix,
+              double2* __restrict__ part) {
+  __shared__ double red[kMetricThreads / 64];
+  const int b = blockIdx.y;
+  double ar = 0.0, ai = 0.0;
+  for (long i = blockIdx.x * (long)kMetricThreads + threadIdx.x; i < npix; i += (long)gridDim.x * kMetricThreads) {
+    const float2 x = u[b * npix + i], y = v[b * npix + i];
+    ar += (double)x.x * y.x + (double)x.y * y.y;  // Re(x conj(y))
+    ai += (double)x.y * y.x - (double)x.x * y.y;  // Im(x conj(y))
+  }
+  const double sr = block_sum(ar, red);
+  __syncthreads();
+  const double si = block_sum(ai, red);
+  if (threadIdx.x == 0) part[b * (long)gridDim.x + blockIdx.x] = make_double2(sr, si);
+}
+
+// c[b] = ds * ordered sum of the partials; p[b] = |c[b]|^2 when p is not null
+__global__ void __launch_bounds__(kMetricThreads)
+k_dot_final(const double2* __restrict__ part, int nper, double ds, double2* __restrict__ c,
+            double* __restrict__ p) {
+  __shared__ double red[kMetricThreads / 64];
+  double ar = 0.0, ai = 0.0;
+  for (int i = threadIdx.x; i < nper; i += kMetricThreads) {
+    const double2 x = part[blockIdx.x * (long)nper + i];
+    ar += x.x;
+    ai += x.y;
+  }
+  const double sr = block_sum(ar, red);
+  __syncthreads();
+  const double si = block_sum(ai, red);
+  if (threadIdx.x == 0) {
+    const double2 r = make_double2(sr * ds, si * ds);
+    c[blockIdx.x] = r;
+    if (p) p[blockIdx.x] = r.x * r.x + r.y * r.y;
+  }
+}
+
+// du_b = v_b * g_b with g_b = dc_b (DotProduct, dot_product.jl:97-106) or 2 c_b dp_b
+// (PowerCoupling, power_coupling.jl:66-69; dp null = ones)
+__global__ void k_dot_pullback(const float2* __restrict__ v, const double2* __restrict__ dc,
+                               const double2* __restrict__ c, const double* __restrict__ dp, long npix,
+                               float2* __restrict__ du) {
+  const int b = blockIdx.y;
+  double gr, gi;
+  if (dc) {
+    gr = dc[b].x;
+    gi = dc[b].y;
+  } else {
+    const double w = 2.0 * (dp ? dp[b] : 1.0);
+    gr = w * c[b].x;
+    gi = w * c[b].y;
+  }
+  const float fr = (float)gr, fi = (float)gi;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
+    const float2 y = v[b * npix + i];
+    du[b * npix + i] = make_float2(y.x * fr - y.y * fi, y.x * fi + y.y * fr);
+  }
+}
+
 __global__ void k_mul_full_d(double2* __restrict__ u, const double2* __restrict__ tab, long npix,
                              int conj) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix;
@@ -1305,6 +1367,53 @@ int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double sc
   return FH_OK;
 }
 
+static int metric_partials(fh_ctx* ctx, size_t bytes) {
+  const size_t need = (bytes + sizeof(double) - 1) / sizeof(double);
+  if (ctx->metric_part_n >= need) return FH_OK;
+  if (ctx->metric_part) FH_HIP(hipFree(ctx->metric_part));
+  ctx->metric_part = nullptr;
+  ctx->metric_part_n = 0;
+  if (hipMalloc(&ctx->metric_part, need * sizeof(double)) != hipSuccess)
+    return fail(FH_E_NOMEM, "metric partials: out of device memory");
+  ctx->metric_part_n = need;
+  return FH_OK;
+}
+
+int fh_dot_product(fh_ctx* ctx, const void* u, const void* v, int nx, int ny, int batch, double ds,
+                   void* c, double* p) {
+  if (!ctx || !u || !v || !c) return fail(FH_E_ARG, "null context, fields or output");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  if (!std::isfinite(ds)) return fail(FH_E_ARG, "ds must be finite");
+  int rc;
+  if ((rc = set_device(ctx)) || (rc = metric_partials(ctx, (size_t)batch * kMetricBlocks * sizeof(double2))))
+    return rc;
+  double2* part = reinterpret_cast<double2*>(ctx->metric_part);
+  const long npix = (long)nx * ny;
+  hipLaunchKernelGGL(k_dot_partial, dim3(kMetricBlocks, batch), dim3(kMetricThreads), 0, ctx->stream,
+                     reinterpret_cast<const float2*>(u), reinterpret_cast<const float2*>(v), npix, part);
+  FH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_dot_final, dim3(batch), dim3(kMetricThreads), 0, ctx->stream, (const double2*)part,
+                     kMetricBlocks, ds, reinterpret_cast<double2*>(c), p);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_dot_product_pullback(fh_ctx* ctx, const void* v, const void* dc, const void* c, const double* dp,
+                            int nx, int ny, int batch, void* du) {
+  if (!ctx || !v || !du) return fail(FH_E_ARG, "null context, field or output");
+  if (!dc && !c) return fail(FH_E_ARG, "need the cotangents dc, or the dot products c for a coupling pullback");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const long npix = (long)nx * ny;
+  const int blocks = (int)std::min<long>((npix + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_dot_pullback, dim3(blocks, batch), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<const float2*>(v), reinterpret_cast<const double2*>(dc),
+                     reinterpret_cast<const double2*>(c), dp, npix, reinterpret_cast<float2*>(du));
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
 int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx, int ny, int batch,
                          int target_batch, const double* dc, double* loss, void* du) {
   if (!ctx || !u || !target || !loss) return fail(FH_E_ARG, "null context, field, target or loss");
@@ -1316,15 +1425,7 @@ int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx
   if ((rc = set_device(ctx))) return rc;
   const int spatial = target_batch == 1 && batch > 1;
   const int nout = spatial ? 1 : batch;
-  const size_t need = (size_t)nout * kMetricBlocks;
-  if (ctx->metric_part_n < need) {
-    if (ctx->metric_part) FH_HIP(hipFree(ctx->metric_part));
-    ctx->metric_part = nullptr;
-    ctx->metric_part_n = 0;
-    if (hipMalloc(&ctx->metric_part, need * sizeof(double)) != hipSuccess)
-      return fail(FH_E_NOMEM, "metric partials: out of device memory");
-    ctx->metric_part_n = need;
-  }
+  if ((rc = metric_partials(ctx, (size_t)nout * kMetricBlocks * sizeof(double)))) return rc;
   const long npix = (long)nx * ny;
   hipLaunchKernelGGL(k_sqint_partial, dim3(kMetricBlocks, nout), dim3(kMetricThreads), 0, ctx->stream,
                      reinterpret_cast<const float2*>(u), target, npix, batch, spatial, dc,

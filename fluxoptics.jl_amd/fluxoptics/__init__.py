@@ -9,13 +9,13 @@ from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficie
 from .components import (AS_BPM, ASProp, FourierOperator, FourierWrapper, Phase, RSProp,
                          phase_propagate_, propagate_phase_backward_)
 from .group import PropagationGroup
-from .metrics import SquaredIntensityDifference
+from .metrics import DotProduct, PowerCoupling, SquaredIntensityDifference
 from .system import OpticalSequence, OpticalSystem, ScalarSource, merge
 
 __all__ = [
     "AS_BPM", "ASProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
     "RSProp", "ScalarField",
-    "OpticalSequence", "OpticalSystem", "PropagationGroup", "ScalarSource", "SquaredIntensityDifference", "merge",
+    "OpticalSequence", "OpticalSystem", "PropagationGroup", "ScalarSource", "SquaredIntensityDifference", "DotProduct", "PowerCoupling", "merge",
     "coupling_efficiency", "phase_propagate_", "propagate_phase_backward_", "propagate",
     "propagate_", "backpropagate_", "rs_valid_distance",
 ]

@@ -65,6 +65,9 @@ SIGNATURES = {
     "fh_propagate_phase_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                             c_int, c_int]),
     "fh_propagate_plain_fp64": (c_int, [c_void_p, c_void_p, c_int]),
+    "fh_dot_product": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p]),
+    "fh_dot_product_pullback": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                        c_void_p]),
     "fh_fft2": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double]),
     "fh_sq_intensity_diff": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p,
                                      c_void_p, c_void_p]),

@@ -2,8 +2,9 @@
 
 ``SquaredIntensityDifference`` mirrors ``src/metrics/squared_intensity_difference.jl``
 (constructor from ``(field, target_intensity)`` pairs, ``compute_metric`` at :78-83 and
-``backpropagate_metric`` at :85-90).  The sums run in libfluxhip (FP64 accumulation in a
-fixed order); nothing is computed on the host.
+``backpropagate_metric`` at :85-90); ``DotProduct`` and ``PowerCoupling`` mirror
+``src/metrics/dot_product.jl`` and ``power_coupling.jl`` (mode-selective).  The sums run in
+libfluxhip (FP64 accumulation in a fixed order); nothing is computed on the host.
 """
 
 from __future__ import annotations
@@ -90,3 +91,100 @@ class SquaredIntensityDifference:
             losses.append(self._run(k, u, None if dcs is None else dcs[k], du))
             cots.append(u if inplace else u.set_field_data(du))
         return tuple(losses), tuple(cots)
+
+
+class DotProduct:
+    """``DotProduct(fields...; mode_selective=true)`` (``metrics/dot_product.jl``): one complex
+    value per mode, c_b = ds * sum u_b conj(v_b) (complex128 device tensors), and the pullback
+    du_b = v_b dc_b.  The non-selective overlap matrix is not on the accelerated path."""
+
+    def __init__(self, *fields: ScalarField, mode_selective: bool = True):
+        if not mode_selective:
+            raise NotImplementedError("the non-selective overlap matrix is not on the accelerated path")
+        if not fields:
+            raise ValueError("DotProduct needs at least one reference field")
+        for v in fields:
+            check_field(v.electric)
+        self.v = [v.copy() for v in fields]
+
+    def _dot(self, k, u, p=None):
+        v = self.v[k]
+        check_field(u.electric)
+        if u.electric.shape != v.electric.shape:
+            raise ValueError("field does not match the metric's reference field")
+        c = torch.empty(u.batch, dtype=torch.complex128, device=u.electric.device)
+        _lib.call("fh_dot_product", context(u.electric.device), _ptr(u.electric), _ptr(v.electric), u.nx, u.ny,
+                  u.batch, u.ds[0] * u.ds[1], _ptr(c), _ptr(p))
+        return c
+
+    def compute_metric(self, *fields: ScalarField):
+        return tuple(self._dot(k, u) for k, u in enumerate(fields))
+
+    __call__ = compute_metric
+
+    def backpropagate_metric(self, fields, dcs):
+        out = []
+        for k, (u, dc) in enumerate(zip(fields, dcs)):
+            dc = torch.as_tensor(dc, dtype=torch.complex128, device=u.electric.device).reshape(-1)
+            dc = (dc.expand(u.batch) if dc.numel() == 1 else dc).contiguous()
+            du = torch.empty_like(u.electric)
+            v = self.v[k]
+            _lib.call("fh_dot_product_pullback", context(u.electric.device), _ptr(v.electric), _ptr(dc), None, None,
+                      u.nx, u.ny, u.batch, _ptr(du))
+            out.append(u.set_field_data(du))
+        return tuple(out)
+
+
+class PowerCoupling:
+    """``PowerCoupling(fields...; mode_selective=true)`` (``metrics/power_coupling.jl``): the
+    targets are copied and normalised to unit power; one value per mode, p_b = |<u_b, v_b>|^2,
+    with the pullback 2 c_b dp_b v_b."""
+
+    def __init__(self, *fields: ScalarField, mode_selective: bool = True):
+        if not mode_selective:
+            raise NotImplementedError("the non-selective coupling matrix is not on the accelerated path")
+        if not fields:
+            raise ValueError("PowerCoupling needs at least one target field")
+        self.v = []
+        for v in fields:
+            check_field(v.electric)
+            self.v.append(v.copy().normalize_power_())
+        self._c = [None] * len(self.v)
+
+    def _run(self, k, u):
+        v = self.v[k]
+        check_field(u.electric)
+        if u.electric.shape != v.electric.shape:
+            raise ValueError("field does not match the metric's target field")
+        c = torch.empty(u.batch, dtype=torch.complex128, device=u.electric.device)
+        p = torch.empty(u.batch, dtype=torch.float64, device=u.electric.device)
+        _lib.call("fh_dot_product", context(u.electric.device), _ptr(u.electric), _ptr(v.electric), u.nx, u.ny,
+                  u.batch, u.ds[0] * u.ds[1], _ptr(c), _ptr(p))
+        self._c[k] = c
+        return p
+
+    def compute_metric(self, *fields: ScalarField):
+        return tuple(self._run(k, u) for k, u in enumerate(fields))
+
+    __call__ = compute_metric
+
+    def backpropagate_metric(self, fields, dps=None, inplace: bool = False):
+        """Cotangent fields 2 c dp v (needs a preceding compute_metric on the same fields)."""
+        out = []
+        for k, u in enumerate(fields):
+            c = self._c[k]
+            if c is None:
+                raise RuntimeError("backpropagate_metric needs compute_metric first")
+            dp = None
+            if dps is not None:
+                dp = torch.as_tensor(dps[k], dtype=torch.float64, device=u.electric.device).reshape(-1)
+                dp = (dp.expand(u.batch) if dp.numel() == 1 else dp).contiguous()
+            du = u.electric if inplace else torch.empty_like(u.electric)
+            _lib.call("fh_dot_product_pullback", context(u.electric.device), _ptr(self.v[k].electric), None, _ptr(c),
+                      _ptr(dp), u.nx, u.ny, u.batch, _ptr(du))
+            out.append(u if inplace else u.set_field_data(du))
+        return tuple(out)
+
+    def value_and_pullback(self, fields, dcs=None, inplace: bool = False):
+        vals = self.compute_metric(*fields)
+        return vals, self.backpropagate_metric(fields, dcs, inplace)

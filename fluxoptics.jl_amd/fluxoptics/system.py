@@ -189,30 +189,32 @@ class OpticalSystem:
         return SystemOutput(u)
 
     # -- gradient
-    def withgradient(self, loss, x: Optional[ScalarField] = None):
+    def withgradient(self, loss, x: Optional[ScalarField] = None, weights=None):
         """Loss value and gradients of every trainable component.
 
-        ``loss`` is a metric with ``value_and_pullback`` (e.g. ``SquaredIntensityDifference``);
-        the objective is the sum of its values.  Returns ``(value, grads)`` where ``value``
-        is a float64 device scalar and ``grads`` maps each trainable component (and a
-        trainable source) to its gradient tensor.
+        ``loss`` is a metric with ``value_and_pullback`` (``SquaredIntensityDifference``,
+        ``PowerCoupling``); the objective is the sum of its values, each times its weight
+        (``weights``: one per value or a scalar, e.g. -1 to maximise a coupling).  Returns
+        ``(value, grads)`` where ``value`` is the weighted sum as a float64 device scalar
+        and ``grads`` maps each trainable component (and a trainable source) to its gradient.
         """
-        losses, grads = self._withgradient_launches(loss, x)
-        return _total(losses), grads
+        losses, grads = self._withgradient_launches(loss, x, weights)
+        return _total(losses, weights), grads
 
-    def _withgradient_launches(self, loss, x):
+    def _withgradient_launches(self, loss, x, weights=None):
         """The device work of ``withgradient``: per-pair loss tensors and the gradients."""
         u = self._start(x)
         comps = list(self.components)
         tape = _run_forward(comps, u, self.direction, save=True)
-        losses, (du,) = loss.value_and_pullback((u,), inplace=True)
+        dcs = None if weights is None else [weights]
+        losses, (du,) = loss.value_and_pullback((u,), dcs, inplace=True)
         grads: Dict = {}
         du = _run_backward(comps, tape, du, self.direction, grads)
         if self.source is not None and self.source.trainable:
             grads[self.source] = self.source.backpropagate_with_gradient(du)
         return losses, grads
 
-    def capture_withgradient(self, loss, x: Optional[ScalarField] = None) -> "CapturedGradient":
+    def capture_withgradient(self, loss, x: Optional[ScalarField] = None, weights=None) -> "CapturedGradient":
         """Record the whole ``withgradient`` launch sequence (forward with saves, metric and
         pullback, adjoint with gradients) once as a HIP graph; ``replay()`` then reruns it with
         one graph launch and returns ``(value, grads)`` in the same buffers every time.
@@ -222,27 +224,32 @@ class OpticalSystem:
         between replays.  One eager call runs first so that every plan, table and workspace
         exists before capture.
         """
-        self._withgradient_launches(loss, x)
+        if weights is not None:  # a device copy that lives as long as the graph
+            weights = torch.as_tensor(weights, dtype=torch.float64).cuda()
+        self._withgradient_launches(loss, x, weights)
         torch.cuda.synchronize()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
-            losses, grads = self._withgradient_launches(loss, x)
-        return CapturedGradient(graph, losses, grads)
+            losses, grads = self._withgradient_launches(loss, x, weights)
+        return CapturedGradient(graph, losses, grads, weights)
 
 
-def _total(losses):
-    return sum(l.sum() for l in losses)
+def _total(losses, weights=None):
+    if weights is None:
+        return sum(l.sum() for l in losses)
+    w = torch.as_tensor(weights, dtype=torch.float64, device=losses[0].device).reshape(-1)
+    return sum((l * w).sum() for l in losses)
 
 
 class CapturedGradient:
     """A recorded ``withgradient`` (see ``OpticalSystem.capture_withgradient``)."""
 
-    def __init__(self, graph, losses, grads):
-        self.graph, self.losses, self.grads = graph, losses, grads
+    def __init__(self, graph, losses, grads, weights=None):
+        self.graph, self.losses, self.grads, self.weights = graph, losses, grads, weights
 
     def replay(self):
         self.graph.replay()
-        return _total(self.losses), self.grads
+        return _total(self.losses, self.weights), self.grads
 
 
 # --------------------------------------------------------------------------- executor

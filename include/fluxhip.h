@@ -203,6 +203,17 @@ int fh_group_propagate(fh_group* group, void* const* fields, int direction);
 int fh_group_graph_count(fh_group* group, int* count);
 int fh_group_destroy(fh_group* group);
 
+/* DotProduct and PowerCoupling, mode-selective (metrics/dot_product.jl:60-106,
+ * power_coupling.jl:55-69): c[b] = ds * sum u_b conj(v_b) into the device complex128 array c
+ * (FP64, fixed summation order), and p[b] = |c[b]|^2 when p is not NULL.  The pullback writes
+ * du_b = v_b * g_b with g_b = dc[b] (DotProduct; dc complex128) or, when dc is NULL,
+ * g_b = 2 c[b] dp[b] (PowerCoupling; dp doubles, NULL = ones).  As in the reference, the
+ * pullback carries no ds factor. */
+int fh_dot_product(fh_ctx* ctx, const void* u, const void* v, int nx, int ny, int batch, double ds,
+                   void* c, double* p);
+int fh_dot_product_pullback(fh_ctx* ctx, const void* v, const void* dc, const void* c, const double* dp,
+                            int nx, int ny, int batch, void* du);
+
 #ifdef __cplusplus
 }
 #endif

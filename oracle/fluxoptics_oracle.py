@@ -425,6 +425,35 @@ def sq_intensity_diff(v, target_intensity):
     return float(vals.sum()), dv.reshape(np.shape(v))
 
 
+def dot_product(u, v, ds):
+    """Mode-selective ``DotProduct`` (``metrics/dot_product.jl:60-72``): per field
+    c_b = sum u_b conj(v_b) * dx * dy (the code's form; its docstring writes <u, v>)."""
+    u, v = _as_batch(np.asarray(u, np.complex128)), _as_batch(np.asarray(v, np.complex128))
+    return (u * np.conj(v)).sum(axis=(-2, -1)) * (ds[0] * ds[1])
+
+
+def dot_product_pullback(v, dc):
+    """``backpropagate_metric`` of DotProduct (``dot_product.jl:97-106``): du_b = v_b * dc_b
+    (the reference applies no ds factor here)."""
+    v = _as_batch(np.asarray(v, np.complex128))
+    return v * np.asarray(dc, np.complex128).reshape(-1, 1, 1)
+
+
+def power_coupling(u, v, ds):
+    """Mode-selective ``PowerCoupling`` (``power_coupling.jl:47-64``): targets normalised to
+    unit power, p_b = |c_b|^2 with c the DotProduct against the normalised targets.  Returns
+    (p, c, v_normalised)."""
+    vh = normalize_power(v, ds)
+    c = dot_product(u, vh, ds)
+    return np.abs(c) ** 2, c, vh
+
+
+def power_coupling_pullback(vh, c, dp=None):
+    """``power_coupling.jl:66-69``: dc = 2 c dp, then the DotProduct pullback."""
+    dp = np.ones(len(c)) if dp is None else np.asarray(dp, np.float64)
+    return dot_product_pullback(vh, 2.0 * c * dp)
+
+
 def half_sq_field_diff(v, t):
     """L = 1/2 sum|v - t|^2 and its cotangent v - t (harness loss for the fwd+adjoint unit)."""
     d = np.asarray(v) - np.asarray(t)

@@ -177,3 +177,54 @@ def test_captured_withgradient_replays_match_eager():
         torch.cuda.synchronize()
         assert v == float(ev), rep
         assert torch.equal(g1, eg[doe1]) and torch.equal(g2, eg[doe2]), rep
+
+
+def test_dot_product_and_power_coupling_on_device():
+    """Values against the oracle (complex FP64 sums of FP32 data) and the pullbacks; the
+    reference's doctest values (1 per normalised mode)."""
+    from test_system_host import _doc_modes
+    u = fo.ScalarField(to_dev(_doc_modes()), (1.0, 1.0), 1.064)
+    (p,) = fo.PowerCoupling(u)(u)
+    assert np.allclose(p.cpu().numpy(), [1.0, 1.0], atol=1e-7)
+    (c,) = fo.DotProduct(u)(u)
+    assert np.allclose(c.cpu().numpy(), [1.0, 1.0], atol=1e-7)
+    b, ny, nx, ds = 3, 128, 256, (0.5, 0.5)
+    u0 = c64(inputs.normal(21, (b, ny, nx)) + 1j * inputs.normal(22, (b, ny, nx)))
+    v0 = c64(inputs.normal(23, (b, ny, nx)) + 1j * inputs.normal(24, (b, ny, nx)))
+    u = fo.ScalarField(to_dev(u0), ds, 0.8)
+    v = fo.ScalarField(to_dev(v0), ds, 0.8)
+    dm = fo.DotProduct(v)
+    (c,) = dm(u)
+    assert np.allclose(c.cpu().numpy(), O.dot_product(u0, v0, ds), rtol=1e-10, atol=0)
+    dc = np.array([0.3 - 0.7j, -1.1 + 0.2j, 2.0 + 0.0j])
+    (du,) = dm.backpropagate_metric([u], [torch.from_numpy(dc)])
+    assert rel_l2(to_host(du.electric), O.dot_product_pullback(v0, dc)) <= FIELD_TOL
+    pm = fo.PowerCoupling(v)
+    (p,) = pm(u)
+    p_ref, c_ref, vh = O.power_coupling(u0, v0, ds)
+    assert np.allclose(p.cpu().numpy(), p_ref, rtol=1e-6, atol=0)  # targets normalised in FP32
+    dp = np.array([1.0, -0.5, 2.0])
+    (du,) = pm.backpropagate_metric([u], [torch.from_numpy(dp)])
+    assert rel_l2(to_host(du.electric), O.power_coupling_pullback(vh, c_ref, dp)) <= FIELD_TOL
+
+
+def test_mode_coupling_system_gradient():
+    """Maximise the power coupled into a shifted Gaussian through Phase -> ASProp: the mask
+    gradient of -PowerCoupling against the oracle chain."""
+    n, ds, lam, z = 256, 1.0, 0.8, 600.0
+    u0 = c64(inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 20.0), ds, ds))
+    t0 = inputs.gaussian(n, n, ds, ds, 15.0, 25.0, -10.0)
+    phi = inputs.uniform(91, (n, n), 0, 1).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), (ds, ds), lam)
+    t = fo.ScalarField(to_dev(t0), (ds, ds), lam)
+    doe = fo.Phase(u, phi, trainable=True, buffered=True)
+    system = fo.ScalarSource(u) | doe | fo.ASProp(u, z)
+    value, grads = system.withgradient(fo.PowerCoupling(t), weights=-1.0)
+    stages = [("phase", phi.astype(np.float64)), ("as", z)]
+    v_ref, saved = O.pipeline_forward(u0, stages, (ds, ds), lam)
+    p_ref, c_ref, vh = O.power_coupling(v_ref, c64(t0), (ds, ds))
+    dv = O.power_coupling_pullback(vh, c_ref, [-1.0])
+    _, (g_ref,) = O.pipeline_backward(dv, stages, (ds, ds), lam, saved)
+    assert abs(float(value) + p_ref.sum()) <= FIELD_TOL * p_ref.sum()
+    torch.cuda.synchronize()
+    assert rel_l2(doe.grad.cpu().numpy(), g_ref) <= GRAD_TOL

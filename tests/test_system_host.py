@@ -96,3 +96,47 @@ def test_fourier_operator_merge_and_wrapper():
     p2 = fo.ASProp(u, 2.0)
     sm = fo.OpticalSystem(None, w, fo.FourierWrapper(u, p2), merge_components=True)
     assert [type(c).__name__ for c in sm.components] == ["FourierOperator", "Phase", "ASProp", "FourierOperator"]
+
+
+def _doc_modes():
+    """``dot_product.jl`` / ``power_coupling.jl`` doctest field: ones(4, 4, 2) with the second
+    mode's rows 3:4 (x) set to -1, power-normalised; as (B, ny, nx): u[1, :, 2:] = -1."""
+    u = np.ones((2, 4, 4), complex)
+    u[1, :, 2:] = -1
+    return O.normalize_power(u, (1.0, 1.0))
+
+
+def test_dot_and_power_coupling_reference_doctests():
+    u = _doc_modes()
+    assert np.allclose(O.dot_product(u, u, (1.0, 1.0)), [1.0, 1.0], atol=1e-15)
+    p, c, vh = O.power_coupling(u, u, (1.0, 1.0))
+    assert np.allclose(p, [1.0, 1.0], atol=1e-15)
+    # the two modes are orthogonal (the non-selective doctest's off-diagonal zeros)
+    assert abs(O.dot_product(u[0], u[1], (1.0, 1.0))[0]) < 1e-15
+
+
+@pytest.mark.parametrize("which", ["dot", "power"])
+def test_dot_and_power_pullbacks_by_finite_differences(which):
+    """At ds = 1 the pullbacks equal the gradient of Re(sum conj(dc) c) (DotProduct) and of
+    sum dp |c|^2 (PowerCoupling), checked by central differences."""
+    b, n = 2, 5
+    u = inputs.normal(11, (b, n, n)) + 1j * inputs.normal(12, (b, n, n))
+    v = inputs.normal(13, (b, n, n)) + 1j * inputs.normal(14, (b, n, n))
+    ds = (1.0, 1.0)
+    if which == "dot":
+        dc = np.array([0.3 - 0.7j, -1.1 + 0.2j])
+        f = lambda w: float(np.real(np.sum(np.conj(dc) * O.dot_product(w, v, ds))))
+        g = O.dot_product_pullback(v, dc)
+    else:
+        dp = np.array([0.5, -2.0])
+        f = lambda w: float(np.sum(dp * O.power_coupling(w, v, ds)[0]))
+        _, c, vh = O.power_coupling(u, v, ds)
+        g = O.power_coupling_pullback(vh, c, dp)
+    h = 1e-6
+    for (bb, i, j) in [(0, 1, 2), (1, 4, 0), (1, 2, 3)]:
+        e = np.zeros_like(u)
+        e[bb, i, j] = h
+        gr = (f(u + e) - f(u - e)) / (2 * h)
+        gi = (f(u + 1j * e) - f(u - 1j * e)) / (2 * h)
+        assert abs(gr - g[bb, i, j].real) <= 1e-6 * (1 + abs(gr))
+        assert abs(gi - g[bb, i, j].imag) <= 1e-6 * (1 + abs(gi))
