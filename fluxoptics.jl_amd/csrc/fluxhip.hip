@@ -366,6 +366,85 @@ __global__ void k_phase_backward(float2* __restrict__ dv, const float* __restric
   }
 }
 
+// Mask (modulators/mask.jl:95-141): u <- u * m (Forward) or u * conj(m) (Backward), m a c64
+// [ny][nx] transmission shared by the batch, saving u first when saved != nullptr.
+__global__ void k_mask_apply(float2* __restrict__ u, const float2* __restrict__ m, float2* __restrict__ saved,
+                             long npix, int batch, int conj) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
+    const float2 w = m[i];
+    for (int b = 0; b < batch; ++b) {
+      const long k = b * npix + i;
+      const float2 v = u[k];
+      if (saved) saved[k] = v;
+      u[k] = conj ? cmulc(v, w) : cmul(v, w);
+    }
+  }
+}
+
+// dm (+)= sum_b conj_direction(dv conj(saved)) from the cotangent at the mask output, then
+// du = dv * conj_direction(m, reverse(direction)) in place (mask.jl:119-141)
+__global__ void k_mask_backward(float2* __restrict__ dv, const float2* __restrict__ m,
+                                const float2* __restrict__ saved, float2* __restrict__ dm, long npix,
+                                int batch, int forward, int acc) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
+    const float2 w = m[i];
+    float gr = 0.f, gi = 0.f;
+    for (int b = 0; b < batch; ++b) {
+      const long k = b * npix + i;
+      const float2 d = dv[k];
+      if (dm) {
+        const float2 g = cmulc(d, saved[k]);  // dv * conj(saved)
+        gr += g.x;
+        gi += forward ? g.y : -g.y;
+      }
+      dv[k] = forward ? cmulc(d, w) : cmul(d, w);
+    }
+    if (dm) dm[i] = acc ? make_float2(dm[i].x + gr, dm[i].y + gi) : make_float2(gr, gi);
+  }
+}
+
+// TeaDOE / TeaReflector (modulators/tea_doe.jl:146-199): per field b with kl_b = 2 pi dn(l_b)/l_b
+// and complex reflectivity r_b, u_b <- u_b * r_b * cis(kl_b h) (Forward) or
+// conj(r_b) * cis(-kl_b h) (Backward); h is an FP32 [ny][nx] height map.
+__global__ void k_tea_apply(float2* __restrict__ u, const float* __restrict__ h, float2* __restrict__ saved,
+                            long npix, int batch, const double* __restrict__ kl, const double2* __restrict__ r,
+                            float s) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
+    const float hi = h[i];
+    for (int b = 0; b < batch; ++b) {
+      const long k = b * npix + i;
+      float sn, cs;
+      sincosf(s * ((float)kl[b] * hi), &sn, &cs);
+      float2 w = make_float2(cs, sn);
+      if (r) w = cmul(w, make_float2((float)r[b].x, s * (float)r[b].y));
+      const float2 v = u[k];
+      if (saved) saved[k] = v;
+      u[k] = cmul(v, w);
+    }
+  }
+}
+
+// du = dv * conj(t) in place, then dh (+)= sum_b s kl_b Im(du conj(saved)) (tea_doe.jl:180-199)
+__global__ void k_tea_backward(float2* __restrict__ dv, const float* __restrict__ h,
+                               const float2* __restrict__ saved, float* __restrict__ dh, long npix, int batch,
+                               const double* __restrict__ kl, const double2* __restrict__ r, float s, int acc) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
+    const float hi = h[i];
+    float g = 0.f;
+    for (int b = 0; b < batch; ++b) {
+      const long k = b * npix + i;
+      float sn, cs;
+      sincosf(-s * ((float)kl[b] * hi), &sn, &cs);
+      float2 w = make_float2(cs, sn);
+      if (r) w = cmul(w, make_float2((float)r[b].x, -s * (float)r[b].y));
+      const float2 du = cmul(dv[k], w);
+      dv[k] = du;
+      if (dh) g += (float)kl[b] * cmulc(du, saved[k]).y;
+    }
+    if (dh) dh[i] = (acc ? dh[i] : 0.f) + s * g;
+  }
+}
+
 // ================================================================ metrics
 // SquaredIntensityDifference (squared_intensity_difference.jl:78-90): d = |u_b|^2 - I_b per
 // field ("full dimensions") or d = sum_b |u_b|^2 - I ("spatial only"); loss = sum d^2 and the
@@ -1410,6 +1489,66 @@ int fh_dot_product_pullback(fh_ctx* ctx, const void* v, const void* dc, const vo
   hipLaunchKernelGGL(k_dot_pullback, dim3(blocks, batch), dim3(256), 0, ctx->stream,
                      reinterpret_cast<const float2*>(v), reinterpret_cast<const double2*>(dc),
                      reinterpret_cast<const double2*>(c), dp, npix, reinterpret_cast<float2*>(du));
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+static int elementwise_blocks(long npix) { return (int)std::min<long>((npix + 255) / 256, 4096); }
+
+int fh_mask_apply(fh_ctx* ctx, void* u, const void* m, int nx, int ny, int batch, int direction, void* saved) {
+  if (!ctx || !u || !m) return fail(FH_E_ARG, "null context, field or mask");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(ctx))) return rc;
+  const long npix = (long)nx * ny;
+  hipLaunchKernelGGL(k_mask_apply, dim3(elementwise_blocks(npix)), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<float2*>(u), reinterpret_cast<const float2*>(m),
+                     reinterpret_cast<float2*>(saved), npix, batch, direction < 0);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_mask_backward(fh_ctx* ctx, void* dv, const void* m, const void* saved, void* dm, int nx, int ny,
+                     int batch, int direction, int accumulate) {
+  if (!ctx || !dv || !m) return fail(FH_E_ARG, "null context, cotangent or mask");
+  if (dm && !saved) return fail(FH_E_ARG, "the mask gradient needs the saved mask input");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(ctx))) return rc;
+  const long npix = (long)nx * ny;
+  hipLaunchKernelGGL(k_mask_backward, dim3(elementwise_blocks(npix)), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<float2*>(dv), reinterpret_cast<const float2*>(m),
+                     reinterpret_cast<const float2*>(saved), reinterpret_cast<float2*>(dm), npix, batch,
+                     direction > 0, accumulate);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_tea_apply(fh_ctx* ctx, void* u, const float* h, int nx, int ny, int batch, const double* kl,
+                 const double* r, int direction, void* saved) {
+  if (!ctx || !u || !h || !kl) return fail(FH_E_ARG, "null context, field, height map or coefficients");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(ctx))) return rc;
+  const long npix = (long)nx * ny;
+  hipLaunchKernelGGL(k_tea_apply, dim3(elementwise_blocks(npix)), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<float2*>(u), h, reinterpret_cast<float2*>(saved), npix, batch, kl,
+                     reinterpret_cast<const double2*>(r), (float)direction);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_tea_backward(fh_ctx* ctx, void* dv, const float* h, const void* saved, float* dh, int nx, int ny,
+                    int batch, const double* kl, const double* r, int direction, int accumulate) {
+  if (!ctx || !dv || !h || !kl) return fail(FH_E_ARG, "null context, cotangent, height map or coefficients");
+  if (dh && !saved) return fail(FH_E_ARG, "the height gradient needs the saved input");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(ctx))) return rc;
+  const long npix = (long)nx * ny;
+  hipLaunchKernelGGL(k_tea_backward, dim3(elementwise_blocks(npix)), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<float2*>(dv), h, reinterpret_cast<const float2*>(saved), dh, npix, batch,
+                     kl, reinterpret_cast<const double2*>(r), (float)direction, accumulate);
   FH_HIP(hipGetLastError());
   return FH_OK;
 }

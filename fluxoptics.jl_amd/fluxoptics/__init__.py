@@ -6,15 +6,16 @@ the arithmetic is the HIP library's (``libfluxhip.so``, C ABI in include/fluxhip
 
 from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficiency,
                      rs_valid_distance)
-from .components import (AS_BPM, ASProp, FourierOperator, FourierWrapper, Phase, RSProp,
-                         phase_propagate_, propagate_phase_backward_)
+from .components import (AS_BPM, ASProp, FourierMask, FourierOperator, FourierPhase, FourierWrapper,
+                         Mask, Phase, RSProp, TeaDOE, TeaReflector, phase_propagate_,
+                         propagate_phase_backward_)
 from .group import PropagationGroup
 from .metrics import DotProduct, PowerCoupling, SquaredIntensityDifference
 from .system import OpticalSequence, OpticalSystem, ScalarSource, merge
 
 __all__ = [
     "AS_BPM", "ASProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
-    "RSProp", "ScalarField",
+    "RSProp", "ScalarField", "Mask", "TeaDOE", "TeaReflector", "FourierPhase", "FourierMask",
     "OpticalSequence", "OpticalSystem", "PropagationGroup", "ScalarSource", "SquaredIntensityDifference", "DotProduct", "PowerCoupling", "merge",
     "coupling_efficiency", "phase_propagate_", "propagate_phase_backward_", "propagate",
     "propagate_", "backpropagate_", "rs_valid_distance",

@@ -363,6 +363,170 @@ class Phase(_PipeComponent):
         return dv, dphi
 
 
+# --------------------------------------------------------------------------- Mask, TeaDOE
+
+def _grid_array(u: ScalarField, ds, f, dtype):
+    """A function (x, y) -> value on the centred grid (``function_to_array``,
+    ``OpticalComponents.jl:927-935``) or an array, as a (ny, nx) tensor on the field's device."""
+    ds = tuple(ds if ds is not None else u.ds)
+    if callable(f):
+        x = _spatial(u.nx, ds[0])[None, :]
+        y = _spatial(u.ny, ds[1])[:, None]
+        arr = np.broadcast_to(np.vectorize(f, otypes=[complex])(x, y), (u.ny, u.nx))
+        if dtype == torch.float32:
+            arr = arr.real
+    else:
+        arr = f
+    t = torch.as_tensor(np.asarray(arr) if not isinstance(arr, torch.Tensor) else arr, device=u.electric.device)
+    t = t.to(dtype).contiguous()
+    if tuple(t.shape) != (u.ny, u.nx):
+        raise ValueError(f"array shape {tuple(t.shape)} does not match the field ({u.ny}, {u.nx})")
+    return t
+
+
+class _Modulator(_PipeComponent):
+    """Trainable elementwise modulators: the component contract of
+    ``OpticalComponents.jl:726-816`` (``propagate_and_save!``,
+    ``backpropagate_with_gradient!`` with a pre-allocated gradient when buffered)."""
+
+    def _init_training(self, u: ScalarField, trainable: bool, buffered: bool):
+        self.trainable = trainable
+        self.buffered = trainable and buffered
+        self.grad = torch.zeros_like(self._param) if self.buffered else None
+        self.saved = torch.empty_like(u.electric) if self.buffered else None
+
+    def propagate(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self.propagate_(u.copy(), direction)
+
+    def backpropagate_(self, dv: ScalarField, direction=Forward) -> ScalarField:
+        return self._backward(dv, None, None, direction, False)
+
+    def alloc_saved_buffer(self, u: ScalarField) -> torch.Tensor:
+        return torch.empty_like(u.electric)
+
+    def propagate_and_save_(self, u: ScalarField, saved: Optional[torch.Tensor] = None,
+                            direction=Forward) -> ScalarField:
+        saved = self.saved if saved is None else saved
+        check_field(saved, "saved buffer")
+        return self._apply(u, saved, direction)
+
+    def backpropagate_with_gradient_(self, dv: ScalarField, saved: torch.Tensor, grad=None,
+                                     direction=Forward, accumulate: bool = False):
+        grad = self.grad if grad is None else grad
+        if grad is None:
+            grad = torch.empty_like(self._param)
+        check_field(saved, "saved buffer")
+        self._backward(dv, saved, grad, direction, accumulate)
+        return dv, grad
+
+    def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self._apply(u, None, direction)
+
+
+class Mask(_Modulator):
+    """``Mask(u, f; trainable, buffered)`` (``modulators/mask.jl``): u -> u * m(x, y), complex
+    transmission (conjugated in the Backward direction); the gradient is complex."""
+
+    def __init__(self, u: ScalarField, f=None, ds=None, *, trainable: bool = False, buffered: bool = False):
+        self.m = _grid_array(u, ds, (lambda x, y: 1.0) if f is None else f, torch.complex64)
+        self._init_training(u, trainable, buffered)
+
+    @property
+    def _param(self):
+        return self.m
+
+    def _apply(self, u, saved, direction):
+        check_field(u.electric)
+        _lib.call("fh_mask_apply", context(u.electric.device), _ptr(u.electric), _ptr(self.m), u.nx, u.ny, u.batch,
+                  dir_sign(direction), _ptr(saved))
+        return u
+
+    def _backward(self, dv, saved, grad, direction, accumulate):
+        check_field(dv.electric)
+        _lib.call("fh_mask_backward", context(dv.electric.device), _ptr(dv.electric), _ptr(self.m), _ptr(saved),
+                  _ptr(grad), dv.nx, dv.ny, dv.batch, dir_sign(direction), int(accumulate))
+        return dv
+
+
+class TeaDOE(_Modulator):
+    """``TeaDOE(u, dn, f; r=1, trainable, buffered)`` (``modulators/tea_doe.jl``): a height map
+    h(x, y) with transmission r(lambda) * cis(2 pi dn(lambda) h / lambda) per field
+    wavelength; ``dn`` and ``r`` are numbers or functions of lambda (evaluated on the host
+    once per wavelength set)."""
+
+    def __init__(self, u: ScalarField, dn, f=None, ds=None, *, r=1.0, trainable: bool = False,
+                 buffered: bool = False):
+        self.h = _grid_array(u, ds, (lambda x, y: 0.0) if f is None else f, torch.float32)
+        self.dn = dn
+        self.r = r
+        self._coef = {}
+        self._init_training(u, trainable, buffered)
+
+    @property
+    def _param(self):
+        return self.h
+
+    def _coefficients(self, u: ScalarField):
+        lams = tuple(u.lambda_list())
+        key = (u.electric.device.index, lams)
+        c = self._coef.get(key)
+        if c is None:
+            dn = [float(self.dn(l)) if callable(self.dn) else float(self.dn) for l in lams]
+            kl = torch.tensor([2.0 * np.pi * d / l for d, l in zip(dn, lams)], dtype=torch.float64,
+                              device=u.electric.device)
+            rs = [complex(self.r(l)) if callable(self.r) else complex(self.r) for l in lams]
+            r = None
+            if any(x != 1 for x in rs):
+                r = torch.tensor([[x.real, x.imag] for x in rs], dtype=torch.float64, device=u.electric.device)
+            c = (kl, r)
+            self._coef[key] = c
+        return c
+
+    def _apply(self, u, saved, direction):
+        check_field(u.electric)
+        kl, r = self._coefficients(u)
+        _lib.call("fh_tea_apply", context(u.electric.device), _ptr(u.electric), _ptr(self.h), u.nx, u.ny, u.batch,
+                  _ptr(kl), _ptr(r), dir_sign(direction), _ptr(saved))
+        return u
+
+    def _backward(self, dv, saved, grad, direction, accumulate):
+        check_field(dv.electric)
+        kl, r = self._coefficients(dv)
+        _lib.call("fh_tea_backward", context(dv.electric.device), _ptr(dv.electric), _ptr(self.h), _ptr(saved),
+                  _ptr(grad), dv.nx, dv.ny, dv.batch, _ptr(kl), _ptr(r), dir_sign(direction), int(accumulate))
+        return dv
+
+
+def TeaReflector(u: ScalarField, f=None, ds=None, *, r=1.0, trainable: bool = False, buffered: bool = False):
+    """``TeaReflector(u, f; r)`` (``tea_doe.jl:110-131``): a TeaDOE with dn = 2 (phase 4 pi h / lambda)."""
+    return TeaDOE(u, 2.0, f, ds, r=r, trainable=trainable, buffered=buffered)
+
+
+def _fourier_array(u: ScalarField, ds, f, dtype):
+    """f(fx, fy) on the FFT-order frequency grid (``function_to_array(..., true)``)."""
+    ds = tuple(ds if ds is not None else u.ds)
+    if not callable(f):
+        return f
+    fx = np.fft.fftfreq(u.nx, ds[0])[None, :]
+    fy = np.fft.fftfreq(u.ny, ds[1])[:, None]
+    return np.broadcast_to(np.vectorize(f, otypes=[complex if dtype == torch.complex64 else float])(fx, fy),
+                           (u.ny, u.nx))
+
+
+def FourierPhase(u: ScalarField, f=None, ds=None, *, trainable: bool = False, buffered: bool = False):
+    """``FourierPhase(u, f)`` (``fourier/fourier_phase.jl``): a Phase on the spectrum,
+    FFT -> Phase(f(fx, fy)) -> IFFT."""
+    arr = _fourier_array(u, ds, (lambda a, b: 0.0) if f is None else f, torch.float32)
+    return FourierWrapper(u, Phase(u, np.asarray(arr, dtype=np.float64), ds, trainable=trainable,
+                                   buffered=buffered))
+
+
+def FourierMask(u: ScalarField, f=None, ds=None, *, trainable: bool = False, buffered: bool = False):
+    """``FourierMask(u, f)`` (``fourier/fourier_mask.jl``): a Mask on the spectrum."""
+    arr = _fourier_array(u, ds, (lambda a, b: 1.0) if f is None else f, torch.complex64)
+    return FourierWrapper(u, Mask(u, np.asarray(arr), ds, trainable=trainable, buffered=buffered))
+
+
 # --------------------------------------------------------------------------- fused pairs
 
 def phase_propagate_(u: ScalarField, phase: Phase, prop: _KernelPropagator,

@@ -23,7 +23,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 import torch
 
 from .components import (AS_BPM, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
-                         check_field)
+                         _Modulator, check_field)
 from .components import phase_propagate_, propagate_phase_backward_
 from .fields import Backward, Forward, ScalarField, dir_sign
 
@@ -281,6 +281,11 @@ def _run_forward(comps: List, u: ScalarField, direction, save: bool) -> list:
                 else torch.empty_like(u.electric)
             c.propagate_and_save_(u, saved, direction)
             tape.append(("phase", (c,), saved))
+        elif save and isinstance(c, _Modulator) and c.trainable:
+            saved = c.saved if (c.saved is not None and not any(t[2] is c.saved for t in tape)) \
+                else torch.empty_like(u.electric)
+            c.propagate_and_save_(u, saved, direction)
+            tape.append(("mod", (c,), saved))
         elif save and isinstance(c, AS_BPM) and c.trainable:
             if any(t[0] == "bpm" and t[1][0] is c for t in tape):
                 raise NotImplementedError("a trainable AS_BPM may appear once in a differentiated system "
@@ -316,6 +321,10 @@ def _run_backward(comps: List, tape: list, du: ScalarField, direction, grads: Di
             (ph,) = cs
             g, acc = _grad_buffer(grads, ph, ph.phi)
             ph.backpropagate_with_gradient_(du, saved, g, direction, accumulate=acc)
+        elif kind == "mod":
+            (c,) = cs
+            g, acc = _grad_buffer(grads, c, c._param)
+            c.backpropagate_with_gradient_(du, saved, g, direction, accumulate=acc)
         elif kind == "bpm":
             (m,) = cs
             if m in grads:

@@ -203,6 +203,23 @@ int fh_group_propagate(fh_group* group, void* const* fields, int direction);
 int fh_group_graph_count(fh_group* group, int* count);
 int fh_group_destroy(fh_group* group);
 
+/* Mask (modulators/mask.jl:95-141): u <- u * m (Forward) or u * conj(m) (Backward) with m a
+ * c64 [nx, ny] device transmission shared by the batch; saved (nullable) receives the input.
+ * fh_mask_backward: dm (+)= sum_b conj_direction(dv * conj(saved)) (complex gradient, from the
+ * cotangent at the mask output), then dv <- dv * conj_direction(m, reverse(direction)). */
+int fh_mask_apply(fh_ctx* ctx, void* u, const void* m, int nx, int ny, int batch, int direction, void* saved);
+int fh_mask_backward(fh_ctx* ctx, void* dv, const void* m, const void* saved, void* dm, int nx, int ny,
+                     int batch, int direction, int accumulate);
+
+/* TeaDOE / TeaReflector (modulators/tea_doe.jl:146-199): per field b, with the device arrays
+ * kl[b] = 2 pi dn(lambda_b) / lambda_b and r (2*batch doubles: re, im; NULL = 1), u_b <- u_b *
+ * r_b * cis(kl_b h) (Forward) or conj(r_b) * cis(-kl_b h) (Backward); h is an FP32 [nx, ny]
+ * height map.  fh_tea_backward: dv <- dv * conj(t), then dh (+)= s * sum_b kl_b Im(dv conj(saved)). */
+int fh_tea_apply(fh_ctx* ctx, void* u, const float* h, int nx, int ny, int batch, const double* kl,
+                 const double* r, int direction, void* saved);
+int fh_tea_backward(fh_ctx* ctx, void* dv, const float* h, const void* saved, float* dh, int nx, int ny,
+                    int batch, const double* kl, const double* r, int direction, int accumulate);
+
 /* DotProduct and PowerCoupling, mode-selective (metrics/dot_product.jl:60-106,
  * power_coupling.jl:55-69): c[b] = ds * sum u_b conj(v_b) into the device complex128 array c
  * (FP64, fixed summation order), and p[b] = |c[b]|^2 when p is not NULL.  The pullback writes

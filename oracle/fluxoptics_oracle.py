@@ -321,6 +321,47 @@ def phase_backward(dv, u_saved, phi, direction: int = 1):
     return du, phase_grad(du, u_saved, direction)
 
 
+def mask_apply(u, m, direction: int = 1):
+    """``Mask`` ``propagate!`` (``modulators/mask.jl:95-101``): u * m, or u * conj(m) Backward."""
+    u = _as_batch(np.asarray(u, np.complex128))
+    m = np.asarray(m, np.complex128)
+    return u * (m if direction > 0 else np.conj(m))
+
+
+def mask_backward(dv, u_saved, m, direction: int = 1):
+    """``Mask`` ``backpropagate!`` with gradient (``mask.jl:112-141``): dm = sum_b
+    conj_direction(dv conj(u_saved)) from the cotangent at the mask output, then
+    du = dv * conj_direction(m, reverse(direction)).  Returns (du, dm)."""
+    dv = _as_batch(np.asarray(dv, np.complex128))
+    g = (dv * np.conj(_as_batch(u_saved))).sum(axis=0)
+    dm = g if direction > 0 else np.conj(g)
+    m = np.asarray(m, np.complex128)
+    return dv * (np.conj(m) if direction > 0 else m), dm
+
+
+def tea_transmission(h, lam: float, dn: float, r: complex = 1.0, direction: int = 1):
+    """``TeaDOE`` transmission (``modulators/tea_doe.jl:146-152``): r cis(2 pi dn h / lam)
+    Forward, conj(r) cis(-2 pi dn h / lam) Backward."""
+    ph = TWO_PI * dn * np.asarray(h, np.float64) / lam
+    return (r if direction > 0 else np.conj(r)) * np.exp(1j * direction * ph)
+
+
+def tea_apply(u, h, lams, dn, r=1.0, direction: int = 1):
+    u = _as_batch(np.asarray(u, np.complex128))
+    lam_b = _lambdas(lams, u.shape[0])
+    return np.stack([u[b] * tea_transmission(h, lam_b[b], dn, r, direction) for b in range(u.shape[0])])
+
+
+def tea_backward(dv, u_saved, h, lams, dn, r=1.0, direction: int = 1):
+    """``tea_doe.jl:180-199``: du = dv * conj(t); dh = s * sum_b (2 pi dn / lam_b) Im(du conj(u_saved))."""
+    dv = _as_batch(np.asarray(dv, np.complex128))
+    lam_b = _lambdas(lams, dv.shape[0])
+    du = np.stack([dv[b] * np.conj(tea_transmission(h, lam_b[b], dn, r, direction)) for b in range(dv.shape[0])])
+    us = _as_batch(u_saved)
+    dh = sum(TWO_PI * dn / lam_b[b] * np.imag(du[b] * np.conj(us[b])) for b in range(dv.shape[0]))
+    return du, direction * dh
+
+
 # ----------------------------------------------------------------------------- AS_BPM
 
 def bpm_kdz(dz: float, tilts=(0.0, 0.0)) -> float:
@@ -474,8 +515,9 @@ def pipeline_forward(u, stages, ds, lams):
     """Run a list of stages forward, saving the inputs of every Phase.
 
     ``OpticalSequence`` forward loop (``optical_sequence.jl:54-60``) with trainable
-    ``Phase`` using ``propagate_and_save!`` (``phasemask.jl:100-113``).  A stage is
-    ``("as", z, n0)``, ``("rs", z, pad_mode)`` or ``("phase", phi)``.
+    modulators using ``propagate_and_save!`` (``phasemask.jl:100-113``).  A stage is
+    ``("as", z, n0)``, ``("rs", z, pad_mode)``, ``("phase", phi)``, ``("mask", m)`` or
+    ``("tea", h, dn, r)``.
     """
     u = _as_batch(np.asarray(u, dtype=np.complex128))
     saved = []
@@ -487,6 +529,12 @@ def pipeline_forward(u, stages, ds, lams):
         elif st[0] == "phase":
             saved.append(u.copy())
             u = phase_apply(u, st[1], 1)
+        elif st[0] == "mask":
+            saved.append(u.copy())
+            u = mask_apply(u, st[1], 1)
+        elif st[0] == "tea":
+            saved.append(u.copy())
+            u = tea_apply(u, st[1], lams, st[2], st[3] if len(st) > 3 else 1.0, 1)
         else:
             raise ValueError(st[0])
     return u, saved
@@ -506,6 +554,14 @@ def pipeline_backward(dv, stages, ds, lams, saved):
         elif st[0] == "phase":
             k -= 1
             du, g = phase_backward(du, saved[k], st[1], 1)
+            grads.append(g)
+        elif st[0] == "mask":
+            k -= 1
+            du, g = mask_backward(du, saved[k], st[1], 1)
+            grads.append(g)
+        elif st[0] == "tea":
+            k -= 1
+            du, g = tea_backward(du, saved[k], st[1], lams, st[2], st[3] if len(st) > 3 else 1.0, 1)
             grads.append(g)
     return du, grads[::-1]
 

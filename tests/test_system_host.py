@@ -11,6 +11,8 @@ import fluxoptics as fo
 from fluxoptics import inputs
 from oracle import fluxoptics_oracle as O
 
+TWO_PI = 2.0 * math.pi
+
 
 def test_sqint_reference_doctest_values():
     """``src/metrics/squared_intensity_difference.jl:19-45``: ones(4,4,2) against zeros gives
@@ -140,3 +142,49 @@ def test_dot_and_power_pullbacks_by_finite_differences(which):
         gi = (f(u + 1j * e) - f(u - 1j * e)) / (2 * h)
         assert abs(gr - g[bb, i, j].real) <= 1e-6 * (1 + abs(gr))
         assert abs(gi - g[bb, i, j].imag) <= 1e-6 * (1 + abs(gi))
+
+
+@pytest.mark.parametrize("direction", [1, -1])
+def test_mask_gradient_by_finite_differences(direction):
+    """The complex mask gradient dL/dRe(m) + i dL/dIm(m) of L = 1/2 |v - t|^2, v = mask(u)."""
+    b, n = 2, 5
+    u = inputs.normal(31, (b, n, n)) + 1j * inputs.normal(32, (b, n, n))
+    t = inputs.normal(33, (b, n, n)) + 1j * inputs.normal(34, (b, n, n))
+    m = inputs.normal(35, (n, n)) + 1j * inputs.normal(36, (n, n))
+    f = lambda mm: O.half_sq_field_diff(O.mask_apply(u, mm, direction), t)[0]
+    dv = O.mask_apply(u, m, direction) - t
+    du, dm = O.mask_backward(dv, u, m, direction)
+    h = 1e-6
+    for (i, j) in [(0, 1), (3, 4), (2, 2)]:
+        e = np.zeros((n, n), complex)
+        e[i, j] = h
+        gr = (f(m + e) - f(m - e)) / (2 * h)
+        gi = (f(m + 1j * e) - f(m - 1j * e)) / (2 * h)
+        assert abs(gr - dm[i, j].real) <= 1e-6 * (1 + abs(gr))
+        assert abs(gi - dm[i, j].imag) <= 1e-6 * (1 + abs(gi))
+    # du is the exact adjoint: <mask(x), y> = <x, mask^H(y)>
+    x, y = inputs.normal(37, (b, n, n)) + 0j, inputs.normal(38, (b, n, n)) + 1j
+    assert np.isclose(np.vdot(y, O.mask_apply(x, m, direction)), np.vdot(O.mask_backward(y, x, m, direction)[0], x))
+
+
+def test_tea_doe_reduces_to_phase_and_gradient_by_fd():
+    """With r = 1 a TeaDOE is the Phase 2 pi dn h / lambda per wavelength; its height gradient
+    matches central differences (with a complex r as well)."""
+    b, n, lams, dn = 2, 6, [0.8, 1.064], 0.45
+    u = inputs.normal(41, (b, n, n)) + 1j * inputs.normal(42, (b, n, n))
+    h = inputs.uniform(43, (n, n), 0.0, 2.0)
+    for k in range(b):
+        ref = np.reshape(O.phase_apply(u[k], TWO_PI * dn * h / lams[k], 1), (n, n))
+        assert np.allclose(O.tea_apply(u[k], h, lams[k], dn)[0], ref, rtol=1e-14, atol=0)
+    r = 0.9 * np.exp(0.3j)
+    t = inputs.normal(44, (b, n, n)) + 1j * inputs.normal(45, (b, n, n))
+    for direction in (1, -1):
+        f = lambda hh: O.half_sq_field_diff(O.tea_apply(u, hh, lams, dn, r, direction), t)[0]
+        dv = O.tea_apply(u, h, lams, dn, r, direction) - t
+        _, dh = O.tea_backward(dv, u, h, lams, dn, r, direction)
+        eps = 1e-6
+        for (i, j) in [(0, 0), (2, 5), (4, 1)]:
+            e = np.zeros((n, n))
+            e[i, j] = eps
+            g = (f(h + e) - f(h - e)) / (2 * eps)
+            assert abs(g - dh[i, j]) <= 1e-6 * (1 + abs(g))
