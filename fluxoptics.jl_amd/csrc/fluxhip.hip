@@ -445,6 +445,46 @@ __global__ void k_tea_backward(float2* __restrict__ dv, const float* __restrict_
   }
 }
 
+// Optimiser step with its proximal operator in one pass over the parameter (n FP32 values;
+// complex parameters are their interleaved real and imaginary parts).  Rules: 0 Descent
+// x -= eta dx; 1 Momentum m = rho m + eta dx, x -= m; 2 Nesterov d = -rho^2 v + (1+rho) eta dx,
+// v = rho v - eta dx, x -= d (Optimisers.jl 0.4.6); 3 Fista (OptimisersExt.jl:31-40) with
+// eta already squared: d = eta dx - beta (x - xk), xk = x, x -= d.  Prox
+// (proximal_operators/*.jl): 0 none, 1 clamp to [p0, p1], 2 soft threshold by p0 around p1,
+// 3 negatives to zero.
+__global__ void k_optim_step(float* __restrict__ x, const float* __restrict__ dx, float* __restrict__ st,
+                             long n, int rule, float eta, float rho, float beta, int prox, float p0, float p1) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    float xi = x[i];
+    const float g = dx[i];
+    if (rule == 0) {
+      xi -= eta * g;
+    } else if (rule == 1) {
+      const float m = rho * st[i] + eta * g;
+      st[i] = m;
+      xi -= m;
+    } else if (rule == 2) {
+      const float v = st[i];
+      const float d = -rho * rho * v + (1.f + rho) * eta * g;
+      st[i] = rho * v - eta * g;
+      xi -= d;
+    } else {
+      const float d = eta * g - beta * (xi - st[i]);
+      st[i] = xi;
+      xi -= d;
+    }
+    if (prox == 1) {
+      xi = xi < p0 ? p0 : (xi > p1 ? p1 : xi);
+    } else if (prox == 2) {
+      const float xc = xi - p1;
+      xi = fabsf(xc) <= p0 ? p1 : (xc > p0 ? xi - p0 : xi + p0);
+    } else if (prox == 3) {
+      xi = xi < 0.f ? 0.f : xi;
+    }
+    x[i] = xi;
+  }
+}
+
 // ================================================================ metrics
 // SquaredIntensityDifference (squared_intensity_difference.jl:78-90): d = |u_b|^2 - I_b per
 // field ("full dimensions") or d = sum_b |u_b|^2 - I ("spatial only"); loss = sum d^2 and the
@@ -1549,6 +1589,22 @@ int fh_tea_backward(fh_ctx* ctx, void* dv, const float* h, const void* saved, fl
   hipLaunchKernelGGL(k_tea_backward, dim3(elementwise_blocks(npix)), dim3(256), 0, ctx->stream,
                      reinterpret_cast<float2*>(dv), h, reinterpret_cast<const float2*>(saved), dh, npix, batch,
                      kl, reinterpret_cast<const double2*>(r), (float)direction, accumulate);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_optim_step(fh_ctx* ctx, int rule, float* x, const float* dx, float* state, long n, double eta,
+                  double rho, double beta, int prox, double p0, double p1) {
+  if (!ctx || !x || !dx) return fail(FH_E_ARG, "null context, parameter or gradient");
+  if (rule < 0 || rule > 3) return fail(FH_E_ARG, "unknown rule %d", rule);
+  if (rule > 0 && !state) return fail(FH_E_ARG, "rule %d needs a state buffer", rule);
+  if (prox < 0 || prox > 3) return fail(FH_E_ARG, "unknown proximal operator %d", prox);
+  if (n <= 0) return fail(FH_E_ARG, "n must be positive");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const int blocks = (int)std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_optim_step, dim3(blocks), dim3(256), 0, ctx->stream, x, dx, state, n, rule, (float)eta,
+                     (float)rho, (float)beta, prox, (float)p0, (float)p1);
   FH_HIP(hipGetLastError());
   return FH_OK;
 }

@@ -12,10 +12,14 @@ from .components import (AS_BPM, ASProp, FourierMask, FourierOperator, FourierPh
 from .group import PropagationGroup
 from .metrics import DotProduct, PowerCoupling, SquaredIntensityDifference
 from .system import OpticalSequence, OpticalSystem, ScalarSource, merge
+from .optim import (ClampProx, CompositeProx, Descent, Fista, IstaProx, Momentum, Nesterov, NoDescent,
+                    PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
     "AS_BPM", "ASProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
     "RSProp", "ScalarField", "Mask", "TeaDOE", "TeaReflector", "FourierPhase", "FourierMask",
+    "ClampProx", "CompositeProx", "Descent", "Fista", "IstaProx", "Momentum", "Nesterov", "NoDescent",
+    "PositiveProx", "ProxRule", "compose", "make_rules", "setup", "update_",
     "OpticalSequence", "OpticalSystem", "PropagationGroup", "ScalarSource", "SquaredIntensityDifference", "DotProduct", "PowerCoupling", "merge",
     "coupling_efficiency", "phase_propagate_", "propagate_phase_backward_", "propagate",
     "propagate_", "backpropagate_", "rs_valid_distance",

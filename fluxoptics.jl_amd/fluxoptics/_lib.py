@@ -75,6 +75,8 @@ SIGNATURES = {
                              c_void_p]),
     "fh_tea_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
                                 c_void_p, c_int, c_int]),
+    "fh_optim_step": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, ctypes.c_long, c_double, c_double,
+                              c_double, c_int, c_double, c_double]),
     "fh_fft2": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double]),
     "fh_sq_intensity_diff": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p,
                                      c_void_p, c_void_p]),

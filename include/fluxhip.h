@@ -220,6 +220,15 @@ int fh_tea_apply(fh_ctx* ctx, void* u, const float* h, int nx, int ny, int batch
 int fh_tea_backward(fh_ctx* ctx, void* dv, const float* h, const void* saved, float* dh, int nx, int ny,
                     int batch, const double* kl, const double* r, int direction, int accumulate);
 
+/* One optimiser step fused with its proximal operator over n FP32 parameter values (complex
+ * parameters as interleaved re, im).  rule: 0 Descent, 1 Momentum(eta, rho), 2 Nesterov(eta,
+ * rho) (Optimisers.jl 0.4.6, the reference's dependency), 3 Fista (OptimisersExt.jl:12-40;
+ * pass eta = the squared rate the reference stores and beta = (t_k - 1)/t_{k+1}); state holds
+ * the rule's buffer (momentum, velocity or the previous iterate; NULL for Descent).  prox
+ * (proximal_operators/): 0 none, 1 ClampProx(p0, p1), 2 IstaProx(p0, p1), 3 PositiveProx. */
+int fh_optim_step(fh_ctx* ctx, int rule, float* x, const float* dx, float* state, long n, double eta,
+                  double rho, double beta, int prox, double p0, double p1);
+
 /* DotProduct and PowerCoupling, mode-selective (metrics/dot_product.jl:60-106,
  * power_coupling.jl:55-69): c[b] = ds * sum u_b conj(v_b) into the device complex128 array c
  * (FP64, fixed summation order), and p[b] = |c[b]|^2 when p is not NULL.  The pullback writes

@@ -509,6 +509,57 @@ def coupling_efficiency(u, v) -> np.ndarray:
     return num / den
 
 
+# ----------------------------------------------------------------------------- optimisation
+
+def prox_clamp(x, lo, hi):
+    """``ClampProx`` (``proximal_operators/clamp_prox.jl:33-35``)."""
+    return np.clip(x, lo, hi)
+
+
+def prox_ista(x, s, c=0.0):
+    """``IstaProx`` (``proximal_operators/ista_prox.jl:1-18``): per real component,
+    c if |x - c| <= s, else x - s or x + s."""
+    def shrink(r):
+        xc = r - c
+        return np.where(np.abs(xc) <= s, c, np.where(xc > s, r - s, r + s))
+    x = np.asarray(x)
+    if np.iscomplexobj(x):
+        return shrink(x.real) + 1j * shrink(x.imag)
+    return shrink(x)
+
+
+def prox_positive(x):
+    """``PositiveProx`` (``proximal_operators/positive_prox.jl:1-7``)."""
+    return np.where(np.asarray(x) < 0, 0.0, x)
+
+
+def fista_t(t):
+    """t_{k+1} of ``Fista`` (``OptimisersExt.jl:34-36``)."""
+    return (1.0 + math.sqrt(1.0 + 4.0 * t * t)) / 2.0
+
+
+def optim_step(rule, x, dx, state=None, t=1.0, eta=0.1, rho=0.9):
+    """One step of a rule on (x, dx) returning (x_new, state_new, t_new).  ``rule``: "descent",
+    "momentum", "nesterov" (Optimisers.jl 0.4.6 ``apply!``) or "fista" (``OptimisersExt.jl:31-40``;
+    ``eta`` is the squared rate the reference stores)."""
+    x = np.asarray(x)
+    dx = np.asarray(dx)
+    if rule == "descent":
+        return x - eta * dx, state, t
+    if rule == "momentum":
+        m = rho * state + eta * dx
+        return x - m, m, t
+    if rule == "nesterov":
+        d = -rho * rho * state + (1.0 + rho) * eta * dx
+        return x - d, rho * state - eta * dx, t
+    if rule == "fista":
+        tn = fista_t(t)
+        beta = (t - 1.0) / tn
+        d = eta * dx - beta * (x - state)
+        return x - d, x.copy(), tn
+    raise ValueError(rule)
+
+
 # ----------------------------------------------------------------------------- pipelines
 
 def pipeline_forward(u, stages, ds, lams):

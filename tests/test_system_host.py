@@ -188,3 +188,28 @@ def test_tea_doe_reduces_to_phase_and_gradient_by_fd():
             e[i, j] = eps
             g = (f(h + e) - f(h - e)) / (2 * eps)
             assert abs(g - dh[i, j]) <= 1e-6 * (1 + abs(g))
+
+
+def test_prox_reference_doctests():
+    """``proximal_operators/clamp_prox.jl``, ``ista_prox.jl``, ``positive_prox.jl`` doctests."""
+    assert np.allclose(O.prox_clamp([-0.5, 0.3, 1.2, 0.8], 0.0, 1.0), [0.0, 0.3, 1.0, 0.8])
+    assert np.allclose(O.prox_ista([-0.5, -0.005, 0.0, 0.003, 0.2], 0.01, 0.0), [-0.49, 0.0, 0.0, 0.0, 0.19])
+    assert np.allclose(O.prox_positive([-0.1, 0.0, 0.5, -0.3, 1.2]), [0.0, 0.0, 0.5, 0.0, 1.2])
+
+
+def test_rule_invariants():
+    """Momentum and Nesterov with rho = 0 are Descent; Fista's first step is Descent with the
+    stored (squared) rate and t_2 = golden ratio; on f(x) = |x|^2 / 2 Fista meets the
+    O(1/k^2) bound f(x_k) <= 2 |x_0|^2 / (eta (k+1)^2)."""
+    x = inputs.normal(51, (7,))
+    g = inputs.normal(52, (7,))
+    d = O.optim_step("descent", x, g, eta=0.3)[0]
+    assert np.allclose(O.optim_step("momentum", x, g, np.zeros(7), eta=0.3, rho=0.0)[0], d, rtol=1e-15)
+    assert np.allclose(O.optim_step("nesterov", x, g, np.zeros(7), eta=0.3, rho=0.0)[0], d, rtol=1e-15)
+    xf, xk, t = O.optim_step("fista", x, g, x.copy(), 1.0, eta=0.3)
+    assert np.allclose(xf, d, rtol=1e-15) and abs(t - (1 + 5 ** 0.5) / 2) < 1e-15
+    eta = 0.5
+    xx, st, t = x.copy(), x.copy(), 1.0
+    for k in range(1, 30):
+        xx, st, t = O.optim_step("fista", xx, xx, st, t, eta=eta)
+        assert 0.5 * np.dot(xx, xx) <= 2 * np.dot(x, x) / (eta * (k + 1) ** 2) + 1e-15
