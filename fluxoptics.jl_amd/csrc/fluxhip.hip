@@ -206,8 +206,14 @@ __device__ inline void tab_freq_index(int i, int j, int nx, int ny, int full, in
 //   mode 2, tilted (:45-62): a = 1/lam'^2 - (fx+f0x)^2 - (fy+f0y)^2, off = [tracked]
 //     (f0x*fx + f0y*fy)*lam'; a >= 0: t = z*(sqrt(a) + off); a < 0: amplitude
 //     exp(-2*pi*|z|*sqrt(-a)) and t = z*off (conj(H(-z)) for z < 0 gives the same).
-// The inverse FFT's 1/(nx*ny) is folded in (scale).
-__global__ void k_as_table(float2* __restrict__ tab, int nx, int ny, int rows, int qcol, int full,
+//   mode 3, geometric shift (shift_kernel.jl:1-12): t = -z*(tan(theta_x)*fx + tan(theta_y)*fy).
+// The inverse FFT's 1/(nx*ny) is folded in (scale).  T2 = float2 rounds the FP64 value
+// once to c64; T2 = double2 keeps it (the BPM slice steps).
+__device__ inline void store_tab(float2* p, double re, double im) { *p = make_float2((float)re, (float)im); }
+__device__ inline void store_tab(double2* p, double re, double im) { *p = make_double2(re, im); }
+
+template <typename T2>
+__global__ void k_as_table(T2* __restrict__ tab, int nx, int ny, int rows, int qcol, int full,
                            double dx, double dy, double lam, double n0, double z, double tx,
                            double ty, int mode, int track, double scale) {
 #pragma clang fp contract(off)
@@ -225,6 +231,8 @@ __global__ void k_as_table(float2* __restrict__ tab, int nx, int ny, int rows, i
     const double a = f2 - fx * fx - fy * fy;
     if (a >= 0.0) t = z * sqrt(a);
     else amp = exp(-2.0 * M_PI * fabs(z) * sqrt(-a));
+  } else if (mode == 3) {
+    t = -z * (tan(tx) * fx + tan(ty) * fy);
   } else {
     const double f0x = sin(tx) / le, f0y = sin(ty) / le;
     const double gx = fx + f0x, gy = fy + f0y;
@@ -246,7 +254,7 @@ __global__ void k_as_table(float2* __restrict__ tab, int nx, int ny, int rows, i
   double sn, cs;
   sincospi(2.0 * r, &sn, &cs);
   const double re = amp * cs, im = amp * sn;
-  tab[(long)i * qcol + j] = make_float2((float)(re * scale), (float)(im * scale));
+  store_tab(tab + (long)i * qcol + j, re * scale, im * scale);
 }
 
 // Full FP64 transfer-function table [ny][nx] for the plain comparator (same formula).
@@ -1197,7 +1205,9 @@ int fh_ctx_destroy(fh_ctx* ctx) {
 int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   if (!out || !d) return fail(FH_E_ARG, "null descriptor or output pointer");
   *out = nullptr;
-  if (d->flags & ~(FH_PROP_PARAXIAL | FH_PROP_TRACK_TILTS)) return fail(FH_E_ARG, "unknown flags 0x%x", d->flags);
+  if (d->flags & ~(FH_PROP_PARAXIAL | FH_PROP_TRACK_TILTS | FH_PROP_SHIFT)) return fail(FH_E_ARG, "unknown flags 0x%x", d->flags);
+  if ((d->flags & FH_PROP_SHIFT) && (d->flags & FH_PROP_PARAXIAL))
+    return fail(FH_E_ARG, "FH_PROP_SHIFT and FH_PROP_PARAXIAL exclude each other");
   if (!d->z) return fail(FH_E_ARG, "null distances");
   fh_plan* p = new fh_plan();
   int rc = plan_common(ctx, d->nx, d->ny, d->batch, d->dx, d->dy, d->lambdas, d->nlambdas, p, d->z, d->nz,
@@ -1214,7 +1224,7 @@ int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   if ((rc = set_device(ctx))) { plan_free(p); return rc; }
   // kernel choice of angular_spectrum.jl:126-139; tilted tables are not even in fx, fy
   const bool tilted = plan_tilted(p);
-  p->mode = (d->flags & FH_PROP_PARAXIAL) ? 1 : (tilted ? 2 : 0);
+  p->mode = (d->flags & FH_PROP_SHIFT) ? 3 : ((d->flags & FH_PROP_PARAXIAL) ? 1 : (tilted ? 2 : 0));
   p->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
   p->full = tilted ? 1 : 0;
   p->qx = p->full ? nx : nx / 2 + 1;
@@ -1225,7 +1235,7 @@ int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   const double scale = 1.0 / ((double)nx * (double)ny);
   for (size_t k = 0; k < p->lam_tables.size(); ++k) {
     dim3 grid((p->qx + 255) / 256, p->qy);
-    hipLaunchKernelGGL(k_as_table, grid, dim3(256), 0, ctx->stream, p->tab + k * p->tabsize, nx, ny,
+    hipLaunchKernelGGL(k_as_table<float2>, grid, dim3(256), 0, ctx->stream, p->tab + k * p->tabsize, nx, ny,
                        p->qx, p->qy, p->full, d->dx, d->dy, p->lam_tables[k], d->n0, p->tab_z[k],
                        p->tab_tx[k], p->tab_ty[k], p->mode, p->track, scale);
     if (hipGetLastError() != hipSuccess) { plan_free(p); return fail(FH_E_HIP, "ASProp table kernel launch failed"); }
@@ -1701,31 +1711,6 @@ int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {
 
 namespace fh {
 
-// FP64 quadrant of the angular-spectrum transfer function (same formula as k_as_table,
-// kept in FP64 for the slice steps).
-__global__ void k_as_table_d(double2* __restrict__ tab, int nx, int ny, int qx, int qy, double dx,
-                             double dy, double lam, double n0, double z, double scale) {
-#pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= qx || j >= qy) return;
-  const double fx = (double)i / ((double)nx * dx);
-  const double fy = (double)j / ((double)ny * dy);
-  const double le = lam / n0;
-  const double f2 = 1.0 / (le * le);
-  const double a = f2 - fx * fx - fy * fy;
-  double re, im;
-  if (a >= 0.0) {
-    const double t = z * sqrt(a);
-    const double r = t - rint(t);
-    sincospi(2.0 * r, &im, &re);
-  } else {
-    re = exp(-2.0 * M_PI * fabs(z) * sqrt(-a));
-    im = 0.0;
-  }
-  tab[(long)i * qy + j] = make_double2(re * scale, im * scale);
-}
-
 template <typename RD, int L>
 static hipError_t launch_bpm_step_l(const LineIO<float2>& io, int batch, const double2* tw,
                                     const DnPhaseOp& ph, float2* save, float2* ckspec, float* ddn,
@@ -1774,10 +1759,11 @@ struct fh_bpm {
   int nx = 0, ny = 0, batch = 0, nslices = 0, every = 1, nck = 0;
   int lx = 0, ly = 0;
   double thickness = 0, n0 = 1, dz = 0;
-  std::vector<double> lam_tables;
+  std::vector<double> lam_tables, tab_tx, tab_ty;
   std::vector<int> field_table;
-  double2* tfull = nullptr;  // FP64 quadrant tables of ASProp(dz; n0), one per wavelength
-  double2* thalf = nullptr;  // ... of ASProp(dz/2; n0)
+  int mode = 0, track = 0, full = 0;  // slice propagator: as k_as_table's mode; full layout if tilted
+  double2* tfull = nullptr;  // FP64 tables of P(dz), one per (wavelength, tilt) key
+  double2* thalf = nullptr;  // ... of P(dz/2)
   int* d_tabidx = nullptr;
   long tabsize = 0;
   int qx = 0, qy = 0;
@@ -1838,7 +1824,8 @@ static int bpm_ypass(fh_bpm* m, float2* f, const double2* tab, int direction) {
   io.in = f; io.out = f; io.in_f = io.out_f = (long)m->nx * m->ny;
   io.in_l = io.out_l = 1; io.in_e = io.out_e = m->nx;
   io.nlines = m->nx; io.in_valid = io.out_count = m->ny;
-  QuadMulT<double2> q{tab, m->d_tabidx, m->tabsize, m->nx, m->ny, m->qy, direction < 0, m->lam_tables.size() == 1};
+  QuadMulT<double2> q{tab, m->d_tabidx, m->tabsize, m->nx, m->ny, m->qy, direction < 0, m->lam_tables.size() == 1,
+                      m->full};
   FH_HIP((launch_conv<float2, double2>(m->ly, io, m->batch, tw, q, m->ctx->stream)));
   return FH_OK;
 }
@@ -1881,29 +1868,45 @@ static int bpm_recompute(fh_bpm* m, const float* dn, int k0, int k1, double sgn,
 
 extern "C" {
 
-int fh_bpm_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy, const double* lambdas,
-                  int nlambdas, double thickness, double n0, int nslices, int checkpoint_every,
-                  fh_bpm** out) {
-  if (!out) return fail(FH_E_ARG, "null output pointer");
+int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* d, double thickness, int nslices, int checkpoint_every,
+                     fh_bpm** out) {
+  if (!out || !d) return fail(FH_E_ARG, "null descriptor or output pointer");
   *out = nullptr;
   if (!ctx) return fail(FH_E_ARG, "null context");
-  if (nslices < 2) return fail(FH_E_ARG, "AS_BPM needs at least 2 slices, got %d", nslices);
+  if (d->flags & ~(FH_PROP_PARAXIAL | FH_PROP_TRACK_TILTS | FH_PROP_SHIFT)) return fail(FH_E_ARG, "unknown flags 0x%x", d->flags);
+  if ((d->flags & FH_PROP_SHIFT) && (d->flags & (FH_PROP_PARAXIAL | FH_PROP_TRACK_TILTS)))
+    return fail(FH_E_ARG, "Shift_BPM takes neither FH_PROP_PARAXIAL nor FH_PROP_TRACK_TILTS");
+  if (d->z || d->nz) return fail(FH_E_ARG, "BPM slice distances follow from thickness and nslices (z must be null)");
+  if (nslices < 2) return fail(FH_E_ARG, "BPM needs at least 2 slices, got %d", nslices);
   if (checkpoint_every < 1) return fail(FH_E_ARG, "checkpoint_every must be >= 1");
   if (!(thickness > 0) || !std::isfinite(thickness)) return fail(FH_E_ARG, "thickness must be positive");
+  const bool shift = (d->flags & FH_PROP_SHIFT) != 0;
+  const double n0 = shift ? 1.0 : d->n0;
   if (!(n0 > 0)) return fail(FH_E_ARG, "n0 must be positive");
-  fh_plan tmp;  // wavelength bookkeeping and argument checks shared with the propagators
-  int rc = plan_common(ctx, nx, ny, batch, dx, dy, lambdas, nlambdas, &tmp);
+  const int nx = d->nx, ny = d->ny, batch = d->batch;
+  fh_plan tmp;  // wavelength/tilt bookkeeping and argument checks shared with the propagators
+  int rc = plan_common(ctx, nx, ny, batch, d->dx, d->dy, d->lambdas, d->nlambdas, &tmp, nullptr, 0, d->tilts_x,
+                       d->tilts_y, d->ntilts);
   if (rc) return rc;
   const int lx = ilog2_exact(nx), ly = ilog2_exact(ny);
   if (lx < kMinLog || lx > kMaxLog || ly < kMinLog || ly > kMaxLog)
-    return fail(FH_E_UNSUPP, "AS_BPM needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << kMinLog, 1 << kMaxLog, nx, ny);
+    return fail(FH_E_UNSUPP, "BPM needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << kMinLog, 1 << kMaxLog, nx, ny);
   if ((rc = set_device(ctx))) return rc;
   fh_bpm* m = new fh_bpm();
   m->ctx = ctx; m->nx = nx; m->ny = ny; m->batch = batch; m->nslices = nslices; m->lx = lx; m->ly = ly;
   m->every = std::min(checkpoint_every, nslices);
   m->thickness = thickness; m->n0 = n0; m->dz = thickness / nslices;
-  m->lam_tables = tmp.lam_tables; m->field_table = tmp.field_table;
-  m->qx = nx / 2 + 1; m->qy = ny / 2 + 1; m->tabsize = (long)m->qx * m->qy;
+  m->lam_tables = tmp.lam_tables; m->tab_tx = tmp.tab_tx; m->tab_ty = tmp.tab_ty;
+  m->field_table = tmp.field_table;
+  // P is ASProp(dz; n0, paraxial) or ShiftProp(dz) (bpm.jl:54-73,140-142,207-217); the
+  // kernel choice of angular_spectrum.jl:126-139
+  const bool tilted = plan_tilted(&tmp);
+  m->mode = shift ? 3 : ((d->flags & FH_PROP_PARAXIAL) ? 1 : (tilted ? 2 : 0));
+  m->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
+  m->full = tilted ? 1 : 0;
+  m->qx = m->full ? nx : nx / 2 + 1;
+  m->qy = m->full ? ny + 1 : ny / 2 + 1;
+  m->tabsize = (long)m->qx * m->qy;
   m->nck = (nslices + m->every - 1) / m->every;
   const size_t ntab = m->lam_tables.size();
   const size_t fsz = (size_t)nx * ny * batch * sizeof(float2);
@@ -1914,29 +1917,43 @@ int fh_bpm_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy, 
       hipMalloc(&m->ckpt, m->nck * fsz) != hipSuccess || hipMalloc(&m->seg, m->every * fsz) != hipSuccess ||
       hipMalloc(&m->work, fsz) != hipSuccess) {
     bpm_free(m);
-    return fail(FH_E_NOMEM, "AS_BPM: out of device memory");
+    return fail(FH_E_NOMEM, "BPM: out of device memory");
   }
   const double scale = 1.0 / ((double)nx * (double)ny);
   for (size_t k = 0; k < ntab; ++k) {
     dim3 grid((m->qx + 255) / 256, m->qy);
-    hipLaunchKernelGGL(k_as_table_d, grid, dim3(256), 0, ctx->stream, m->tfull + k * m->tabsize, nx, ny, m->qx,
-                       m->qy, dx, dy, m->lam_tables[k], n0, m->dz, scale);
-    hipLaunchKernelGGL(k_as_table_d, grid, dim3(256), 0, ctx->stream, m->thalf + k * m->tabsize, nx, ny, m->qx,
-                       m->qy, dx, dy, m->lam_tables[k], n0, m->dz / 2, scale);
+    hipLaunchKernelGGL(k_as_table<double2>, grid, dim3(256), 0, ctx->stream, m->tfull + k * m->tabsize, nx, ny,
+                       m->qx, m->qy, m->full, d->dx, d->dy, m->lam_tables[k], n0, m->dz, m->tab_tx[k],
+                       m->tab_ty[k], m->mode, m->track, scale);
+    hipLaunchKernelGGL(k_as_table<double2>, grid, dim3(256), 0, ctx->stream, m->thalf + k * m->tabsize, nx, ny,
+                       m->qx, m->qy, m->full, d->dx, d->dy, m->lam_tables[k], n0, m->dz / 2, m->tab_tx[k],
+                       m->tab_ty[k], m->mode, m->track, scale);
   }
-  // kdz = 2*pi*dz / prod(cos(theta)) with on-axis fields (bpm.jl:3-6,70)
-  const double kdz = 2.0 * M_PI * m->dz;
+  // kdz = 2*pi*dz / (cos(theta_x) cos(theta_y)) per field (bpm.jl:3-6,70), over the vacuum
+  // wavelength (bpm.jl:235-242)
   std::vector<double> kl(batch);
-  for (int b = 0; b < batch; ++b) kl[b] = kdz / lambdas[nlambdas == 1 ? 0 : b];
+  for (int b = 0; b < batch; ++b) {
+    const int t = m->field_table[b];
+    kl[b] = 2.0 * M_PI * m->dz / (cos(m->tab_tx[t]) * cos(m->tab_ty[t])) / m->lam_tables[t];
+  }
   if (hipGetLastError() != hipSuccess ||
       hipMemcpy(m->kl, kl.data(), batch * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(m->d_tabidx, m->field_table.data(), batch * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
     bpm_free(m);
-    return fail(FH_E_HIP, "AS_BPM: table set-up failed");
+    return fail(FH_E_HIP, "BPM: table set-up failed");
   }
   m->bytes = (m->nck + m->every + 1) * fsz + 2 * tbytes;
   *out = m;
   return FH_OK;
+}
+
+int fh_bpm_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy, const double* lambdas,
+                  int nlambdas, double thickness, double n0, int nslices, int checkpoint_every,
+                  fh_bpm** out) {
+  fh_prop_desc d{};
+  d.nx = nx; d.ny = ny; d.batch = batch; d.dx = dx; d.dy = dy;
+  d.lambdas = lambdas; d.nlambdas = nlambdas; d.n0 = n0;
+  return fh_bpm_create_ex(ctx, &d, thickness, nslices, checkpoint_every, out);
 }
 
 int fh_bpm_destroy(fh_bpm* m) {

@@ -1,13 +1,14 @@
 """fluxoptics (MI355X): the FluxOptics.jl propagation path on libfluxhip.
 
-Host-side mirror of the reference's interface for ASProp, RSProp, Phase and AS_BPM;
+Host-side mirror of the reference's interface for the propagators (ASProp, RSProp,
+ShiftProp), modulators, BPM volumes (AS_BPM, Shift_BPM), metrics and optimisers;
 the arithmetic is the HIP library's (``libfluxhip.so``, C ABI in include/fluxhip.h).
 """
 
 from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficiency,
                      rs_valid_distance)
-from .components import (AS_BPM, ASProp, FourierMask, FourierOperator, FourierPhase, FourierWrapper,
-                         Mask, Phase, RSProp, TeaDOE, TeaReflector, phase_propagate_,
+from .components import (AS_BPM, BPM, ASProp, FourierMask, FourierOperator, FourierPhase, FourierWrapper,
+                         Mask, Phase, RSProp, ShiftProp, Shift_BPM, TeaDOE, TeaReflector, phase_propagate_,
                          propagate_phase_backward_)
 from .group import PropagationGroup
 from .metrics import DotProduct, PowerCoupling, SquaredIntensityDifference
@@ -16,8 +17,8 @@ from .optim import (ClampProx, CompositeProx, Descent, Fista, IstaProx, Momentum
                     PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
-    "AS_BPM", "ASProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
-    "RSProp", "ScalarField", "Mask", "TeaDOE", "TeaReflector", "FourierPhase", "FourierMask",
+    "AS_BPM", "BPM", "Shift_BPM", "ASProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
+    "RSProp", "ScalarField", "ShiftProp", "Mask", "TeaDOE", "TeaReflector", "FourierPhase", "FourierMask",
     "ClampProx", "CompositeProx", "Descent", "Fista", "IstaProx", "Momentum", "Nesterov", "NoDescent",
     "PositiveProx", "ProxRule", "compose", "make_rules", "setup", "update_",
     "OpticalSequence", "OpticalSystem", "PropagationGroup", "ScalarSource", "SquaredIntensityDifference", "DotProduct", "PowerCoupling", "merge",

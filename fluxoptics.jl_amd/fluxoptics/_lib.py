@@ -23,6 +23,7 @@ c_double_p = ctypes.POINTER(ctypes.c_double)
 
 FH_PROP_PARAXIAL = 1
 FH_PROP_TRACK_TILTS = 2
+FH_PROP_SHIFT = 4
 
 
 class PropDesc(ctypes.Structure):
@@ -82,6 +83,7 @@ SIGNATURES = {
                                      c_void_p, c_void_p]),
     "fh_bpm_create": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_double_p, c_int,
                               c_double, c_double, c_int, c_int, ctypes.POINTER(c_void_p)]),
+    "fh_bpm_create_ex": (c_int, [c_void_p, ctypes.POINTER(PropDesc), c_double, c_int, c_int, ctypes.POINTER(c_void_p)]),
     "fh_bpm_destroy": (c_int, [c_void_p]),
     "fh_bpm_workspace_bytes": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
     "fh_bpm_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int]),

@@ -204,6 +204,24 @@ class ASProp(_KernelPropagator):
         return self
 
 
+class ShiftProp(_KernelPropagator):
+    """``ShiftProp(u, z)`` (``freespace_propagators/shift_kernel.jl:47-117``): the geometric
+    shift of each field by z tan(theta) according to its tilts, no diffraction, applied as the
+    Fourier-domain phase cis(-2 pi z (tan(theta_x) fx + tan(theta_y) fy))."""
+
+    _create_ex = "fh_asprop_create_ex"
+
+    def __init__(self, u: ScalarField, z, ds=None, *, use_cache: bool = True,
+                 double_precision_kernel: Optional[bool] = None):
+        if double_precision_kernel is False:
+            raise NotImplementedError("transfer functions are always computed in FP64")
+        super().__init__(u, ds, z)
+        self.n0 = 1.0
+
+    def _flags(self) -> int:
+        return _lib.FH_PROP_SHIFT
+
+
 class RSProp(_KernelPropagator):
     """Rayleigh-Sommerfeld propagation (``rayleigh_sommerfeld.jl:128-164``).
 
@@ -549,20 +567,26 @@ def propagate_phase_backward_(dv: ScalarField, phase: Phase, prop: _KernelPropag
 
 # --------------------------------------------------------------------------- AS_BPM
 
-class AS_BPM(_PipeComponent):
-    """Split-step beam propagation through a graded-index volume (``bpm.jl:130-142``).
+class BPM(_PipeComponent):
+    """Split-step beam propagation through a graded-index volume (``BPM``, ``bpm.jl:8-74``);
+    built as ``AS_BPM`` or ``Shift_BPM``.
 
     ``dn`` has shape (nslices, ny, nx) (the reference's ``dn[nx, ny, nslices]``);
-    ``thickness`` is the total length, ``n0`` the background index.  For the gradient
-    the library keeps one checkpoint every ``checkpoint_every`` slices instead of every
-    slice input and recomputes one segment at a time in the adjoint.
+    ``thickness`` is the total length.  kdz = 2 pi dz / (cos(theta_x) cos(theta_y)) is taken
+    from the template field's tilts, as the reference does at construction (``bpm.jl:70``), so
+    a propagated field must carry the template's tilts.  For the gradient the library keeps
+    one checkpoint every ``checkpoint_every`` slices instead of every slice input and
+    recomputes one segment at a time in the adjoint.  ``aperture`` is accepted and, as in the
+    reference (whose ``propagate!`` never applies ``aperture_mask``), has no effect.
     """
+
+    _flags = 0
 
     def __init__(self, u: ScalarField, thickness: float, n0: float, dn, *, trainable: bool = False,
                  buffered: bool = False, checkpoint_every: int = 20, use_cache: bool = True,
-                 paraxial: bool = False, aperture: Optional[Callable] = None):
-        if paraxial:
-            raise NotImplementedError("paraxial AS_BPM is not on the accelerated path")
+                 aperture: Optional[Callable] = None, double_precision_kernel: Optional[bool] = None):
+        if double_precision_kernel is False:
+            raise NotImplementedError("slice transfer functions are always FP64")
         check_field(u.electric)
         device = u.electric.device
         self.dn = torch.as_tensor(np.asarray(dn) if not isinstance(dn, torch.Tensor) else dn,
@@ -570,29 +594,50 @@ class AS_BPM(_PipeComponent):
         if self.dn.dim() != 3 or tuple(self.dn.shape[1:]) != (u.ny, u.nx):
             raise ValueError("dn must have shape (nslices, ny, nx)")
         if self.dn.shape[0] < 2:
-            raise ValueError("AS_BPM needs at least 2 slices (bpm.jl:40)")
+            raise ValueError("BPM needs at least 2 slices (bpm.jl:40)")
         self.nslices = int(self.dn.shape[0])
         self.thickness, self.n0 = float(thickness), float(n0)
         self.nx, self.ny, self.ds = u.nx, u.ny, u.ds
+        self.tilts = tuple(tuple(t) for t in u.tilt_lists())
         self.checkpoint_every = int(checkpoint_every)
         self.trainable = trainable
         self.grad = torch.zeros_like(self.dn) if (trainable and buffered) else None
         self._objs = {}
+
+    def _tilts_for(self, u: ScalarField):
+        tilts = tuple(tuple(t) for t in u.tilt_lists())
+        same = tilts == self.tilts
+        if not same and len(set(self.tilts[0])) == 1 and len(set(self.tilts[1])) == 1:
+            same = all(v == self.tilts[0][0] for v in tilts[0]) and all(v == self.tilts[1][0] for v in tilts[1])
+        if not same:
+            raise ValueError("the field's tilts differ from the template's, for which kdz was computed")
+        return tilts
 
     def _obj(self, u: ScalarField) -> ctypes.c_void_p:
         if (u.nx, u.ny) != (self.nx, self.ny):
             raise ValueError("field grid does not match the BPM volume")
         check_field(u.electric)
         lams = tuple(u.lambda_list())
-        key = (u.electric.device.index, u.batch, lams)
+        tilts = self._tilts_for(u)
+        key = (u.electric.device.index, u.batch, lams, tilts)
         ctx = context(u.electric.device)
         m = self._objs.get(key)
         if m is None:
             m = ctypes.c_void_p()
-            arr = (ctypes.c_double * len(lams))(*lams)
-            _lib.call("fh_bpm_create", ctx, self.nx, self.ny, u.batch, self.ds[0], self.ds[1], arr,
-                      len(lams), self.thickness, self.n0, self.nslices, self.checkpoint_every,
-                      ctypes.byref(m))
+            arr = lambda v: (ctypes.c_double * len(v))(*v)
+            d = _lib.PropDesc()
+            d.nx, d.ny, d.batch = self.nx, self.ny, u.batch
+            d.dx, d.dy = self.ds
+            keep = (arr(lams), arr(tilts[0]), arr(tilts[1]))
+            d.lambdas = ctypes.cast(keep[0], _lib.c_double_p)
+            d.nlambdas = len(lams)
+            d.tilts_x = ctypes.cast(keep[1], _lib.c_double_p)
+            d.tilts_y = ctypes.cast(keep[2], _lib.c_double_p)
+            d.ntilts = u.batch
+            d.n0 = self.n0
+            d.flags = self._flags
+            _lib.call("fh_bpm_create_ex", ctx, ctypes.byref(d), self.thickness, self.nslices,
+                      self.checkpoint_every, ctypes.byref(m))
             self._objs[key] = m
         return m
 
@@ -632,3 +677,24 @@ class AS_BPM(_PipeComponent):
         _lib.call("fh_bpm_backward", self._obj(du), _ptr(du.electric), _ptr(self.dn), _ptr(ddn),
                   dir_sign(direction))
         return du, ddn
+
+
+class AS_BPM(BPM):
+    """``AS_BPM(u, thickness, n0, dn; paraxial=false, trainable, buffered, aperture)``
+    (``bpm.jl:130-142``): the slice propagator is ASProp(dz; n0) with the reference's kernel
+    choice (paraxial, on-axis or tilted by the field's tilts; tilts untracked)."""
+
+    def __init__(self, u: ScalarField, thickness: float, n0: float, dn, *, paraxial: bool = False, **kw):
+        super().__init__(u, thickness, n0, dn, **kw)
+        self.paraxial = bool(paraxial)
+        self._flags = _lib.FH_PROP_PARAXIAL if paraxial else 0
+
+
+class Shift_BPM(BPM):
+    """``Shift_BPM(u, thickness, dn; trainable, buffered, aperture)`` (``bpm.jl:207-217``): the
+    slice propagator is ShiftProp(dz), a geometric shift by dz tan(theta) per field."""
+
+    _flags = _lib.FH_PROP_SHIFT
+
+    def __init__(self, u: ScalarField, thickness: float, dn, **kw):
+        super().__init__(u, thickness, 1.0, dn, **kw)

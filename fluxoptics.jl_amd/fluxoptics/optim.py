@@ -15,7 +15,7 @@ from typing import Dict, Iterable, Optional
 import torch
 
 from . import _lib
-from .components import AS_BPM, _Modulator, _ptr, context, Phase
+from .components import BPM, _Modulator, _ptr, context, Phase
 from .system import OpticalSystem, ScalarSource
 
 _RULE_IDS = {"Descent": 0, "Momentum": 1, "Nesterov": 2, "Fista": 3}
@@ -106,7 +106,7 @@ def _params(component):
         return [("phi", component.phi)]
     if isinstance(component, _Modulator):
         return [(type(component).__name__, component._param)]
-    if isinstance(component, AS_BPM):
+    if isinstance(component, BPM):
         return [("dn", component.dn)]
     if isinstance(component, ScalarSource):
         return [("u0", component.u0.electric)]

@@ -22,7 +22,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 
 import torch
 
-from .components import (AS_BPM, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
+from .components import (BPM, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
                          _Modulator, check_field)
 from .components import phase_propagate_, propagate_phase_backward_
 from .fields import Backward, Forward, ScalarField, dir_sign
@@ -286,9 +286,9 @@ def _run_forward(comps: List, u: ScalarField, direction, save: bool) -> list:
                 else torch.empty_like(u.electric)
             c.propagate_and_save_(u, saved, direction)
             tape.append(("mod", (c,), saved))
-        elif save and isinstance(c, AS_BPM) and c.trainable:
+        elif save and isinstance(c, BPM) and c.trainable:
             if any(t[0] == "bpm" and t[1][0] is c for t in tape):
-                raise NotImplementedError("a trainable AS_BPM may appear once in a differentiated system "
+                raise NotImplementedError("a trainable BPM may appear once in a differentiated system "
                                           "(it keeps one set of checkpoints)")
             c.propagate_and_save_(u, direction)
             tape.append(("bpm", (c,), None))

@@ -77,6 +77,7 @@ int fh_rsprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double d
  * ASProp only. */
 #define FH_PROP_PARAXIAL 1    /* ASProp(...; paraxial=true): as_paraxial_kernel */
 #define FH_PROP_TRACK_TILTS 2 /* track_tilts=true */
+#define FH_PROP_SHIFT 4       /* ShiftProp (shift_kernel.jl): H = cis(-2 pi z (tan(tx) fx + tan(ty) fy)) */
 typedef struct fh_prop_desc {
   int nx, ny, batch;
   double dx, dy;
@@ -93,7 +94,9 @@ typedef struct fh_prop_desc {
 
 /* ASProp with the reference's kernel choice (angular_spectrum.jl:126-139): paraxial
  * (as_paraxial_kernel, :27-35) when FH_PROP_PARAXIAL is set, else on-axis (as_kernel) when
- * every tilt is zero, else tilted (tilted_as_kernel, :45-62), with per-field z. */
+ * every tilt is zero, else tilted (tilted_as_kernel, :45-62), with per-field z.  With
+ * FH_PROP_SHIFT the plan is a ShiftProp (shift_kernel.jl:1-117): the geometric shift by
+ * z tan(theta) of each field, as a Fourier-domain linear phase. */
 int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, fh_plan** out);
 
 /* RSProp with per-field z and tilted fields (rs_tilted_kernel, rayleigh_sommerfeld.jl:14-27,
@@ -175,6 +178,15 @@ int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx
 int fh_bpm_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
                   const double* lambdas, int nlambdas, double thickness, double n0,
                   int nslices, int checkpoint_every, fh_bpm** out);
+
+/* The general BPM constructor (bpm.jl:54-73): the field template's tilts (per field or
+ * shared) and the slice propagator from desc->flags -- ASProp(dz; n0) with the reference's
+ * kernel choice and FH_PROP_PARAXIAL / FH_PROP_TRACK_TILTS (AS_BPM, bpm.jl:130-142), or
+ * ShiftProp(dz) with FH_PROP_SHIFT (Shift_BPM, bpm.jl:207-217; desc->n0 unused).  kdz is
+ * 2*pi*dz / (cos(theta_x) cos(theta_y)) per field (bpm.jl:3-6,70).  desc->z must be NULL:
+ * the slice distances follow from thickness and nslices. */
+int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, double thickness, int nslices,
+                     int checkpoint_every, fh_bpm** out);
 int fh_bpm_destroy(fh_bpm* bpm);
 
 /* Device memory owned by the BPM object (checkpoints, segment, tables), in bytes. */

@@ -141,6 +141,26 @@ def tilted_as_kernel(fx, fy, lam: float, tx: float, ty: float, track: bool, n0: 
     return np.exp(1j * (TWO_PI * z) * (np.sqrt(arg) + off))
 
 
+def shift_kernel(fx, fy, tx: float, ty: float, z: float):
+    """``shift_kernel`` (``freespace_propagators/shift_kernel.jl:1-6``):
+    cis(-2 pi z (tan(tx) fx + tan(ty) fy))."""
+    return np.exp(-1j * TWO_PI * z * (math.tan(tx) * np.asarray(fx) + math.tan(ty) * np.asarray(fy)))
+
+
+def shiftprop(u, ds, z, tilts, direction: int = 1):
+    """``ShiftProp``: FFT2 -> shift kernel (conjugate for Backward) -> IFFT2, per field tilts."""
+    u = _as_batch(np.asarray(u, dtype=np.complex128))
+    b, ny, nx = u.shape
+    z_b = _per_field(z, b)
+    tx_b, ty_b = _per_field(tilts[0], b), _per_field(tilts[1], b)
+    fx = fftfreq(nx, ds[0])[None, :]
+    fy = fftfreq(ny, ds[1])[:, None]
+    v = ft2(u)
+    for k in range(b):
+        v[k] *= conj_direction(shift_kernel(fx, fy, tx_b[k], ty_b[k], z_b[k]), direction)
+    return ift2(v)
+
+
 def as_transfer(nx: int, ny: int, dx: float, dy: float, lam: float, n0: float, z: float,
                 tilts=(0.0, 0.0), paraxial: bool = False, track: bool = False):
     """H sampled on the FFT-order frequency grid, shape (ny, nx).
@@ -374,52 +394,70 @@ def _dn_phase(dn_s, kdz, lam_b, s):
     return np.exp(1j * (s * kdz / lam_b)[:, None, None] * np.asarray(dn_s, np.float64)[None])
 
 
-def bpm_forward(u, dn, thickness: float, n0: float, lams, ds, direction: int = 1):
-    """``AS_BPM`` split-step forward; returns (v, saved) with saved[s] = slice-s input.
+def _bpm_setup(b: int, thickness: float, ns: int, lams, tilts):
+    """dz, per-field kdz (``bpm.jl:3-6,70``, with each field's tilts) and vacuum lambdas."""
+    if ns < 2:
+        raise ValueError("BPM needs at least 2 slices (bpm.jl:40)")
+    dz = thickness / ns
+    tx = _per_field(0.0 if tilts is None else tilts[0], b)
+    ty = _per_field(0.0 if tilts is None else tilts[1], b)
+    kdz = np.array([bpm_kdz(dz, (tx[k], ty[k])) for k in range(b)])
+    return dz, kdz, _lambdas(lams, b)
 
-    ``bpm.jl:54-73`` (construction: dz = L/N, P(dz) and P(dz/2) are ASProps with n0),
-    ``bpm.jl:244-260`` (half step, N-1 x [save, phase, full step], save, phase, half step).
-    ``dn`` has shape (N, ny, nx) (the reference's dn[nx, ny, N]).
+
+def _bpm_prop(u, ds, lam_b, z, n0, direction, tilts, paraxial, track, shift):
+    # the slice propagator P: ASProp(z; n0, paraxial) or ShiftProp(z) (bpm.jl:54-73,140-142,207-217)
+    if shift:
+        return shiftprop(u, ds, z, (0.0, 0.0) if tilts is None else tilts, direction)
+    return asprop(u, ds, lam_b, z, n0, direction, tilts=tilts, paraxial=paraxial, track=track)
+
+
+def bpm_forward(u, dn, thickness: float, n0: float, lams, ds, direction: int = 1, tilts=None,
+                paraxial: bool = False, track: bool = False, shift: bool = False):
+    """``AS_BPM`` / ``Shift_BPM`` split-step forward; returns (v, saved) with saved[s] =
+    slice-s input.
+
+    ``bpm.jl:54-73`` (construction: dz = L/N, P(dz) and P(dz/2) are ASProps with n0, or
+    ShiftProps for ``shift``), ``bpm.jl:244-260`` (half step, N-1 x [save, phase, full step],
+    save, phase, half step).  ``dn`` has shape (N, ny, nx) (the reference's dn[nx, ny, N]).
     """
     u = _as_batch(np.asarray(u, dtype=np.complex128)).copy()
     dn = np.asarray(dn, np.float64)
     ns = dn.shape[0]
-    if ns < 2:
-        raise ValueError("AS_BPM needs at least 2 slices (bpm.jl:40)")
-    dz = thickness / ns
-    kdz = bpm_kdz(dz)
-    lam_b = _lambdas(lams, u.shape[0])
+    dz, kdz, lam_b = _bpm_setup(u.shape[0], thickness, ns, lams, tilts)
+    prop = lambda a, z: _bpm_prop(a, ds, lam_b, z, n0, direction, tilts, paraxial, track, shift)
     saved = []
-    u = asprop(u, ds, lam_b, dz / 2, n0, direction)
+    u = prop(u, dz / 2)
     for s in range(ns - 1):
         saved.append(u.copy())
         u = u * _dn_phase(dn[s], kdz, lam_b, direction)
-        u = asprop(u, ds, lam_b, dz, n0, direction)
+        u = prop(u, dz)
     saved.append(u.copy())
     u = u * _dn_phase(dn[ns - 1], kdz, lam_b, direction)
-    u = asprop(u, ds, lam_b, dz / 2, n0, direction)
+    u = prop(u, dz / 2)
     return u, saved
 
 
-def bpm_backward(dv, dn, thickness: float, n0: float, lams, ds, saved, direction: int = 1):
-    """``AS_BPM`` adjoint with the dn gradient (``bpm.jl:275-312``); returns (du, ddn).
+def bpm_backward(dv, dn, thickness: float, n0: float, lams, ds, saved, direction: int = 1, tilts=None,
+                 paraxial: bool = False, track: bool = False, shift: bool = False):
+    """``AS_BPM`` / ``Shift_BPM`` adjoint with the dn gradient (``bpm.jl:275-312``);
+    returns (du, ddn).
 
-    ddn[s] = s * kdz/lam * sum_b Im(du_s * conj(saved[s])), du_s being the cotangent
+    ddn[s] = s * kdz_b/lam_b * sum_b Im(du_s * conj(saved[s])), du_s being the cotangent
     after un-applying slice s's phase.
     """
     du = _as_batch(np.asarray(dv, dtype=np.complex128)).copy()
     dn = np.asarray(dn, np.float64)
     ns = dn.shape[0]
-    dz = thickness / ns
-    kdz = bpm_kdz(dz)
-    lam_b = _lambdas(lams, du.shape[0])
+    dz, kdz, lam_b = _bpm_setup(du.shape[0], thickness, ns, lams, tilts)
+    prop = lambda a, z: _bpm_prop(a, ds, lam_b, z, n0, -direction, tilts, paraxial, track, shift)
     ddn = np.zeros_like(dn)
     coef = (direction * kdz / lam_b)[:, None, None]
-    du = asprop(du, ds, lam_b, dz / 2, n0, -direction)
+    du = prop(du, dz / 2)
     for s in range(ns - 1, -1, -1):
         du = du * _dn_phase(dn[s], kdz, lam_b, -direction)
         ddn[s] = (coef * np.imag(du * np.conj(saved[s]))).sum(axis=0)
-        du = asprop(du, ds, lam_b, dz if s > 0 else dz / 2, n0, -direction)
+        du = prop(du, dz if s > 0 else dz / 2)
     return du, ddn
 
 

@@ -106,3 +106,46 @@ def test_bpm_config4_graded_index_volume():
     print(f"config4: field {ev:.2e} adjoint {edu:.2e} ddn {eg:.2e}; library workspace {ws / 2**20:.1f} MiB "
           f"(a full save of every slice input is {ns * n * n * 8 / 2**20:.0f} MiB)")
     assert ev <= 1e-5 and edu <= 1e-5 and eg <= 1e-4
+
+
+@pytest.mark.parametrize("kind,every", [("tilted", 3), ("paraxial", 1), ("shift", 2)])
+def test_general_bpm_forward_adjoint_gradient(kind, every):
+    """Tilted and paraxial AS_BPM and Shift_BPM, per-field tilts and wavelengths (per-field
+    kdz = 2 pi dz / (cos tx cos ty)), against the oracle."""
+    n, ns, batch = 128, 6, 2
+    print(f"[bpm {kind}] n={n} slices={ns} every={every}", flush=True)
+    ds, n0, L = (0.5, 0.5), 1.5, 18.0
+    lams = [1.0, 0.9]
+    tilts = ([0.03, -0.02], [0.01, 0.04])
+    kw = dict(tilts=tilts, paraxial=kind == "paraxial", shift=kind == "shift")
+    dn = smooth_dn(ns, n, 2.0, 0.01, 6)
+    u0 = c64(np.stack([tilted_gaussian(n, 0.5, 1.0, 12.0, 1.0)] * batch))
+    t0 = c64(0.01 * random_field((batch, n, n), 12))
+    u = fo.ScalarField(to_dev(u0), ds, lams, tilts)
+    if kind == "shift":
+        bpm = fo.Shift_BPM(u, L, dn, trainable=True, buffered=True, checkpoint_every=every)
+    else:
+        bpm = fo.AS_BPM(u, L, n0, dn, paraxial=kind == "paraxial", trainable=True, buffered=True,
+                        checkpoint_every=every)
+    bpm.propagate_and_save_(u, fo.Forward)
+    v = to_host(u.electric)
+    vref, saved = O.bpm_forward(u0, dn.astype(np.float64), L, n0, lams, ds, **kw)
+    assert rel_l2(v, vref) <= 1e-5
+    dv0 = c64(v - t0)
+    du = fo.ScalarField(to_dev(dv0), ds, lams, tilts)
+    bpm.backpropagate_with_gradient_(du, None, fo.Forward)
+    duref, ddnref = O.bpm_backward(dv0, dn.astype(np.float64), L, n0, lams, ds, saved, **kw)
+    assert rel_l2(to_host(du.electric), duref) <= 1e-5
+    torch.cuda.synchronize()
+    assert rel_l2(bpm.grad.cpu().numpy(), ddnref) <= 1e-4
+    w = bpm.propagate(fo.ScalarField(to_dev(u0), ds, lams, tilts), fo.Backward)
+    wref, _ = O.bpm_forward(u0, dn.astype(np.float64), L, n0, lams, ds, direction=-1, **kw)
+    assert rel_l2(to_host(w.electric), wref) <= 1e-5
+
+
+def test_bpm_rejects_other_tilts():
+    n = 64
+    u = fo.ScalarField(to_dev(c64(random_field((1, n, n), 13))), (0.5, 0.5), 1.0, (0.01, 0.0))
+    bpm = fo.Shift_BPM(u, 10.0, np.zeros((3, n, n), np.float32))
+    with pytest.raises(ValueError):
+        bpm.propagate(fo.ScalarField(u.electric.clone(), (0.5, 0.5), 1.0, (0.02, 0.0)))

@@ -78,3 +78,26 @@ def test_per_field_distances(kind):
     else:
         ref = lambda a, d: O.rsprop(a, ds, lam, zs, d, "2n")
     _check(p, u0, ds, lam, (0.0, 0.0), ref, f"per-field z {kind}")
+
+
+@pytest.mark.parametrize("n", [256, 1024])
+def test_shiftprop(n):
+    """``ShiftProp``: per-field geometric shifts by z tan(theta), both directions."""
+    ds, lams, z = (0.5, 0.5), [0.64, 0.538, 0.455], 300.0
+    print(f"[ShiftProp] n={n}", flush=True)
+    tilts = ([0.01, -0.02, 0.0], [0.015, 0.0, -0.005])
+    u0 = c64(speckle_batch(3, n, n, 0.5, 0.5, 0.1 * n, 55))
+    u = fo.ScalarField(to_dev(u0), ds, lams, tilts)
+    p = fo.ShiftProp(u, z)
+    _check(p, u0, ds, lams, tilts, lambda a, d: O.shiftprop(a, ds, z, tilts, d), "ShiftProp")
+
+
+def test_shiftprop_whole_pixels_is_a_roll():
+    """A shift of whole pixels on the device equals a periodic roll of the input."""
+    n, dx, z = 128, 0.5, 40.0
+    print("[ShiftProp roll]", flush=True)
+    tilts = ([math.atan(5 * dx / z)], [math.atan(-3 * dx / z)])
+    u0 = c64(random_field((1, n, n), 56))
+    u = fo.ScalarField(to_dev(u0), (dx, dx), 1.0, tilts)
+    v = fo.ShiftProp(u, z).propagate(u, fo.Forward)
+    assert rel_l2(to_host(v.electric), np.roll(u0, (-3, 5), axis=(1, 2))) <= FIELD_TOL

@@ -196,3 +196,52 @@ def test_bpm_adjoint_and_dn_gradient():
     _, ddn = O.bpm_backward(dv, dn, L, n0, lams, ds, saved)
     idx = [int(i) for i in inputs.splitmix64(89, 12) % np.uint64(ns * n * n)]
     _fd_check(loss, dn, idx, ddn, h=1e-5)
+
+
+def test_shift_bpm_closed_form():
+    """Shift_BPM with a uniform dn = c and whole-pixel shifts: a periodic roll by
+    thickness*tan(theta) times cis(2 pi L c / (lam cos(tx) cos(ty))) (bpm.jl:3-6,70,235-242)."""
+    n, dx, lam, L, ns, c = 32, 0.5, 1.0, 40.0, 8, 0.002
+    u = _cplx((2, n, n), 91)
+    tilts = ([math.atan(4 * dx / L), math.atan(-6 * dx / L)], [math.atan(2 * dx / L), 0.0])
+    v, _ = O.bpm_forward(u, np.full((ns, n, n), c), L, 1.0, lam, (dx, dx), tilts=tilts, shift=True)
+    shifts = [(2, 4), (0, -6)]
+    for b in range(2):
+        kdz = O.bpm_kdz(L, (tilts[0][b], tilts[1][b]))
+        ref = np.roll(u[b], shifts[b], axis=(0, 1)) * np.exp(1j * kdz * c / lam)
+        assert np.max(np.abs(v[b] - ref)) < 1e-11
+
+
+def test_tilted_bpm_reduces_to_asprop():
+    """With dn = 0 the tilted (and paraxial) AS_BPM is one ASProp over the thickness with the
+    same kernel choice: transfer functions of one tilt compose over z."""
+    n, ds, lams, n0, L, ns = 32, (0.5, 0.5), [1.0, 0.8], 1.3, 30.0, 6
+    u = _cplx((2, n, n), 93)
+    tilts = ([0.05, -0.03], [0.0, 0.02])
+    for paraxial in (False, True):
+        v, _ = O.bpm_forward(u, np.zeros((ns, n, n)), L, n0, lams, ds, tilts=tilts, paraxial=paraxial)
+        ref = O.asprop(u, ds, lams, L, n0, tilts=tilts, paraxial=paraxial)
+        assert np.max(np.abs(v - ref)) < 1e-11
+
+
+@pytest.mark.parametrize("kind", ["tilted", "paraxial", "shift"])
+def test_general_bpm_adjoint_and_dn_gradient(kind):
+    n, ds, lams, n0, L, ns = 16, (0.5, 0.5), [1.0, 0.9], 1.5, 12.0, 4
+    tilts = ([0.08, -0.05], [0.03, 0.1])
+    kw = dict(tilts=tilts, paraxial=kind == "paraxial", shift=kind == "shift")
+    u = _cplx((2, n, n), 95)
+    t = _cplx((2, n, n), 96)
+    dn = 0.01 * inputs.normal(97, (ns, n, n))
+    v, saved = O.bpm_forward(u, dn, L, n0, lams, ds, **kw)
+    w = _cplx((2, n, n), 98)
+    du, _ = O.bpm_backward(w, dn, L, n0, lams, ds, saved, **kw)
+    assert abs(np.vdot(v, w) - np.vdot(u, du)) < 1e-11 * abs(np.vdot(v, w))
+
+    def loss(d):
+        vv, _ = O.bpm_forward(u, d, L, n0, lams, ds, **kw)
+        return O.half_sq_field_diff(vv, t)[0]
+
+    _, dv = O.half_sq_field_diff(v, t)
+    _, ddn = O.bpm_backward(dv, dn, L, n0, lams, ds, saved, **kw)
+    idx = [int(i) for i in inputs.splitmix64(99, 10) % np.uint64(ns * n * n)]
+    _fd_check(loss, dn, idx, ddn, h=1e-5)

@@ -96,3 +96,16 @@ def test_per_field_distances():
     v = O.asprop(u, (dx, dx), lam, zs)
     for k in range(3):
         assert np.array_equal(v[k], O.asprop(u[k], (dx, dx), lam, zs[k])[0])
+
+
+def test_shiftprop_is_a_whole_pixel_roll():
+    """``ShiftProp`` moves a field by z tan(theta); with that a whole number of pixels it is
+    exactly a periodic roll (both axes, per field), and Backward undoes it."""
+    n, dx = 32, 0.5
+    u = inputs.normal(61, (2, n, n)) + 1j * inputs.normal(62, (2, n, n))
+    z = 10.0
+    tilts = ([math.atan(3 * dx / z), math.atan(-2 * dx / z)], [math.atan(-1 * dx / z), 0.0])
+    v = O.shiftprop(u, (dx, dx), z, tilts)
+    assert np.linalg.norm(v[0] - np.roll(u[0], (-1, 3), axis=(0, 1))) <= 1e-12 * np.linalg.norm(u[0])
+    assert np.linalg.norm(v[1] - np.roll(u[1], (0, -2), axis=(0, 1))) <= 1e-12 * np.linalg.norm(u[1])
+    assert np.linalg.norm(O.shiftprop(v, (dx, dx), z, tilts, -1) - u) <= 1e-12 * np.linalg.norm(u)
