@@ -374,6 +374,34 @@ __global__ void k_phase_backward(float2* __restrict__ dv, const float* __restric
   }
 }
 
+// offset_tilts! (Fields.jl:280-293): u_b <- u_b * cis(2*pi*(cx_b*x + cy_b*y)) with
+// cx_b = (sin(theta_x,from) - sin(theta_x,to)) / lambda_b (cycles per unit length) and the
+// centred grid x = (i - (nx-1)/2)*dx (GridUtils.jl:56-60).  The phase is taken in cycles in
+// FP64 and reduced to [-1/2, 1/2] before sincospi, so its error does not grow with the grid
+// extent.  Fields are handled kTiltChunk at a time (coefficients passed by value).
+constexpr int kTiltChunk = 32;
+struct TiltCoef {
+  double cx[kTiltChunk], cy[kTiltChunk];
+};
+
+__global__ void k_offset_tilts(float2* __restrict__ u, int nx, int ny, double dx, double dy, TiltCoef c,
+                               int nb) {
+  const long npix = (long)nx * ny;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % nx), iy = (int)(i / nx);
+    const double x = ((double)ix - 0.5 * (double)(nx - 1)) * dx;
+    const double y = ((double)iy - 0.5 * (double)(ny - 1)) * dy;
+    for (int b = 0; b < nb; ++b) {
+      const double t = c.cx[b] * x + c.cy[b] * y;
+      const double r = t - rint(t);
+      double sn, cs;
+      sincospi(2.0 * r, &sn, &cs);
+      const long k = b * npix + i;
+      u[k] = cmul(u[k], make_float2((float)cs, (float)sn));
+    }
+  }
+}
+
 // Mask (modulators/mask.jl:95-141): u <- u * m (Forward) or u * conj(m) (Backward), m a c64
 // [ny][nx] transmission shared by the batch, saving u first when saved != nullptr.
 __global__ void k_mask_apply(float2* __restrict__ u, const float2* __restrict__ m, float2* __restrict__ saved,
@@ -1544,6 +1572,40 @@ int fh_dot_product_pullback(fh_ctx* ctx, const void* v, const void* dc, const vo
 }
 
 static int elementwise_blocks(long npix) { return (int)std::min<long>((npix + 255) / 256, 4096); }
+
+int fh_offset_tilts(fh_ctx* ctx, void* u, int nx, int ny, int batch, double dx, double dy, const double* lambdas,
+                    int nlambdas, const double* from_x, const double* from_y, const double* to_x,
+                    const double* to_y, int ntilts) {
+  if (!ctx || !u) return fail(FH_E_ARG, "null context or field");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  if (!(dx > 0) || !(dy > 0)) return fail(FH_E_ARG, "sampling must be positive");
+  if (!lambdas || (nlambdas != 1 && nlambdas != batch))
+    return fail(FH_E_ARG, "need 1 or batch (=%d) wavelengths, got %d", batch, nlambdas);
+  if (!from_x || !from_y || !to_x || !to_y || (ntilts != 1 && ntilts != batch))
+    return fail(FH_E_ARG, "need 1 or batch (=%d) tilt pairs on both sides, got %d", batch, ntilts);
+  for (int k = 0; k < nlambdas; ++k)
+    if (!(lambdas[k] > 0)) return fail(FH_E_ARG, "wavelengths must be positive");
+  for (int k = 0; k < ntilts; ++k)
+    for (double a : {from_x[k], from_y[k], to_x[k], to_y[k]})
+      if (!std::isfinite(a) || fabs(a) >= M_PI / 2) return fail(FH_E_ARG, "tilt angles must be finite radians in (-pi/2, pi/2)");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const long npix = (long)nx * ny;
+  for (int b0 = 0; b0 < batch; b0 += kTiltChunk) {
+    const int nb = std::min(kTiltChunk, batch - b0);
+    TiltCoef c{};
+    for (int j = 0; j < nb; ++j) {
+      const int b = b0 + j, t = ntilts == 1 ? 0 : b;
+      const double lam = lambdas[nlambdas == 1 ? 0 : b];
+      c.cx[j] = (sin(from_x[t]) - sin(to_x[t])) / lam;
+      c.cy[j] = (sin(from_y[t]) - sin(to_y[t])) / lam;
+    }
+    hipLaunchKernelGGL(k_offset_tilts, dim3(elementwise_blocks(npix)), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<float2*>(u) + b0 * npix, nx, ny, dx, dy, c, nb);
+    FH_HIP(hipGetLastError());
+  }
+  return FH_OK;
+}
 
 int fh_mask_apply(fh_ctx* ctx, void* u, const void* m, int nx, int ny, int batch, int direction, void* saved) {
   if (!ctx || !u || !m) return fail(FH_E_ARG, "null context, field or mask");

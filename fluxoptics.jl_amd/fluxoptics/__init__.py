@@ -8,7 +8,8 @@ the arithmetic is the HIP library's (``libfluxhip.so``, C ABI in include/fluxhip
 from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficiency,
                      rs_valid_distance)
 from .components import (AS_BPM, BPM, ASProp, FourierMask, FourierOperator, FourierPhase, FourierWrapper,
-                         Mask, Phase, RSProp, ShiftProp, Shift_BPM, TeaDOE, TeaReflector, phase_propagate_,
+                         Mask, Phase, RSProp, ShiftProp, Shift_BPM, TeaDOE, TeaReflector, TiltAnchor,
+                         phase_propagate_,
                          propagate_phase_backward_)
 from .group import PropagationGroup
 from .metrics import DotProduct, PowerCoupling, SquaredIntensityDifference
@@ -17,7 +18,7 @@ from .optim import (ClampProx, CompositeProx, Descent, Fista, IstaProx, Momentum
                     PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
-    "AS_BPM", "BPM", "Shift_BPM", "ASProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
+    "AS_BPM", "BPM", "Shift_BPM", "TiltAnchor", "ASProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
     "RSProp", "ScalarField", "ShiftProp", "Mask", "TeaDOE", "TeaReflector", "FourierPhase", "FourierMask",
     "ClampProx", "CompositeProx", "Descent", "Fista", "IstaProx", "Momentum", "Nesterov", "NoDescent",
     "PositiveProx", "ProxRule", "compose", "make_rules", "setup", "update_",

@@ -567,6 +567,28 @@ def propagate_phase_backward_(dv: ScalarField, phase: Phase, prop: _KernelPropag
 
 # --------------------------------------------------------------------------- AS_BPM
 
+class TiltAnchor(_PipeComponent):
+    """``TiltAnchor(u; trainable=false, buffered=false)`` (``tilt_anchor.jl:28-77``): resets
+    the tilts of the fields passing through to the template's (``offset_tilts!``), in either
+    direction.  It has no parameters.  Inside a differentiated system the reverse pass
+    restores the tilts the field carried on entry (``backpropagate_with_gradient!`` with the
+    saved tilts), which is the exact adjoint of the forward carrier multiplication."""
+
+    def __init__(self, u: ScalarField, trainable: bool = False, buffered: bool = False):
+        self.tilts = tuple(list(t) if isinstance(t, list) else t for t in u.tilts)
+        self.trainable = False
+
+    def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return u.offset_tilts_(self.tilts)
+
+    def propagate(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self.propagate_(u.copy(), direction)
+
+    def backpropagate_(self, du: ScalarField, direction=Forward) -> ScalarField:
+        """``backpropagate!`` (``tilt_anchor.jl:70-72``): offsets to the anchor's tilts."""
+        return du.offset_tilts_(self.tilts)
+
+
 class BPM(_PipeComponent):
     """Split-step beam propagation through a graded-index volume (``BPM``, ``bpm.jl:8-74``);
     built as ``AS_BPM`` or ``Shift_BPM``.

@@ -7,6 +7,7 @@ in C order, byte-identical to the reference's column-major ``electric[nx, ny, B]
 
 from __future__ import annotations
 
+import ctypes
 import math
 from typing import Sequence, Union
 
@@ -115,6 +116,25 @@ class ScalarField:
     def set_field_tilts(self, tilts) -> "ScalarField":
         """New field over the same data with other tilts (``Fields.jl:220-230``)."""
         return ScalarField(self.electric, self.ds, self.lambdas, tilts)
+
+    def offset_tilts_(self, tilts) -> "ScalarField":
+        """``offset_tilts!(u, tilts)`` (``Fields.jl:280-293``): multiplies each field by
+        cis(2 pi / lambda ((sin(theta_x) - sin(theta_x')) x + (sin(theta_y) - sin(theta_y')) y))
+        on the device and takes the new tilts; nothing happens when they equal the current ones."""
+        from . import _lib
+        from .components import check_field, context
+        new = ScalarField(self.electric[..., :1, :1], self.ds, self.lambdas, tilts)  # parses the tilts
+        if tuple(new.tilts) == tuple(self.tilts):
+            return self
+        check_field(self.electric)
+        arr = lambda v: (ctypes.c_double * len(v))(*v)
+        (fx, fy), (tx, ty) = self.tilt_lists(), new.tilt_lists()
+        lams = self.lambda_list()
+        _lib.call("fh_offset_tilts", context(self.electric.device), ctypes.c_void_p(self.electric.data_ptr()),
+                  self.nx, self.ny, self.batch, self.ds[0], self.ds[1], arr(lams), len(lams), arr(fx), arr(fy),
+                  arr(tx), arr(ty), self.batch)
+        self.tilts = new.tilts
+        return self
 
     def power(self) -> torch.Tensor:
         """Per-field power sum|u|^2 dx dy, shape (B,) (``Fields.jl:843-846``)."""

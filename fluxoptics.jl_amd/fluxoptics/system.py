@@ -22,7 +22,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 
 import torch
 
-from .components import (BPM, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
+from .components import (BPM, TiltAnchor, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
                          _Modulator, check_field)
 from .components import phase_propagate_, propagate_phase_backward_
 from .fields import Backward, Forward, ScalarField, dir_sign
@@ -286,6 +286,9 @@ def _run_forward(comps: List, u: ScalarField, direction, save: bool) -> list:
                 else torch.empty_like(u.electric)
             c.propagate_and_save_(u, saved, direction)
             tape.append(("mod", (c,), saved))
+        elif save and isinstance(c, TiltAnchor):
+            tape.append(("anchor", (c,), u.tilts))
+            c.propagate_(u, direction)
         elif save and isinstance(c, BPM) and c.trainable:
             if any(t[0] == "bpm" and t[1][0] is c for t in tape):
                 raise NotImplementedError("a trainable BPM may appear once in a differentiated system "
@@ -325,6 +328,8 @@ def _run_backward(comps: List, tape: list, du: ScalarField, direction, grads: Di
             (c,) = cs
             g, acc = _grad_buffer(grads, c, c._param)
             c.backpropagate_with_gradient_(du, saved, g, direction, accumulate=acc)
+        elif kind == "anchor":
+            du.offset_tilts_(saved)
         elif kind == "bpm":
             (m,) = cs
             if m in grads:

@@ -215,6 +215,14 @@ int fh_group_propagate(fh_group* group, void* const* fields, int direction);
 int fh_group_graph_count(fh_group* group, int* count);
 int fh_group_destroy(fh_group* group);
 
+/* offset_tilts!(u, tilts) (Fields.jl:280-293), the operation of TiltAnchor (tilt_anchor.jl:
+ * 56-77): u_b <- u_b * cis(2*pi/lambda_b * ((sin(fx_b) - sin(tx_b)) x + (sin(fy_b) - sin(ty_b)) y))
+ * on the centred grid (GridUtils.jl:56-60), from the field's tilts (from_x, from_y) to new ones
+ * (to_x, to_y); host arrays of 1 or batch entries (ntilts), lambdas of 1 or batch. */
+int fh_offset_tilts(fh_ctx* ctx, void* u, int nx, int ny, int batch, double dx, double dy,
+                    const double* lambdas, int nlambdas, const double* from_x, const double* from_y,
+                    const double* to_x, const double* to_y, int ntilts);
+
 /* Mask (modulators/mask.jl:95-141): u <- u * m (Forward) or u * conj(m) (Backward) with m a
  * c64 [nx, ny] device transmission shared by the batch; saved (nullable) receives the input.
  * fh_mask_backward: dm (+)= sum_b conj_direction(dv * conj(saved)) (complex gradient, from the

@@ -141,6 +141,23 @@ def tilted_as_kernel(fx, fy, lam: float, tx: float, ty: float, track: bool, n0: 
     return np.exp(1j * (TWO_PI * z) * (np.sqrt(arg) + off))
 
 
+def offset_tilts(u, ds, lams, from_tilts, to_tilts):
+    """``offset_tilts!`` (``Fields.jl:280-293``): u_b * cis(2 pi / lam_b ((sin(fx_b) - sin(tx_b)) x
+    + (sin(fy_b) - sin(ty_b)) y)) on the centred grid ``spatial_vectors``."""
+    u = _as_batch(np.asarray(u, dtype=np.complex128))
+    b, ny, nx = u.shape
+    lam_b = _lambdas(lams, b)
+    xv, yv = spatial_vectors((nx, ny), ds)
+    out = np.empty_like(u)
+    for k in range(b):
+        f0 = [_per_field(from_tilts[i], b)[k] for i in (0, 1)]
+        t0 = [_per_field(to_tilts[i], b)[k] for i in (0, 1)]
+        ph = TWO_PI / lam_b[k] * ((math.sin(f0[0]) - math.sin(t0[0])) * xv[None, :]
+                                  + (math.sin(f0[1]) - math.sin(t0[1])) * yv[:, None])
+        out[k] = u[k] * np.exp(1j * ph)
+    return out
+
+
 def shift_kernel(fx, fy, tx: float, ty: float, z: float):
     """``shift_kernel`` (``freespace_propagators/shift_kernel.jl:1-6``):
     cis(-2 pi z (tan(tx) fx + tan(ty) fy))."""
@@ -605,14 +622,17 @@ def pipeline_forward(u, stages, ds, lams):
 
     ``OpticalSequence`` forward loop (``optical_sequence.jl:54-60``) with trainable
     modulators using ``propagate_and_save!`` (``phasemask.jl:100-113``).  A stage is
-    ``("as", z, n0)``, ``("rs", z, pad_mode)``, ``("phase", phi)``, ``("mask", m)`` or
-    ``("tea", h, dn, r)``.
+    ``("as", z, n0[, tilts])``, ``("rs", z, pad_mode)``, ``("phase", phi)``, ``("mask", m)``,
+    ``("tea", h, dn, r)`` or ``("anchor", from_tilts, to_tilts)`` (``TiltAnchor``, whose
+    reverse restores the entry tilts, ``tilt_anchor.jl:74-77``).
     """
     u = _as_batch(np.asarray(u, dtype=np.complex128))
     saved = []
     for st in stages:
         if st[0] == "as":
-            u = asprop(u, ds, lams, st[1], st[2] if len(st) > 2 else 1.0, 1)
+            u = asprop(u, ds, lams, st[1], st[2] if len(st) > 2 else 1.0, 1, tilts=st[3] if len(st) > 3 else None)
+        elif st[0] == "anchor":
+            u = offset_tilts(u, ds, lams, st[1], st[2])
         elif st[0] == "rs":
             u = rsprop(u, ds, lams, st[1], 1, st[2] if len(st) > 2 else "2n-1")
         elif st[0] == "phase":
@@ -637,7 +657,9 @@ def pipeline_backward(dv, stages, ds, lams, saved):
     k = len(saved)
     for st in reversed(stages):
         if st[0] == "as":
-            du = asprop(du, ds, lams, st[1], st[2] if len(st) > 2 else 1.0, -1)
+            du = asprop(du, ds, lams, st[1], st[2] if len(st) > 2 else 1.0, -1, tilts=st[3] if len(st) > 3 else None)
+        elif st[0] == "anchor":
+            du = offset_tilts(du, ds, lams, st[2], st[1])
         elif st[0] == "rs":
             du = rsprop(du, ds, lams, st[1], -1, st[2] if len(st) > 2 else "2n-1")
         elif st[0] == "phase":

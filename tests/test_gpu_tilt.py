@@ -6,8 +6,10 @@ import warnings
 
 import numpy as np
 import pytest
+import torch
 
 import fluxoptics as fo
+from fluxoptics import inputs
 from oracle import fluxoptics_oracle as O
 
 from _util import c64, random_field, rel_l2, speckle_batch, to_dev, to_host
@@ -101,3 +103,47 @@ def test_shiftprop_whole_pixels_is_a_roll():
     u = fo.ScalarField(to_dev(u0), (dx, dx), 1.0, tilts)
     v = fo.ShiftProp(u, z).propagate(u, fo.Forward)
     assert rel_l2(to_host(v.electric), np.roll(u0, (-3, 5), axis=(1, 2))) <= FIELD_TOL
+
+
+def test_offset_tilts_on_device():
+    """``offset_tilts!`` on the device: 40 fields (two coefficient chunks), per-field
+    wavelengths and tilts, non-square grid; then back to the original tilts."""
+    nx, ny, b = 128, 64, 40
+    print("[offset_tilts] 40 x 64x128", flush=True)
+    ds = (0.5, 0.7)
+    lams = [0.5 + 0.01 * k for k in range(b)]
+    tx = [0.002 * (k - 20) for k in range(b)]
+    ty = [0.03 - 0.001 * k for k in range(b)]
+    u0 = c64(random_field((b, ny, nx), 57))
+    u = fo.ScalarField(to_dev(u0), ds, lams, (tx, ty))
+    u.offset_tilts_((0.01, 0.0))
+    assert u.tilts == (0.01, 0.0)
+    ref = O.offset_tilts(u0, ds, lams, (tx, ty), (0.01, 0.0))
+    assert rel_l2(to_host(u.electric), ref) <= FIELD_TOL
+    u.offset_tilts_((tx, ty))
+    assert rel_l2(to_host(u.electric), u0) <= FIELD_TOL
+
+
+def test_system_with_tilt_anchor_gradient():
+    """Tilted source -> ASProp (tilted kernel) -> TiltAnchor(0) -> Phase -> ASProp (on-axis):
+    loss and Phase gradient against the oracle chain; the cotangent's tilts are restored."""
+    n, ds, lam = 256, (0.5, 0.5), 0.8
+    print("[tilt anchor system] 256^2 x 2", flush=True)
+    th = ([0.03, -0.02], [0.01, 0.02])
+    u0 = c64(speckle_batch(2, n, n, 0.5, 0.5, 25.0, 58))
+    phi = inputs.uniform(59, (n, n), 0, 2 * math.pi).astype(np.float32)
+    tgt = (np.abs(c64(speckle_batch(2, n, n, 0.5, 0.5, 25.0, 60))) ** 2).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), ds, lam, th)
+    flat = fo.ScalarField(to_dev(u0), ds, lam)
+    ph = fo.Phase(u, phi, trainable=True, buffered=True)
+    system = fo.ScalarSource(u) | fo.ASProp(u, 300.0) | fo.TiltAnchor(flat) | ph | fo.ASProp(flat, 200.0)
+    metric = fo.SquaredIntensityDifference((u, tgt))
+    stages = [("as", 300.0, 1.0, th), ("anchor", th, (0.0, 0.0)), ("phase", phi.astype(np.float64)),
+              ("as", 200.0)]
+    v_ref, saved = O.pipeline_forward(u0, stages, ds, lam)
+    vals_ref, dv_ref = O.sq_intensity_diff_values(v_ref, tgt.astype(np.float64))
+    _, (g_ref,) = O.pipeline_backward(dv_ref, stages, ds, lam, saved)
+    value, grads = system.withgradient(metric)
+    assert abs(float(value) - vals_ref.sum()) <= FIELD_TOL * vals_ref.sum()
+    torch.cuda.synchronize()
+    assert rel_l2(ph.grad.cpu().numpy(), g_ref) <= 1e-4
