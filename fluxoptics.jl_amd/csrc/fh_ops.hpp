@@ -63,6 +63,25 @@ struct PhaseTimeOp {
   }
 };
 
+// Separable chirp on the time side of an x pass (CollinsProp's A and D chirps,
+// collins_integral.jl:5-43): u(b, y, x) *= s * cx[t][x] * cy[t][y] (conjugated when conj),
+// t = idx[b] the field's chirp table.  Positions x >= nx (zero padding) are left alone.
+struct ChirpOp {
+  const float2* cx;  // [tables][nx]
+  const float2* cy;  // [tables][ny]
+  const int* idx;    // [batch]
+  int nx, ny;
+  int conj;
+  float s;
+  __device__ __forceinline__ float2 operator()(float2 v, int b, int y, int x) const {
+    if (x >= nx) return v;
+    const int t = idx[b];
+    const float2 h = cmul(cx[(long)t * nx + x], cy[(long)t * ny + y]);
+    const float2 w = conj ? cmulc(v, h) : cmul(v, h);
+    return make_float2(w.x * s, w.y * s);
+  }
+};
+
 // Multiply by a real constant (the 1/(nx*ny) of an inverse 2-D FFT, in the last pass).
 struct ScaleOp {
   float s;

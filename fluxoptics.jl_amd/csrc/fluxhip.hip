@@ -322,6 +322,39 @@ __global__ void k_rs_sample(double2* __restrict__ d, int nx, int ny, double dx, 
   d[(long)j * nx2 + i] = make_double2(re, im);
 }
 
+// CollinsProp's convolution kernel (collins_integral.jl:17-21, sampled as ConvolutionKernel
+// does, convolution_kernel.jl:13-18, on the 2n grid with offsets i <= n ? i : i - 2n):
+// h = nrm * cis(pi*(ax*x^2 + ay*y^2)/(b*lam)), phase taken in cycles and reduced in FP64;
+// nrm = dx*dy/(i*b) = prod(ds ./ sqrt(i*b)) (collins_integral.jl:122-123).
+__global__ void k_collins_sample(double2* __restrict__ d, int nx, int ny, double dx, double dy, double lam,
+                                 double ax, double ay, double b, double nrm_re, double nrm_im) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int nx2 = 2 * nx;
+  if (i >= nx2) return;
+  const int oi = i <= nx ? i : i - nx2;
+  const int oj = j <= ny ? j : j - 2 * ny;
+  const double x = oi * dx, y = oj * dy;
+  const double cyc = (ax * x * x + ay * y * y) / (2.0 * b * lam);
+  double sn, cs;
+  sincospi(2.0 * (cyc - rint(cyc)), &sn, &cs);
+  d[(long)j * nx2 + i] = make_double2(nrm_re * cs - nrm_im * sn, nrm_re * sn + nrm_im * cs);
+}
+
+// One line of a separable chirp on the centred grid x_i = (i - (n-1)/2)*ds:
+// out[i] = amp * cis(2*pi*(q*x^2 + f*x)), the cycles reduced in FP64 before sincospi.
+__global__ void k_chirp_line(float2* __restrict__ out, int n, double ds, double q, double f, double amp) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = ((double)i - 0.5 * (double)(n - 1)) * ds;
+  const double t = q * x * x + f * x;
+  double sn, cs;
+  sincospi(2.0 * (t - rint(t)), &sn, &cs);
+  out[i] = make_float2((float)(amp * cs), (float)(amp * sn));
+}
+
 // Table of an FP64 spectrum [ny2][nx2] on the padded grid, scaled and rounded: the even
 // quadrant [0..nx2/2]x[0..ny2/2] or the full layout (see tab_freq_index).
 __global__ void k_tab_extract(float2* __restrict__ tab, const double2* __restrict__ d, int nx2, int ny2,
@@ -788,7 +821,7 @@ struct fh_ctx {
 
 struct fh_plan {
   fh_ctx* ctx = nullptr;
-  int kind = 0;  // 0 = AS, 1 = RS
+  int kind = 0;  // 0 = AS, 1 = RS, 2 = Collins (RS geometry: zero padding to 2n)
   int nx = 0, ny = 0, batch = 0;
   int lx = 0, ly = 0;   // log2 of the transform lengths (padded for RS)
   double dx = 0, dy = 0, z = 0, n0 = 1;
@@ -807,6 +840,8 @@ struct fh_plan {
   double2* plain_tab = nullptr;    // plain FP64 comparator (AS): full tables
   int conv_var = -1;               // y-pass variant (-1 = built-in default)
   int xf_var = -1, xi_var = -1;    // x-pass variants (forward, inverse)
+  float2* chirp = nullptr;         // Collins: A_x, A_y, D_x, D_y chirp lines per table
+  double alpha_x = 1, alpha_y = 1; // Collins: output / input sampling
 };
 
 namespace fh {
@@ -928,13 +963,23 @@ static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
 template <typename Op>
 static int fast_xfwd(fh_ctx* c, int logn, bool half_in, const FastIO& io, int batch, Op op, bool lanes) {
   if (half_in) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, true, false, Op>(c, io, batch, op, lanes))) }
-  FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, false, false, Op>(c, io, batch, op, lanes)))
+  if constexpr (std::is_same<Op, ChirpOp>::value) {
+    return fail(FH_E_UNSUPP, "chirp prologue needs a padded plan");
+  } else {
+    FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, false, false, Op>(c, io, batch, op, lanes)))
+  }
 }
 
-// inverse x FFT; half_out: only n/2 points are stored (RS crop)
-static int fast_xinv(fh_ctx* c, int logn, bool half_out, const FastIO& io, int batch, bool lanes) {
-  if (half_out) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, true, NoOp>(c, io, batch, NoOp{}, lanes))) }
-  FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, false, NoOp>(c, io, batch, NoOp{}, lanes)))
+// inverse x FFT; half_out: only n/2 points are stored (RS crop).  The chirp op only runs on
+// padded (Collins) plans, so its unpadded variants are not instantiated.
+template <typename Op>
+static int fast_xinv(fh_ctx* c, int logn, bool half_out, const FastIO& io, int batch, Op op, bool lanes) {
+  if (half_out) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, true, Op>(c, io, batch, op, lanes))) }
+  if constexpr (std::is_same<Op, ChirpOp>::value) {
+    return fail(FH_E_UNSUPP, "chirp epilogue needs a padded plan");
+  } else {
+    FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, false, Op>(c, io, batch, op, lanes)))
+  }
 }
 
 // x-pass variants: 0 = lanes over positions of a line, 1 = lanes over the workgroup's lines
@@ -1113,6 +1158,7 @@ static void plan_free(fh_plan* p) {
   if (p->d_tabidx) (void)hipFree(p->d_tabidx);
   if (p->scratch) (void)hipFree(p->scratch);
   if (p->plain_tab) (void)hipFree(p->plain_tab);
+  if (p->chirp) (void)hipFree(p->chirp);
   delete p;
 }
 
@@ -1146,7 +1192,7 @@ template <typename XOp>
 static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1) {
   fh_ctx* c = p->ctx;
   const int nxl = 1 << p->lx;
-  const bool rs = p->kind == 1;
+  const bool rs = p->kind != 0;
   float2* mid = rs ? p->scratch : u;
   const long midf = (long)nxl * p->ny;
   int rc;
@@ -1172,16 +1218,44 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
   return vars[v].fn(c, io, p->batch, q, rs);
 }
 
-static int run_xifft(fh_plan* p, float2* u) {
+template <typename XOp>
+static int run_xifft_op(fh_plan* p, float2* u, XOp xop) {
   const int nxl = 1 << p->lx;
-  const bool rs = p->kind == 1;
+  const bool rs = p->kind != 0;
   float2* mid = rs ? p->scratch : u;
   FastIO io{};
   io.in = mid; io.out = u; io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
   io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1; io.in_blk = 1; io.out_blk = 0;
   io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
   const int xv = p->xi_var >= 0 ? p->xi_var : default_x_variant(p->lx);
-  return fast_xinv(p->ctx, p->lx, rs, io, p->batch, xv == 1);
+  return fast_xinv(p->ctx, p->lx, rs, io, p->batch, xop, xv == 1);
+}
+
+// CollinsProp's chirps for one direction (collins_integral.jl:155-189, reversed for
+// Backward): Forward applies A before the convolution and D after it; Backward applies
+// conj(D) before and conj(A) after, times alpha_x*alpha_y = the ratio of the reference's
+// backward normalisation prod(ds'/sqrt(-i b)) to the conjugate of the forward one folded
+// into the convolution table.
+static ChirpOp collins_op(const fh_plan* p, int direction, bool pre) {
+  const long nt = (long)p->lam_tables.size();
+  const float2* ax = p->chirp;
+  const float2* ay = ax + nt * p->nx;
+  const float2* dx = ay + nt * p->ny;
+  const float2* dy = dx + nt * p->nx;
+  const bool a = (direction > 0) == pre;
+  const float s = (direction < 0 && !pre) ? (float)(p->alpha_x * p->alpha_y) : 1.f;
+  return ChirpOp{a ? ax : dx, a ? ay : dy, p->d_tabidx, p->nx, p->ny, direction < 0, s};
+}
+
+// The passes of a plan's own propagation (chirps included for Collins).
+static int run_prop_plan(fh_plan* p, float2* u, int direction, int only = -1) {
+  if (p->kind == 2) return run_prop(p, u, direction, collins_op(p, direction, true), only);
+  return run_prop(p, u, direction, NoOp{}, only);
+}
+
+static int run_xifft(fh_plan* p, float2* u, int direction) {
+  if (p->kind == 2) return run_xifft_op(p, u, collins_op(p, direction, false));
+  return run_xifft_op(p, u, NoOp{});
 }
 
 }  // namespace fh
@@ -1342,6 +1416,86 @@ int fh_rsprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   return FH_OK;
 }
 
+int fh_collins_create(fh_ctx* ctx, const fh_prop_desc* d, double dx_out, double dy_out, double a, double b,
+                      double dd, fh_plan** out) {
+  if (!out || !d) return fail(FH_E_ARG, "null descriptor or output pointer");
+  *out = nullptr;
+  if (d->flags & ~FH_PROP_TRACK_TILTS) return fail(FH_E_ARG, "CollinsProp takes FH_PROP_TRACK_TILTS only (flags 0x%x)", d->flags);
+  if (d->z || d->nz) return fail(FH_E_ARG, "CollinsProp takes (A, B, D), not distances (z must be null)");
+  if (!std::isfinite(a) || !std::isfinite(b) || !std::isfinite(dd) || b == 0.0)
+    return fail(FH_E_ARG, "CollinsProp needs finite A, B, D with B != 0");
+  if (!(dx_out > 0) || !(dy_out > 0) || !std::isfinite(dx_out) || !std::isfinite(dy_out))
+    return fail(FH_E_ARG, "output sampling must be positive");
+  fh_plan* p = new fh_plan();
+  int rc = plan_common(ctx, d->nx, d->ny, d->batch, d->dx, d->dy, d->lambdas, d->nlambdas, p, nullptr, 0,
+                       d->tilts_x, d->tilts_y, d->ntilts);
+  if (rc) { plan_free(p); return rc; }
+  const int nx = d->nx, ny = d->ny;
+  p->kind = 2;
+  p->lx = ilog2_exact(2 * nx); p->ly = ilog2_exact(2 * ny);
+  if (p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
+    plan_free(p);
+    return fail(FH_E_UNSUPP, "CollinsProp needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << (kMinLog - 1), 1 << (kMaxLog - 1), nx, ny);
+  }
+  if ((rc = set_device(ctx))) { plan_free(p); return rc; }
+  const int nx2 = 2 * nx, ny2 = 2 * ny;
+  const double ax = dx_out / d->dx, ay = dy_out / d->dy;  // alpha = ds'/ds (collins_integral.jl:72)
+  p->alpha_x = ax; p->alpha_y = ay;
+  p->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
+  // the convolution kernel is even in x and y: quadrant tables
+  p->qx = nx + 1; p->qy = ny + 1;
+  p->tabsize = (long)p->qx * p->qy;
+  const long nt = (long)p->lam_tables.size();
+  if (hipMalloc(&p->tab, p->tabsize * nt * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&p->chirp, nt * 2 * (nx + ny) * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&p->scratch, (size_t)nx2 * ny * d->batch * sizeof(float2)) != hipSuccess) {
+    plan_free(p);
+    return fail(FH_E_NOMEM, "CollinsProp tables and workspace: out of device memory");
+  }
+  double2* dbuf = nullptr;
+  if (hipMalloc(&dbuf, (size_t)nx2 * ny2 * sizeof(double2)) != hipSuccess) {
+    plan_free(p);
+    return fail(FH_E_NOMEM, "CollinsProp FP64 kernel buffer: out of device memory");
+  }
+  const double scale = 1.0 / ((double)nx2 * (double)ny2);
+  const double nrm_im = -d->dx * d->dy / b;  // dx*dy/(i*b)
+  const bool tilted = plan_tilted(p);
+  float2* cax = p->chirp;
+  float2* cay = cax + nt * nx;
+  float2* cdx = cay + nt * ny;
+  float2* cdy = cdx + nt * nx;
+  for (long k = 0; k < nt && rc == FH_OK; ++k) {
+    const double lam = p->lam_tables[k];
+    hipLaunchKernelGGL(k_collins_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, dbuf, nx, ny,
+                       d->dx, d->dy, lam, ax, ay, b, 0.0, nrm_im);
+    if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "Collins sample kernel launch failed"); break; }
+    if ((rc = fft2_d(ctx, dbuf, nx2, ny2, p->lx, p->ly))) break;
+    hipLaunchKernelGGL(k_tab_extract, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
+                       p->tab + k * p->tabsize, dbuf, nx2, ny2, p->qx, p->qy, 0, scale);
+    // chirps (collins_integral.jl:5-36) on the input grid, in cycles: A = cis(pi*x^2*(a -
+    // alpha)/(b*lam))/lam, D = cis(pi*x^2*alpha*(d*alpha - 1)/(b*lam)); a tilted field whose
+    // tilts are not tracked adds cis(+-2*pi*x*sin(theta)/lam) to A (+) and D (-)
+    const bool lin = tilted && !p->track;
+    const double fx = lin ? sin(p->tab_tx[k]) / lam : 0.0, fy = lin ? sin(p->tab_ty[k]) / lam : 0.0;
+    const double bl = 2.0 * b * lam;
+    hipLaunchKernelGGL(k_chirp_line, dim3((nx + 255) / 256), dim3(256), 0, ctx->stream, cax + k * nx, nx, d->dx,
+                       (a - ax) / bl, fx, 1.0 / lam);
+    hipLaunchKernelGGL(k_chirp_line, dim3((ny + 255) / 256), dim3(256), 0, ctx->stream, cay + k * ny, ny, d->dy,
+                       (a - ay) / bl, fy, 1.0);
+    hipLaunchKernelGGL(k_chirp_line, dim3((nx + 255) / 256), dim3(256), 0, ctx->stream, cdx + k * nx, nx, d->dx,
+                       ax * (dd * ax - 1.0) / bl, -fx, 1.0);
+    hipLaunchKernelGGL(k_chirp_line, dim3((ny + 255) / 256), dim3(256), 0, ctx->stream, cdy + k * ny, ny, d->dy,
+                       ay * (dd * ay - 1.0) / bl, -fy, 1.0);
+    if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "Collins table kernel launch failed"); break; }
+  }
+  if (rc == FH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(FH_E_HIP, "Collins table generation failed");
+  (void)hipFree(dbuf);
+  if (rc == FH_OK) rc = upload_tabidx(p);
+  if (rc) { plan_free(p); return rc; }
+  *out = p;
+  return FH_OK;
+}
+
 int fh_rsprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
                      const double* lambdas, int nlambdas, double z, fh_plan** out) {
   fh_prop_desc d{};
@@ -1362,8 +1516,8 @@ int fh_propagate(fh_plan* p, void* u, int direction) {
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(u);
-  if ((rc = run_prop(p, f, direction, NoOp{}))) return rc;
-  return run_xifft(p, f);
+  if ((rc = run_prop_plan(p, f, direction))) return rc;
+  return run_xifft(p, f, direction);
 }
 
 int fh_propagate_pass(fh_plan* p, void* u, int direction, int pass) {
@@ -1372,8 +1526,8 @@ int fh_propagate_pass(fh_plan* p, void* u, int direction, int pass) {
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(u);
-  if (pass < 2) return run_prop(p, f, direction, NoOp{}, pass);
-  return run_xifft(p, f);
+  if (pass < 2) return run_prop_plan(p, f, direction, pass);
+  return run_xifft(p, f, direction);
 }
 
 int fh_plan_variants(fh_plan* p, int pass, int* count, int* current) {
@@ -1413,7 +1567,7 @@ int fh_autotune(fh_plan* p, void* u, int reps, char* report, int report_len) {
     int n = kNumXVariants;
     const FastConvVariant* vars = pass == 1 ? fast_conv_variants(p->ly, &n) : nullptr;
     int* sel = pass == 0 ? &p->xf_var : (pass == 1 ? &p->conv_var : &p->xi_var);
-    auto run = [&]() { return pass == 2 ? run_xifft(p, f) : run_prop(p, f, 1, NoOp{}, pass); };
+    auto run = [&]() { return pass == 2 ? run_xifft(p, f, 1) : run_prop_plan(p, f, 1, pass); };
     char buf[200];
     snprintf(buf, sizeof buf, "%s\"%s\": {\"variants\": [", pass ? ", " : "", kPassName[pass]);
     js += buf;
@@ -1454,9 +1608,10 @@ int fh_phase_propagate(fh_plan* p, void* u, const float* phi, void* saved, int d
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(u);
+  if (p->kind == 2) return fail(FH_E_UNSUPP, "Phase fuses with ASProp and RSProp plans only");
   PhaseTimeOp op{phi, reinterpret_cast<float2*>(saved), (long)p->nx * p->ny, p->nx, (float)direction};
   if ((rc = run_prop(p, f, direction, op))) return rc;
-  return run_xifft(p, f);
+  return run_xifft(p, f, direction);
 }
 
 int fh_phase_apply(fh_ctx* ctx, void* u, const float* phi, int nx, int ny, int batch, int direction,
@@ -1711,6 +1866,7 @@ int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const vo
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(dv);
+  if (p->kind == 2) return fail(FH_E_UNSUPP, "Phase fuses with ASProp and RSProp plans only");
   // adjoint propagation = the propagator in the reverse direction
   if ((rc = run_prop(p, f, -direction, NoOp{}))) return rc;
   const int nxl = 1 << p->lx;
@@ -2134,8 +2290,8 @@ static int group_capture(fh_group* g, void* const* fields, int direction, fh_gro
   for (size_t i = 0; e == hipSuccess && rc == FH_OK && i < g->plans.size(); ++i) {
     c->stream = g->side[i % g->side.size()];
     float2* f = reinterpret_cast<float2*>(fields[i]);
-    rc = run_prop(g->plans[i], f, direction, NoOp{});
-    if (rc == FH_OK) rc = run_xifft(g->plans[i], f);
+    rc = run_prop_plan(g->plans[i], f, direction);
+    if (rc == FH_OK) rc = run_xifft(g->plans[i], f, direction);
   }
   c->stream = user;
   for (size_t s = 0; e == hipSuccess && s < g->side.size(); ++s) {

@@ -7,7 +7,7 @@ the arithmetic is the HIP library's (``libfluxhip.so``, C ABI in include/fluxhip
 
 from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficiency,
                      rs_valid_distance)
-from .components import (AS_BPM, BPM, ASProp, FourierMask, FourierOperator, FourierPhase, FourierWrapper,
+from .components import (AS_BPM, BPM, ASProp, CollinsProp, FourierLens, ParaxialProp, FourierMask, FourierOperator, FourierPhase, FourierWrapper,
                          Mask, Phase, RSProp, ShiftProp, Shift_BPM, TeaDOE, TeaReflector, TiltAnchor,
                          phase_propagate_,
                          propagate_phase_backward_)
@@ -18,7 +18,7 @@ from .optim import (ClampProx, CompositeProx, Descent, Fista, IstaProx, Momentum
                     PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
-    "AS_BPM", "BPM", "Shift_BPM", "TiltAnchor", "ASProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
+    "AS_BPM", "BPM", "Shift_BPM", "TiltAnchor", "ASProp", "CollinsProp", "FourierLens", "ParaxialProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
     "RSProp", "ScalarField", "ShiftProp", "Mask", "TeaDOE", "TeaReflector", "FourierPhase", "FourierMask",
     "ClampProx", "CompositeProx", "Descent", "Fista", "IstaProx", "Momentum", "Nesterov", "NoDescent",
     "PositiveProx", "ProxRule", "compose", "make_rules", "setup", "update_",

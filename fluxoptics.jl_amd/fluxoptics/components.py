@@ -124,9 +124,12 @@ class _KernelPropagator(_PipeComponent):
         if p is None:
             p = ctypes.c_void_p()
             desc = self._desc(u.batch, lams, tilts, self._flags())
-            _lib.call(self._create_ex, ctx, ctypes.byref(desc), ctypes.byref(p))
+            self._create(ctx, desc, p)
             self._plans[key] = p
         return p
+
+    def _create(self, ctx, desc, p) -> None:
+        _lib.call(self._create_ex, ctx, ctypes.byref(desc), ctypes.byref(p))
 
     def __del__(self):
         try:
@@ -242,6 +245,84 @@ class RSProp(_KernelPropagator):
             warnings.warn(f"RSProp: propagation distance z={zmin} is below critical distance "
                           f"zc={zc}. Numerical artifacts expected. Consider using ASProp or finer "
                           "sampling (dx < lambda/2).")
+
+
+class CollinsProp(_KernelPropagator):
+    """``CollinsProp(u, ds', (A, B, D))`` or ``CollinsProp(u, ds, ds', (A, B, D))``
+    (``collins_integral.jl:200-301``): the Collins integral of the ABCD system with
+    AD - BC = 1, resampled from ``ds`` (default ``u.ds``) to ``ds'``.  One fused plan: the A
+    chirp rides on the first x FFT, the zero-padded convolution on the y pass, the D chirp
+    on the last x IFFT.  Forward relabels the field's sampling to ``ds'``, Backward to
+    ``ds``.  Backward (and so the pullback, ``freespace.jl:61-63``) conjugates every factor
+    and normalises by prod(ds' / sqrt(-i B)) as the reference does, which is alpha_x alpha_y
+    times the plain adjoint when ds' differs from ds."""
+
+    _create_ex = "fh_collins_create"
+
+    def __init__(self, u: ScalarField, *args, use_cache: bool = True, track_tilts: bool = False,
+                 double_precision_kernel: Optional[bool] = None):
+        if len(args) == 2:
+            ds, ds_out, abd = None, args[0], args[1]
+        elif len(args) == 3:
+            ds, ds_out, abd = args
+        else:
+            raise TypeError("CollinsProp(u, ds_out, (A, B, D)) or CollinsProp(u, ds, ds_out, (A, B, D))")
+        if double_precision_kernel is False:
+            raise NotImplementedError("chirps and kernels are always computed in FP64")
+        super().__init__(u, ds, 0.0)
+        self.ds_out = (float(ds_out[0]), float(ds_out[1]))
+        a, b, d = (float(v) for v in abd)
+        if b == 0.0:
+            raise ValueError("CollinsProp cannot be used with B = 0")
+        self.abd = (a, b, d)
+        self.track_tilts = bool(track_tilts)
+
+    def _desc(self, batch, lams, tilts, flags) -> "_lib.PropDesc":
+        d = super()._desc(batch, lams, tilts, flags)
+        d.z, d.nz = None, 0
+        return d
+
+    def _create(self, ctx, desc, p) -> None:
+        _lib.call("fh_collins_create", ctx, ctypes.byref(desc), self.ds_out[0], self.ds_out[1], *self.abd,
+                  ctypes.byref(p))
+
+    def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
+        super().propagate_(u, direction)
+        u.ds = self.ds_out if dir_sign(direction) > 0 else self.ds
+        return u
+
+
+def FourierLens(u: ScalarField, *args, use_cache: bool = True,
+                double_precision_kernel: Optional[bool] = None) -> CollinsProp:
+    """``FourierLens(u, ds', fl)`` or ``FourierLens(u, ds, ds', fl)`` (``fourier_lens.jl:38-64``):
+    the ideal Fourier lens, ``CollinsProp`` with ABCD = (0, fl, 0)."""
+    if len(args) == 2:
+        return CollinsProp(u, args[0], (0.0, args[1], 0.0), use_cache=use_cache,
+                           double_precision_kernel=double_precision_kernel)
+    if len(args) == 3:
+        return CollinsProp(u, args[0], args[1], (0.0, args[2], 0.0), use_cache=use_cache,
+                           double_precision_kernel=double_precision_kernel)
+    raise TypeError("FourierLens(u, ds_out, fl) or FourierLens(u, ds, ds_out, fl)")
+
+
+def ParaxialProp(u: ScalarField, *args, use_cache: bool = True, filter: Optional[Callable] = None,
+                 double_precision_kernel: Optional[bool] = None):
+    """``ParaxialProp(u, z)``, ``ParaxialProp(u, ds', z)`` or ``ParaxialProp(u, ds, ds', z)``
+    (``paraxial_propagation.jl:33-82``): the paraxial ASProp without resampling, else
+    ``CollinsProp`` with ABCD = (1, z, 1)."""
+    if len(args) == 1:
+        return ASProp(u, args[0], paraxial=True, filter=filter, double_precision_kernel=double_precision_kernel)
+    if len(args) == 2:
+        ds, ds_out, z = tuple(u.ds), args[0], args[1]
+    elif len(args) == 3:
+        ds, ds_out, z = args
+    else:
+        raise TypeError("ParaxialProp(u, z), ParaxialProp(u, ds_out, z) or ParaxialProp(u, ds, ds_out, z)")
+    ds, ds_out = tuple(float(v) for v in ds), tuple(float(v) for v in ds_out)
+    if ds == ds_out:
+        return ASProp(u, z, ds, paraxial=True, filter=filter, double_precision_kernel=double_precision_kernel)
+    return CollinsProp(u, ds, ds_out, (1.0, z, 1.0), use_cache=use_cache,
+                       double_precision_kernel=double_precision_kernel)
 
 
 # --------------------------------------------------------------------------- Fourier operators

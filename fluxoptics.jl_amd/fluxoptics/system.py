@@ -22,7 +22,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 
 import torch
 
-from .components import (BPM, TiltAnchor, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
+from .components import (BPM, CollinsProp, TiltAnchor, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
                          _Modulator, check_field)
 from .components import phase_propagate_, propagate_phase_backward_
 from .fields import Backward, Forward, ScalarField, dir_sign
@@ -255,7 +255,8 @@ class CapturedGradient:
 # --------------------------------------------------------------------------- executor
 
 def _is_prop(c) -> bool:
-    return isinstance(c, _KernelPropagator)
+    """A propagator whose plan a preceding Phase can ride on (its first pass has no chirp)."""
+    return isinstance(c, _KernelPropagator) and not isinstance(c, CollinsProp)
 
 
 def _run_forward(comps: List, u: ScalarField, direction, save: bool) -> list:

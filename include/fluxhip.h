@@ -103,6 +103,18 @@ int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, fh_plan** out);
  * selected at :90-96 when a tilt is nonzero).  flags: FH_PROP_TRACK_TILTS only. */
 int fh_rsprop_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, fh_plan** out);
 
+/* CollinsProp(u, ds, ds', (A, B, D)) (collins_integral.jl:200-301; FourierLens is (0, f, 0),
+ * fourier_lens.jl:38-48; ParaxialProp with resampling is (1, z, 1)): the A chirp, the
+ * zero-padded convolution with cis(pi*(ax x^2 + ay y^2)/(B lambda)) normalised by
+ * prod(ds ./ sqrt(i B)), and the D chirp, alpha = ds'/ds, chirps on the input grid.  In the
+ * plan the chirps ride on the first and last x passes, so it costs what an RSProp does.
+ * Backward conjugates every factor and normalises by prod(ds' ./ sqrt(-i B)), as the
+ * reference does (collins_integral.jl:189).  desc->z must be NULL; flags:
+ * FH_PROP_TRACK_TILTS only (untracked tilts add cis(+-2 pi x sin(theta)/lambda) to A, D).
+ * The caller relabels the field's sampling (ds' after Forward, ds after Backward). */
+int fh_collins_create(fh_ctx* ctx, const fh_prop_desc* desc, double dx_out, double dy_out, double a,
+                      double b, double d, fh_plan** out);
+
 int fh_plan_destroy(fh_plan* plan);
 
 /* propagate!(u, p, direction): in place on a c64 [nx, ny, batch] device array

@@ -399,6 +399,90 @@ def tea_backward(dv, u_saved, h, lams, dn, r=1.0, direction: int = 1):
     return du, direction * dh
 
 
+# ----------------------------------------------------------------------------- CollinsProp
+
+def collins_chirps(nx: int, ny: int, ds, lam: float, alpha, a: float, b: float, d: float, tilts=None,
+                   track: bool = False):
+    """The A and D chirps on the centred input grid (``collins_integral.jl:5-43``):
+    A = cis(pi (x^2 (a - ax) + y^2 (a - ay)) / (b lam)) / lam and
+    D = cis(pi (x^2 ax (d ax - 1) + y^2 ay (d ay - 1)) / (b lam)); a tilted field whose tilts
+    are not tracked adds cis(2 pi (x f0x + y f0y)) to A and its conjugate to D (f0 = sin/lam)."""
+    xv, yv = spatial_vectors((nx, ny), ds)
+    x, y = xv[None, :], yv[:, None]
+    A = np.exp(1j * math.pi * (x ** 2 * (a - alpha[0]) + y ** 2 * (a - alpha[1])) / (b * lam)) / lam
+    D = np.exp(1j * math.pi * (x ** 2 * alpha[0] * (d * alpha[0] - 1) + y ** 2 * alpha[1] * (d * alpha[1] - 1))
+               / (b * lam))
+    if tilts is not None and not track and (tilts[0] != 0 or tilts[1] != 0):
+        lin = np.exp(1j * TWO_PI * (x * math.sin(tilts[0]) / lam + y * math.sin(tilts[1]) / lam))
+        A, D = A * lin, D * np.conj(lin)
+    return A, D
+
+
+def collins_conv_spectrum(nx: int, ny: int, ds, lam: float, alpha, b: float):
+    """FFT of ``collins_convolution_kernel`` cis(pi (x^2 ax + y^2 ay) / (b lam)) sampled on the
+    (2n-1) grid of offsets circshift((1-n):(n-1), n) * ds (``convolution_kernel.jl:13-18``,
+    ``abstract_kernel.jl:15-25``)."""
+    sx = np.roll(np.arange(1 - nx, nx), nx) * ds[0]
+    sy = np.roll(np.arange(1 - ny, ny), ny) * ds[1]
+    h = np.exp(1j * math.pi * (sx[None, :] ** 2 * alpha[0] + sy[:, None] ** 2 * alpha[1]) / (b * lam))
+    return np.fft.fft2(h)
+
+
+def collinsprop(u, ds, ds_out, abd, lams, direction: int = 1, tilts=None, track: bool = False):
+    """``CollinsProp`` (``collins_integral.jl:262-296``): the sequence A chirp, pad to (2n-1),
+    FFT, convolution spectrum times prod(ds ./ sqrt(i b)), IFFT, crop, D chirp; Backward runs it
+    reversed with every factor conjugated and prod(ds' ./ sqrt(-i b)) (``:189-198``)."""
+    u = _as_batch(np.asarray(u, dtype=np.complex128))
+    nb, ny, nx = u.shape
+    a, b, d = abd
+    alpha = (ds_out[0] / ds[0], ds_out[1] / ds[1])
+    lam_b = _lambdas(lams, nb)
+    tx = _per_field(0.0 if tilts is None else tilts[0], nb)
+    ty = _per_field(0.0 if tilts is None else tilts[1], nb)
+    on_axis = all(v == 0 for v in tx) and all(v == 0 for v in ty)
+    out = np.empty_like(u)
+    for k in range(nb):
+        A, D = collins_chirps(nx, ny, ds, lam_b[k], alpha, a, b, d, None if on_axis else (tx[k], ty[k]), track)
+        H = collins_conv_spectrum(nx, ny, ds, lam_b[k], alpha, b)
+        if direction > 0:
+            pre, post, nrm = A, D, np.prod([complex(v) / np.sqrt(1j * b) for v in ds])
+        else:
+            pre, post, H = np.conj(D), np.conj(A), np.conj(H)
+            nrm = np.prod([complex(v) / np.sqrt(-1j * b) for v in ds_out])
+        w = np.zeros((2 * ny - 1, 2 * nx - 1), np.complex128)
+        w[:ny, :nx] = u[k] * pre
+        w = np.fft.ifft2(np.fft.fft2(w) * H) * nrm
+        out[k] = w[:ny, :nx] * post
+    return out
+
+
+def collins_direct(u, ds, ds_out, abd, lams, tilts=None):
+    """Brute-force Collins integral (Collins, JOSA 60, 1168 (1970)) on the grids:
+    U(x'_m) = dx dy / (i lam b) sum_k u_k c(x_k) exp(i pi (a x_k^2 - 2 x_k x'_m + d x'_m^2) / (lam b))
+    / c(x~_m), x'_m = alpha x~_m with x~ the centred input grid and c the tilt carrier."""
+    u = _as_batch(np.asarray(u, dtype=np.complex128))
+    nb, ny, nx = u.shape
+    a, b, d = abd
+    alpha = (ds_out[0] / ds[0], ds_out[1] / ds[1])
+    lam_b = _lambdas(lams, nb)
+    xv, yv = spatial_vectors((nx, ny), ds)
+    out = np.empty_like(u)
+    for k in range(nb):
+        lam = lam_b[k]
+        f0 = (0.0, 0.0) if tilts is None else (math.sin(_per_field(tilts[0], nb)[k]) / lam,
+                                               math.sin(_per_field(tilts[1], nb)[k]) / lam)
+        kx = np.exp(1j * math.pi * (a * xv[None, :] ** 2 - 2 * xv[None, :] * (alpha[0] * xv[:, None])
+                                    + d * (alpha[0] * xv[:, None]) ** 2) / (lam * b))  # [m, k]
+        ky = np.exp(1j * math.pi * (a * yv[None, :] ** 2 - 2 * yv[None, :] * (alpha[1] * yv[:, None])
+                                    + d * (alpha[1] * yv[:, None]) ** 2) / (lam * b))
+        cx = np.exp(1j * TWO_PI * f0[0] * xv)
+        cy = np.exp(1j * TWO_PI * f0[1] * yv)
+        w = u[k] * cy[:, None] * cx[None, :]
+        v = ky @ w @ kx.T
+        out[k] = v / (cy[:, None] * cx[None, :]) * ds[0] * ds[1] / (1j * lam * b)
+    return out
+
+
 # ----------------------------------------------------------------------------- AS_BPM
 
 def bpm_kdz(dz: float, tilts=(0.0, 0.0)) -> float:
@@ -623,8 +707,9 @@ def pipeline_forward(u, stages, ds, lams):
     ``OpticalSequence`` forward loop (``optical_sequence.jl:54-60``) with trainable
     modulators using ``propagate_and_save!`` (``phasemask.jl:100-113``).  A stage is
     ``("as", z, n0[, tilts])``, ``("rs", z, pad_mode)``, ``("phase", phi)``, ``("mask", m)``,
-    ``("tea", h, dn, r)`` or ``("anchor", from_tilts, to_tilts)`` (``TiltAnchor``, whose
-    reverse restores the entry tilts, ``tilt_anchor.jl:74-77``).
+    ``("tea", h, dn, r)``, ``("anchor", from_tilts, to_tilts)`` (``TiltAnchor``, whose
+    reverse restores the entry tilts, ``tilt_anchor.jl:74-77``) or ``("collins", ds, ds', abd)``
+    (whose reverse is CollinsProp's Backward, as the reference's pullback).
     """
     u = _as_batch(np.asarray(u, dtype=np.complex128))
     saved = []
@@ -633,6 +718,8 @@ def pipeline_forward(u, stages, ds, lams):
             u = asprop(u, ds, lams, st[1], st[2] if len(st) > 2 else 1.0, 1, tilts=st[3] if len(st) > 3 else None)
         elif st[0] == "anchor":
             u = offset_tilts(u, ds, lams, st[1], st[2])
+        elif st[0] == "collins":
+            u = collinsprop(u, st[1], st[2], st[3], lams, 1)
         elif st[0] == "rs":
             u = rsprop(u, ds, lams, st[1], 1, st[2] if len(st) > 2 else "2n-1")
         elif st[0] == "phase":
@@ -660,6 +747,8 @@ def pipeline_backward(dv, stages, ds, lams, saved):
             du = asprop(du, ds, lams, st[1], st[2] if len(st) > 2 else 1.0, -1, tilts=st[3] if len(st) > 3 else None)
         elif st[0] == "anchor":
             du = offset_tilts(du, ds, lams, st[2], st[1])
+        elif st[0] == "collins":
+            du = collinsprop(du, st[1], st[2], st[3], lams, -1)
         elif st[0] == "rs":
             du = rsprop(du, ds, lams, st[1], -1, st[2] if len(st) > 2 else "2n-1")
         elif st[0] == "phase":
