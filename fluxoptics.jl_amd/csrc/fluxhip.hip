@@ -355,6 +355,62 @@ __global__ void k_chirp_line(float2* __restrict__ out, int n, double ds, double 
   out[i] = make_float2((float)(amp * cs), (float)(amp * sn));
 }
 
+// CZT parameters of one dimension (FFTutils.jl:30-50): a = rho_a e^{i theta_a},
+// w = rho_w e^{i theta_w}; an untransformed dimension has on = 0.
+struct CztDim {
+  int on, center;
+  double la, ta, lw, tw;  // log(rho_a), theta_a, log(rho_w), theta_w
+};
+
+// exp(re) * cis(im) with the angle reduced in FP64 cycles before sincospi.
+__device__ inline double2 cexp_reduced(double re, double im) {
+  const double t = im / (2.0 * M_PI);
+  double sn, cs;
+  sincospi(2.0 * (t - rint(t)), &sn, &cs);
+  const double m = exp(re);
+  return make_double2(m * cs, m * sn);
+}
+
+// The Bluestein kernel h[f] = w^{-f^2/2} of prepare_czt_data (FFTutils.jl:41), separable,
+// sampled on the 2n grid with offsets f = i <= n ? i : i - 2n (the (2n-1)-grid values at
+// |f| < n; the offset -n is never reached by the cropped convolution); an untransformed
+// dimension contributes a unit impulse at f = 0.
+__device__ inline double2 czt_h(const CztDim& d, int f) {
+  if (!d.on) return make_double2(f == 0 ? 1.0 : 0.0, 0.0);
+  const double q = -0.5 * (double)f * (double)f;
+  return cexp_reduced(q * d.lw, q * d.tw);
+}
+
+__global__ void k_czt_sample(double2* __restrict__ out, int nx, int ny, CztDim dx, CztDim dy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int nx2 = 2 * nx;
+  if (i >= nx2) return;
+  const int oi = i <= nx ? i : i - nx2;
+  const int oj = j <= ny ? j : j - 2 * ny;
+  const double2 a = czt_h(dx, oi), b = czt_h(dy, oj);
+  out[(long)j * nx2 + i] = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Input (w^{k^2/2} a^{-k}) or output (w^{k^2/2}) chirp line, k = i or i - (n-1)/2 when centred
+// on the grid (FFTutils.jl:33-40); 1 on an untransformed dimension.
+__global__ void k_czt_chirp(float2* __restrict__ out, int n, CztDim d, int input) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!d.on) {
+    out[i] = make_float2(1.f, 0.f);
+    return;
+  }
+  const double k = d.center ? (double)i - 0.5 * (double)(n - 1) : (double)i;
+  double re = 0.5 * k * k * d.lw, im = 0.5 * k * k * d.tw;
+  if (input) {
+    re -= k * d.la;
+    im -= k * d.ta;
+  }
+  const double2 v = cexp_reduced(re, im);
+  out[i] = make_float2((float)v.x, (float)v.y);
+}
+
 // Table of an FP64 spectrum [ny2][nx2] on the padded grid, scaled and rounded: the even
 // quadrant [0..nx2/2]x[0..ny2/2] or the full layout (see tab_freq_index).
 __global__ void k_tab_extract(float2* __restrict__ tab, const double2* __restrict__ d, int nx2, int ny2,
@@ -1489,6 +1545,73 @@ int fh_collins_create(fh_ctx* ctx, const fh_prop_desc* d, double dx_out, double 
     if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "Collins table kernel launch failed"); break; }
   }
   if (rc == FH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(FH_E_HIP, "Collins table generation failed");
+  (void)hipFree(dbuf);
+  if (rc == FH_OK) rc = upload_tabidx(p);
+  if (rc) { plan_free(p); return rc; }
+  *out = p;
+  return FH_OK;
+}
+
+int fh_czt_create(fh_ctx* ctx, int nx, int ny, int batch, int dims, const double* a, const double* w,
+                  int center_on_grid, fh_plan** out) {
+  if (!out) return fail(FH_E_ARG, "null output pointer");
+  *out = nullptr;
+  if (dims < 1 || dims > 3) return fail(FH_E_ARG, "dims must be 1 (x), 2 (y) or 3 (both), got %d", dims);
+  if (!a || !w) return fail(FH_E_ARG, "null a or w");
+  CztDim dd[2];
+  for (int k = 0; k < 2; ++k) {
+    const double ra = a[2 * k], ta = a[2 * k + 1], rw = w[2 * k], tw = w[2 * k + 1];
+    dd[k].on = (dims >> k) & 1;
+    dd[k].center = center_on_grid ? 1 : 0;
+    if (dd[k].on && (!(ra > 0) || !(rw > 0) || !std::isfinite(ra) || !std::isfinite(rw) || !std::isfinite(ta) ||
+                     !std::isfinite(tw)))
+      return fail(FH_E_ARG, "CZT needs finite a and w with positive magnitudes");
+    dd[k].la = dd[k].on ? log(ra) : 0.0; dd[k].ta = dd[k].on ? ta : 0.0;
+    dd[k].lw = dd[k].on ? log(rw) : 0.0; dd[k].tw = dd[k].on ? tw : 0.0;
+  }
+  const double one = 1.0;
+  fh_plan* p = new fh_plan();
+  int rc = plan_common(ctx, nx, ny, batch, 1.0, 1.0, &one, 1, p);
+  if (rc) { plan_free(p); return rc; }
+  p->kind = 2;  // the padded chirp / convolution / chirp plan of CollinsProp, with unit scale
+  p->lx = ilog2_exact(2 * nx); p->ly = ilog2_exact(2 * ny);
+  if (p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
+    plan_free(p);
+    return fail(FH_E_UNSUPP, "CZT needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << (kMinLog - 1), 1 << (kMaxLog - 1), nx, ny);
+  }
+  if ((rc = set_device(ctx))) { plan_free(p); return rc; }
+  const int nx2 = 2 * nx, ny2 = 2 * ny;
+  p->qx = nx + 1; p->qy = ny + 1;  // h depends on f^2: even, quadrant tables
+  p->tabsize = (long)p->qx * p->qy;
+  if (hipMalloc(&p->tab, p->tabsize * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&p->chirp, 2 * (nx + ny) * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&p->scratch, (size_t)nx2 * ny * batch * sizeof(float2)) != hipSuccess) {
+    plan_free(p);
+    return fail(FH_E_NOMEM, "CZT tables and workspace: out of device memory");
+  }
+  double2* dbuf = nullptr;
+  if (hipMalloc(&dbuf, (size_t)nx2 * ny2 * sizeof(double2)) != hipSuccess) {
+    plan_free(p);
+    return fail(FH_E_NOMEM, "CZT FP64 kernel buffer: out of device memory");
+  }
+  hipLaunchKernelGGL(k_czt_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, dbuf, nx, ny, dd[0], dd[1]);
+  if (hipGetLastError() != hipSuccess) rc = fail(FH_E_HIP, "CZT sample kernel launch failed");
+  if (rc == FH_OK) rc = fft2_d(ctx, dbuf, nx2, ny2, p->lx, p->ly);
+  if (rc == FH_OK) {
+    hipLaunchKernelGGL(k_tab_extract, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream, p->tab, dbuf,
+                       nx2, ny2, p->qx, p->qy, 0, 1.0 / ((double)nx2 * (double)ny2));
+    // chirp lines in the plan's A_x, A_y (input) and D_x, D_y (output) slots
+    float2* cax = p->chirp;
+    float2* cay = cax + nx;
+    float2* cdx = cay + ny;
+    float2* cdy = cdx + nx;
+    hipLaunchKernelGGL(k_czt_chirp, dim3((nx + 255) / 256), dim3(256), 0, ctx->stream, cax, nx, dd[0], 1);
+    hipLaunchKernelGGL(k_czt_chirp, dim3((ny + 255) / 256), dim3(256), 0, ctx->stream, cay, ny, dd[1], 1);
+    hipLaunchKernelGGL(k_czt_chirp, dim3((nx + 255) / 256), dim3(256), 0, ctx->stream, cdx, nx, dd[0], 0);
+    hipLaunchKernelGGL(k_czt_chirp, dim3((ny + 255) / 256), dim3(256), 0, ctx->stream, cdy, ny, dd[1], 0);
+    if (hipGetLastError() != hipSuccess) rc = fail(FH_E_HIP, "CZT table kernel launch failed");
+  }
+  if (rc == FH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(FH_E_HIP, "CZT table generation failed");
   (void)hipFree(dbuf);
   if (rc == FH_OK) rc = upload_tabidx(p);
   if (rc) { plan_free(p); return rc; }

@@ -11,6 +11,7 @@ from .components import (AS_BPM, BPM, ASProp, CollinsProp, FourierLens, Paraxial
                          Mask, Phase, RSProp, ShiftProp, Shift_BPM, TeaDOE, TeaReflector, TiltAnchor,
                          phase_propagate_,
                          propagate_phase_backward_)
+from .czt import CZTPlan, plan_czt, plan_czt_
 from .group import PropagationGroup
 from .metrics import DotProduct, PowerCoupling, SquaredIntensityDifference
 from .system import OpticalSequence, OpticalSystem, ScalarSource, merge
@@ -18,7 +19,7 @@ from .optim import (ClampProx, CompositeProx, Descent, Fista, IstaProx, Momentum
                     PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
-    "AS_BPM", "BPM", "Shift_BPM", "TiltAnchor", "ASProp", "CollinsProp", "FourierLens", "ParaxialProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
+    "CZTPlan", "plan_czt", "plan_czt_", "AS_BPM", "BPM", "Shift_BPM", "TiltAnchor", "ASProp", "CollinsProp", "FourierLens", "ParaxialProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
     "RSProp", "ScalarField", "ShiftProp", "Mask", "TeaDOE", "TeaReflector", "FourierPhase", "FourierMask",
     "ClampProx", "CompositeProx", "Descent", "Fista", "IstaProx", "Momentum", "Nesterov", "NoDescent",
     "PositiveProx", "ProxRule", "compose", "make_rules", "setup", "update_",

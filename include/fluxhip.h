@@ -115,6 +115,17 @@ int fh_rsprop_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, fh_plan** out);
 int fh_collins_create(fh_ctx* ctx, const fh_prop_desc* desc, double dx_out, double dy_out, double a,
                       double b, double d, fh_plan** out);
 
+/* plan_czt(x, dims, a, w; center_on_grid) (FFTutils.jl:30-177): the chirp z-transform
+ * X[m] = sum_k x[k] a^-k w^(m k) along the dimensions in the bitmask dims (1 = x, 2 = y),
+ * with k, m shifted by (n-1)/2 when center_on_grid, by Bluestein's identity on the padded
+ * chirp / convolution / chirp plan.  a and w are host arrays in polar form
+ * {rho_x, theta_x, rho_y, theta_y} (convert_czt_arg); the reference's defaults are
+ * a = (1, 0) and w = (1, -2 pi / n) per dimension, for which the CZT is the DFT.
+ * fh_propagate(plan, x, +1) applies czt!, fh_propagate(plan, x, -1) the adjoint plan's czt!
+ * (FFTutils.jl:143-155). */
+int fh_czt_create(fh_ctx* ctx, int nx, int ny, int batch, int dims, const double* a, const double* w,
+                  int center_on_grid, fh_plan** out);
+
 int fh_plan_destroy(fh_plan* plan);
 
 /* propagate!(u, p, direction): in place on a c64 [nx, ny, batch] device array

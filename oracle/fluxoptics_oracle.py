@@ -483,6 +483,56 @@ def collins_direct(u, ds, ds_out, abd, lams, tilts=None):
     return out
 
 
+# ----------------------------------------------------------------------------- CZT
+
+def _czt_data(n: int, a, w, center: bool):
+    """``prepare_czt_data`` (``FFTutils.jl:30-50``) for one dimension, N = 2n - 1: input chirp
+    w^{k^2/2} a^{-k}, output chirp w^{k^2/2} (k shifted by (n-1)/2 when centred) and the kernel
+    w^{-f^2/2} at f = N fftfreq(N, 1); a, w in polar form (rho, theta)."""
+    N = 2 * n - 1
+    k = np.arange(N, dtype=np.float64)
+    if center:
+        k = k - (n - 1) / 2.0
+    f = np.round(N * np.fft.fftfreq(N, 1.0))
+    (ra, ta), (rw, tw) = a, w
+    lw, la = math.log(rw), math.log(ra)
+    inp = np.exp(k ** 2 / 2 * lw - k * la) * np.exp(1j * (k ** 2 / 2 * tw - k * ta))
+    outc = np.exp(k ** 2 / 2 * lw) * np.exp(1j * (k ** 2 / 2 * tw))
+    h = np.exp(-f ** 2 / 2 * lw) * np.exp(-1j * (f ** 2 / 2 * tw))
+    return inp, outc, h
+
+
+def czt(x, dims, a=None, w=None, center_on_grid: bool = False, adjoint: bool = False):
+    """``czt!`` of a ``CZTPlan`` (``FFTutils.jl:52-141``) and of its adjoint plan (``:143-155``)
+    on an array [..., ny, nx]; ``dims`` holds 1 (x) and/or 2 (y); ``a``, ``w`` are per-dimension
+    (rho, theta) pairs, defaulting to (1, 0) and (1, -2 pi / n)."""
+    x = np.asarray(x, dtype=np.complex128)
+    shape = x.shape
+    ns = {1: shape[-1], 2: shape[-2]}
+    axis = {1: -1, 2: -2}
+    a = a or {1: (1.0, 0.0), 2: (1.0, 0.0)}
+    w = w or {1: (1.0, -TWO_PI / ns[1]), 2: (1.0, -TWO_PI / ns[2])}
+    tmp_shape = list(shape)
+    for d in dims:
+        tmp_shape[axis[d]] = 2 * ns[d] - 1
+    inp = np.ones(tmp_shape[-2:], np.complex128)
+    outc = np.ones(tmp_shape[-2:], np.complex128)
+    h = np.ones(tmp_shape[-2:], np.complex128)
+    for d in dims:
+        i1, o1, h1 = _czt_data(ns[d], a[d], w[d], center_on_grid)
+        sh = (1, -1) if d == 1 else (-1, 1)
+        inp, outc, h = inp * i1.reshape(sh), outc * o1.reshape(sh), h * h1.reshape(sh)
+    ax = tuple(axis[d] for d in dims)
+    hf = np.fft.fftn(h, axes=ax)
+    t = np.zeros(tmp_shape, np.complex128)
+    t[..., :shape[-2], :shape[-1]] = x
+    if adjoint:
+        t = np.fft.ifftn(np.fft.fftn(t * np.conj(outc), axes=ax) * np.conj(hf), axes=ax) * np.conj(inp)
+    else:
+        t = np.fft.ifftn(np.fft.fftn(t * inp, axes=ax) * hf, axes=ax) * outc
+    return t[..., :shape[-2], :shape[-1]]
+
+
 # ----------------------------------------------------------------------------- AS_BPM
 
 def bpm_kdz(dz: float, tilts=(0.0, 0.0)) -> float:
