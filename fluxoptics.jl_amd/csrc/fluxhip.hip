@@ -680,6 +680,75 @@ k_sum_partials(const double* __restrict__ part, int nper, double* __restrict__ o
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// The last pass of a propagation fused with SquaredIntensityDifference
+// (squared_intensity_difference.jl:78-90) and the first pass of the adjoint propagation: per
+// x line of the 4x4-blocked spectrum, IFFT -> u; d = |u|^2 - I (FP64), loss partial += d^2,
+// du = 4 dc d u -> FFT -> stored back in place.  The inverse transform leaves its points in
+// the forward transform's input order (its last radix is the forward's first), so the two
+// transforms meet in registers.  HALF (zero-padded RS plans): points x >= nx of the IFFT are
+// the discarded wrap of the linear convolution; du is zero there, as the adjoint's padding
+// wants.  One FP64 partial per workgroup, summed in a fixed order afterwards.
+template <typename RDF, int L, bool HALF>
+__global__ void __launch_bounds__(L * RDF::P)
+k_fast_sqint(FastIO io, const float2* __restrict__ tabi, const float2* __restrict__ tabf,
+             const float* __restrict__ I, long tstride, int nx, const double* __restrict__ dc,
+             double* __restrict__ part) {
+  using RI = typename RDF::Rev;
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  constexpr int RL = LastRadix<RI>::value;
+  constexpr int RE = LastRadix<RDF>::value;
+  static_assert(RL == RDF::R0, "the inverse transform must end in the forward transform's input order");
+  __shared__ double red[(L * P + 63) / 64];
+  extern __shared__ unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  int l, t;
+  thread_map<L, P, false>(threadIdx.x, l, t);
+  const int b = blockIdx.y;
+  const int li = blockIdx.x * L + l;
+  const bool live = li < io.nlines;
+  float2* s = smem + l * LdsLine<n, L>::size;
+  const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
+  const int vin = fast_off<false>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
+  const int kin = fast_k<false>(io.in_l, io.in_e, io.in_blk) * 8;
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V / RI::R0; ++m)
+#pragma unroll
+    for (int r = 0; r < RI::R0; ++r)
+      v[m * RI::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RI::R0)) * kin);
+  fast_fft<float2, RI, true>(v, s, t, tabi);
+  const double g = dc ? 4.0 * dc[b] : 4.0;
+  const float* Ib = I + (long)b * tstride + (long)(live ? li : 0) * nx;
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int x = t + P * m + r * (n / RL);
+      const float2 u = v[m * RL + r];
+      float2 w = make_float2(0.f, 0.f);
+      if (live && (!HALF || x < nx)) {
+        const double d = (double)u.x * u.x + (double)u.y * u.y - (double)Ib[x];
+        acc += d * d;
+        const float f = (float)(g * d);
+        w = make_float2(f * u.x, f * u.y);
+      }
+      v[m * RL + r] = w;
+    }
+  fast_fft<float2, RDF, false>(v, s, t, tabf);
+  if (live) {
+    const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
+    const int vout = fast_off<false>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
+    const int kout = fast_k<false>(io.out_l, io.out_e, io.out_blk) * 8;
+#pragma unroll
+    for (int m = 0; m < V / RE; ++m)
+#pragma unroll
+      for (int r = 0; r < RE; ++r) buf_store(rout, vout, (P * m + r * (n / RE)) * kout, v[m * RE + r]);
+  }
+  const double sum = block_sum(acc, red);
+  if (threadIdx.x == 0) part[b * (long)gridDim.x + blockIdx.x] = sum;
+}
+
 // DotProduct / PowerCoupling (metrics/dot_product.jl:60-95, power_coupling.jl:55-64), one
 // value per mode: c_b = ds * sum u_b conj(v_b) in FP64 (block partials, then an ordered sum).
 __global__ void __launch_bounds__(kMetricThreads)
@@ -1036,6 +1105,49 @@ static int fast_xinv(fh_ctx* c, int logn, bool half_out, const FastIO& io, int b
   } else {
     FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, false, Op>(c, io, batch, op, lanes)))
   }
+}
+
+// The fused metric pass (k_fast_sqint): lines per workgroup halve like the x passes'; the
+// partials buffer must hold batch * ceil(ny / L) values for the L the launch ends with, which
+// sqint_parts() reports for the same arguments.
+template <int LOGN, bool HALF, int L>
+static int fast_sqint_l(fh_ctx* c, const FastIO& io, int batch, const float* I, long tstride, int nx,
+                        const double* dc, double* part, long* nparts) {
+  using RDF = typename Cfg<float, LOGN, false>::RD;
+  using RI = typename RDF::Rev;
+  if constexpr (L >= 8 && (L / 2) * RDF::P >= 64) {
+    if ((long)((io.nlines + L - 1) / L) * batch < kFillWGs)
+      return fast_sqint_l<LOGN, HALF, L / 2>(c, io, batch, I, tstride, nx, dc, part, nparts);
+  }
+  const long gx = (io.nlines + L - 1) / L;
+  if (!part) {
+    *nparts = gx;
+    return FH_OK;
+  }
+  const float2 *tabi = nullptr, *tabf = nullptr;
+  int rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &tabi);
+  if (!rc) rc = get_stage_tw(c, RDF::n, RDF::R0, RDF::R1, RDF::R2, false, &tabf);
+  if (rc) return rc;
+  auto kern = k_fast_sqint<RDF, L, HALF>;
+  const size_t lds = (size_t)L * LdsLine<RDF::n, L>::size * sizeof(float2);
+  FH_HIP(ensure_lds(kern, lds));
+  hipLaunchKernelGGL(kern, dim3(gx, batch), dim3(L * RDF::P), lds, c->stream, io, tabi, tabf, I, tstride, nx, dc,
+                     part);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+template <int LOGN, bool HALF>
+static int fast_sqint_t(fh_ctx* c, const FastIO& io, int batch, const float* I, long tstride, int nx,
+                        const double* dc, double* part, long* nparts) {
+  return fast_sqint_l<LOGN, HALF, Cfg<float, LOGN, false>::L>(c, io, batch, I, tstride, nx, dc, part, nparts);
+}
+
+// part == nullptr: only report the number of partials per field in *nparts
+static int fast_sqint(fh_ctx* c, int logn, bool half, const FastIO& io, int batch, const float* I, long tstride,
+                      int nx, const double* dc, double* part, long* nparts) {
+  if (half) { FH_LOGN_SWITCH_INT(logn, (fast_sqint_t<LG, true>(c, io, batch, I, tstride, nx, dc, part, nparts))) }
+  FH_LOGN_SWITCH_INT(logn, (fast_sqint_t<LG, false>(c, io, batch, I, tstride, nx, dc, part, nparts)))
 }
 
 // x-pass variants: 0 = lanes over positions of a line, 1 = lanes over the workgroup's lines
@@ -1803,7 +1915,9 @@ int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double sc
 }
 
 static int metric_partials(fh_ctx* ctx, size_t bytes) {
-  const size_t need = (bytes + sizeof(double) - 1) / sizeof(double);
+  // grown once to a generous size, so that graphs captured after the first calls keep a
+  // valid pointer
+  const size_t need = std::max<size_t>((bytes + sizeof(double) - 1) / sizeof(double), 1 << 16);
   if (ctx->metric_part_n >= need) return FH_OK;
   if (ctx->metric_part) FH_HIP(hipFree(ctx->metric_part));
   ctx->metric_part = nullptr;
@@ -2003,6 +2117,57 @@ int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const vo
   io.in_blk = 1; io.out_blk = 0;
   FH_HIP(launch_xifft_phase_grad(p->lx, io, p->batch, twx, phi, reinterpret_cast<const float2*>(saved),
                                  dphi, p->nx, (float)direction, accumulate, p->ctx->stream));
+  return FH_OK;
+}
+
+int fh_sqint_block(fh_plan* p, void* u, const float* target, int target_batch, const double* dc, double* loss,
+                   int direction, const float* phi, void* saved, float* dphi, int accumulate) {
+  if (!p || !u || !target || !loss) return fail(FH_E_ARG, "null plan, field, target or loss");
+  if (p->kind == 2) return fail(FH_E_UNSUPP, "the fused metric block takes ASProp and RSProp plans");
+  if (target_batch != p->batch && !(target_batch == 1 && p->batch == 1))
+    return fail(FH_E_UNSUPP, "the fused metric block needs one target per field (target batch %d, batch %d)",
+                target_batch, p->batch);
+  if ((saved || dphi) && !phi) return fail(FH_E_ARG, "a saved input or mask gradient needs the mask");
+  if (dphi && !saved) return fail(FH_E_ARG, "the mask gradient needs the saved mask input");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
+  fh_ctx* c = p->ctx;
+  float2* f = reinterpret_cast<float2*>(u);
+  // forward: [Phase in the first pass] x FFT, y pass
+  if (phi) {
+    PhaseTimeOp op{phi, reinterpret_cast<float2*>(saved), (long)p->nx * p->ny, p->nx, (float)direction};
+    if ((rc = run_prop(p, f, direction, op))) return rc;
+  } else if ((rc = run_prop(p, f, direction, NoOp{}))) {
+    return rc;
+  }
+  // fused x IFFT + metric + pullback + x FFT of the adjoint, in place on the spectrum
+  const int nxl = 1 << p->lx;
+  const bool rs = p->kind != 0;
+  float2* mid = rs ? p->scratch : f;
+  FastIO io{};
+  io.in = mid; io.out = mid; io.in_f = io.out_f = (long)nxl * p->ny;
+  io.in_l = io.out_l = nxl; io.in_e = io.out_e = 1; io.in_blk = io.out_blk = 1;
+  io.nlines = p->ny; io.in_bytes = io.out_bytes = io.in_f * 8;
+  const long tstride = target_batch == 1 ? 0 : (long)p->nx * p->ny;
+  long nparts = 0;
+  if ((rc = fast_sqint(c, p->lx, rs, io, p->batch, target, tstride, p->nx, dc, nullptr, &nparts))) return rc;
+  if ((rc = metric_partials(c, (size_t)p->batch * nparts * sizeof(double)))) return rc;
+  if ((rc = fast_sqint(c, p->lx, rs, io, p->batch, target, tstride, p->nx, dc, c->metric_part, &nparts))) return rc;
+  hipLaunchKernelGGL(k_sum_partials, dim3(p->batch), dim3(kMetricThreads), 0, c->stream,
+                     (const double*)c->metric_part, (int)nparts, loss);
+  FH_HIP(hipGetLastError());
+  // adjoint: y pass with the conjugate kernel, then x IFFT [un-applying the Phase, its gradient]
+  if ((rc = run_prop(p, f, -direction, NoOp{}, 1))) return rc;
+  if (!phi) return run_xifft(p, f, -direction);
+  const float2* twx = nullptr;
+  if ((rc = get_twiddles(c, nxl, &twx))) return rc;
+  LineIO<float2> lio{};
+  lio.in = mid; lio.out = f; lio.in_f = (long)nxl * p->ny; lio.out_f = (long)p->nx * p->ny;
+  lio.in_l = nxl; lio.out_l = p->nx; lio.in_e = lio.out_e = 1;
+  lio.nlines = p->ny; lio.in_valid = nxl; lio.out_count = p->nx;
+  lio.in_blk = 1; lio.out_blk = 0;
+  FH_HIP(launch_xifft_phase_grad(p->lx, lio, p->batch, twx, phi, reinterpret_cast<const float2*>(saved), dphi,
+                                 p->nx, (float)direction, accumulate, c->stream));
   return FH_OK;
 }
 

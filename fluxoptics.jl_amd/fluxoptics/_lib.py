@@ -89,6 +89,8 @@ SIGNATURES = {
                                   c_double, ctypes.POINTER(c_void_p)]),
     "fh_czt_create": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double_p, c_double_p, c_int,
                               ctypes.POINTER(c_void_p)]),
+    "fh_sqint_block": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                               c_void_p, c_void_p, c_int]),
     "fh_bpm_create_ex": (c_int, [c_void_p, ctypes.POINTER(PropDesc), c_double, c_int, c_int, ctypes.POINTER(c_void_p)]),
     "fh_bpm_destroy": (c_int, [c_void_p]),
     "fh_bpm_workspace_bytes": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t)]),

@@ -23,9 +23,11 @@ from typing import Callable, Dict, List, Optional, Sequence
 import torch
 
 from .components import (BPM, CollinsProp, TiltAnchor, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
-                         _Modulator, check_field)
+                         _Modulator, _ptr, check_field)
 from .components import phase_propagate_, propagate_phase_backward_
 from .fields import Backward, Forward, ScalarField, dir_sign
+from .metrics import SquaredIntensityDifference
+from . import _lib
 
 
 class ScalarSource:
@@ -189,7 +191,7 @@ class OpticalSystem:
         return SystemOutput(u)
 
     # -- gradient
-    def withgradient(self, loss, x: Optional[ScalarField] = None, weights=None):
+    def withgradient(self, loss, x: Optional[ScalarField] = None, weights=None, fuse: bool = True):
         """Loss value and gradients of every trainable component.
 
         ``loss`` is a metric with ``value_and_pullback`` (``SquaredIntensityDifference``,
@@ -197,19 +199,29 @@ class OpticalSystem:
         (``weights``: one per value or a scalar, e.g. -1 to maximise a coupling).  Returns
         ``(value, grads)`` where ``value`` is the weighted sum as a float64 device scalar
         and ``grads`` maps each trainable component (and a trainable source) to its gradient.
+        A system ending in [Phase ->] ASProp/RSProp with a one-pair SquaredIntensityDifference
+        (one target per field) runs its tail as one fused block (``fh_sqint_block``: two HBM
+        passes fewer); ``fuse=False`` runs the components and the metric one by one.
         """
-        losses, grads = self._withgradient_launches(loss, x, weights)
+        losses, grads = self._withgradient_launches(loss, x, weights, fuse)
         return _total(losses, weights), grads
 
-    def _withgradient_launches(self, loss, x, weights=None):
+    def _withgradient_launches(self, loss, x, weights=None, fuse: bool = True):
         """The device work of ``withgradient``: per-pair loss tensors and the gradients."""
         u = self._start(x)
         comps = list(self.components)
-        tape = _run_forward(comps, u, self.direction, save=True)
-        dcs = None if weights is None else [weights]
-        losses, (du,) = loss.value_and_pullback((u,), dcs, inplace=True)
+        tail = _fusable_tail(comps, loss, u, self.direction) if fuse else None
         grads: Dict = {}
-        du = _run_backward(comps, tape, du, self.direction, grads)
+        if tail is not None:
+            head, ph, prop = tail
+            tape = _run_forward(head, u, self.direction, save=True)
+            losses = (_sqint_block(loss, u, ph, prop, tape, grads, weights),)
+            du = _run_backward(head, tape, u, self.direction, grads)
+        else:
+            tape = _run_forward(comps, u, self.direction, save=True)
+            dcs = None if weights is None else [weights]
+            losses, (du,) = loss.value_and_pullback((u,), dcs, inplace=True)
+            du = _run_backward(comps, tape, du, self.direction, grads)
         if self.source is not None and self.source.trainable:
             grads[self.source] = self.source.backpropagate_with_gradient(du)
         return losses, grads
@@ -257,6 +269,51 @@ class CapturedGradient:
 def _is_prop(c) -> bool:
     """A propagator whose plan a preceding Phase can ride on (its first pass has no chirp)."""
     return isinstance(c, _KernelPropagator) and not isinstance(c, CollinsProp)
+
+
+def _fusable_tail(comps: List, loss, u: ScalarField, direction):
+    """(head, Phase or None, propagator) when the system ends in [Phase ->] ASProp/RSProp and
+    the loss is a one-pair SquaredIntensityDifference with one target per field; else None."""
+    if dir_sign(direction) < 0 or not comps or not isinstance(loss, SquaredIntensityDifference):
+        return None
+    if len(loss.targets) != 1 or not _is_prop(comps[-1]):
+        return None
+    t = loss.targets[0]
+    tb = 1 if t.dim() == 2 else t.shape[0]
+    if not (tb == u.batch or (tb == 1 and u.batch == 1)):
+        return None
+    if len(comps) >= 2 and isinstance(comps[-2], Phase):
+        return comps[:-2], comps[-2], comps[-1]
+    return comps[:-1], None, comps[-1]
+
+
+def _sqint_block(loss, u: ScalarField, ph, prop, tape, grads: Dict, weights):
+    """The fused tail (``fh_sqint_block``); u ends as the cotangent at the tail's input."""
+    t = loss.targets[0]
+    tb = 1 if t.dim() == 2 else t.shape[0]
+    if t.shape[-2:] != (u.ny, u.nx):
+        raise ValueError("field does not match the metric's target")
+    nout = u.batch if tb == u.batch else 1
+    dev = u.electric.device
+    out = torch.empty(nout, dtype=torch.float64, device=dev)
+    dc = None
+    if weights is not None:
+        dc = torch.as_tensor(weights, dtype=torch.float64, device=dev).reshape(-1)
+        dc = (dc.expand(nout) if dc.numel() == 1 else dc).contiguous()
+        if dc.numel() != nout:
+            raise ValueError(f"dc must hold {nout} values")
+    phi = saved = g = None
+    acc = False
+    if ph is not None:
+        phi = ph.phi
+        if ph.trainable:
+            saved = ph.saved if (ph.saved is not None and not any(e[2] is ph.saved for e in tape)) \
+                else torch.empty_like(u.electric)
+            g, acc = _grad_buffer(grads, ph, ph.phi)
+    plan = prop.plan(u)
+    _lib.call("fh_sqint_block", plan, _ptr(u.electric), _ptr(t), tb, _ptr(dc), _ptr(out), dir_sign(Forward),
+              _ptr(phi), _ptr(saved), _ptr(g), int(acc))
+    return out
 
 
 def _run_forward(comps: List, u: ScalarField, direction, save: bool) -> list:

@@ -191,6 +191,18 @@ int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double sc
 int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx, int ny, int batch,
                          int target_batch, const double* dc, double* loss, void* du);
 
+/* The tail of a differentiated system ending in [Phase ->] propagator -> SquaredIntensityDifference,
+ * fused: propagate u (the Phase, when phi is given, rides on the first pass and saves its input to
+ * saved), then one pass does the last x IFFT, the metric values (loss[b], one per field; the target
+ * has the field's full dims, or both are single), the pullback 4 dc (|u|^2 - I) u and the first x FFT
+ * of the adjoint propagation; the adjoint then finishes (un-applying the Phase and accumulating its
+ * gradient into dphi when given, as fh_propagate_phase_backward).  u ends as the cotangent at the
+ * block's input.  Equals fh_phase_propagate / fh_propagate + fh_sq_intensity_diff +
+ * fh_propagate_phase_backward / fh_propagate(-direction) with two HBM passes fewer.  ASProp and
+ * RSProp plans. */
+int fh_sqint_block(fh_plan* plan, void* u, const float* target, int target_batch, const double* dc,
+                   double* loss, int direction, const float* phi, void* saved, float* dphi, int accumulate);
+
 /* AS_BPM: split-step beam propagation through nslices slices of index perturbation dn
  * (AS_BPM(u, thickness, n0, dn), bpm.jl:130-142 and :8-74): dz = thickness/nslices,
  * P(dz) and P(dz/2) are ASProps in index n0, the slice phase is

@@ -1,0 +1,86 @@
+"""The fused [Phase ->] propagator -> SquaredIntensityDifference block (fh_sqint_block) against
+the same system run component by component (fuse=False) and against the oracle chain."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import fluxoptics as fo
+from fluxoptics import inputs
+from oracle import fluxoptics_oracle as O
+from _util import c64, rel_l2, speckle_batch, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(system, metric, comps, weights=None):
+    v1, g1 = system.withgradient(metric, weights=weights, fuse=True)
+    g1 = {c: g.clone() for c, g in g1.items()}
+    v0, g0 = system.withgradient(metric, weights=weights, fuse=False)
+    torch.cuda.synchronize()
+    assert abs(float(v1) - float(v0)) <= 1e-6 * abs(float(v0))
+    for c in comps:
+        assert rel_l2(g1[c].cpu().numpy(), g0[c].cpu().numpy()) <= 1e-5
+
+
+@pytest.mark.parametrize("kind,n,tail_phase", [("as", 1024, True), ("as", 256, False), ("rs", 256, True),
+                                                ("rs", 128, False)])
+def test_fused_tail_matches_unfused(kind, n, tail_phase):
+    print(f"[fused metric] {kind} n={n} phase={tail_phase}", flush=True)
+    ds, lams = (1.0, 1.0), [0.8, 0.6, 0.5]
+    u0 = c64(speckle_batch(3, n, n, 1.0, 1.0, 0.12 * n, 151))
+    tgt = (np.abs(c64(speckle_batch(3, n, n, 1.0, 1.0, 0.12 * n, 155))) ** 2).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), ds, lams)
+    prop = (lambda z: fo.ASProp(u, z)) if kind == "as" else (lambda z: fo.RSProp(u, z))
+    p1 = fo.Phase(u, inputs.uniform(152, (n, n), 0, 2 * math.pi).astype(np.float32), trainable=True)
+    p2 = fo.Phase(u, inputs.uniform(153, (n, n), 0, 2 * math.pi).astype(np.float32), trainable=tail_phase)
+    system = fo.ScalarSource(u) | p1 | prop(400.0 * n / 256) | p2 | prop(300.0 * n / 256)
+    if not tail_phase:
+        system = fo.ScalarSource(u) | p1 | prop(400.0 * n / 256) | prop(300.0 * n / 256)
+    metric = fo.SquaredIntensityDifference((u, tgt))
+    _compare(system, metric, [p1, p2] if tail_phase else [p1], weights=[1.0, -0.5, 2.0])
+
+
+def test_fused_tail_single_field_spatial_target_and_oracle():
+    """Batch 1 with a 2-D target (the spatial-only form, fusable for one field); the fused
+    loss and gradient against the oracle chain."""
+    n, ds, lam, z = 512, (0.5, 0.5), 0.8, 600.0
+    print("[fused metric] single field vs oracle", flush=True)
+    u0 = c64(speckle_batch(1, n, n, 0.5, 0.5, 40.0, 157))[0]
+    phi = inputs.uniform(158, (n, n), 0, 2 * math.pi).astype(np.float32)
+    tgt = (np.abs(c64(speckle_batch(1, n, n, 0.5, 0.5, 40.0, 159))[0]) ** 2).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    ph = fo.Phase(u, phi, trainable=True, buffered=True)
+    system = fo.ScalarSource(u) | ph | fo.ASProp(u, z)
+    metric = fo.SquaredIntensityDifference((u, tgt))
+    stages = [("phase", phi.astype(np.float64)), ("as", z)]
+    v_ref, saved = O.pipeline_forward(u0, stages, ds, lam)
+    vals_ref, dv_ref = O.sq_intensity_diff_values(v_ref, tgt.astype(np.float64))
+    _, (g_ref,) = O.pipeline_backward(dv_ref, stages, ds, lam, saved)
+    value, grads = system.withgradient(metric)
+    assert abs(float(value) - vals_ref.sum()) <= 1e-5 * vals_ref.sum()
+    torch.cuda.synchronize()
+    assert rel_l2(ph.grad.cpu().numpy(), g_ref) <= 1e-4
+
+
+def test_captured_fused_tail_replays():
+    n = 256
+    print("[fused metric] captured replays", flush=True)
+    u0 = c64(speckle_batch(2, n, n, 1.0, 1.0, 30.0, 161))
+    tgt = (np.abs(c64(speckle_batch(2, n, n, 1.0, 1.0, 30.0, 163))) ** 2).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), (1.0, 1.0), [0.8, 0.7])
+    ph = fo.Phase(u, inputs.uniform(162, (n, n), 0, 2 * math.pi).astype(np.float32), trainable=True,
+                  buffered=True)
+    system = fo.ScalarSource(u) | ph | fo.RSProp(u, 500.0)
+    metric = fo.SquaredIntensityDifference((u, tgt))
+    v0, g0 = system.withgradient(metric)
+    v0, g0 = float(v0), g0[ph].clone()
+    cap = system.capture_withgradient(metric)
+    for _ in range(3):
+        v, g = cap.replay()
+        torch.cuda.synchronize()
+        assert float(v) == v0
+        assert torch.equal(g[ph], g0)
+        torch.zeros(1, device="cuda").sum()  # eager work between replays
