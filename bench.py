@@ -157,6 +157,36 @@ def workload_beam_splitter(torch, fo, inputs):
                     "one recorded graph per iteration (eager host-driven launches beside it)"}
 
 
+def workload_train_step(torch, fo, inputs):
+    """The headline grid as a training step: 8 fields of 2048x2048 (ds 1, lambda 0.8),
+    Phase (trainable, shared) -> ASProp(5000), per-field intensity targets; value + mask
+    gradient per iteration, the fused tail block (fh_sqint_block) against the same system run
+    component by component, each as one recorded graph."""
+    import numpy as np
+    n, b, ds, lam, z = 2048, 8, 1.0, 0.8, 5000.0
+    g = inputs.gaussian(n, n, ds, ds, 200.0)
+    u0 = np.stack([g * np.exp(1j * inputs.uniform(171 + k, (n, n), 0, 0.5)) for k in range(b)]).astype(np.complex64)
+    tgt = np.stack([np.abs(inputs.gaussian(n, n, ds, ds, 200.0, 40.0 * k, 0.0)) ** 2 for k in range(b)])
+    u = fo.ScalarField(torch.from_numpy(u0).cuda(), (ds, ds), lam)
+    ph = fo.Phase(u, inputs.uniform(170, (n, n), 0, 2 * math.pi).astype(np.float32), trainable=True,
+                  buffered=True)
+    system = fo.ScalarSource(u) | ph | fo.ASProp(u, z)
+    metric = fo.SquaredIntensityDifference((u, tgt.astype(np.float32)))
+    out = {}
+    for fuse in (True, False):
+        system.withgradient(metric, fuse=fuse)
+        w = system._withgradient_launches  # record the chosen path as one graph
+        system.withgradient(metric, fuse=fuse)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            w(metric, None, None, fuse)
+        out["fused" if fuse else "unfused"] = _timed_host(graph.replay, 30, torch)
+    return {"ms_per_iter": out["fused"], "iters_per_s": 1e3 / out["fused"], "unfused_ms_per_iter": out["unfused"],
+            "what": "8 x 2048^2 Phase->ASProp, loss + mask gradient; fused tail (5 passes + phase-gradient "
+                    "epilogue) vs component by component (3 + metric + 3 passes), each one recorded graph"}
+
+
 def workload_rgb_cascade(torch, fo, inputs):
     """1024x1024 x 3 wavelengths, Phase -> ASProp(1e4) -> Phase -> ASProp(1e4) -> Phase,
     intensity loss, three mask gradients."""
@@ -346,6 +376,7 @@ def main():
         from fluxoptics import inputs as inp
         out["workloads"] = {
             "beam_splitter_512": workload_beam_splitter(torch, fo, inp),
+            "train_step_2048x8": workload_train_step(torch, fo, inp),
             "rgb_cascade_1024x3": workload_rgb_cascade(torch, fo, inp),
             "as_bpm_512x400": workload_bpm(torch, fo, inp),
             "uneven_batch_64": workload_uneven_batch(torch, fo, inp),

@@ -821,16 +821,19 @@ __global__ void k_mul_full_d(double2* __restrict__ u, const double2* __restrict_
 }
 
 // ================================================================ fused x-IFFT + Phase adjoint + gradient
-// One workgroup owns L rows and loops over the fields of the batch so the mask gradient
-// is summed in registers, in field order, and written once.
+// x IFFT of the 4x4-blocked spectrum into natural rows, u <- u * cis(-s*phi) (un-applying
+// the Phase, phasemask.jl:115-150) and dphi (+)= s * sum_b Im(du_b * conj(saved_b)).  One
+// workgroup owns L rows and loops over the fields of the batch, so the mask gradient is
+// summed in registers, in field order, and written once; the phase factors are taken once
+// per position for the whole batch and the next field's line is loaded while this one is
+// transformed.  Rows of RS plans hold 2nx points of which the first nx are kept.
 template <typename RDF, int L>
 __global__ void __launch_bounds__(L * RDF::P)
-k_xifft_phase_grad(LineIO<float2> io, int batch, const float2* __restrict__ tw,
-                   const float* __restrict__ phi, const float2* __restrict__ saved,
-                   float* __restrict__ dphi, int nx, float s, int acc) {
-  using RD = typename RDF::Rev;
-  constexpr int n = RD::n, V = RD::V, P = RD::P;
-  constexpr int RL = (RD::NST == 3) ? RD::R2 : ((RD::NST == 2) ? RD::R1 : RD::R0);
+k_fast_phase_grad(FastIO io, int batch, const float2* __restrict__ tab, const float* __restrict__ phi,
+                  const float2* __restrict__ saved, float* __restrict__ dphi, int nx, float s, int acc) {
+  using RI = typename RDF::Rev;
+  constexpr int n = RI::n, V = RI::V, P = RI::P;
+  constexpr int RL = LastRadix<RI>::value;
   extern __shared__ unsigned char smem_raw[];
   float2* smem = reinterpret_cast<float2*>(smem_raw);
   int l, t;
@@ -838,87 +841,70 @@ k_xifft_phase_grad(LineIO<float2> io, int batch, const float2* __restrict__ tw,
   const int li = blockIdx.x * L + l;
   const bool live = li < io.nlines;
   float2* sl = smem + l * LdsLine<n, L>::size;
+  const int vin = fast_off<false>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
+  const int kin = fast_k<false>(io.in_l, io.in_e, io.in_blk) * 8;
+  const int vout = fast_off<false>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
+  const int kout = fast_k<false>(io.out_l, io.out_e, io.out_blk) * 8;
+  float2 w[V];
   float g[V];
 #pragma unroll
-  for (int k = 0; k < V; ++k) g[k] = 0.f;
+  for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int x = t + P * m + r * (n / RL);
+      float sn = 0.f, cs = 1.f;
+      if (live && x < nx) sincosf(-s * phi[(long)li * nx + x], &sn, &cs);
+      w[m * RL + r] = make_float2(cs, sn);
+      g[m * RL + r] = 0.f;
+    }
+  float2 nxt[V];
+  {
+    const auto rin = make_rsrc(io.in, io.in_bytes);
+#pragma unroll
+    for (int m = 0; m < V / RI::R0; ++m)
+#pragma unroll
+      for (int r = 0; r < RI::R0; ++r) nxt[m * RI::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RI::R0)) * kin);
+  }
   for (int b = 0; b < batch; ++b) {
-    const float2* src = io.in + b * io.in_f;
     float2 v[V];
 #pragma unroll
-    for (int m = 0; m < V / RD::R0; ++m)
+    for (int k = 0; k < V; ++k) v[k] = nxt[k];
+    if (b + 1 < batch) {
+      const auto rin = make_rsrc(io.in + (b + 1) * io.in_f, io.in_bytes);
 #pragma unroll
-      for (int r = 0; r < RD::R0; ++r) {
-        const int pos = reg_pos<n, RD::R0, P>(t, m, r);
-        float2 x = make_float2(0.f, 0.f);
-        if (live && pos < io.in_valid) x = src[elem_addr<false>(li, pos, io.in_l, io.in_e, io.in_blk)];
-        v[m * RD::R0 + r] = x;
-      }
-    line_fft<float2, RD, true>(v, sl, t, tw);
-    float2* dst = io.out + b * io.out_f + (long)li * io.out_l;
-    const float2* sv = saved + b * io.out_f + (long)li * io.out_l;
+      for (int m = 0; m < V / RI::R0; ++m)
 #pragma unroll
-    for (int m = 0; m < V / RL; ++m)
+        for (int r = 0; r < RI::R0; ++r) nxt[m * RI::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RI::R0)) * kin);
+    }
+    fast_fft<float2, RI, true>(v, sl, t, tab);
+    if (live) {
+      const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
+      const auto rsv = make_rsrc(saved ? saved + b * io.out_f : io.out, io.out_bytes);
 #pragma unroll
-      for (int r = 0; r < RL; ++r) {
-        const int pos = reg_pos<n, RL, P>(t, m, r);
-        if (live && pos < io.out_count) {
-          float sn, cs;
-          sincosf(-s * phi[(long)li * nx + pos], &sn, &cs);
-          const float2 du = cmul(v[m * RL + r], make_float2(cs, sn));
-          dst[pos * io.out_e] = du;
-          if (dphi) g[m * RL + r] += cmulc(du, sv[pos * io.out_e]).y;
+      for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const int c = P * m + r * (n / RL);
+          if (t + c < nx) {
+            const float2 du = cmul(v[m * RL + r], w[m * RL + r]);
+            if (dphi) g[m * RL + r] += cmulc(du, buf_load(rsv, vout, c * kout)).y;
+            buf_store(rout, vout, c * kout, du);
+          }
         }
-      }
+    }
   }
   if (dphi && live) {
 #pragma unroll
     for (int m = 0; m < V / RL; ++m)
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
-        const int pos = reg_pos<n, RL, P>(t, m, r);
-        if (pos < io.out_count) {
-          const long k = (long)li * nx + pos;
+        const int x = t + P * m + r * (n / RL);
+        if (x < nx) {
+          const long k = (long)li * nx + x;
           dphi[k] = (acc ? dphi[k] : 0.f) + s * g[m * RL + r];
         }
       }
   }
-}
-
-// one workgroup per L rows, looping over the batch; L halves (down to one full wave) while
-// there would be fewer than kFillWGs workgroups, but stays a multiple of 4: the pass reads
-// the blocked spectrum and writes natural rows in place, so a workgroup must own whole 4-row
-// groups
-template <typename RD, int L>
-static hipError_t launch_xifft_phase_grad_l(const LineIO<float2>& io, int batch, const float2* tw,
-                                            const float* phi, const float2* saved, float* dphi,
-                                            int nx, float s, int acc, hipStream_t st) {
-  if constexpr (L >= 8 && (L / 2) * RD::P >= 64) {
-    if ((long)((io.nlines + L - 1) / L) < kFillWGs)
-      return launch_xifft_phase_grad_l<RD, L / 2>(io, batch, tw, phi, saved, dphi, nx, s, acc, st);
-  }
-  auto kern = k_xifft_phase_grad<RD, L>;
-  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n, L>::size * sizeof(float2) : 0;
-  hipError_t e = ensure_lds(kern, lds);
-  if (e != hipSuccess) return e;
-  dim3 grid((io.nlines + L - 1) / L, 1);
-  hipLaunchKernelGGL(kern, grid, dim3(L * RD::P), lds, st, io, batch, tw, phi, saved, dphi, nx, s,
-                     acc);
-  return hipGetLastError();
-}
-
-template <int LOGN>
-static hipError_t launch_xifft_phase_grad_t(const LineIO<float2>& io, int batch, const float2* tw,
-                                            const float* phi, const float2* saved, float* dphi,
-                                            int nx, float s, int acc, hipStream_t st) {
-  using CF = Cfg<float, LOGN, false>;
-  return launch_xifft_phase_grad_l<typename CF::RD, CF::L>(io, batch, tw, phi, saved, dphi, nx, s, acc, st);
-}
-
-static hipError_t launch_xifft_phase_grad(int logn, const LineIO<float2>& io, int batch,
-                                          const float2* tw, const float* phi, const float2* saved,
-                                          float* dphi, int nx, float s, int acc, hipStream_t st) {
-  FH_LOGN_SWITCH(logn, (launch_xifft_phase_grad_t<LG>(io, batch, tw, phi, saved, dphi, nx, s, acc,
-                                                      st)))
 }
 
 // ================================================================ host objects
@@ -1148,6 +1134,40 @@ static int fast_sqint(fh_ctx* c, int logn, bool half, const FastIO& io, int batc
                       int nx, const double* dc, double* part, long* nparts) {
   if (half) { FH_LOGN_SWITCH_INT(logn, (fast_sqint_t<LG, true>(c, io, batch, I, tstride, nx, dc, part, nparts))) }
   FH_LOGN_SWITCH_INT(logn, (fast_sqint_t<LG, false>(c, io, batch, I, tstride, nx, dc, part, nparts)))
+}
+
+template <int LOGN, int L>
+static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float* phi, const float2* saved,
+                             float* dphi, int nx, float s, int acc) {
+  using RDF = typename Cfg<float, LOGN, false>::RD;
+  using RI = typename RDF::Rev;
+  if constexpr (L >= 8 && (L / 2) * RDF::P >= 64) {
+    if ((long)((io.nlines + L - 1) / L) < kFillWGs)
+      return fast_phase_grad_l<LOGN, L / 2>(c, io, batch, phi, saved, dphi, nx, s, acc);
+  }
+  const float2* tab = nullptr;
+  int rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &tab);
+  if (rc) return rc;
+  auto kern = k_fast_phase_grad<RDF, L>;
+  const size_t lds = (size_t)L * LdsLine<RDF::n, L>::size * sizeof(float2);
+  FH_HIP(ensure_lds(kern, lds));
+  hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L), dim3(L * RDF::P), lds, c->stream, io, batch, tab, phi,
+                     saved, dphi, nx, s, acc);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+template <int LOGN>
+static int fast_phase_grad_t(fh_ctx* c, const FastIO& io, int batch, const float* phi, const float2* saved,
+                             float* dphi, int nx, float s, int acc) {
+  return fast_phase_grad_l<LOGN, Cfg<float, LOGN, false>::L>(c, io, batch, phi, saved, dphi, nx, s, acc);
+}
+
+// x IFFT of the blocked spectrum into natural rows, un-applying a Phase and accumulating its
+// gradient over the batch (the adjoint end of a Phase -> propagator pair)
+static int fast_phase_grad(fh_ctx* c, int logn, const FastIO& io, int batch, const float* phi, const float2* saved,
+                           float* dphi, int nx, float s, int acc) {
+  FH_LOGN_SWITCH_INT(logn, (fast_phase_grad_t<LG>(c, io, batch, phi, saved, dphi, nx, s, acc)))
 }
 
 // x-pass variants: 0 = lanes over positions of a line, 1 = lanes over the workgroup's lines
@@ -1424,6 +1444,20 @@ static int run_prop_plan(fh_plan* p, float2* u, int direction, int only = -1) {
 static int run_xifft(fh_plan* p, float2* u, int direction) {
   if (p->kind == 2) return run_xifft_op(p, u, collins_op(p, direction, false));
   return run_xifft_op(p, u, NoOp{});
+}
+
+// The adjoint's last pass for a Phase -> propagator pair: x IFFT of the blocked spectrum,
+// the Phase un-applied, its gradient summed over the batch (fh_propagate_phase_backward).
+static int phase_grad_pass(fh_plan* p, float2* u, const float* phi, const void* saved, float* dphi, int direction,
+                           int accumulate) {
+  const int nxl = 1 << p->lx;
+  FastIO io{};
+  io.in = p->kind == 0 ? u : p->scratch; io.out = u;
+  io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
+  io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1; io.in_blk = 1; io.out_blk = 0;
+  io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
+  return fast_phase_grad(p->ctx, p->lx, io, p->batch, phi, reinterpret_cast<const float2*>(saved), dphi, p->nx,
+                         (float)direction, accumulate);
 }
 
 }  // namespace fh
@@ -2106,18 +2140,7 @@ int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const vo
   if (p->kind == 2) return fail(FH_E_UNSUPP, "Phase fuses with ASProp and RSProp plans only");
   // adjoint propagation = the propagator in the reverse direction
   if ((rc = run_prop(p, f, -direction, NoOp{}))) return rc;
-  const int nxl = 1 << p->lx;
-  const float2* twx = nullptr;
-  if ((rc = get_twiddles(p->ctx, nxl, &twx))) return rc;
-  float2* mid = p->kind == 0 ? f : p->scratch;
-  LineIO<float2> io{};
-  io.in = mid; io.out = f; io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
-  io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1;
-  io.nlines = p->ny; io.in_valid = nxl; io.out_count = p->nx;
-  io.in_blk = 1; io.out_blk = 0;
-  FH_HIP(launch_xifft_phase_grad(p->lx, io, p->batch, twx, phi, reinterpret_cast<const float2*>(saved),
-                                 dphi, p->nx, (float)direction, accumulate, p->ctx->stream));
-  return FH_OK;
+  return phase_grad_pass(p, f, phi, saved, dphi, direction, accumulate);
 }
 
 int fh_sqint_block(fh_plan* p, void* u, const float* target, int target_batch, const double* dc, double* loss,
@@ -2159,16 +2182,7 @@ int fh_sqint_block(fh_plan* p, void* u, const float* target, int target_batch, c
   // adjoint: y pass with the conjugate kernel, then x IFFT [un-applying the Phase, its gradient]
   if ((rc = run_prop(p, f, -direction, NoOp{}, 1))) return rc;
   if (!phi) return run_xifft(p, f, -direction);
-  const float2* twx = nullptr;
-  if ((rc = get_twiddles(c, nxl, &twx))) return rc;
-  LineIO<float2> lio{};
-  lio.in = mid; lio.out = f; lio.in_f = (long)nxl * p->ny; lio.out_f = (long)p->nx * p->ny;
-  lio.in_l = nxl; lio.out_l = p->nx; lio.in_e = lio.out_e = 1;
-  lio.nlines = p->ny; lio.in_valid = nxl; lio.out_count = p->nx;
-  lio.in_blk = 1; lio.out_blk = 0;
-  FH_HIP(launch_xifft_phase_grad(p->lx, lio, p->batch, twx, phi, reinterpret_cast<const float2*>(saved), dphi,
-                                 p->nx, (float)direction, accumulate, c->stream));
-  return FH_OK;
+  return phase_grad_pass(p, f, phi, saved, dphi, direction, accumulate);
 }
 
 int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {
