@@ -716,9 +716,20 @@ k_fast_sqint(FastIO io, const float2* __restrict__ tabi, const float2* __restric
 #pragma unroll
     for (int r = 0; r < RI::R0; ++r)
       v[m * RI::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RI::R0)) * kin);
-  fast_fft<float2, RI, true>(v, s, t, tabi);
+  // the target values are loaded up front, in flight during the inverse transform (the
+  // inverse leaves point (m, r) at position t + P m + r n/RL, known now); the target row is
+  // nx long, so positions past it (HALF) read a clamped, unused address
   const double g = dc ? 4.0 * dc[b] : 4.0;
   const float* Ib = I + (long)b * tstride + (long)(live ? li : 0) * nx;
+  float tg[V];
+#pragma unroll
+  for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int x = t + P * m + r * (n / RL);
+      tg[m * RL + r] = Ib[(!HALF || x < nx) ? x : 0];
+    }
+  fast_fft<float2, RI, true>(v, s, t, tabi);
   double acc = 0.0;
 #pragma unroll
   for (int m = 0; m < V / RL; ++m)
@@ -728,7 +739,7 @@ k_fast_sqint(FastIO io, const float2* __restrict__ tabi, const float2* __restric
       const float2 u = v[m * RL + r];
       float2 w = make_float2(0.f, 0.f);
       if (live && (!HALF || x < nx)) {
-        const double d = (double)u.x * u.x + (double)u.y * u.y - (double)Ib[x];
+        const double d = (double)u.x * u.x + (double)u.y * u.y - (double)tg[m * RL + r];
         acc += d * d;
         const float f = (float)(g * d);
         w = make_float2(f * u.x, f * u.y);
@@ -827,7 +838,7 @@ __global__ void k_mul_full_d(double2* __restrict__ u, const double2* __restrict_
 // summed in registers, in field order, and written once; the phase factors are taken once
 // per position for the whole batch and the next field's line is loaded while this one is
 // transformed.  Rows of RS plans hold 2nx points of which the first nx are kept.
-template <typename RDF, int L>
+template <typename RDF, int L, bool GRAD>
 __global__ void __launch_bounds__(L * RDF::P)
 k_fast_phase_grad(FastIO io, int batch, const float2* __restrict__ tab, const float* __restrict__ phi,
                   const float2* __restrict__ saved, float* __restrict__ dphi, int nx, float s, int acc) {
@@ -845,55 +856,65 @@ k_fast_phase_grad(FastIO io, int batch, const float2* __restrict__ tab, const fl
   const int kin = fast_k<false>(io.in_l, io.in_e, io.in_blk) * 8;
   const int vout = fast_off<false>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
   const int kout = fast_k<false>(io.out_l, io.out_e, io.out_blk) * 8;
-  float2 w[V];
-  float g[V];
+  // the mask values stay in registers; cis(-s*phi) is re-taken per field (cheap next to the
+  // memory traffic, and it keeps the register count down)
+  float ph[V], g[V];
 #pragma unroll
   for (int m = 0; m < V / RL; ++m)
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int x = t + P * m + r * (n / RL);
-      float sn = 0.f, cs = 1.f;
-      if (live && x < nx) sincosf(-s * phi[(long)li * nx + x], &sn, &cs);
-      w[m * RL + r] = make_float2(cs, sn);
+      ph[m * RL + r] = (live && x < nx) ? -s * phi[(long)li * nx + x] : 0.f;
       g[m * RL + r] = 0.f;
     }
+  // the next field's spectrum, and this field's saved input, are in flight while this field
+  // is transformed; the load past the last field re-reads it (unconditional, so that no load
+  // sits under a branch)
   float2 nxt[V];
-  {
-    const auto rin = make_rsrc(io.in, io.in_bytes);
+  const auto load_spec = [&](int b) {
+    const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
 #pragma unroll
     for (int m = 0; m < V / RI::R0; ++m)
 #pragma unroll
       for (int r = 0; r < RI::R0; ++r) nxt[m * RI::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RI::R0)) * kin);
-  }
+  };
+  load_spec(0);
   for (int b = 0; b < batch; ++b) {
-    float2 v[V];
+    float2 v[V], sv[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) v[k] = nxt[k];
-    if (b + 1 < batch) {
-      const auto rin = make_rsrc(io.in + (b + 1) * io.in_f, io.in_bytes);
+    if constexpr (GRAD) {
+      const auto rsv = make_rsrc(saved + b * io.out_f, io.out_bytes);
 #pragma unroll
-      for (int m = 0; m < V / RI::R0; ++m)
+      for (int m = 0; m < V / RL; ++m)
 #pragma unroll
-        for (int r = 0; r < RI::R0; ++r) nxt[m * RI::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RI::R0)) * kin);
+        for (int r = 0; r < RL; ++r) {
+          const int c = P * m + r * (n / RL);
+          sv[m * RL + r] = buf_load(rsv, vout, (t + c < nx ? c : 0) * kout);
+        }
     }
+    load_spec(b + 1 < batch ? b + 1 : b);
     fast_fft<float2, RI, true>(v, sl, t, tab);
+    // loads and stores share one counter: wait while only old operations are outstanding
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     if (live) {
       const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
-      const auto rsv = make_rsrc(saved ? saved + b * io.out_f : io.out, io.out_bytes);
 #pragma unroll
       for (int m = 0; m < V / RL; ++m)
 #pragma unroll
         for (int r = 0; r < RL; ++r) {
           const int c = P * m + r * (n / RL);
           if (t + c < nx) {
-            const float2 du = cmul(v[m * RL + r], w[m * RL + r]);
-            if (dphi) g[m * RL + r] += cmulc(du, buf_load(rsv, vout, c * kout)).y;
+            float sn, cs;
+            sincosf(ph[m * RL + r], &sn, &cs);
+            const float2 du = cmul(v[m * RL + r], make_float2(cs, sn));
+            if constexpr (GRAD) g[m * RL + r] += cmulc(du, sv[m * RL + r]).y;
             buf_store(rout, vout, c * kout, du);
           }
         }
     }
   }
-  if (dphi && live) {
+  if (GRAD && live) {
 #pragma unroll
     for (int m = 0; m < V / RL; ++m)
 #pragma unroll
@@ -1148,7 +1169,7 @@ static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float
   const float2* tab = nullptr;
   int rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &tab);
   if (rc) return rc;
-  auto kern = k_fast_phase_grad<RDF, L>;
+  auto kern = dphi ? k_fast_phase_grad<RDF, L, true> : k_fast_phase_grad<RDF, L, false>;
   const size_t lds = (size_t)L * LdsLine<RDF::n, L>::size * sizeof(float2);
   FH_HIP(ensure_lds(kern, lds));
   hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L), dim3(L * RDF::P), lds, c->stream, io, batch, tab, phi,
