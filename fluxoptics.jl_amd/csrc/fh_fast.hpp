@@ -14,6 +14,7 @@
 #pragma once
 
 #include "fh_lines.hpp"
+#include "fh_ops.hpp"
 
 namespace fh {
 
@@ -201,11 +202,24 @@ k_fast_fft(FastIO io, const float2* __restrict__ tab, TimeOp op) {
       v[m * RD::R0 + r] = x;
     }
   if (!INV && live) {
+    if constexpr (TwoPhaseOp<TimeOp>::value) {
+      typename TimeOp::Pre pre[V];
 #pragma unroll
-    for (int m = 0; m < V / RD::R0; ++m)
+      for (int m = 0; m < V / RD::R0; ++m)
 #pragma unroll
-      for (int r = 0; r < RD::R0; ++r)
-        v[m * RD::R0 + r] = op(v[m * RD::R0 + r], b, li, t + P * m + r * (n / RD::R0));
+        for (int r = 0; r < RD::R0; ++r) pre[m * RD::R0 + r] = op.pre(b, li, t + P * m + r * (n / RD::R0));
+#pragma unroll
+      for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+        for (int r = 0; r < RD::R0; ++r)
+          v[m * RD::R0 + r] = op.apply(v[m * RD::R0 + r], pre[m * RD::R0 + r], b, li, t + P * m + r * (n / RD::R0));
+    } else {
+#pragma unroll
+      for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+        for (int r = 0; r < RD::R0; ++r)
+          v[m * RD::R0 + r] = op(v[m * RD::R0 + r], b, li, t + P * m + r * (n / RD::R0));
+    }
   }
   fast_fft<float2, RD, INV>(v, s, t, tab);
   const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);

@@ -47,21 +47,35 @@ struct FullMulD {
 
 // u * cis(s*phi[y, x]), optionally saving the input first.  The line index is the row y
 // and the line position is x; phi is one nx*ny mask shared by every field.
+// Two-phase form (pre, then apply; see TwoPhaseOp): the mask loads and phase factors come
+// first, for every point of a thread, so that they are in flight with the field's loads
+// rather than queued behind the stores of the saved input, which may alias them.
 struct PhaseTimeOp {
   const float* phi;
   float2* saved;   // nullable: [B][ny][nx]
   long fstride;    // nx*ny
   int nx;
   float s;         // +1 Forward, -1 Backward
-  __device__ __forceinline__ float2 operator()(float2 v, int b, int y, int x) const {
-    if (x >= nx) return v;
-    const long i = (long)y * nx + x;
-    if (saved) saved[b * fstride + i] = v;
+  using Pre = float2;
+  __device__ __forceinline__ float2 pre(int, int y, int x) const {
+    if (x >= nx) return make_float2(1.f, 0.f);
     float sn, cs;
-    sincosf(s * phi[i], &sn, &cs);
-    return cmul(v, make_float2(cs, sn));
+    sincosf(s * phi[(long)y * nx + x], &sn, &cs);
+    return make_float2(cs, sn);
+  }
+  __device__ __forceinline__ float2 apply(float2 v, float2 f, int b, int y, int x) const {
+    if (x >= nx) return v;
+    if (saved) saved[b * fstride + (long)y * nx + x] = v;
+    return cmul(v, f);
+  }
+  __device__ __forceinline__ float2 operator()(float2 v, int b, int y, int x) const {
+    return apply(v, pre(b, y, x), b, y, x);
   }
 };
+
+// Time-side ops with a separate load phase (pre) and apply phase.
+template <typename Op> struct TwoPhaseOp : std::false_type {};
+template <> struct TwoPhaseOp<PhaseTimeOp> : std::true_type {};
 
 // Separable chirp on the time side of an x pass (CollinsProp's A and D chirps,
 // collins_integral.jl:5-43): u(b, y, x) *= s * cx[t][x] * cy[t][y] (conjugated when conj),
