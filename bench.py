@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the secondary workloads (beam splitter, RGB cascade, AS_BPM volume, "
                          "uneven batch)")
+    ap.add_argument("--workload", default=None,
+                    help="only run the named secondary workload (e.g. train_step_2048x8) and print "
+                         "its JSON object (for profiling it alone)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 512 MiB device copy (the reference kernel that calibrates "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE into bytes) and exit")
@@ -300,6 +303,17 @@ def main():
     import fluxoptics as fo
     from fluxoptics import _lib
     from fluxoptics.components import _ptr
+
+    if args.workload:
+        from fluxoptics import inputs as inp
+        fns = {"beam_splitter_512": workload_beam_splitter, "train_step_2048x8": workload_train_step,
+               "rgb_cascade_1024x3": workload_rgb_cascade, "as_bpm_512x400": workload_bpm,
+               "uneven_batch_64": workload_uneven_batch}
+        if args.workload not in fns:
+            sys.exit(f"bench.py: unknown workload {args.workload!r} (one of {', '.join(fns)})")
+        torch.cuda.set_device(0)
+        print(json.dumps({args.workload: fns[args.workload](torch, fo, inp)}))
+        return
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
