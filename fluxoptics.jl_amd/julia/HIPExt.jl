@@ -9,7 +9,8 @@
 module HIPExt
 
 using AMDGPU: ROCArray, AMDGPU
-using ..FluxOptics: ScalarField, ASProp, RSProp, Phase, AS_BPM, Forward, Backward, Direction
+using ..FluxOptics: ScalarField, ASProp, RSProp, ShiftProp, CollinsProp, Phase, AS_BPM, Forward, Backward, Direction,
+                    set_field_ds!
 import ..FluxOptics: propagate!, backpropagate!, propagate_and_save!, backpropagate_with_gradient!
 import ChainRulesCore
 
@@ -56,6 +57,7 @@ struct PropDesc
 end
 const FH_PROP_PARAXIAL = Cint(1)
 const FH_PROP_TRACK_TILTS = Cint(2)
+const FH_PROP_SHIFT = Cint(4)
 
 # tilts as the reference stores them (one collection per axis, Fields.jl:34-37), promoted
 tilts_f64(u::ScalarField, batch) = map(c -> (v = Float64.(collect(vec(c))); length(v) == 1 ? fill(v[1], batch) : v),
@@ -102,8 +104,44 @@ function propagate!(u::ScalarField{<:ROCArray{ComplexF32}}, p::RSProp, direction
     u
 end
 
-backpropagate!(u::ScalarField{<:ROCArray{ComplexF32}}, p::Union{ASProp, RSProp}, direction::Type{<:Direction}) =
-    propagate!(u, p, reverse(direction))
+# ShiftProp (shift_kernel.jl:91-117): (ft, ShiftKernel, ift), z from the kernel component
+function propagate!(u::ScalarField{<:ROCArray{ComplexF32}}, p::ShiftProp, direction::Type{<:Direction})
+    k = p.optical_components[2]
+    plan = prop_plan(p, u, Float64(k.z), 1.0, FH_PROP_SHIFT, :fh_asprop_create_ex)
+    check(ccall((:fh_propagate, libfluxhip), Cint, (PlanPtr, Ptr{Cvoid}, Cint), plan, pointer(u.electric), dir_sign(direction)), :fh_propagate)
+    u
+end
+
+# CollinsProp (collins_integral.jl:262-296): (a_chirp, pad, ft, conv, ift, crop, d_chirp) as one
+# plan; A, B, ds from the A chirp, D and ds' from the D chirp; the field's sampling is relabelled
+# as the chirp components do (set_field_ds!).
+const COLLINS = IdDict{Any, Dict{Any, PlanPtr}}()
+function propagate!(u::ScalarField{<:ROCArray{ComplexF32}}, p::CollinsProp, direction::Type{<:Direction})
+    ac, dc = p.optical_components[1], p.optical_components[end]
+    nx, ny = size(u.electric, 1), size(u.electric, 2)
+    batch = div(length(u.electric), nx * ny)
+    lams = lambdas_f64(u)
+    tx, ty = tilts_f64(u, batch)
+    ds, dsp = Float64.(ac.ds), Float64.(dc.ds′)
+    cache = get!(COLLINS, p) do; Dict{Any, PlanPtr}() end
+    plan = get!(cache, (batch, lams, tx, ty)) do
+        r = Ref{PlanPtr}(C_NULL)
+        GC.@preserve lams tx ty begin
+            d = Ref(PropDesc(nx, ny, batch, ds[1], ds[2], pointer(lams), length(lams), C_NULL, 0,
+                             pointer(tx), pointer(ty), batch, 1.0, ac.track_tilts ? FH_PROP_TRACK_TILTS : Cint(0)))
+            check(ccall((:fh_collins_create, libfluxhip), Cint,
+                        (CtxPtr, Ptr{PropDesc}, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{PlanPtr}),
+                        context(), d, dsp[1], dsp[2], Float64(ac.a), Float64(ac.b), Float64(dc.d), r), :fh_collins_create)
+        end
+        r[]
+    end
+    check(ccall((:fh_propagate, libfluxhip), Cint, (PlanPtr, Ptr{Cvoid}, Cint), plan, pointer(u.electric), dir_sign(direction)), :fh_propagate)
+    set_field_ds!(u, direction == Forward ? dsp : ds)
+    u
+end
+
+backpropagate!(u::ScalarField{<:ROCArray{ComplexF32}}, p::Union{ASProp, RSProp, ShiftProp, CollinsProp},
+               direction::Type{<:Direction}) = propagate!(u, p, reverse(direction))
 
 # Phase (phasemask.jl:90-150)
 function propagate!(u::ScalarField{<:ROCArray{ComplexF32}}, p::Phase, direction::Type{<:Direction})
