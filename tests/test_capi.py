@@ -81,3 +81,49 @@ def test_prop_desc_layout_matches_header(tmp_path):
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(_lib.PropDesc)] + [getattr(_lib.PropDesc, n).offset for n in names]
     assert got == want
+
+
+def _desc(nx=64, ny=64, batch=1, flags=0, z=None):
+    d = _lib.PropDesc()
+    d.nx, d.ny, d.batch, d.dx, d.dy = nx, ny, batch, 1.0, 1.0
+    d._lam = (ctypes.c_double * 1)(0.8)
+    d.lambdas, d.nlambdas = ctypes.cast(d._lam, _lib.c_double_p), 1
+    if z is not None:
+        d._z = (ctypes.c_double * 1)(z)
+        d.z, d.nz = ctypes.cast(d._z, _lib.c_double_p), 1
+    d.n0, d.flags = 1.0, flags
+    return d
+
+
+def test_new_entry_points_validate_arguments():
+    """Collins, CZT, general BPM, fused metric block and offset_tilts reject bad arguments
+    (with a message) before any device work."""
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    ref = ctypes.byref
+    # CollinsProp: B = 0, distances given, bad output sampling, unknown flags, no context
+    assert lib.fh_collins_create(None, ref(_desc()), 1.0, 1.0, 1.0, 0.0, 1.0, ref(out)) == -1
+    assert b"B != 0" in lib.fh_last_error()
+    assert lib.fh_collins_create(None, ref(_desc(z=10.0)), 1.0, 1.0, 1.0, 100.0, 1.0, ref(out)) == -1
+    assert lib.fh_collins_create(None, ref(_desc()), -1.0, 1.0, 1.0, 100.0, 1.0, ref(out)) == -1
+    assert lib.fh_collins_create(None, ref(_desc(flags=1)), 1.0, 1.0, 1.0, 100.0, 1.0, ref(out)) == -1
+    assert lib.fh_collins_create(None, ref(_desc()), 1.0, 1.0, 1.0, 100.0, 1.0, ref(out)) == -1
+    assert b"context" in lib.fh_last_error()
+    # CZT: dims outside {1, 2, 3}, non-positive magnitudes
+    a = (ctypes.c_double * 4)(1.0, 0.0, 1.0, 0.0)
+    w = (ctypes.c_double * 4)(1.0, -0.1, 1.0, -0.1)
+    bad = (ctypes.c_double * 4)(0.0, 0.0, 1.0, 0.0)
+    assert lib.fh_czt_create(None, 64, 64, 1, 0, a, w, 0, ref(out)) == -1
+    assert b"dims" in lib.fh_last_error()
+    assert lib.fh_czt_create(None, 64, 64, 1, 3, bad, w, 0, ref(out)) == -1
+    assert b"magnitudes" in lib.fh_last_error()
+    # general BPM: distances given, too few slices, shift with paraxial
+    assert lib.fh_bpm_create_ex(None, ref(_desc(z=1.0)), 10.0, 4, 1, ref(out)) == -1
+    assert lib.fh_bpm_create_ex(None, ref(_desc()), 10.0, 1, 1, ref(out)) == -1
+    assert lib.fh_bpm_create_ex(None, ref(_desc(flags=4 | 1)), 10.0, 4, 1, ref(out)) == -1
+    assert lib.fh_asprop_create_ex(None, ref(_desc(flags=4 | 1, z=1.0)), ref(out)) == -1
+    # fused metric block and offset_tilts: null arguments
+    assert lib.fh_sqint_block(None, None, None, 1, None, None, 1, None, None, None, 0) == -1
+    lam = (ctypes.c_double * 1)(0.8)
+    t0 = (ctypes.c_double * 1)(0.0)
+    assert lib.fh_offset_tilts(None, None, 4, 4, 1, 1.0, 1.0, lam, 1, t0, t0, t0, t0, 1) == -1
