@@ -48,6 +48,9 @@ struct FastIO {
   int in_blk, out_blk;   // 4x4-blocked layout (elem_addr)
   int nlines;
   long in_bytes, out_bytes;  // bytes of one field (descriptor bound)
+  // zero-padded lines (IN_HALF / OUT_HALF): points read / stored per line, at most n/2 (the
+  // padded length of an RS plan is a power of two >= 2 n_valid - 1)
+  int in_valid = 1 << 30, out_valid = 1 << 30;
 };
 
 // Offset of (line li, position pos) and the per-position stride K such that
@@ -168,8 +171,9 @@ template <typename RD> struct LastRadix {
 // ---------------------------------------------------------------- kernels
 
 // Line FFT (forward or inverse), c64 global -> global, with a time-side op (prologue of a
-// forward transform, epilogue of an inverse one).  IN_HALF: only the first n/2 points of
-// each line are read (the rest are zero); OUT_HALF: only the first n/2 are stored.
+// forward transform, epilogue of an inverse one).  IN_HALF: only the first io.in_valid <= n/2
+// points of each line are read (the rest are zero); OUT_HALF: only the first io.out_valid <=
+// n/2 are stored.  The n/2 bound is a compile-time predicate; the count a runtime one.
 // LANES: the workgroup's L lines sit on adjacent lanes (a wave instruction then covers whole
 // 128-byte lines of the 4x4-blocked layout, and 4 rows x 128 bytes of the natural one);
 // otherwise a wave covers 64 consecutive positions of one line.
@@ -198,7 +202,7 @@ k_fast_fft(FastIO io, const float2* __restrict__ tab, TimeOp op) {
     for (int r = 0; r < RD::R0; ++r) {
       const int c = P * m + r * (n / RD::R0);
       float2 x = make_float2(0.f, 0.f);
-      if (!IN_HALF || c < n / 2) x = buf_load(rin, vin, c * kin);
+      if (!IN_HALF || (c < n / 2 && t + c < io.in_valid)) x = buf_load(rin, vin, c * kin);
       v[m * RD::R0 + r] = x;
     }
   if (!INV && live) {
@@ -233,7 +237,7 @@ k_fast_fft(FastIO io, const float2* __restrict__ tab, TimeOp op) {
         const int c = P * m + r * (n / RL);
         float2 x = v[m * RL + r];
         if (INV) x = op(x, b, li, t + c);
-        if (!OUT_HALF || c < n / 2) buf_store(rout, vout, c * kout, x);
+        if (!OUT_HALF || (c < n / 2 && t + c < io.out_valid)) buf_store(rout, vout, c * kout, x);
       }
   }
 }
@@ -265,7 +269,7 @@ k_fast_conv(FastIO io, const float2* __restrict__ tabf, const float2* __restrict
     for (int r = 0; r < RDF::R0; ++r) {
       const int c = P * m + r * (n / RDF::R0);
       float2 x = make_float2(0.f, 0.f);
-      if (!IN_HALF || c < n / 2) x = buf_load(rin, vin, c * kin);
+      if (!IN_HALF || (c < n / 2 && t + c < io.in_valid)) x = buf_load(rin, vin, c * kin);
       v[m * RDF::R0 + r] = x;
     }
   fast_fft<float2, RDF, false>(v, s, t, tabf);
@@ -288,7 +292,7 @@ k_fast_conv(FastIO io, const float2* __restrict__ tabf, const float2* __restrict
 #pragma unroll
       for (int r = 0; r < RE; ++r) {
         const int c = P * m + r * (n / RE);
-        if (!OUT_HALF || c < n / 2) buf_store(rout, vout, c * kout, v[m * RE + r]);
+        if (!OUT_HALF || (c < n / 2 && t + c < io.out_valid)) buf_store(rout, vout, c * kout, v[m * RE + r]);
       }
   }
 }
@@ -341,20 +345,20 @@ __device__ __forceinline__ void yconv_field(float2* v, float2* s, int t, int li,
 #pragma unroll
       for (int r = 0; r < RE; ++r) {
         const int c = P * m + r * (n / RE);
-        if (!OUT_HALF || c < n / 2) buf_store(rout, vout, c * kout, v[m * RE + r]);
+        if (!OUT_HALF || (c < n / 2 && t + c < io.out_valid)) buf_store(rout, vout, c * kout, v[m * RE + r]);
       }
   }
 }
 
 template <int V, int R0, int n, int P, bool IN_HALF>
-__device__ __forceinline__ void yconv_load(float2* v, const FastIO& io, int b, int vin, int kin) {
+__device__ __forceinline__ void yconv_load(float2* v, const FastIO& io, int b, int vin, int kin, int t) {
   const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
 #pragma unroll
   for (int m = 0; m < V / R0; ++m)
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
       const int c = P * m + r * (n / R0);
-      v[m * R0 + r] = (!IN_HALF || c < n / 2) ? buf_load(rin, vin, c * kin) : make_float2(0.f, 0.f);
+      v[m * R0 + r] = (!IN_HALF || (c < n / 2 && t + c < io.in_valid)) ? buf_load(rin, vin, c * kin) : make_float2(0.f, 0.f);
     }
 }
 
@@ -393,7 +397,7 @@ k_fast_yconv(FastIO io, int batch, int fpw, const float2* __restrict__ tabf,
     // them would hold every twiddle in registers and halve the resident workgroups)
     asm volatile("" ::: "memory");
     float2 v[V];
-    yconv_load<V, RDF::R0, n, P, IN_HALF>(v, io, b, vin, kin);
+    yconv_load<V, RDF::R0, n, P, IN_HALF>(v, io, b, vin, kin, t);
     yconv_field<RDF, L, OUT_HALF>(v, s, t, li, live, b, io, tabf, tabi, qt, vout, kout, qoff_d, qoff_m);
   }
 }
@@ -516,7 +520,7 @@ __device__ __forceinline__ void ypipe_field(float2* v, const float2* h, float2* 
 #pragma unroll
       for (int r = 0; r < RE; ++r) {
         const int c = P * m + r * (n / RE);
-        if (!OUT_HALF || c < n / 2) buf_store(rout, vout, c * kout, v[m * RE + r]);
+        if (!OUT_HALF || (c < n / 2 && t + c < io.out_valid)) buf_store(rout, vout, c * kout, v[m * RE + r]);
       }
   }
 }
@@ -553,29 +557,29 @@ k_fast_ypipe(FastIO io, int batch, const float2* __restrict__ tabf, const float2
   pf.load(tabf, t);
   pi.load(tabi, t);
   float2 va[V], vb[V], ha[V];
-  yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, 0, vin, kin);
+  yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, 0, vin, kin, t);
   ypipe_loadh<RDF>(ha, qt, 0, qoff_d, qoff_m);
   // The prefetches are unconditional (past the last field they re-read field batch-1, which
   // is harmless): a load under a branch makes the wait-count pass drain every outstanding
   // load at the join, which would serialise the prefetch with the arithmetic.
   if constexpr (ONE_TAB) {
     for (int b = 0; b < batch; b += 2) {
-      yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b + 1 < batch ? b + 1 : batch - 1, vin, kin);
+      yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b + 1 < batch ? b + 1 : batch - 1, vin, kin, t);
       ypipe_field<RDF, OUT_HALF>(va, ha, s, t, live, b, io, pf, pi, qt.conj, vout, kout);
       if (b + 1 >= batch) break;
-      yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b + 2 < batch ? b + 2 : batch - 1, vin, kin);
+      yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b + 2 < batch ? b + 2 : batch - 1, vin, kin, t);
       ypipe_field<RDF, OUT_HALF>(vb, ha, s, t, live, b + 1, io, pf, pi, qt.conj, vout, kout);
     }
   } else {
     float2 hb[V];
     for (int b = 0; b < batch; b += 2) {
       const int b1 = b + 1 < batch ? b + 1 : batch - 1;
-      yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b1, vin, kin);
+      yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b1, vin, kin, t);
       ypipe_loadh<RDF>(hb, qt, b1, qoff_d, qoff_m);
       ypipe_field<RDF, OUT_HALF>(va, ha, s, t, live, b, io, pf, pi, qt.conj, vout, kout);
       if (b + 1 >= batch) break;
       const int b2 = b + 2 < batch ? b + 2 : batch - 1;
-      yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b2, vin, kin);
+      yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b2, vin, kin, t);
       ypipe_loadh<RDF>(ha, qt, b2, qoff_d, qoff_m);
       ypipe_field<RDF, OUT_HALF>(vb, hb, s, t, live, b + 1, io, pf, pi, qt.conj, vout, kout);
     }

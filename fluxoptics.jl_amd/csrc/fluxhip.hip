@@ -288,14 +288,14 @@ __global__ void k_as_table_full_d(double2* __restrict__ tab, int nx, int ny, dou
 // i - 2nx above, so the samples are even in both axes (offset +-nx is never read by a
 // cropped output of the linear convolution).  The phase k*r is reduced in cycles:
 // r/lam - rint(r/lam).
-__global__ void k_rs_sample(double2* __restrict__ d, int nx, int ny, double dx, double dy,
+__global__ void k_rs_sample(double2* __restrict__ d, int nx2, int ny2, double dx, double dy,
                             double lam, double z, double nrm, double tx, double ty, int tilt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
-  const int nx2 = 2 * nx;
   if (i >= nx2) return;
-  const int oi = i <= nx ? i : i - nx2;
-  const int oj = j <= ny ? j : j - 2 * ny;
+  // offsets in FFT order on the padded grid; the cropped convolution uses |offset| < n only
+  const int oi = i <= nx2 / 2 ? i : i - nx2;
+  const int oj = j <= ny2 / 2 ? j : j - ny2;
   const double x = oi * dx, y = oj * dy;
   const double az = fabs(z);
   const double r = sqrt(x * x + y * y + az * az);
@@ -598,6 +598,33 @@ __global__ void k_optim_step(float* __restrict__ x, const float* __restrict__ dx
       st[i] = xi;
       xi -= d;
     }
+    if (prox == 1) {
+      xi = xi < p0 ? p0 : (xi > p1 ? p1 : xi);
+    } else if (prox == 2) {
+      const float xc = xi - p1;
+      xi = fabsf(xc) <= p0 ? p1 : (xc > p0 ? xi - p0 : xi + p0);
+    } else if (prox == 3) {
+      xi = xi < 0.f ? 0.f : xi;
+    }
+    x[i] = xi;
+  }
+}
+
+// Adam (Optimisers.jl 0.4.6, the reference's dependency): m = b1 m + (1 - b1) g,
+// v = b2 v + (1 - b2) g^2 (stored in FP32), x -= eta * m / (1 - bt1) / (sqrt(v / (1 - bt2)) + eps),
+// each formed in FP64 from the FP32 values as Julia's Float64-scalar broadcasts do; bt1, bt2 are
+// b1^t, b2^t of the current step.  state: m in [0, n), v in [n, 2n).  Then the proximal operator.
+__global__ void k_adam_step(float* __restrict__ x, const float* __restrict__ dx, float* __restrict__ st, long n,
+                            double eta, double b1, double b2, double eps, double bt1, double bt2, int prox,
+                            float p0, float p1) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double g = dx[i];
+    const float m = (float)(b1 * (double)st[i] + (1.0 - b1) * g);
+    const float v = (float)(b2 * (double)st[n + i] + (1.0 - b2) * (g * g));
+    st[i] = m;
+    st[n + i] = v;
+    const double d = (double)m / (1.0 - bt1) / (sqrt((double)v / (1.0 - bt2)) + eps) * eta;
+    float xi = (float)((double)x[i] - d);
     if (prox == 1) {
       xi = xi < p0 ? p0 : (xi > p1 ? p1 : xi);
     } else if (prox == 2) {
@@ -934,6 +961,13 @@ static int ilog2_exact(int n) {
   if (n <= 0 || (n & (n - 1))) return -1;
   int l = 0;
   while ((1 << l) < n) ++l;
+  return l;
+}
+
+// smallest l with 2^l >= n (n >= 1)
+static int ilog2_ceil(int n) {
+  int l = 0;
+  while ((1 << l) < n && l < 30) ++l;
   return l;
 }
 
@@ -1395,21 +1429,25 @@ static int fft2_d(fh_ctx* c, double2* d, int nx2, int ny2, int lx, int ly) {
 
 // The passes of one propagation of fields u (or, fused, of Phase then propagation):
 // pass 0 (x FFT) and pass 1 (y FFT * kernel * IFFT); `only` >= 0 runs just that pass.
-// Between passes the spectrum sits in the 4x4-blocked layout (elem_addr); for RS it sits
-// in the (2nx) x ny workspace and only its first ny rows are ever stored.
+// Between passes the spectrum sits in the 4x4-blocked layout (elem_addr); for padded plans
+// (RS, Collins, CZT) it sits in the nxl x ny workspace (rows rounded up to whole 4-row
+// groups) and only its first ny rows are ever stored.
+static long mid_rows(const fh_plan* p) { return p->kind == 0 ? p->ny : ((p->ny + 3) / 4) * 4; }
+
 template <typename XOp>
 static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1) {
   fh_ctx* c = p->ctx;
   const int nxl = 1 << p->lx;
   const bool rs = p->kind != 0;
   float2* mid = rs ? p->scratch : u;
-  const long midf = (long)nxl * p->ny;
+  const long midf = (long)nxl * mid_rows(p);
   int rc;
   FastIO io{};
   if (only < 0 || only == 0) {
     io.in = u; io.out = mid; io.in_f = (long)p->nx * p->ny; io.out_f = midf;
     io.in_l = p->nx; io.out_l = nxl; io.in_e = io.out_e = 1; io.in_blk = 0; io.out_blk = 1;
     io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = midf * 8;
+    io.in_valid = p->nx;
     const int xv = p->xf_var >= 0 ? p->xf_var : default_x_variant(p->lx);
     if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop, xv == 1))) return rc;
   }
@@ -1419,6 +1457,7 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
   io.in = mid; io.out = mid; io.in_f = io.out_f = midf;
   io.in_l = io.out_l = 1; io.in_e = io.out_e = nxl; io.in_blk = io.out_blk = 1;
   io.nlines = nxl; io.in_bytes = io.out_bytes = midf * 8;
+  io.in_valid = io.out_valid = p->ny;
   int nvar = 0;
   const FastConvVariant* vars = fast_conv_variants(p->ly, &nvar);
   if (!vars) return fail(FH_E_UNSUPP, "y length 2^%d unsupported", p->ly);
@@ -1433,9 +1472,10 @@ static int run_xifft_op(fh_plan* p, float2* u, XOp xop) {
   const bool rs = p->kind != 0;
   float2* mid = rs ? p->scratch : u;
   FastIO io{};
-  io.in = mid; io.out = u; io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
+  io.in = mid; io.out = u; io.in_f = (long)nxl * mid_rows(p); io.out_f = (long)p->nx * p->ny;
   io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1; io.in_blk = 1; io.out_blk = 0;
   io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
+  io.out_valid = p->nx;
   const int xv = p->xi_var >= 0 ? p->xi_var : default_x_variant(p->lx);
   return fast_xinv(p->ctx, p->lx, rs, io, p->batch, xop, xv == 1);
 }
@@ -1474,7 +1514,7 @@ static int phase_grad_pass(fh_plan* p, float2* u, const float* phi, const void* 
   const int nxl = 1 << p->lx;
   FastIO io{};
   io.in = p->kind == 0 ? u : p->scratch; io.out = u;
-  io.in_f = (long)nxl * p->ny; io.out_f = (long)p->nx * p->ny;
+  io.in_f = (long)nxl * mid_rows(p); io.out_f = (long)p->nx * p->ny;
   io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1; io.in_blk = 1; io.out_blk = 0;
   io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
   return fast_phase_grad(p->ctx, p->lx, io, p->batch, phi, reinterpret_cast<const float2*>(saved), dphi, p->nx,
@@ -1591,26 +1631,30 @@ int fh_rsprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
     if (z == 0.0) { plan_free(p); return fail(FH_E_ARG, "RSProp needs a finite, nonzero z"); }
   const int nx = d->nx, ny = d->ny;
   p->kind = 1; p->z = p->tab_z[0];
-  p->lx = ilog2_exact(2 * nx); p->ly = ilog2_exact(2 * ny);
-  if (p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
+  // zero padding to the power of two >= 2n - 1 (the reference pads to exactly 2n - 1,
+  // rayleigh_sommerfeld.jl:135-156; any length >= 2n - 1 gives the same cropped linear
+  // convolution), so any n up to half the longest transform
+  p->lx = ilog2_ceil(2 * nx - 1); p->ly = ilog2_ceil(2 * ny - 1);
+  if (nx < 2 || ny < 2 || p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
     plan_free(p);
-    return fail(FH_E_UNSUPP, "RSProp needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << (kMinLog - 1), 1 << (kMaxLog - 1), nx, ny);
+    return fail(FH_E_UNSUPP, "RSProp needs 2 <= n with 2n - 1 <= %d and a padded length >= %d (got %d x %d)",
+                1 << kMaxLog, 1 << kMinLog, nx, ny);
   }
   if ((rc = set_device(ctx))) { plan_free(p); return rc; }
-  const int nx2 = 2 * nx, ny2 = 2 * ny;
+  const int nx2 = 1 << p->lx, ny2 = 1 << p->ly;
   // tilted kernels (rayleigh_sommerfeld.jl:90-96) carry a linear phase unless tracked: odd spectra
   const bool tilted = plan_tilted(p);
   p->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
   p->mode = tilted ? 2 : 0;
   p->full = (tilted && !p->track) ? 1 : 0;
-  p->qx = p->full ? nx2 : nx + 1;
-  p->qy = p->full ? ny2 + 1 : ny + 1;
+  p->qx = p->full ? nx2 : nx2 / 2 + 1;
+  p->qy = p->full ? ny2 + 1 : ny2 / 2 + 1;
   p->tabsize = (long)p->qx * p->qy;
   if (hipMalloc(&p->tab, p->tabsize * p->lam_tables.size() * sizeof(float2)) != hipSuccess) {
     plan_free(p);
     return fail(FH_E_NOMEM, "RSProp table: out of device memory");
   }
-  if (hipMalloc(&p->scratch, (size_t)nx2 * ny * d->batch * sizeof(float2)) != hipSuccess) {
+  if (hipMalloc(&p->scratch, (size_t)nx2 * mid_rows(p) * d->batch * sizeof(float2)) != hipSuccess) {
     plan_free(p);
     return fail(FH_E_NOMEM, "RSProp workspace: out of device memory");
   }
@@ -1623,7 +1667,7 @@ int fh_rsprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   const double nrm = d->dx * d->dy / (2.0 * M_PI);
   for (size_t k = 0; k < p->lam_tables.size() && rc == FH_OK; ++k) {
     const int tilt = (p->tab_tx[k] != 0.0 || p->tab_ty[k] != 0.0) && !p->track;
-    hipLaunchKernelGGL(k_rs_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, dd, nx, ny,
+    hipLaunchKernelGGL(k_rs_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, dd, nx2, ny2,
                        d->dx, d->dy, p->lam_tables[k], p->tab_z[k], nrm, p->tab_tx[k], p->tab_ty[k], tilt);
     if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "RS sample kernel launch failed"); break; }
     if ((rc = fft2_d(ctx, dd, nx2, ny2, p->lx, p->ly))) break;
@@ -2128,6 +2172,21 @@ int fh_optim_step(fh_ctx* ctx, int rule, float* x, const float* dx, float* state
   return FH_OK;
 }
 
+int fh_adam_step(fh_ctx* ctx, float* x, const float* dx, float* state, long n, double eta, double beta1,
+                 double beta2, double eps, double bt1, double bt2, int prox, double p0, double p1) {
+  if (!ctx || !x || !dx || !state) return fail(FH_E_ARG, "null context, parameter, gradient or state");
+  if (n <= 0) return fail(FH_E_ARG, "n must be positive");
+  if (prox < 0 || prox > 3) return fail(FH_E_ARG, "unknown proximal operator %d", prox);
+  if (!(bt1 < 1.0) || !(bt2 < 1.0)) return fail(FH_E_ARG, "bias-correction powers must be below 1");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const int blocks = (int)std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_adam_step, dim3(blocks), dim3(256), 0, ctx->stream, x, dx, state, n, eta, beta1, beta2, eps,
+                     bt1, bt2, prox, (float)p0, (float)p1);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
 int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx, int ny, int batch,
                          int target_batch, const double* dc, double* loss, void* du) {
   if (!ctx || !u || !target || !loss) return fail(FH_E_ARG, "null context, field, target or loss");
@@ -2189,7 +2248,7 @@ int fh_sqint_block(fh_plan* p, void* u, const float* target, int target_batch, c
   const bool rs = p->kind != 0;
   float2* mid = rs ? p->scratch : f;
   FastIO io{};
-  io.in = mid; io.out = mid; io.in_f = io.out_f = (long)nxl * p->ny;
+  io.in = mid; io.out = mid; io.in_f = io.out_f = (long)nxl * mid_rows(p);
   io.in_l = io.out_l = nxl; io.in_e = io.out_e = 1; io.in_blk = io.out_blk = 1;
   io.nlines = p->ny; io.in_bytes = io.out_bytes = io.in_f * 8;
   const long tstride = target_batch == 1 ? 0 : (long)p->nx * p->ny;

@@ -15,13 +15,13 @@ from .czt import CZTPlan, plan_czt, plan_czt_
 from .group import PropagationGroup
 from .metrics import DotProduct, PowerCoupling, SquaredIntensityDifference
 from .system import OpticalSequence, OpticalSystem, ScalarSource, merge
-from .optim import (ClampProx, CompositeProx, Descent, Fista, IstaProx, Momentum, Nesterov, NoDescent,
+from .optim import (Adam, ClampProx, CompositeProx, Descent, Fista, IstaProx, Momentum, Nesterov, NoDescent,
                     PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
     "CZTPlan", "plan_czt", "plan_czt_", "AS_BPM", "BPM", "Shift_BPM", "TiltAnchor", "ASProp", "CollinsProp", "FourierLens", "ParaxialProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
     "RSProp", "ScalarField", "ShiftProp", "Mask", "TeaDOE", "TeaReflector", "FourierPhase", "FourierMask",
-    "ClampProx", "CompositeProx", "Descent", "Fista", "IstaProx", "Momentum", "Nesterov", "NoDescent",
+    "Adam", "ClampProx", "CompositeProx", "Descent", "Fista", "IstaProx", "Momentum", "Nesterov", "NoDescent",
     "PositiveProx", "ProxRule", "compose", "make_rules", "setup", "update_",
     "OpticalSequence", "OpticalSystem", "PropagationGroup", "ScalarSource", "SquaredIntensityDifference", "DotProduct", "PowerCoupling", "merge",
     "coupling_efficiency", "phase_propagate_", "propagate_phase_backward_", "propagate",

@@ -43,6 +43,14 @@ class Nesterov:
         self.eta, self.rho = float(eta), float(rho)
 
 
+class Adam:
+    """m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, x -= eta m/(1 - b1^t)/(sqrt(v/(1 - b2^t)) + eps)
+    (Optimisers.jl ``Adam``; defaults 0.001, (0.9, 0.999), 1e-8)."""
+
+    def __init__(self, eta: float = 0.001, beta=(0.9, 0.999), epsilon: float = 1e-8):
+        self.eta, self.beta, self.epsilon = float(eta), (float(beta[0]), float(beta[1])), float(epsilon)
+
+
 class Fista:
     """``Fista(eta)`` (``OptimisersExt.jl:12-40``): stores eta^2 as the reference does; with
     t_1 = 1, t_{k+1} = (1 + sqrt(1 + 4 t_k^2)) / 2 and beta = (t_k - 1) / t_{k+1}:
@@ -142,6 +150,9 @@ class _Leaf:
         self.state = None
         if isinstance(base, (Momentum, Nesterov)):
             self.state = torch.zeros_like(tensor)
+        elif isinstance(base, Adam):
+            self.state = torch.zeros((2,) + tuple(tensor.shape), dtype=tensor.dtype, device=tensor.device)
+            self.bt = base.beta  # b1^t, b2^t of the next step
         elif isinstance(base, Fista):
             self.state = tensor.detach().clone()
 
@@ -188,6 +199,19 @@ def _step(leaf: _Leaf, grad: torch.Tensor) -> None:
     if leaf.state is not None:
         sr = torch.view_as_real(leaf.state) if leaf.state.is_complex() else leaf.state
     n = xr.numel()
+    ctx = context(x.device)
+    if isinstance(base, Adam):
+        if x.is_complex():
+            raise NotImplementedError("Adam is applied to real parameters here")
+        first = proxes[0] if proxes else None
+        _lib.call("fh_adam_step", ctx, _ptr(xr), _ptr(gr), _ptr(leaf.state), n, base.eta, base.beta[0], base.beta[1],
+                  base.epsilon, leaf.bt[0], leaf.bt[1], first.code if first else 0, first.p0 if first else 0.0,
+                  first.p1 if first else 0.0)
+        leaf.bt = (leaf.bt[0] * base.beta[0], leaf.bt[1] * base.beta[1])
+        zero = torch.zeros_like(xr)
+        for p in proxes[1:]:
+            _lib.call("fh_optim_step", ctx, 0, _ptr(xr), _ptr(zero), None, n, 0.0, 0.0, 0.0, p.code, p.p0, p.p1)
+        return
     eta = base.eta
     rho = getattr(base, "rho", 0.0)
     beta = 0.0
@@ -195,7 +219,6 @@ def _step(leaf: _Leaf, grad: torch.Tensor) -> None:
         tn = (1.0 + math.sqrt(1.0 + 4.0 * leaf.t * leaf.t)) / 2.0
         beta = (leaf.t - 1.0) / tn
         leaf.t = tn
-    ctx = context(x.device)
     first = proxes[0] if proxes else None
     _lib.call("fh_optim_step", ctx, _RULE_IDS[type(base).__name__], _ptr(xr), _ptr(gr), _ptr(sr), n, eta, rho, beta,
               first.code if first else 0, first.p0 if first else 0.0, first.p1 if first else 0.0)

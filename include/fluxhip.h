@@ -42,8 +42,8 @@ int fh_version(void);
 /* Message of the calling thread's last failure ("" if none). */
 const char* fh_last_error(void);
 
-/* Largest supported transform length per axis (RS pads to 2n, so RS fields are limited
- * to half of it). */
+/* Largest supported transform length per axis (ASProp: n itself, a power of two; RS pads
+ * to the power of two >= 2n - 1, so RS fields are limited to half of it). */
 int fh_max_length(void);
 
 /* Context: one HIP device and one stream (NULL = the device's default stream).
@@ -62,10 +62,10 @@ int fh_asprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double d
                      const double* lambdas, int nlambdas, double z, double n0,
                      fh_plan** out);
 
-/* RSProp: Rayleigh-Sommerfeld propagator (rayleigh_sommerfeld.jl:128-164).  The
- * reference pads to (2n-1)^2; this library pads to (2n)^2, which yields the same
- * cropped linear convolution.  The kernel's FFT is computed in FP64 on the device
- * and rounded once. */
+/* RSProp: Rayleigh-Sommerfeld propagator (rayleigh_sommerfeld.jl:128-164), any nx, ny with
+ * 2n - 1 <= fh_max_length().  The reference pads to (2n-1)^2; this library pads each axis to
+ * the power of two >= 2n - 1 ((2n)^2 for power-of-two n), which yields the same cropped linear
+ * convolution.  The kernel's FFT is computed in FP64 on the device and rounded once. */
 int fh_rsprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
                      const double* lambdas, int nlambdas, double z, fh_plan** out);
 
@@ -283,6 +283,14 @@ int fh_tea_backward(fh_ctx* ctx, void* dv, const float* h, const void* saved, fl
  * (proximal_operators/): 0 none, 1 ClampProx(p0, p1), 2 IstaProx(p0, p1), 3 PositiveProx. */
 int fh_optim_step(fh_ctx* ctx, int rule, float* x, const float* dx, float* state, long n, double eta,
                   double rho, double beta, int prox, double p0, double p1);
+
+/* Adam (Optimisers.jl 0.4.6, the reference's dependency; the rule of its published
+ * beam-splitter example): m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, x -= eta m/(1 - bt1) /
+ * (sqrt(v/(1 - bt2)) + eps), then the proximal operator as in fh_optim_step.  state holds 2n
+ * FP32 values (m, then v), zero before the first step; bt1 = b1^t and bt2 = b2^t for step t
+ * (t = 1 first). */
+int fh_adam_step(fh_ctx* ctx, float* x, const float* dx, float* state, long n, double eta, double beta1,
+                 double beta2, double eps, double bt1, double bt2, int prox, double p0, double p1);
 
 /* DotProduct and PowerCoupling, mode-selective (metrics/dot_product.jl:60-106,
  * power_coupling.jl:55-69): c[b] = ds * sum u_b conj(v_b) into the device complex128 array c

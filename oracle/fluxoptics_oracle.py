@@ -749,6 +749,18 @@ def optim_step(rule, x, dx, state=None, t=1.0, eta=0.1, rho=0.9):
     raise ValueError(rule)
 
 
+def adam_step(x, dx, m, v, bt, eta=0.001, beta=(0.9, 0.999), eps=1e-8):
+    """Optimisers.jl 0.4.6 ``Adam`` ``apply!`` on FP32 parameters: m, v updated and stored in
+    FP32, the step formed in FP64, x rounded back to FP32; ``bt`` = (b1^t, b2^t) of this step.
+    Returns (x, m, v, bt_next)."""
+    g = np.asarray(dx, np.float64)
+    m = (beta[0] * np.asarray(m, np.float64) + (1 - beta[0]) * g).astype(np.float32)
+    v = (beta[1] * np.asarray(v, np.float64) + (1 - beta[1]) * g * g).astype(np.float32)
+    d = m.astype(np.float64) / (1 - bt[0]) / (np.sqrt(v.astype(np.float64) / (1 - bt[1])) + eps) * eta
+    xn = (np.asarray(x, np.float64) - d).astype(np.float32)
+    return xn, m, v, (bt[0] * beta[0], bt[1] * beta[1])
+
+
 # ----------------------------------------------------------------------------- pipelines
 
 def pipeline_forward(u, stages, ds, lams):

@@ -213,3 +213,21 @@ def test_rule_invariants():
     for k in range(1, 30):
         xx, st, t = O.optim_step("fista", xx, xx, st, t, eta=eta)
         assert 0.5 * np.dot(xx, xx) <= 2 * np.dot(x, x) / (eta * (k + 1) ** 2) + 1e-15
+
+
+def test_oracle_adam_closed_forms():
+    """Adam's first step is eta * g / (|g| + eps) (bias corrections undo (1 - b)); with b1 = b2 = 0
+    every step is that sign-like step; the state stays FP32."""
+    from oracle import fluxoptics_oracle as O
+    x = np.linspace(-1, 1, 11).astype(np.float32)
+    g = np.linspace(-3, 2, 11).astype(np.float32)
+    z = np.zeros(11, np.float32)
+    xn, m, v, bt = O.adam_step(x, g, z, z, (0.9, 0.999), eta=0.1)
+    ref = x.astype(np.float64) - 0.1 * g / (np.abs(g.astype(np.float64)) + 1e-8)
+    assert np.max(np.abs(xn - ref)) <= 1e-6 and m.dtype == np.float32 and bt == (0.81, 0.999 ** 2)
+    xk = x
+    for _ in range(3):
+        xk2, m0, v0, _ = O.adam_step(xk, g, z, z, (0.0, 0.0), eta=0.05, beta=(0.0, 0.0))
+        assert np.max(np.abs(xk2 - (xk.astype(np.float64) - 0.05 * np.sign(g) * (np.abs(g) / (np.abs(g) + 1e-8)))))\
+            <= 1e-6
+        xk = xk2
