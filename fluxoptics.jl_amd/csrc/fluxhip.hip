@@ -612,11 +612,13 @@ __global__ void k_optim_step(float* __restrict__ x, const float* __restrict__ dx
 
 // Adam (Optimisers.jl 0.4.6, the reference's dependency): m = b1 m + (1 - b1) g,
 // v = b2 v + (1 - b2) g^2 (stored in FP32), x -= eta * m / (1 - bt1) / (sqrt(v / (1 - bt2)) + eps),
-// each formed in FP64 from the FP32 values as Julia's Float64-scalar broadcasts do; bt1, bt2 are
-// b1^t, b2^t of the current step.  state: m in [0, n), v in [n, 2n).  Then the proximal operator.
+// each formed in FP64 from the FP32 values as Julia's Float64-scalar broadcasts do; bt = {b1^t,
+// b2^t} of the current step, in device memory so that a recorded step replays correctly
+// (k_adam_advance moves it on).  state: m in [0, n), v in [n, 2n).  Then the proximal operator.
 __global__ void k_adam_step(float* __restrict__ x, const float* __restrict__ dx, float* __restrict__ st, long n,
-                            double eta, double b1, double b2, double eps, double bt1, double bt2, int prox,
+                            double eta, double b1, double b2, double eps, const double* __restrict__ bt, int prox,
                             float p0, float p1) {
+  const double bt1 = bt[0], bt2 = bt[1];
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const double g = dx[i];
     const float m = (float)(b1 * (double)st[i] + (1.0 - b1) * g);
@@ -634,6 +636,13 @@ __global__ void k_adam_step(float* __restrict__ x, const float* __restrict__ dx,
       xi = xi < 0.f ? 0.f : xi;
     }
     x[i] = xi;
+  }
+}
+
+__global__ void k_adam_advance(double* bt, double b1, double b2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    bt[0] *= b1;
+    bt[1] *= b2;
   }
 }
 
@@ -2173,16 +2182,19 @@ int fh_optim_step(fh_ctx* ctx, int rule, float* x, const float* dx, float* state
 }
 
 int fh_adam_step(fh_ctx* ctx, float* x, const float* dx, float* state, long n, double eta, double beta1,
-                 double beta2, double eps, double bt1, double bt2, int prox, double p0, double p1) {
-  if (!ctx || !x || !dx || !state) return fail(FH_E_ARG, "null context, parameter, gradient or state");
+                 double beta2, double eps, double* bt, int prox, double p0, double p1) {
+  if (!ctx || !x || !dx || !state || !bt) return fail(FH_E_ARG, "null context, parameter, gradient, state or powers");
   if (n <= 0) return fail(FH_E_ARG, "n must be positive");
   if (prox < 0 || prox > 3) return fail(FH_E_ARG, "unknown proximal operator %d", prox);
-  if (!(bt1 < 1.0) || !(bt2 < 1.0)) return fail(FH_E_ARG, "bias-correction powers must be below 1");
+  if (!(beta1 >= 0.0 && beta1 < 1.0) || !(beta2 >= 0.0 && beta2 < 1.0))
+    return fail(FH_E_ARG, "Adam needs 0 <= beta < 1");
   int rc;
   if ((rc = set_device(ctx))) return rc;
   const int blocks = (int)std::min<long>((n + 255) / 256, 8192);
   hipLaunchKernelGGL(k_adam_step, dim3(blocks), dim3(256), 0, ctx->stream, x, dx, state, n, eta, beta1, beta2, eps,
-                     bt1, bt2, prox, (float)p0, (float)p1);
+                     (const double*)bt, prox, (float)p0, (float)p1);
+  FH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_adam_advance, dim3(1), dim3(64), 0, ctx->stream, bt, beta1, beta2);
   FH_HIP(hipGetLastError());
   return FH_OK;
 }

@@ -152,14 +152,19 @@ class _Leaf:
             self.state = torch.zeros_like(tensor)
         elif isinstance(base, Adam):
             self.state = torch.zeros((2,) + tuple(tensor.shape), dtype=tensor.dtype, device=tensor.device)
-            self.bt = base.beta  # b1^t, b2^t of the next step
+            # b1^t, b2^t of the next step, on the device (advanced by each step)
+            self.bt = torch.tensor(base.beta, dtype=torch.float64, device=tensor.device)
         elif isinstance(base, Fista):
             self.state = tensor.detach().clone()
 
 
-def setup(rules: Dict, model, default_rule=None):
+def setup(rules, model, default_rule=None):
     """``setup(rules, [default_rule,] model)`` (``OptimisersExt.jl:95-140``): one optimiser state
-    per trainable parameter; parameters without a rule get ``default_rule`` (NoDescent)."""
+    per trainable parameter; parameters without a rule get ``default_rule`` (NoDescent).
+    ``setup(rule, model)`` with a single rule applies it to every parameter, as
+    ``FluxOptics.setup(Optimisers.Adam(0.1), system)`` does."""
+    if not isinstance(rules, dict):
+        rules, default_rule = {}, rules
     default_rule = NoDescent() if default_rule is None else default_rule
     leaves = []
     for c in _trainable_components(model):
@@ -205,9 +210,8 @@ def _step(leaf: _Leaf, grad: torch.Tensor) -> None:
             raise NotImplementedError("Adam is applied to real parameters here")
         first = proxes[0] if proxes else None
         _lib.call("fh_adam_step", ctx, _ptr(xr), _ptr(gr), _ptr(leaf.state), n, base.eta, base.beta[0], base.beta[1],
-                  base.epsilon, leaf.bt[0], leaf.bt[1], first.code if first else 0, first.p0 if first else 0.0,
+                  base.epsilon, _ptr(leaf.bt), first.code if first else 0, first.p0 if first else 0.0,
                   first.p1 if first else 0.0)
-        leaf.bt = (leaf.bt[0] * base.beta[0], leaf.bt[1] * base.beta[1])
         zero = torch.zeros_like(xr)
         for p in proxes[1:]:
             _lib.call("fh_optim_step", ctx, 0, _ptr(xr), _ptr(zero), None, n, 0.0, 0.0, 0.0, p.code, p.p0, p.p1)

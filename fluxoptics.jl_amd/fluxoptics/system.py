@@ -226,6 +226,13 @@ class OpticalSystem:
             grads[self.source] = self.source.backpropagate_with_gradient(du)
         return losses, grads
 
+    def capture_train_step(self, loss, opt_state, x: Optional[ScalarField] = None, weights=None) -> "CapturedGradient":
+        """Record one optimisation iteration -- ``withgradient`` and ``update_(opt_state, ...)``
+        -- as one HIP graph; each ``replay()`` runs an iteration (parameters and optimiser state
+        move on) and returns ``(value, grads)`` of that iteration.  The recording itself changes
+        nothing; one eager ``withgradient`` runs first so that every plan and workspace exists."""
+        return _capture_train_step(self, loss, opt_state, x, weights)
+
     def capture_withgradient(self, loss, x: Optional[ScalarField] = None, weights=None) -> "CapturedGradient":
         """Record the whole ``withgradient`` launch sequence (forward with saves, metric and
         pullback, adjoint with gradients) once as a HIP graph; ``replay()`` then reruns it with
@@ -244,6 +251,23 @@ class OpticalSystem:
         with torch.cuda.graph(graph):
             losses, grads = self._withgradient_launches(loss, x, weights)
         return CapturedGradient(graph, losses, grads, weights)
+
+
+def _capture_train_step(system, loss, opt_state, x=None, weights=None):
+    from .optim import Fista, update_
+    for leaf in opt_state:
+        if isinstance(leaf.base, Fista):
+            raise NotImplementedError("Fista's step coefficient is a host value per step; record steps with "
+                                      "Descent, Momentum, Nesterov or Adam")
+    if weights is not None:
+        weights = torch.as_tensor(weights, dtype=torch.float64).cuda()
+    system._withgradient_launches(loss, x, weights)  # plans, tables, workspaces; parameters unchanged
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        losses, grads = system._withgradient_launches(loss, x, weights)
+        update_(opt_state, system, grads)
+    return CapturedGradient(graph, losses, grads, weights)
 
 
 def _total(losses, weights=None):

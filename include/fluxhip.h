@@ -287,10 +287,11 @@ int fh_optim_step(fh_ctx* ctx, int rule, float* x, const float* dx, float* state
 /* Adam (Optimisers.jl 0.4.6, the reference's dependency; the rule of its published
  * beam-splitter example): m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, x -= eta m/(1 - bt1) /
  * (sqrt(v/(1 - bt2)) + eps), then the proximal operator as in fh_optim_step.  state holds 2n
- * FP32 values (m, then v), zero before the first step; bt1 = b1^t and bt2 = b2^t for step t
- * (t = 1 first). */
+ * FP32 values (m, then v), zero before the first step; bt is a device array {b1^t, b2^t} for
+ * step t ({b1, b2} before the first step) that the call advances on the stream, so a recorded
+ * training step replays correctly. */
 int fh_adam_step(fh_ctx* ctx, float* x, const float* dx, float* state, long n, double eta, double beta1,
-                 double beta2, double eps, double bt1, double bt2, int prox, double p0, double p1);
+                 double beta2, double eps, double* bt, int prox, double p0, double p1);
 
 /* DotProduct and PowerCoupling, mode-selective (metrics/dot_product.jl:60-106,
  * power_coupling.jl:55-69): c[b] = ds * sum u_b conj(v_b) into the device complex128 array c

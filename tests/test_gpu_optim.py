@@ -104,3 +104,27 @@ def test_training_loop_matches_oracle():
     torch.cuda.synchronize()
     assert rel_l2(doe.phi.cpu().numpy(), phi) <= 1e-3
     assert np.abs(phi).max() > 1e-2  # the loop moved the mask
+
+
+@pytest.mark.parametrize("pname,prox,pfn", PROX, ids=[p[0] for p in PROX])
+def test_adam_steps(pname, prox, pfn):
+    """Adam (Optimisers.jl rule) fused with each proximal operator, 5 steps, against the
+    oracle's step (FP32 state, FP64 step formation)."""
+    n = 4096
+    x0 = inputs.normal(251, (n,)).astype(np.float32)
+    x = torch.from_numpy(x0.copy()).cuda()
+    comp = fo.Phase.__new__(fo.Phase)
+    comp.phi, comp.trainable = x, True
+    rule = fo.Adam(0.1)
+    state = fo.setup({comp: fo.ProxRule(rule, prox) if prox is not None else rule}, [comp])
+    xr = x0.copy()
+    m = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    bt = rule.beta
+    for k in range(5):
+        g = inputs.normal(260 + k, (n,)).astype(np.float32)
+        fo.update_(state, [comp], {comp: torch.from_numpy(g).cuda()})
+        xr, m, v, bt = O.adam_step(xr, g, m, v, bt, eta=rule.eta, beta=rule.beta, eps=rule.epsilon)
+        xr = pfn(xr.astype(np.float64)).astype(np.float32)
+    torch.cuda.synchronize()
+    assert np.allclose(x.cpu().numpy(), xr, rtol=1e-6, atol=1e-6)
