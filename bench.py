@@ -228,6 +228,56 @@ def workload_published_splitter(torch, fo, inputs):
                     "as 200 replays of one recorded iteration"}
 
 
+def workload_published_rgb(torch, fo, inputs):
+    """The reference's RGB DOE notebook (notebooks/03_RGB_beam_shaping.ipynb, timed at its cell
+    27: 200 iterations in 6.196 s on an unrecorded CUDA GPU): 800x800 (600x600 images padded by
+    100), ds 0.5 um, lambda {0.640, 0.538, 0.455} as one 3-field batch with Gaussians (w0 40 um,
+    red x2); RSProp(3000) -> TeaDOE -> ... (5 RSProps, 4 trainable TeaDOE with dn(lambda) =
+    0.508/0.512/0.518); SquaredIntensityDifference against per-wavelength targets; ProxRule(Fista(200),
+    ClampProx(-0.5, 0.5)).  The notebook's target images are replaced by synthetic intensity
+    patterns of the same shape (normalised to unit sum, the head pattern x4)."""
+    import warnings
+    import numpy as np
+    n, ds, lams = 800, 0.5, [0.640, 0.538, 0.455]
+    dn_tab = {0.455: 1.518 - 1, 0.538: 1.512 - 1, 0.640: 1.508 - 1}
+    dn = lambda lam: dn_tab[round(lam, 3)]
+    g = inputs.gaussian(n, n, ds, ds, 40.0)
+    u0 = np.stack([2 * g, g, g]).astype(np.complex64)
+    x = inputs.grid(n, ds)
+    r = np.hypot(x[None, :], x[:, None])
+    head = np.exp(-((r - 120.0) / 12.0) ** 2)
+    eyes = np.exp(-(np.hypot(x[None, :] - 45, x[:, None] - 40) / 15.0) ** 2) + \
+        np.exp(-(np.hypot(x[None, :] + 45, x[:, None] - 40) / 15.0) ** 2)
+    mouth = np.exp(-((r - 70.0) / 8.0) ** 2) * (x[:, None] < -20)
+    tg = np.stack([4 * head / head.sum(), eyes / eyes.sum(), mouth / mouth.sum()]).astype(np.float32)
+    u = fo.ScalarField(torch.from_numpy(u0).cuda(), (ds, ds), lams)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        p = fo.RSProp(u, 3000.0)
+    does = [fo.TeaDOE(u, dn, trainable=True, buffered=True) for _ in range(4)]
+    system = fo.ScalarSource(u) | p | does[0] | p | does[1] | p | does[2] | p | does[3] | p
+    metric = fo.SquaredIntensityDifference((u, tg))
+    for d in does:
+        d.h.zero_()
+    opt = fo.setup(fo.ProxRule(fo.Fista(200), fo.ClampProx(-0.5, 0.5)), system)
+    _, grads = system.withgradient(metric)   # warm-up, as the notebook's cell 25
+    fo.update_(opt, system, grads)
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(200):
+        val, grads = system.withgradient(metric)
+        fo.update_(opt, system, grads)
+        losses.append(val)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    return {"seconds_200_iterations": sec, "ms_per_iter": sec * 5.0, "loss_first": float(losses[0]),
+            "loss_last": float(losses[-1]), "published_gpu_seconds": 6.196,
+            "published_hardware": "CUDA GPU, model not recorded", "speedup_vs_published": 6.196 / sec,
+            "what": "800^2 x 3 lambda, 5 RSProp(3000) + 4 TeaDOE, intensity loss, Fista(200) + clamp, 200 "
+                    "iterations of withgradient + update_, host-driven (RS padded to 2048^2); synthetic targets"}
+
+
 def workload_train_step(torch, fo, inputs):
     """The headline grid as a training step: 8 fields of 2048x2048 (ds 1, lambda 0.8),
     Phase (trainable, shared) -> ASProp(5000), per-field intensity targets; value + mask
@@ -374,7 +424,7 @@ def main():
 
     if args.workload:
         from fluxoptics import inputs as inp
-        fns = {"published_splitter_250": workload_published_splitter, "beam_splitter_512": workload_beam_splitter, "train_step_2048x8": workload_train_step,
+        fns = {"published_splitter_250": workload_published_splitter, "published_rgb_800x3": workload_published_rgb, "beam_splitter_512": workload_beam_splitter, "train_step_2048x8": workload_train_step,
                "rgb_cascade_1024x3": workload_rgb_cascade, "as_bpm_512x400": workload_bpm,
                "uneven_batch_64": workload_uneven_batch}
         if args.workload not in fns:
@@ -458,6 +508,7 @@ def main():
         from fluxoptics import inputs as inp
         out["workloads"] = {
             "published_splitter_250": workload_published_splitter(torch, fo, inp),
+            "published_rgb_800x3": workload_published_rgb(torch, fo, inp),
             "beam_splitter_512": workload_beam_splitter(torch, fo, inp),
             "train_step_2048x8": workload_train_step(torch, fo, inp),
             "rgb_cascade_1024x3": workload_rgb_cascade(torch, fo, inp),

@@ -383,19 +383,30 @@ def tea_transmission(h, lam: float, dn: float, r: complex = 1.0, direction: int 
     return (r if direction > 0 else np.conj(r)) * np.exp(1j * direction * ph)
 
 
+def _dn_of(dn, lam: float, b: int):
+    # dn(lambda) as TeaDOE takes it: a number, a function of lambda or one value per field
+    if callable(dn):
+        return float(dn(lam))
+    if isinstance(dn, (list, tuple, np.ndarray)):
+        return float(dn[b])
+    return float(dn)
+
+
 def tea_apply(u, h, lams, dn, r=1.0, direction: int = 1):
     u = _as_batch(np.asarray(u, np.complex128))
     lam_b = _lambdas(lams, u.shape[0])
-    return np.stack([u[b] * tea_transmission(h, lam_b[b], dn, r, direction) for b in range(u.shape[0])])
+    return np.stack([u[b] * tea_transmission(h, lam_b[b], _dn_of(dn, lam_b[b], b), r, direction)
+                     for b in range(u.shape[0])])
 
 
 def tea_backward(dv, u_saved, h, lams, dn, r=1.0, direction: int = 1):
     """``tea_doe.jl:180-199``: du = dv * conj(t); dh = s * sum_b (2 pi dn / lam_b) Im(du conj(u_saved))."""
     dv = _as_batch(np.asarray(dv, np.complex128))
     lam_b = _lambdas(lams, dv.shape[0])
-    du = np.stack([dv[b] * np.conj(tea_transmission(h, lam_b[b], dn, r, direction)) for b in range(dv.shape[0])])
+    dns = [_dn_of(dn, lam_b[b], b) for b in range(dv.shape[0])]
+    du = np.stack([dv[b] * np.conj(tea_transmission(h, lam_b[b], dns[b], r, direction)) for b in range(dv.shape[0])])
     us = _as_batch(u_saved)
-    dh = sum(TWO_PI * dn / lam_b[b] * np.imag(du[b] * np.conj(us[b])) for b in range(dv.shape[0]))
+    dh = sum(TWO_PI * dns[b] / lam_b[b] * np.imag(du[b] * np.conj(us[b])) for b in range(dv.shape[0]))
     return du, direction * dh
 
 
