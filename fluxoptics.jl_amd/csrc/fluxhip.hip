@@ -212,20 +212,13 @@ __device__ inline void tab_freq_index(int i, int j, int nx, int ny, int full, in
 __device__ inline void store_tab(float2* p, double re, double im) { *p = make_float2((float)re, (float)im); }
 __device__ inline void store_tab(double2* p, double re, double im) { *p = make_double2(re, im); }
 
-template <typename T2>
-__global__ void k_as_table(T2* __restrict__ tab, int nx, int ny, int rows, int qcol, int full,
-                           double dx, double dy, double lam, double n0, double z, double tx,
-                           double ty, int mode, int track, double scale) {
+// The transfer function at (fx, fy) in cycles of phase (t) and amplitude (amp); see above.
+__device__ inline void as_transfer(double fx, double fy, double lam, double n0, double z, double tx, double ty,
+                                   int mode, int track, double& amp, double& t) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= rows || j >= qcol) return;
-  int kx, ky;
-  tab_freq_index(i, j, nx, ny, full, kx, ky);
-  const double fx = (double)kx / ((double)nx * dx);
-  const double fy = (double)ky / ((double)ny * dy);
   const double le = lam / n0;
-  double amp = 1.0, t = 0.0;
+  amp = 1.0;
+  t = 0.0;
   if (mode == 0) {
     const double f2 = 1.0 / (le * le);
     const double a = f2 - fx * fx - fy * fy;
@@ -250,11 +243,94 @@ __global__ void k_as_table(T2* __restrict__ tab, int nx, int ny, int rows, int q
       }
     }
   }
+}
+
+template <typename T2>
+__global__ void k_as_table(T2* __restrict__ tab, int nx, int ny, int rows, int qcol, int full,
+                           double dx, double dy, double lam, double n0, double z, double tx,
+                           double ty, int mode, int track, double scale) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= rows || j >= qcol) return;
+  int kx, ky;
+  tab_freq_index(i, j, nx, ny, full, kx, ky);
+  const double fx = (double)kx / ((double)nx * dx);
+  const double fy = (double)ky / ((double)ny * dy);
+  double amp, t;
+  as_transfer(fx, fy, lam, n0, z, tx, ty, mode, track, amp, t);
   const double r = t - rint(t);
   double sn, cs;
   sincospi(2.0 * r, &sn, &cs);
   const double re = amp * cs, im = amp * sn;
   store_tab(tab + (long)i * qcol + j, re * scale, im * scale);
+}
+
+// ASProp on grids that are not powers of two runs as a zero-padded linear convolution: the
+// n-point circular convolution with h = IDFT_n(H) equals the linear convolution with h
+// extended periodically over offsets |d| < n, which the padded (RS-geometry) plan computes.
+// k_as_grid_d: H on the field's own n-point frequency grid [ny][nx] (FFT order, fftfreq for
+// any n), FP64, scaled by 1/(nx ny) (the inverse DFT's normalisation).
+__global__ void k_as_grid_d(double2* __restrict__ out, int nx, int ny, double dx, double dy, double lam, double n0,
+                            double z, double tx, double ty, int mode, int track) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx) return;
+  const int kx = i < (nx + 1) / 2 ? i : i - nx;
+  const int ky = j < (ny + 1) / 2 ? j : j - ny;
+  double amp, t;
+  as_transfer((double)kx / ((double)nx * dx), (double)ky / ((double)ny * dy), lam, n0, z, tx, ty, mode, track, amp, t);
+  const double r = t - rint(t);
+  double sn, cs;
+  sincospi(2.0 * r, &sn, &cs);
+  const double sc = 1.0 / ((double)nx * (double)ny);
+  out[(long)j * nx + i] = make_double2(amp * cs * sc, amp * sn * sc);
+}
+
+// Direct inverse DFT along one axis (unnormalised, sign +), FP64: out[line][m] =
+// sum_k in[line][k] e^{+2 pi i k m / n}, with (k m) mod n indexing an exact twiddle table.
+// Lines are `nlines` apart by `lstride`, points by `estride`.
+__global__ void k_idft_lines_d(const double2* __restrict__ in, double2* __restrict__ out, int n, int nlines,
+                               long lstride, long estride, const double2* __restrict__ w) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int line = blockIdx.y;
+  if (m >= n || line >= nlines) return;
+  const double2* src = in + (long)line * lstride;
+  double re = 0.0, im = 0.0;
+  long km = 0;
+  for (int k = 0; k < n; ++k) {
+    const double2 a = src[(long)k * estride];
+    const double2 t = w[km];
+    re += a.x * t.x - a.y * t.y;
+    im += a.x * t.y + a.y * t.x;
+    km += m;
+    if (km >= n) km -= n;
+  }
+  out[(long)line * lstride + (long)m * estride] = make_double2(re, im);
+}
+
+// e^{+2 pi i j / n}, j < n
+__global__ void k_unit_roots_d(double2* __restrict__ w, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double sn, cs;
+  sincospi(2.0 * (double)j / (double)n, &sn, &cs);
+  w[j] = make_double2(cs, sn);
+}
+
+// h [ny][nx] extended periodically over offsets |d| < n onto the padded [ny2][nx2] grid
+// (offsets in FFT order, i <= nx2/2 ? i : i - nx2); zero elsewhere.
+__global__ void k_circ_embed_d(double2* __restrict__ out, const double2* __restrict__ h, int nx, int ny, int nx2,
+                               int ny2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx2) return;
+  const int ox = i <= nx2 / 2 ? i : i - nx2;
+  const int oy = j <= ny2 / 2 ? j : j - ny2;
+  double2 v = make_double2(0.0, 0.0);
+  if (ox > -nx && ox < nx && oy > -ny && oy < ny) v = h[(long)(oy < 0 ? oy + ny : oy) * nx + (ox < 0 ? ox + nx : ox)];
+  out[(long)j * nx2 + i] = v;
 }
 
 // Full FP64 transfer-function table [ny][nx] for the plain comparator (same formula).
@@ -1576,6 +1652,54 @@ int fh_ctx_destroy(fh_ctx* ctx) {
   return FH_OK;
 }
 
+// Tables of a padded ASProp plan (grids that are not powers of two): per key, H on the
+// n-point grid, its inverse 2-D DFT h (FP64, direct sums), h extended periodically onto the
+// padded grid, the padded FFT (FP64), rounded once; the plan then runs the padded passes.
+static int as_padded_tables(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p) {
+  const int nx = d->nx, ny = d->ny;
+  const int nx2 = 1 << p->lx, ny2 = 1 << p->ly;
+  p->kind = 1;
+  p->qx = p->full ? nx2 : nx2 / 2 + 1;
+  p->qy = p->full ? ny2 + 1 : ny2 / 2 + 1;
+  p->tabsize = (long)p->qx * p->qy;
+  const long nt = (long)p->lam_tables.size();
+  if (hipMalloc(&p->tab, p->tabsize * nt * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&p->scratch, (size_t)nx2 * mid_rows(p) * d->batch * sizeof(float2)) != hipSuccess)
+    return fail(FH_E_NOMEM, "ASProp tables and workspace: out of device memory");
+  double2 *hn = nullptr, *tmp = nullptr, *wx = nullptr, *wy = nullptr, *dbuf = nullptr;
+  int rc = FH_OK;
+  const size_t nn = (size_t)nx * ny;
+  if (hipMalloc(&hn, nn * sizeof(double2)) != hipSuccess || hipMalloc(&tmp, nn * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&wx, nx * sizeof(double2)) != hipSuccess || hipMalloc(&wy, ny * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&dbuf, (size_t)nx2 * ny2 * sizeof(double2)) != hipSuccess)
+    rc = fail(FH_E_NOMEM, "ASProp FP64 table buffers: out of device memory");
+  if (rc == FH_OK) {
+    hipLaunchKernelGGL(k_unit_roots_d, dim3((nx + 255) / 256), dim3(256), 0, ctx->stream, wx, nx);
+    hipLaunchKernelGGL(k_unit_roots_d, dim3((ny + 255) / 256), dim3(256), 0, ctx->stream, wy, ny);
+  }
+  for (long k = 0; k < nt && rc == FH_OK; ++k) {
+    hipLaunchKernelGGL(k_as_grid_d, dim3((nx + 255) / 256, ny), dim3(256), 0, ctx->stream, hn, nx, ny, d->dx, d->dy,
+                       p->lam_tables[k], d->n0, p->tab_z[k], p->tab_tx[k], p->tab_ty[k], p->mode, p->track);
+    hipLaunchKernelGGL(k_idft_lines_d, dim3((nx + 255) / 256, ny), dim3(256), 0, ctx->stream, (const double2*)hn, tmp,
+                       nx, ny, (long)nx, 1L, (const double2*)wx);
+    hipLaunchKernelGGL(k_idft_lines_d, dim3((ny + 255) / 256, nx), dim3(256), 0, ctx->stream, (const double2*)tmp, hn,
+                       ny, nx, 1L, (long)nx, (const double2*)wy);
+    hipLaunchKernelGGL(k_circ_embed_d, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, dbuf,
+                       (const double2*)hn, nx, ny, nx2, ny2);
+    if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "ASProp padded table kernels failed to launch"); break; }
+    if ((rc = fft2_d(ctx, dbuf, nx2, ny2, p->lx, p->ly))) break;
+    hipLaunchKernelGGL(k_tab_extract, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
+                       p->tab + k * p->tabsize, (const double2*)dbuf, nx2, ny2, p->qx, p->qy, p->full,
+                       1.0 / ((double)nx2 * (double)ny2));
+    if (hipGetLastError() != hipSuccess) rc = fail(FH_E_HIP, "ASProp table kernel launch failed");
+  }
+  if (rc == FH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(FH_E_HIP, "ASProp table generation failed");
+  for (void* q : {(void*)hn, (void*)tmp, (void*)wx, (void*)wy, (void*)dbuf})
+    if (q) (void)hipFree(q);
+  if (rc == FH_OK) rc = upload_tabidx(p);
+  return rc;
+}
+
 int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   if (!out || !d) return fail(FH_E_ARG, "null descriptor or output pointer");
   *out = nullptr;
@@ -1590,10 +1714,17 @@ int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   const int nx = d->nx, ny = d->ny;
   if (!(d->n0 > 0)) { plan_free(p); return fail(FH_E_ARG, "n0 must be positive"); }
   p->kind = 0; p->z = p->tab_z[0]; p->n0 = d->n0;
-  p->lx = ilog2_exact(nx); p->ly = ilog2_exact(ny);
-  if (p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
+  const bool pow2 = ilog2_exact(nx) >= 0 && ilog2_exact(ny) >= 0;
+  if (pow2) {
+    p->lx = ilog2_exact(nx); p->ly = ilog2_exact(ny);
+  } else {
+    // any other grid: the padded convolution form (as_padded_tables)
+    p->lx = ilog2_ceil(2 * nx - 1); p->ly = ilog2_ceil(2 * ny - 1);
+  }
+  if (nx < 2 || ny < 2 || p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
     plan_free(p);
-    return fail(FH_E_UNSUPP, "ASProp needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << kMinLog, 1 << kMaxLog, nx, ny);
+    return fail(FH_E_UNSUPP, "ASProp needs power-of-two nx, ny in [%d, %d], or other sizes with 2n - 1 <= %d "
+                "(got %d x %d)", 1 << kMinLog, 1 << kMaxLog, 1 << kMaxLog, nx, ny);
   }
   if ((rc = set_device(ctx))) { plan_free(p); return rc; }
   // kernel choice of angular_spectrum.jl:126-139; tilted tables are not even in fx, fy
@@ -1601,6 +1732,11 @@ int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   p->mode = (d->flags & FH_PROP_SHIFT) ? 3 : ((d->flags & FH_PROP_PARAXIAL) ? 1 : (tilted ? 2 : 0));
   p->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
   p->full = tilted ? 1 : 0;
+  if (!pow2) {
+    if ((rc = as_padded_tables(ctx, d, p))) { plan_free(p); return rc; }
+    *out = p;
+    return FH_OK;
+  }
   p->qx = p->full ? nx : nx / 2 + 1;
   p->qy = p->full ? ny + 1 : ny / 2 + 1;
   p->tabsize = (long)p->qx * p->qy;

@@ -54,7 +54,10 @@ int fh_ctx_synchronize(fh_ctx* ctx);
 int fh_ctx_destroy(fh_ctx* ctx);
 
 /* ASProp: angular-spectrum propagator for fields of shape [nx, ny, batch].
- * lambdas has nlambdas entries: 1 (shared) or batch (one per field).
+ * lambdas has nlambdas entries: 1 (shared) or batch (one per field).  Power-of-two grids run
+ * the three-pass FFT form; any other grid (2n - 1 <= fh_max_length()) runs as the equivalent
+ * zero-padded linear convolution with h = IDFT_n(H) extended periodically (the n-point
+ * circular convolution the reference's FFT computes), with tables built in FP64.
  * Replaces ASProp(u, ds, z; n0) with use_cache=true, double_precision_kernel=true
  * (src/optical_components/freespace_propagators/angular_spectrum.jl:193-229,1-12):
  * the transfer function is computed once in FP64 and rounded to c64. */

@@ -174,9 +174,12 @@ def test_reference_properties_on_device():
 
 
 def test_error_paths_on_device():
-    u = fo.ScalarField(to_dev(np.zeros((48, 48))), (1.0, 1.0), 0.8)
-    with pytest.raises(fo._lib.FluxHipError) if hasattr(fo, "_lib") else pytest.raises(Exception):
+    # 2n - 1 beyond the longest transform: refused (any smaller size runs, padded if needed)
+    u = fo.ScalarField(to_dev(np.zeros((16, 3000))), (1.0, 1.0), 0.8)
+    with pytest.raises(fo._lib.FluxHipError):
         fo.ASProp(u, 10.0).propagate_(u)
+    with pytest.raises(fo._lib.FluxHipError):
+        fo.RSProp(u, 10.0).propagate_(u)
     u2 = fo.ScalarField(to_dev(np.zeros((64, 64))), (1.0, 1.0), 0.8)
     with pytest.raises(ValueError):
         fo.ASProp(u, 10.0).propagate_(u2)
