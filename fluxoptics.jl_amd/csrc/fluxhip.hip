@@ -399,18 +399,17 @@ __global__ void k_rs_sample(double2* __restrict__ d, int nx2, int ny2, double dx
 }
 
 // CollinsProp's convolution kernel (collins_integral.jl:17-21, sampled as ConvolutionKernel
-// does, convolution_kernel.jl:13-18, on the 2n grid with offsets i <= n ? i : i - 2n):
+// does, convolution_kernel.jl:13-18, on the padded grid with offsets i <= M/2 ? i : i - M):
 // h = nrm * cis(pi*(ax*x^2 + ay*y^2)/(b*lam)), phase taken in cycles and reduced in FP64;
 // nrm = dx*dy/(i*b) = prod(ds ./ sqrt(i*b)) (collins_integral.jl:122-123).
-__global__ void k_collins_sample(double2* __restrict__ d, int nx, int ny, double dx, double dy, double lam,
+__global__ void k_collins_sample(double2* __restrict__ d, int nx2, int ny2, double dx, double dy, double lam,
                                  double ax, double ay, double b, double nrm_re, double nrm_im) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
-  const int nx2 = 2 * nx;
   if (i >= nx2) return;
-  const int oi = i <= nx ? i : i - nx2;
-  const int oj = j <= ny ? j : j - 2 * ny;
+  const int oi = i <= nx2 / 2 ? i : i - nx2;
+  const int oj = j <= ny2 / 2 ? j : j - ny2;
   const double x = oi * dx, y = oj * dy;
   const double cyc = (ax * x * x + ay * y * y) / (2.0 * b * lam);
   double sn, cs;
@@ -448,8 +447,9 @@ __device__ inline double2 cexp_reduced(double re, double im) {
 }
 
 // The Bluestein kernel h[f] = w^{-f^2/2} of prepare_czt_data (FFTutils.jl:41), separable,
-// sampled on the 2n grid with offsets f = i <= n ? i : i - 2n (the (2n-1)-grid values at
-// |f| < n; the offset -n is never reached by the cropped convolution); an untransformed
+// sampled on the padded grid (a power of two >= 2n - 1) with offsets f = i <= M/2 ? i : i - M
+// (the (2n-1)-grid values at |f| < n; other offsets are never reached by the cropped
+// convolution); an untransformed
 // dimension contributes a unit impulse at f = 0.
 __device__ inline double2 czt_h(const CztDim& d, int f) {
   if (!d.on) return make_double2(f == 0 ? 1.0 : 0.0, 0.0);
@@ -457,13 +457,16 @@ __device__ inline double2 czt_h(const CztDim& d, int f) {
   return cexp_reduced(q * d.lw, q * d.tw);
 }
 
-__global__ void k_czt_sample(double2* __restrict__ out, int nx, int ny, CztDim dx, CztDim dy) {
+__global__ void k_czt_sample(double2* __restrict__ out, int nx, int ny, int nx2, int ny2, CztDim dx, CztDim dy) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
-  const int nx2 = 2 * nx;
   if (i >= nx2) return;
-  const int oi = i <= nx ? i : i - nx2;
-  const int oj = j <= ny ? j : j - 2 * ny;
+  const int oi = i <= nx2 / 2 ? i : i - nx2;
+  const int oj = j <= ny2 / 2 ? j : j - ny2;
+  if (oi <= -nx || oi >= nx || oj <= -ny || oj >= ny) {  // unused; |w| != 1 could overflow there
+    out[(long)j * nx2 + i] = make_double2(0.0, 0.0);
+    return;
+  }
   const double2 a = czt_h(dx, oi), b = czt_h(dy, oj);
   out[(long)j * nx2 + i] = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -564,6 +567,13 @@ __global__ void k_offset_tilts(float2* __restrict__ u, int nx, int ny, double dx
       const long k = b * npix + i;
       u[k] = cmul(u[k], make_float2((float)cs, (float)sn));
     }
+  }
+}
+
+__global__ void k_scale_c64(float2* __restrict__ u, long n, float s) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float2 v = u[i];
+    u[i] = make_float2(v.x * s, v.y * s);
   }
 }
 
@@ -1068,6 +1078,7 @@ struct fh_ctx {
   std::mutex mu;
   double* metric_part = nullptr;  // metric block partials
   size_t metric_part_n = 0;
+  std::map<std::vector<int>, fh_plan*> dft_plans;  // (nx, ny, batch) -> DFT-as-CZT plan (fh_fft2)
 };
 
 struct fh_plan {
@@ -1647,6 +1658,7 @@ int fh_ctx_destroy(fh_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
   for (auto& kv : ctx->stage_tw) (void)hipFree(kv.second);
+  for (auto& kv : ctx->dft_plans) plan_free(kv.second);
   if (ctx->metric_part) (void)hipFree(ctx->metric_part);
   delete ctx;
   return FH_OK;
@@ -1844,23 +1856,23 @@ int fh_collins_create(fh_ctx* ctx, const fh_prop_desc* d, double dx_out, double 
   if (rc) { plan_free(p); return rc; }
   const int nx = d->nx, ny = d->ny;
   p->kind = 2;
-  p->lx = ilog2_exact(2 * nx); p->ly = ilog2_exact(2 * ny);
-  if (p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
+  p->lx = ilog2_ceil(2 * nx - 1); p->ly = ilog2_ceil(2 * ny - 1);
+  if (nx < 2 || ny < 2 || p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
     plan_free(p);
-    return fail(FH_E_UNSUPP, "CollinsProp needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << (kMinLog - 1), 1 << (kMaxLog - 1), nx, ny);
+    return fail(FH_E_UNSUPP, "CollinsProp needs 2 <= n with 2n - 1 <= %d (got %d x %d)", 1 << kMaxLog, nx, ny);
   }
   if ((rc = set_device(ctx))) { plan_free(p); return rc; }
-  const int nx2 = 2 * nx, ny2 = 2 * ny;
+  const int nx2 = 1 << p->lx, ny2 = 1 << p->ly;
   const double ax = dx_out / d->dx, ay = dy_out / d->dy;  // alpha = ds'/ds (collins_integral.jl:72)
   p->alpha_x = ax; p->alpha_y = ay;
   p->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
   // the convolution kernel is even in x and y: quadrant tables
-  p->qx = nx + 1; p->qy = ny + 1;
+  p->qx = nx2 / 2 + 1; p->qy = ny2 / 2 + 1;
   p->tabsize = (long)p->qx * p->qy;
   const long nt = (long)p->lam_tables.size();
   if (hipMalloc(&p->tab, p->tabsize * nt * sizeof(float2)) != hipSuccess ||
       hipMalloc(&p->chirp, nt * 2 * (nx + ny) * sizeof(float2)) != hipSuccess ||
-      hipMalloc(&p->scratch, (size_t)nx2 * ny * d->batch * sizeof(float2)) != hipSuccess) {
+      hipMalloc(&p->scratch, (size_t)nx2 * mid_rows(p) * d->batch * sizeof(float2)) != hipSuccess) {
     plan_free(p);
     return fail(FH_E_NOMEM, "CollinsProp tables and workspace: out of device memory");
   }
@@ -1878,7 +1890,7 @@ int fh_collins_create(fh_ctx* ctx, const fh_prop_desc* d, double dx_out, double 
   float2* cdy = cdx + nt * nx;
   for (long k = 0; k < nt && rc == FH_OK; ++k) {
     const double lam = p->lam_tables[k];
-    hipLaunchKernelGGL(k_collins_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, dbuf, nx, ny,
+    hipLaunchKernelGGL(k_collins_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, dbuf, nx2, ny2,
                        d->dx, d->dy, lam, ax, ay, b, 0.0, nrm_im);
     if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "Collins sample kernel launch failed"); break; }
     if ((rc = fft2_d(ctx, dbuf, nx2, ny2, p->lx, p->ly))) break;
@@ -1930,18 +1942,18 @@ int fh_czt_create(fh_ctx* ctx, int nx, int ny, int batch, int dims, const double
   int rc = plan_common(ctx, nx, ny, batch, 1.0, 1.0, &one, 1, p);
   if (rc) { plan_free(p); return rc; }
   p->kind = 2;  // the padded chirp / convolution / chirp plan of CollinsProp, with unit scale
-  p->lx = ilog2_exact(2 * nx); p->ly = ilog2_exact(2 * ny);
-  if (p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
+  p->lx = ilog2_ceil(2 * nx - 1); p->ly = ilog2_ceil(2 * ny - 1);
+  if (nx < 2 || ny < 2 || p->lx < kMinLog || p->lx > kMaxLog || p->ly < kMinLog || p->ly > kMaxLog) {
     plan_free(p);
-    return fail(FH_E_UNSUPP, "CZT needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << (kMinLog - 1), 1 << (kMaxLog - 1), nx, ny);
+    return fail(FH_E_UNSUPP, "CZT needs 2 <= n with 2n - 1 <= %d (got %d x %d)", 1 << kMaxLog, nx, ny);
   }
   if ((rc = set_device(ctx))) { plan_free(p); return rc; }
-  const int nx2 = 2 * nx, ny2 = 2 * ny;
-  p->qx = nx + 1; p->qy = ny + 1;  // h depends on f^2: even, quadrant tables
+  const int nx2 = 1 << p->lx, ny2 = 1 << p->ly;
+  p->qx = nx2 / 2 + 1; p->qy = ny2 / 2 + 1;  // h depends on f^2: even, quadrant tables
   p->tabsize = (long)p->qx * p->qy;
   if (hipMalloc(&p->tab, p->tabsize * sizeof(float2)) != hipSuccess ||
       hipMalloc(&p->chirp, 2 * (nx + ny) * sizeof(float2)) != hipSuccess ||
-      hipMalloc(&p->scratch, (size_t)nx2 * ny * batch * sizeof(float2)) != hipSuccess) {
+      hipMalloc(&p->scratch, (size_t)nx2 * mid_rows(p) * batch * sizeof(float2)) != hipSuccess) {
     plan_free(p);
     return fail(FH_E_NOMEM, "CZT tables and workspace: out of device memory");
   }
@@ -1950,7 +1962,8 @@ int fh_czt_create(fh_ctx* ctx, int nx, int ny, int batch, int dims, const double
     plan_free(p);
     return fail(FH_E_NOMEM, "CZT FP64 kernel buffer: out of device memory");
   }
-  hipLaunchKernelGGL(k_czt_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, dbuf, nx, ny, dd[0], dd[1]);
+  hipLaunchKernelGGL(k_czt_sample, dim3((nx2 + 255) / 256, ny2), dim3(256), 0, ctx->stream, dbuf, nx, ny, nx2, ny2,
+                     dd[0], dd[1]);
   if (hipGetLastError() != hipSuccess) rc = fail(FH_E_HIP, "CZT sample kernel launch failed");
   if (rc == FH_OK) rc = fft2_d(ctx, dbuf, nx2, ny2, p->lx, p->ly);
   if (rc == FH_OK) {
@@ -2130,9 +2143,33 @@ int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double sc
   if (sign != 1 && sign != -1) return fail(FH_E_ARG, "sign must be -1 (forward) or +1 (inverse)");
   if (!std::isfinite(scale)) return fail(FH_E_ARG, "scale must be finite");
   const int lx = ilog2_exact(nx), ly = ilog2_exact(ny);
-  if (lx < kMinLog || lx > kMaxLog || ly < kMinLog || ly > kMaxLog)
-    return fail(FH_E_UNSUPP, "FFT2 needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << kMinLog, 1 << kMaxLog, nx, ny);
   int rc;
+  if (lx < kMinLog || lx > kMaxLog || ly < kMinLog || ly > kMaxLog) {
+    // other sizes: the DFT as a chirp z-transform (a = 1, w = e^{-2 pi i/n}; its adjoint plan is
+    // the unnormalised inverse), one cached plan per (nx, ny, batch)
+    const std::vector<int> key{nx, ny, batch};
+    fh_plan* dp = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      auto it = ctx->dft_plans.find(key);
+      if (it != ctx->dft_plans.end()) dp = it->second;
+    }
+    if (!dp) {
+      const double a[4] = {1.0, 0.0, 1.0, 0.0};
+      const double w[4] = {1.0, -2.0 * M_PI / nx, 1.0, -2.0 * M_PI / ny};
+      if ((rc = fh_czt_create(ctx, nx, ny, batch, 3, a, w, 0, &dp))) return rc;
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      ctx->dft_plans[key] = dp;
+    }
+    if ((rc = fh_propagate(dp, u, sign < 0 ? 1 : -1))) return rc;
+    if (scale != 1.0) {
+      const long n = (long)nx * ny * batch;
+      hipLaunchKernelGGL(k_scale_c64, dim3((int)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, ctx->stream,
+                         reinterpret_cast<float2*>(u), n, (float)scale);
+      FH_HIP(hipGetLastError());
+    }
+    return FH_OK;
+  }
   if ((rc = set_device(ctx))) return rc;
   const float2 *twx = nullptr, *twy = nullptr;
   if ((rc = get_twiddles(ctx, nx, &twx)) || (rc = get_twiddles(ctx, ny, &twy))) return rc;

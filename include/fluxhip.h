@@ -110,7 +110,8 @@ int fh_rsprop_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, fh_plan** out);
  * fourier_lens.jl:38-48; ParaxialProp with resampling is (1, z, 1)): the A chirp, the
  * zero-padded convolution with cis(pi*(ax x^2 + ay y^2)/(B lambda)) normalised by
  * prod(ds ./ sqrt(i B)), and the D chirp, alpha = ds'/ds, chirps on the input grid.  In the
- * plan the chirps ride on the first and last x passes, so it costs what an RSProp does.
+ * plan the chirps ride on the first and last x passes, so it costs what an RSProp does; any n
+ * with 2n - 1 <= fh_max_length().
  * Backward conjugates every factor and normalises by prod(ds' ./ sqrt(-i B)), as the
  * reference does (collins_integral.jl:189).  desc->z must be NULL; flags:
  * FH_PROP_TRACK_TILTS only (untracked tilts add cis(+-2 pi x sin(theta)/lambda) to A, D).
@@ -121,7 +122,8 @@ int fh_collins_create(fh_ctx* ctx, const fh_prop_desc* desc, double dx_out, doub
 /* plan_czt(x, dims, a, w; center_on_grid) (FFTutils.jl:30-177): the chirp z-transform
  * X[m] = sum_k x[k] a^-k w^(m k) along the dimensions in the bitmask dims (1 = x, 2 = y),
  * with k, m shifted by (n-1)/2 when center_on_grid, by Bluestein's identity on the padded
- * chirp / convolution / chirp plan.  a and w are host arrays in polar form
+ * chirp / convolution / chirp plan (any n with 2n - 1 <= fh_max_length()).  a and w are host
+ * arrays in polar form
  * {rho_x, theta_x, rho_y, theta_y} (convert_czt_arg); the reference's defaults are
  * a = (1, 0) and w = (1, -2 pi / n) per dimension, for which the CZT is the DFT.
  * fh_propagate(plan, x, +1) applies czt!, fh_propagate(plan, x, -1) the adjoint plan's czt!
@@ -181,7 +183,8 @@ int fh_propagate_plain_fp64(fh_plan* plan, void* u_c128, int direction);
 /* FourierOperator (fourier_operator.jl:65-79, FFTutils.jl:13-28): in-place 2-D DFT of a c64
  * [nx, ny, batch] device array, u <- scale * sum u e^{sign 2 pi i jk/n}.  The reference's FFT
  * is sign = -1, scale = 1; its IFFT is sign = +1, scale = 1/(nx*ny); the adjoint of
- * (sign, scale) is (-sign, scale). */
+ * (sign, scale) is (-sign, scale).  Power-of-two sizes run the line FFTs; other sizes (2n - 1 <=
+ * fh_max_length()) the DFT as a chirp z-transform (a plan cached in the context). */
 int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double scale);
 
 /* SquaredIntensityDifference (squared_intensity_difference.jl:78-90, compute_metric and

@@ -84,3 +84,33 @@ def test_asprop_any_size_training_tail():
         assert abs(float(value) - vals_ref.sum()) <= FIELD_TOL * vals_ref.sum()
         torch.cuda.synchronize()
         assert rel_l2(ph.grad.cpu().numpy(), g_ref) <= 1e-4
+
+
+@pytest.mark.parametrize("nx,ny", [(250, 250), (90, 70), (33, 20)])
+def test_fourier_operator_any_size(nx, ny):
+    """fh_fft2 on sizes that are not powers of two (the DFT as a chirp z-transform): FFT, IFFT
+    and the FourierOperator adjoint identity."""
+    print(f"[fft2 any size] {nx}x{ny}", flush=True)
+    x0 = c64(random_field((2, ny, nx), 191))
+    u = fo.ScalarField(to_dev(x0), (1.0, 1.0), 0.8)
+    F = fo.FourierOperator(u, True)
+    v = F.propagate(u, fo.Forward)
+    assert rel_l2(to_host(v.electric), np.fft.fft2(x0)) <= 1e-5
+    w = F.propagate(v, fo.Backward)
+    assert rel_l2(to_host(w.electric), x0) <= 1e-5
+
+
+def test_collins_and_czt_any_size():
+    nx, ny, ds = 90, 70, (0.5, 0.6)
+    print("[collins, czt any size]", flush=True)
+    u0 = c64(random_field((2, ny, nx), 193))
+    u = fo.ScalarField(to_dev(u0), ds, [0.8, 0.6])
+    p = fo.CollinsProp(u, (0.7, 0.45), (0.8, 900.0, -0.3))
+    v = p.propagate(u, fo.Forward)
+    assert rel_l2(to_host(v.electric), O.collinsprop(u0, ds, (0.7, 0.45), (0.8, 900.0, -0.3), [0.8, 0.6], 1)) <= 1e-5
+    x = to_dev(u0)
+    a = ((1.0005, 0.3), (0.9995, -0.2))
+    w = ((1.0, -0.011), (1.0001, 0.017))
+    got = fo.plan_czt(x, (1, 2), a, w, center_on_grid=True) * x
+    ref = O.czt(u0, (1, 2), {1: a[0], 2: a[1]}, {1: w[0], 2: w[1]}, True)
+    assert rel_l2(to_host(got), ref) <= 1e-5
