@@ -231,3 +231,44 @@ def test_oracle_adam_closed_forms():
         assert np.max(np.abs(xk2 - (xk.astype(np.float64) - 0.05 * np.sign(g) * (np.abs(g) / (np.abs(g) + 1e-8)))))\
             <= 1e-6
         xk = xk2
+
+
+def test_host_dispatch_and_argument_parsing():
+    """Host-only logic of the newer components (no device work): ParaxialProp's choice between
+    ASProp and CollinsProp, FourierLens/CollinsProp argument forms, CZT argument conversion,
+    setup(rule, model), TiltAnchor tilts, ShiftProp options."""
+    import cmath
+    import fluxoptics as fo
+    from fluxoptics import czt as czt_mod
+    u = fo.ScalarField(torch.zeros((2, 32, 32), dtype=torch.complex64), (0.5, 0.5), [0.8, 0.6],
+                       ([0.01, 0.02], 0.0))
+    assert isinstance(fo.ParaxialProp(u, 100.0), fo.ASProp) and fo.ParaxialProp(u, 100.0).paraxial
+    assert isinstance(fo.ParaxialProp(u, (0.5, 0.5), 100.0), fo.ASProp)
+    c = fo.ParaxialProp(u, (1.0, 0.25), 100.0)
+    assert isinstance(c, fo.CollinsProp) and c.abd == (1.0, 100.0, 1.0) and c.ds_out == (1.0, 0.25)
+    lens = fo.FourierLens(u, (0.3, 0.3), 500.0)
+    assert lens.abd == (0.0, 500.0, 0.0) and lens.ds == (0.5, 0.5)
+    lens2 = fo.FourierLens(u, (0.25, 0.25), (0.3, 0.3), 500.0)
+    assert lens2.ds == (0.25, 0.25)
+    with pytest.raises(ValueError):
+        fo.CollinsProp(u, (1.0, 1.0), (1.0, 0.0, 1.0))
+    with pytest.raises(TypeError):
+        fo.CollinsProp(u, (1.0, 1.0))
+    with pytest.raises(NotImplementedError):
+        fo.ShiftProp(u, 10.0, double_precision_kernel=False)
+    # CZT argument forms (convert_czt_arg)
+    assert czt_mod._convert_czt_arg(2j) == [(2.0, cmath.phase(2j))] * 2
+    assert czt_mod._convert_czt_arg((1.0, -1.0)) == [(1.0, 0.0), (1.0, math.pi)]
+    assert czt_mod._convert_czt_arg(((1.5, 0.2), (0.5, -0.1))) == [(1.5, 0.2), (0.5, -0.1)]
+    with pytest.raises(ValueError):
+        czt_mod._convert_czt_arg((1.0, 2.0, 3.0))
+    # setup(rule, model) gives every trainable parameter the rule
+    p1 = fo.Phase(u, np.zeros((32, 32), np.float32), trainable=True)
+    p2 = fo.Phase(u, np.zeros((32, 32), np.float32), trainable=True)
+    st = fo.setup(fo.Descent(0.5), [p1, p2])
+    assert len(st) == 2 and all(isinstance(l.base, fo.Descent) for l in st)
+    st = fo.setup(fo.Adam(0.1), [p1])
+    assert st[0].state.shape == (2, 32, 32)
+    # TiltAnchor keeps the template's tilts
+    a = fo.TiltAnchor(u)
+    assert a.tilts == ([0.01, 0.02], 0.0) and not a.trainable
