@@ -2558,6 +2558,11 @@ struct fh_bpm {
   float2* work = nullptr;    // [B][ny][nx] recompute work field
   int saved_dir = 0;         // direction of the last saving forward (0 = none)
   size_t bytes = 0;
+  // grids that are not powers of two: the slice propagators are general plans (padded form)
+  // and the slice phase is an elementwise kernel; checkpoints then hold fields, not x-spectra
+  int generic = 0;
+  fh_plan* pfull = nullptr;
+  fh_plan* phalf = nullptr;
 };
 
 namespace fh {
@@ -2575,7 +2580,50 @@ static void bpm_free(fh_bpm* m) {
   for (void* q : {(void*)m->tfull, (void*)m->thalf, (void*)m->d_tabidx, (void*)m->kl, (void*)m->ckpt,
                   (void*)m->seg, (void*)m->work})
     if (q) (void)hipFree(q);
+  plan_free(m->pfull);
+  plan_free(m->phalf);
   delete m;
+}
+
+// Slice phase of the generic BPM (apply_dn_slice!, bpm.jl:235-242): u_b *= cis(s kl_b dn),
+// the phase formed in FP64; save (nullable) receives the input first.
+__global__ void k_bpm_phase(float2* __restrict__ u, const float* __restrict__ dn, const double* __restrict__ kl,
+                            double s, float2* __restrict__ save, long npix, int batch) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
+    const double d = dn[i];
+    for (int b = 0; b < batch; ++b) {
+      const long k = b * npix + i;
+      const float2 v = u[k];
+      if (save) save[k] = v;
+      double sn, cs;
+      sincos(s * kl[b] * d, &sn, &cs);
+      u[k] = make_float2((float)(v.x * cs - v.y * sn), (float)(v.x * sn + v.y * cs));
+    }
+  }
+}
+
+// Its adjoint with the dn gradient (bpm.jl:275-312): du_b *= cis(-s kl_b dn), then
+// ddn = s sum_b kl_b Im(du_b conj(saved_b)), summed over the batch in field order.
+__global__ void k_bpm_phase_grad(float2* __restrict__ du, const float* __restrict__ dn, const double* __restrict__ kl,
+                                 double s, const float2* __restrict__ saved, float* __restrict__ ddn, long npix,
+                                 int batch) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
+    const double d = dn[i];
+    double g = 0.0;
+    for (int b = 0; b < batch; ++b) {
+      const long k = b * npix + i;
+      const float2 v = du[k];
+      double sn, cs;
+      sincos(-s * kl[b] * d, &sn, &cs);
+      const float2 w = make_float2((float)(v.x * cs - v.y * sn), (float)(v.x * sn + v.y * cs));
+      du[k] = w;
+      if (ddn) {
+        const float2 sv = saved[k];
+        g += kl[b] * ((double)w.y * sv.x - (double)w.x * sv.y);  // Im(w conj(sv))
+      }
+    }
+    if (ddn) ddn[i] = (float)(s * g);
+  }
 }
 
 // x FFT of rows, optionally with slice k's phase as prologue
@@ -2649,6 +2697,63 @@ static int bpm_recompute(fh_bpm* m, const float* dn, int k0, int k1, double sgn,
   return FH_OK;
 }
 
+static int bpm_phase(fh_bpm* m, float2* f, const float* dn_k, double sgn, float2* save) {
+  const long npix = (long)m->nx * m->ny;
+  hipLaunchKernelGGL(k_bpm_phase, dim3(elementwise_blocks(npix)), dim3(256), 0, m->ctx->stream, f, dn_k,
+                     (const double*)m->kl, sgn, save, npix, m->batch);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+// Generic forward (bpm.jl:244-260): P(dz/2); per slice [checkpoint], phase, P(dz) (P(dz/2) after
+// the last slice).  Checkpoint c holds the field entering slice c * every.
+static int bpm_generic_forward(fh_bpm* m, float2* f, const float* dn, int direction, int save) {
+  const long npix = (long)m->nx * m->ny;
+  const long fsz = npix * m->batch;
+  int rc;
+  if ((rc = fh_propagate(m->phalf, f, direction))) return rc;
+  for (int k = 0; k < m->nslices; ++k) {
+    if (save && k % m->every == 0)
+      FH_HIP(hipMemcpyAsync(m->ckpt + (long)(k / m->every) * fsz, f, fsz * sizeof(float2), hipMemcpyDeviceToDevice,
+                            m->ctx->stream));
+    if ((rc = bpm_phase(m, f, dn + (long)k * npix, direction, nullptr))) return rc;
+    if ((rc = fh_propagate(k + 1 < m->nslices ? m->pfull : m->phalf, f, direction))) return rc;
+  }
+  m->saved_dir = save ? direction : 0;
+  return FH_OK;
+}
+
+// Generic adjoint (bpm.jl:289-312) with the dn gradient; slice inputs are recomputed one segment
+// at a time from the checkpoints with the forward's own launches.
+static int bpm_generic_backward(fh_bpm* m, float2* f, const float* dn, float* ddn, int direction) {
+  const long npix = (long)m->nx * m->ny;
+  const long fsz = npix * m->batch;
+  int rc;
+  if ((rc = fh_propagate(m->phalf, f, -direction))) return rc;
+  for (int k = m->nslices - 1; k >= 0; --k) {
+    const float2* saved = nullptr;
+    if (ddn) {
+      const int k0 = (k / m->every) * m->every;
+      if (k == m->nslices - 1 || k % m->every == m->every - 1) {
+        const int k1 = std::min(k0 + m->every, m->nslices);
+        FH_HIP(hipMemcpyAsync(m->work, m->ckpt + (long)(k0 / m->every) * fsz, fsz * sizeof(float2),
+                              hipMemcpyDeviceToDevice, m->ctx->stream));
+        for (int j = k0; j < k1; ++j) {
+          if ((rc = bpm_phase(m, m->work, dn + (long)j * npix, direction, m->seg + (long)(j - k0) * fsz))) return rc;
+          if (j + 1 < k1 && (rc = fh_propagate(m->pfull, m->work, direction))) return rc;
+        }
+      }
+      saved = m->seg + (long)(k - k0) * fsz;
+    }
+    hipLaunchKernelGGL(k_bpm_phase_grad, dim3(elementwise_blocks(npix)), dim3(256), 0, m->ctx->stream, f,
+                       dn + (long)k * npix, (const double*)m->kl, (double)direction, saved,
+                       ddn ? ddn + (long)k * npix : nullptr, npix, m->batch);
+    FH_HIP(hipGetLastError());
+    if ((rc = fh_propagate(k > 0 ? m->pfull : m->phalf, f, -direction))) return rc;
+  }
+  return FH_OK;
+}
+
 }  // namespace fh
 
 extern "C" {
@@ -2674,8 +2779,7 @@ int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* d, double thickness, int n
                        d->tilts_y, d->ntilts);
   if (rc) return rc;
   const int lx = ilog2_exact(nx), ly = ilog2_exact(ny);
-  if (lx < kMinLog || lx > kMaxLog || ly < kMinLog || ly > kMaxLog)
-    return fail(FH_E_UNSUPP, "BPM needs power-of-two nx, ny in [%d, %d] (got %d x %d)", 1 << kMinLog, 1 << kMaxLog, nx, ny);
+  const bool pow2 = !(lx < kMinLog || lx > kMaxLog || ly < kMinLog || ly > kMaxLog);
   if ((rc = set_device(ctx))) return rc;
   fh_bpm* m = new fh_bpm();
   m->ctx = ctx; m->nx = nx; m->ny = ny; m->batch = batch; m->nslices = nslices; m->lx = lx; m->ly = ly;
@@ -2683,6 +2787,37 @@ int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* d, double thickness, int n
   m->thickness = thickness; m->n0 = n0; m->dz = thickness / nslices;
   m->lam_tables = tmp.lam_tables; m->tab_tx = tmp.tab_tx; m->tab_ty = tmp.tab_ty;
   m->field_table = tmp.field_table;
+  if (!pow2) {
+    // other grids: general plans for P(dz) and P(dz/2) (the padded convolution form) and an
+    // elementwise slice phase
+    m->generic = 1;
+    fh_prop_desc pd = *d;
+    double zf = m->dz, zh = m->dz / 2;
+    pd.n0 = n0; pd.nz = 1;
+    pd.z = &zf;
+    if ((rc = fh_asprop_create_ex(ctx, &pd, &m->pfull))) { bpm_free(m); return rc; }
+    pd.z = &zh;
+    if ((rc = fh_asprop_create_ex(ctx, &pd, &m->phalf))) { bpm_free(m); return rc; }
+    m->nck = (nslices + m->every - 1) / m->every;
+    const size_t fsz = (size_t)nx * ny * batch * sizeof(float2);
+    if (hipMalloc(&m->kl, batch * sizeof(double)) != hipSuccess || hipMalloc(&m->ckpt, m->nck * fsz) != hipSuccess ||
+        hipMalloc(&m->seg, m->every * fsz) != hipSuccess || hipMalloc(&m->work, fsz) != hipSuccess) {
+      bpm_free(m);
+      return fail(FH_E_NOMEM, "BPM: out of device memory");
+    }
+    std::vector<double> kl(batch);
+    for (int b = 0; b < batch; ++b) {
+      const int t = m->field_table[b];
+      kl[b] = 2.0 * M_PI * m->dz / (cos(m->tab_tx[t]) * cos(m->tab_ty[t])) / m->lam_tables[t];
+    }
+    if (hipMemcpy(m->kl, kl.data(), batch * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      bpm_free(m);
+      return fail(FH_E_HIP, "BPM: set-up failed");
+    }
+    m->bytes = (m->nck + m->every + 1) * fsz;
+    *out = m;
+    return FH_OK;
+  }
   // P is ASProp(dz; n0, paraxial) or ShiftProp(dz) (bpm.jl:54-73,140-142,207-217); the
   // kernel choice of angular_spectrum.jl:126-139
   const bool tilted = plan_tilted(&tmp);
@@ -2759,6 +2894,7 @@ int fh_bpm_forward(fh_bpm* m, void* u, const float* dn, int direction, int save)
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(m->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(u);
+  if (m->generic) return bpm_generic_forward(m, f, dn, direction, save);
   const long npix = (long)m->nx * m->ny;
   const long fsz = npix * m->batch;
   const double sgn = direction;
@@ -2783,6 +2919,7 @@ int fh_bpm_backward(fh_bpm* m, void* du, const float* dn, float* ddn, int direct
   if (ddn && m->saved_dir != direction)
     return fail(FH_E_ARG, "the dn gradient needs a saving fh_bpm_forward in the same direction first");
   float2* f = reinterpret_cast<float2*>(du);
+  if (m->generic) return bpm_generic_backward(m, f, dn, ddn, direction);
   const long npix = (long)m->nx * m->ny;
   const long fsz = npix * m->batch;
   const double sgn = direction;
