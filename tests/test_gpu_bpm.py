@@ -81,9 +81,9 @@ def test_bpm_checkpointing_matches_full_save():
         assert np.array_equal(d, grads[0][1])
 
 
-def test_bpm_config4_graded_index_volume():
+def test_bpm_graded_index_volume_512x400():
     """512^2, ds=0.5, lam=1, n_bulk=1.5, 400 slices of dz=1, checkpoint every 20."""
-    print("[bpm config4] 512^2 x 400 slices", flush=True)
+    print("[bpm graded-index volume] 512^2 x 400 slices", flush=True)
     n, ns, ds, lam, n0 = 512, 400, (0.5, 0.5), 1.0, 1.5
     dn = smooth_dn(ns, n, 3.0, 0.01, 4)
     u0 = c64(tilted_gaussian(n, 0.5, lam, 80.0, 2.0))
@@ -103,7 +103,7 @@ def test_bpm_config4_graded_index_volume():
     torch.cuda.synchronize()
     eg = rel_l2(bpm.grad.cpu().numpy(), ddnref)
     ws = bpm.workspace_bytes(u)
-    print(f"config4: field {ev:.2e} adjoint {edu:.2e} ddn {eg:.2e}; library workspace {ws / 2**20:.1f} MiB "
+    print(f"512^2 x 400 volume: field {ev:.2e} adjoint {edu:.2e} ddn {eg:.2e}; library workspace {ws / 2**20:.1f} MiB "
           f"(a full save of every slice input is {ns * n * n * 8 / 2**20:.0f} MiB)")
     assert ev <= 1e-5 and edu <= 1e-5 and eg <= 1e-4
 

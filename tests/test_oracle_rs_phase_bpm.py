@@ -142,7 +142,7 @@ def test_phase_gradient_finite_difference():
     _fd_check(loss, phi, [int(i) for i in idx], grads[0])
 
 
-def test_config1_style_pipeline_gradient():
+def test_rs_phase_cascade_pipeline_gradient():
     """RS -> Phase -> RS -> Phase -> RS with sum (|v|^2 - I)^2, both mask gradients."""
     n, ds, lam = 20, (1.0, 1.0), 1.064
     u = O.normalize_power(inputs.gaussian(n, n, 1.0, 1.0, 3.0), ds)
