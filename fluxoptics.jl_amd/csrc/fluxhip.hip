@@ -2563,6 +2563,11 @@ struct fh_bpm {
   int generic = 0;
   fh_plan* pfull = nullptr;
   fh_plan* phalf = nullptr;
+  // the dn adjoint recomputes segment s-1 on a second stream while segment s's gradient steps
+  // run on the caller's (double-buffered segments, event-ordered)
+  float2* seg2 = nullptr;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
 };
 
 namespace fh {
@@ -2582,6 +2587,10 @@ static void bpm_free(fh_bpm* m) {
     if (q) (void)hipFree(q);
   plan_free(m->pfull);
   plan_free(m->phalf);
+  if (m->seg2) (void)hipFree(m->seg2);
+  for (hipEvent_t e : {m->ev_fork, m->ev_rec[0], m->ev_rec[1], m->ev_used[0], m->ev_used[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (m->side) (void)hipStreamDestroy(m->side);
   delete m;
 }
 
@@ -2684,14 +2693,14 @@ static int bpm_step_grad(fh_bpm* m, float2* f, const float* dn_k, double sgn, co
 
 // Slice inputs k0 .. k1-1 into seg[0 ..], replayed from the x-spectrum checkpoint of slice
 // k0 with the forward's own kernels, so they equal the forward's values bit for bit.
-static int bpm_recompute(fh_bpm* m, const float* dn, int k0, int k1, double sgn, int direction) {
+static int bpm_recompute(fh_bpm* m, const float* dn, int k0, int k1, double sgn, int direction, float2* seg) {
   const long npix = (long)m->nx * m->ny;
   const long fsz = npix * m->batch;
   FH_HIP(hipMemcpyAsync(m->work, m->ckpt + (long)(k0 / m->every) * fsz, fsz * sizeof(float2),
                         hipMemcpyDeviceToDevice, m->ctx->stream));
   int rc;
   for (int k = k0; k < k1; ++k) {
-    if ((rc = bpm_step(m, m->work, dn + (long)k * npix, sgn, m->seg + (long)(k - k0) * fsz, nullptr))) return rc;
+    if ((rc = bpm_step(m, m->work, dn + (long)k * npix, sgn, seg + (long)(k - k0) * fsz, nullptr))) return rc;
     if (k + 1 < k1 && (rc = bpm_ypass(m, m->work, m->tfull, direction))) return rc;
   }
   return FH_OK;
@@ -2835,9 +2844,18 @@ int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* d, double thickness, int n
       hipMalloc(&m->d_tabidx, batch * sizeof(int)) != hipSuccess ||
       hipMalloc(&m->kl, batch * sizeof(double)) != hipSuccess ||
       hipMalloc(&m->ckpt, m->nck * fsz) != hipSuccess || hipMalloc(&m->seg, m->every * fsz) != hipSuccess ||
-      hipMalloc(&m->work, fsz) != hipSuccess) {
+      hipMalloc(&m->seg2, m->every * fsz) != hipSuccess || hipMalloc(&m->work, fsz) != hipSuccess) {
     bpm_free(m);
     return fail(FH_E_NOMEM, "BPM: out of device memory");
+  }
+  if (hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev_rec[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev_rec[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev_used[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev_used[1], hipEventDisableTiming) != hipSuccess) {
+    bpm_free(m);
+    return fail(FH_E_HIP, "BPM: stream or event creation failed");
   }
   const double scale = 1.0 / ((double)nx * (double)ny);
   for (size_t k = 0; k < ntab; ++k) {
@@ -2862,7 +2880,7 @@ int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* d, double thickness, int n
     bpm_free(m);
     return fail(FH_E_HIP, "BPM: table set-up failed");
   }
-  m->bytes = (m->nck + m->every + 1) * fsz + 2 * tbytes;
+  m->bytes = (m->nck + 2 * m->every + 1) * fsz + 2 * tbytes;
   *out = m;
   return FH_OK;
 }
@@ -2925,21 +2943,51 @@ int fh_bpm_backward(fh_bpm* m, void* du, const float* dn, float* ddn, int direct
   const double sgn = direction;
   // reverse loop of bpm.jl:289-312: adjoint of the trailing P(dz/2), then per slice
   // [IFFT_x, un-apply phase (+ gradient), FFT_x] + adjoint y pass, then IFFT_x
+  if (!ddn) {
+    if ((rc = bpm_xfft(m, f, nullptr, sgn))) return rc;
+    if ((rc = bpm_ypass(m, f, m->thalf, -direction))) return rc;
+    for (int k = m->nslices - 1; k >= 0; --k) {
+      if ((rc = bpm_step(m, f, dn + (long)k * npix, -sgn, nullptr, nullptr))) return rc;
+      if ((rc = bpm_ypass(m, f, k > 0 ? m->tfull : m->thalf, -direction))) return rc;
+    }
+    return bpm_xifft(m, f);
+  }
+  // with the gradient: segment s's slice inputs are recomputed on the second stream into
+  // buffer s % 2 while the caller's stream runs segment s+1's gradient steps from the other
+  // buffer; events order each recompute after the forward's checkpoints (fork) and after the
+  // last use of its buffer, and each segment's steps after its recompute.  Every side-stream
+  // operation is waited on by the caller's stream, so the work also records into one graph.
+  hipStream_t main = m->ctx->stream;
+  float2* bufs[2] = {m->seg, m->seg2};
+  const int nseg = m->nck;
+  FH_HIP(hipEventRecord(m->ev_fork, main));
+  FH_HIP(hipStreamWaitEvent(m->side, m->ev_fork, 0));
+  auto recompute = [&](int sg) -> int {
+    const int k0 = sg * m->every, k1 = std::min(k0 + m->every, m->nslices);
+    m->ctx->stream = m->side;
+    const int r = bpm_recompute(m, dn, k0, k1, sgn, direction, bufs[sg % 2]);
+    m->ctx->stream = main;
+    if (r) return r;
+    FH_HIP(hipEventRecord(m->ev_rec[sg % 2], m->side));
+    return FH_OK;
+  };
+  if ((rc = recompute(nseg - 1))) return rc;
   if ((rc = bpm_xfft(m, f, nullptr, sgn))) return rc;
   if ((rc = bpm_ypass(m, f, m->thalf, -direction))) return rc;
-  for (int k = m->nslices - 1; k >= 0; --k) {
-    if (ddn) {
-      const int k0 = (k / m->every) * m->every;
-      if (k == m->nslices - 1 || k % m->every == m->every - 1) {
-        const int k1 = std::min(k0 + m->every, m->nslices);
-        if ((rc = bpm_recompute(m, dn, k0, k1, sgn, direction))) return rc;
-      }
-      if ((rc = bpm_step_grad(m, f, dn + (long)k * npix, sgn, m->seg + (long)(k - k0) * fsz, ddn + (long)k * npix)))
-        return rc;
-    } else {
-      if ((rc = bpm_step(m, f, dn + (long)k * npix, -sgn, nullptr, nullptr))) return rc;
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    if (sg >= 1) {
+      if (sg + 1 < nseg) FH_HIP(hipStreamWaitEvent(m->side, m->ev_used[(sg + 1) % 2], 0));
+      if ((rc = recompute(sg - 1))) return rc;
     }
-    if ((rc = bpm_ypass(m, f, k > 0 ? m->tfull : m->thalf, -direction))) return rc;
+    FH_HIP(hipStreamWaitEvent(main, m->ev_rec[sg % 2], 0));
+    const int k0 = sg * m->every, k1 = std::min(k0 + m->every, m->nslices);
+    for (int k = k1 - 1; k >= k0; --k) {
+      if ((rc = bpm_step_grad(m, f, dn + (long)k * npix, sgn, bufs[sg % 2] + (long)(k - k0) * fsz,
+                              ddn + (long)k * npix)))
+        return rc;
+      if ((rc = bpm_ypass(m, f, k > 0 ? m->tfull : m->thalf, -direction))) return rc;
+    }
+    FH_HIP(hipEventRecord(m->ev_used[sg % 2], main));
   }
   return bpm_xifft(m, f);
 }
