@@ -495,7 +495,7 @@ __device__ __forceinline__ void ypipe_loadh(float2* h, const QuadTab& qt, int b,
 
 // One field: FFT with cur's registers, multiply by h, IFFT, store; the caller has already
 // issued the loads of the next field (data and transfer function) into other registers.
-template <typename RDF, bool OUT_HALF>
+template <typename RDF, bool OUT_HALF, int HS = 1>
 __device__ __forceinline__ void ypipe_field(float2* v, const float2* h, float2* s, int t, bool live, int b,
                                             const FastIO& io, const PlanBases<RDF>& pf,
                                             const PlanBases<typename RDF::Rev>& pi, int conj, int vout,
@@ -506,7 +506,7 @@ __device__ __forceinline__ void ypipe_field(float2* v, const float2* h, float2* 
   constexpr int RE = LastRadix<RI>::value;
   fast_fft_b<RDF, false>(v, s, t, pf);
 #pragma unroll
-  for (int k = 0; k < V; ++k) v[k] = conj ? cmulc(v[k], h[k]) : cmul(v[k], h[k]);
+  for (int k = 0; k < V; ++k) v[k] = conj ? cmulc(v[k], h[k * HS]) : cmul(v[k], h[k * HS]);
   fast_fft_b<RI, true>(v, s, t, pi);
   // Loads and stores share the vector-memory counter, and a wait for a load with stores
   // still pending drains everything.  Wait here, before this field's stores, while every
@@ -530,7 +530,7 @@ __device__ __forceinline__ void ypipe_field(float2* v, const float2* h, float2* 
 // register set (ping-pong over two unrolled iterations).  ONE_TAB: every field uses the same
 // transfer function (one wavelength), so its values are loaded once and kept in registers;
 // otherwise they are prefetched with the data.
-template <typename RDF, int L, bool IN_HALF, bool OUT_HALF, bool ONE_TAB>
+template <typename RDF, int L, bool IN_HALF, bool OUT_HALF, bool ONE_TAB, bool H_LDS = false>
 __global__ void __launch_bounds__(L * RDF::P)
 k_fast_ypipe(FastIO io, int batch, const float2* __restrict__ tabf, const float2* __restrict__ tabi,
              QuadTab qt) {
@@ -562,7 +562,21 @@ k_fast_ypipe(FastIO io, int batch, const float2* __restrict__ tabf, const float2
   // The prefetches are unconditional (past the last field they re-read field batch-1, which
   // is harmless): a load under a branch makes the wait-count pass drain every outstanding
   // load at the join, which would serialise the prefetch with the arithmetic.
-  if constexpr (ONE_TAB) {
+  if constexpr (ONE_TAB && H_LDS) {
+    // the shared transfer function lives in LDS after the exchange areas, one slot per
+    // (value, thread): each thread reads back only what it wrote, so no barrier is needed
+    constexpr int LP = L * P;
+    float2* hs = smem + L * LdsLine<n, L>::size + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < V; ++k) hs[k * LP] = ha[k];
+    for (int b = 0; b < batch; b += 2) {
+      yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b + 1 < batch ? b + 1 : batch - 1, vin, kin, t);
+      ypipe_field<RDF, OUT_HALF, LP>(va, hs, s, t, live, b, io, pf, pi, qt.conj, vout, kout);
+      if (b + 1 >= batch) break;
+      yconv_load<V, RDF::R0, n, P, IN_HALF>(va, io, b + 2 < batch ? b + 2 : batch - 1, vin, kin, t);
+      ypipe_field<RDF, OUT_HALF, LP>(vb, hs, s, t, live, b + 1, io, pf, pi, qt.conj, vout, kout);
+    }
+  } else if constexpr (ONE_TAB) {
     for (int b = 0; b < batch; b += 2) {
       yconv_load<V, RDF::R0, n, P, IN_HALF>(vb, io, b + 1 < batch ? b + 1 : batch - 1, vin, kin, t);
       ypipe_field<RDF, OUT_HALF>(va, ha, s, t, live, b, io, pf, pi, qt.conj, vout, kout);

@@ -1371,7 +1371,7 @@ static const FastConvVariant kFast6[] = {{"8x8 L16", fast_conv_v<8, 8, 1, 16>}};
 static const FastConvVariant kFast7[] = {{"16x8 L16", fast_conv_v<16, 8, 1, 16>}};
 static const FastConvVariant kFast8[] = {{"16x16 L16", fast_conv_v<16, 16, 1, 16>},
                                          {"16x16 L8", fast_conv_v<16, 16, 1, 8>}};
-template <int A, int B, int C_, int L>
+template <int A, int B, int C_, int L, bool H_LDS = false>
 static int fast_pipe_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool half) {
   using RDF = Radices<A, B, C_>;
   using RI = typename RDF::Rev;
@@ -1380,17 +1380,21 @@ static int fast_pipe_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool 
   if ((rc = get_stage_tw(c, RDF::n, RDF::R0, RDF::R1, RDF::R2, false, &tf)) ||
       (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
     return rc;
-  const size_t lds = (size_t)L * LdsLine<RDF::n, L>::size * sizeof(float2);
+  const size_t lds0 = (size_t)L * LdsLine<RDF::n, L>::size * sizeof(float2);
   QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab, op.full,
              op.full ? op.ny / 2 : 0};
   dim3 grid((io.nlines + L - 1) / L);
-  auto launch = [&](auto kern) -> int {
+  auto launch = [&](auto kern, size_t lds) -> int {
     FH_HIP(ensure_lds(kern, lds));
     hipLaunchKernelGGL(kern, grid, dim3(L * RDF::P), lds, c->stream, io, batch, tf, ti, qt);
     return FH_OK;
   };
-  if (half) rc = qt.one_tab ? launch(k_fast_ypipe<RDF, L, true, true, true>) : launch(k_fast_ypipe<RDF, L, true, true, false>);
-  else rc = qt.one_tab ? launch(k_fast_ypipe<RDF, L, false, false, true>) : launch(k_fast_ypipe<RDF, L, false, false, false>);
+  // the LDS-resident transfer function (H_LDS) applies to one-table batches only
+  const size_t ldsh = lds0 + (H_LDS ? (size_t)L * RDF::P * RDF::V * sizeof(float2) : 0);
+  if (half) rc = qt.one_tab ? launch(k_fast_ypipe<RDF, L, true, true, true, H_LDS>, ldsh)
+                            : launch(k_fast_ypipe<RDF, L, true, true, false>, lds0);
+  else rc = qt.one_tab ? launch(k_fast_ypipe<RDF, L, false, false, true, H_LDS>, ldsh)
+                       : launch(k_fast_ypipe<RDF, L, false, false, false>, lds0);
   if (rc) return rc;
   FH_HIP(hipGetLastError());
   return FH_OK;
@@ -1408,14 +1412,15 @@ static const FastConvVariant kFast11[] = {{"8x16x16 L4 f1", fast_conv_v<8, 16, 1
                                           {"pipe 8x16x16 L4", fast_pipe_v<8, 16, 16, 4>},
                                           {"pipe 16x16x8 L4", fast_pipe_v<16, 16, 8, 4>},
                                           {"16x16x8 L4 f1", fast_conv_v<16, 16, 8, 4, 1>},
-                                          {"16x16x8 L4 f2", fast_conv_v<16, 16, 8, 4, 2>}};
+                                          {"16x16x8 L4 f2", fast_conv_v<16, 16, 8, 4, 2>},
+                                          {"pipe 16x16x8 L4 hlds", fast_pipe_v<16, 16, 8, 4, true>}};
 static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16, 16, 4, 1>},
                                           {"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
                                           {"16x16x16 L2 f1", fast_conv_v<16, 16, 16, 2, 1>}};
 
 // default variant index for a y length and whether the batch shares one transfer function
 static int default_conv_variant(int logn, bool one_tab) {
-  if (one_tab && logn == 11) return 2;  // pipe 16x16x8 L4
+  if (one_tab && logn == 11) return 5;  // pipe 16x16x8 L4, transfer function in LDS
   if (logn == 12) return 1;             // 16x16x16 L4
   return 0;
 }
@@ -1429,7 +1434,7 @@ static const FastConvVariant* fast_conv_variants(int logn, int* count) {
     case 8: *count = 2; return kFast8;
     case 9: *count = 3; return kFast9;
     case 10: *count = 4; return kFast10;
-    case 11: *count = 5; return kFast11;
+    case 11: *count = 6; return kFast11;
     case 12: *count = 3; return kFast12;
     default: *count = 0; return nullptr;
   }
