@@ -492,14 +492,18 @@ __global__ void k_czt_chirp(float2* __restrict__ out, int n, CztDim d, int input
 
 // Table of an FP64 spectrum [ny2][nx2] on the padded grid, scaled and rounded: the even
 // quadrant [0..nx2/2]x[0..ny2/2] or the full layout (see tab_freq_index).
-__global__ void k_tab_extract(float2* __restrict__ tab, const double2* __restrict__ d, int nx2, int ny2,
+template <typename T>
+__global__ void k_tab_extract(T* __restrict__ tab, const double2* __restrict__ d, int nx2, int ny2,
                               int rows, int qcol, int full, double scale) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= rows || j >= qcol) return;
   const int ky = (full && j > ny2 / 2) ? 3 * ny2 / 2 - j : j;
   const double2 v = d[(long)ky * nx2 + i];
-  tab[(long)i * qcol + j] = make_float2((float)(v.x * scale), (float)(v.y * scale));
+  T r;
+  r.x = v.x * scale;
+  r.y = v.y * scale;
+  tab[(long)i * qcol + j] = r;
 }
 
 // ================================================================ elementwise kernels
@@ -1100,6 +1104,7 @@ struct fh_plan {
   int* d_tabidx = nullptr;
   float2* scratch = nullptr;       // RS: (2nx) x ny x batch
   double2* plain_tab = nullptr;    // plain FP64 comparator (AS): full tables
+  double2* tab_d = nullptr;        // padded AS, on request: the tables before rounding to c64
   int conv_var = -1;               // y-pass variant (-1 = built-in default)
   int xf_var = -1, xi_var = -1;    // x-pass variants (forward, inverse)
   float2* chirp = nullptr;         // Collins: A_x, A_y, D_x, D_y chirp lines per table
@@ -1502,6 +1507,7 @@ static void plan_free(fh_plan* p) {
   if (p->d_tabidx) (void)hipFree(p->d_tabidx);
   if (p->scratch) (void)hipFree(p->scratch);
   if (p->plain_tab) (void)hipFree(p->plain_tab);
+  if (p->tab_d) (void)hipFree(p->tab_d);
   if (p->chirp) (void)hipFree(p->chirp);
   delete p;
 }
@@ -1672,7 +1678,9 @@ int fh_ctx_destroy(fh_ctx* ctx) {
 // Tables of a padded ASProp plan (grids that are not powers of two): per key, H on the
 // n-point grid, its inverse 2-D DFT h (FP64, direct sums), h extended periodically onto the
 // padded grid, the padded FFT (FP64), rounded once; the plan then runs the padded passes.
-static int as_padded_tables(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p) {
+// fp64: also keep the tables unrounded in p->tab_d (FP64-compute users such as the general
+// AS_BPM slice steps, whose hundreds of steps would otherwise repeat the same rounding).
+static int as_padded_tables(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p, bool fp64) {
   const int nx = d->nx, ny = d->ny;
   const int nx2 = 1 << p->lx, ny2 = 1 << p->ly;
   p->kind = 1;
@@ -1681,7 +1689,8 @@ static int as_padded_tables(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p) {
   p->tabsize = (long)p->qx * p->qy;
   const long nt = (long)p->lam_tables.size();
   if (hipMalloc(&p->tab, p->tabsize * nt * sizeof(float2)) != hipSuccess ||
-      hipMalloc(&p->scratch, (size_t)nx2 * mid_rows(p) * d->batch * sizeof(float2)) != hipSuccess)
+      hipMalloc(&p->scratch, (size_t)nx2 * mid_rows(p) * d->batch * sizeof(float2)) != hipSuccess ||
+      (fp64 && hipMalloc(&p->tab_d, p->tabsize * nt * sizeof(double2)) != hipSuccess))
     return fail(FH_E_NOMEM, "ASProp tables and workspace: out of device memory");
   double2 *hn = nullptr, *tmp = nullptr, *wx = nullptr, *wy = nullptr, *dbuf = nullptr;
   int rc = FH_OK;
@@ -1705,9 +1714,13 @@ static int as_padded_tables(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p) {
                        (const double2*)hn, nx, ny, nx2, ny2);
     if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "ASProp padded table kernels failed to launch"); break; }
     if ((rc = fft2_d(ctx, dbuf, nx2, ny2, p->lx, p->ly))) break;
-    hipLaunchKernelGGL(k_tab_extract, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_tab_extract<float2>, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
                        p->tab + k * p->tabsize, (const double2*)dbuf, nx2, ny2, p->qx, p->qy, p->full,
                        1.0 / ((double)nx2 * (double)ny2));
+    if (fp64)
+      hipLaunchKernelGGL(k_tab_extract<double2>, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
+                         p->tab_d + k * p->tabsize, (const double2*)dbuf, nx2, ny2, p->qx, p->qy, p->full,
+                         1.0 / ((double)nx2 * (double)ny2));
     if (hipGetLastError() != hipSuccess) rc = fail(FH_E_HIP, "ASProp table kernel launch failed");
   }
   if (rc == FH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(FH_E_HIP, "ASProp table generation failed");
@@ -1717,7 +1730,7 @@ static int as_padded_tables(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p) {
   return rc;
 }
 
-int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
+static int asprop_create(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out, bool fp64_tables) {
   if (!out || !d) return fail(FH_E_ARG, "null descriptor or output pointer");
   *out = nullptr;
   if (d->flags & ~(FH_PROP_PARAXIAL | FH_PROP_TRACK_TILTS | FH_PROP_SHIFT)) return fail(FH_E_ARG, "unknown flags 0x%x", d->flags);
@@ -1750,7 +1763,7 @@ int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   p->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
   p->full = tilted ? 1 : 0;
   if (!pow2) {
-    if ((rc = as_padded_tables(ctx, d, p))) { plan_free(p); return rc; }
+    if ((rc = as_padded_tables(ctx, d, p, fp64_tables))) { plan_free(p); return rc; }
     *out = p;
     return FH_OK;
   }
@@ -1770,6 +1783,10 @@ int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   if ((rc = upload_tabidx(p))) { plan_free(p); return rc; }
   *out = p;
   return FH_OK;
+}
+
+int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
+  return asprop_create(ctx, d, out, false);
 }
 
 int fh_asprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
@@ -1833,7 +1850,7 @@ int fh_rsprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
                        d->dx, d->dy, p->lam_tables[k], p->tab_z[k], nrm, p->tab_tx[k], p->tab_ty[k], tilt);
     if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "RS sample kernel launch failed"); break; }
     if ((rc = fft2_d(ctx, dd, nx2, ny2, p->lx, p->ly))) break;
-    hipLaunchKernelGGL(k_tab_extract, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_tab_extract<float2>, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
                        p->tab + k * p->tabsize, dd, nx2, ny2, p->qx, p->qy, p->full, scale);
     if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "RS table kernel launch failed"); break; }
   }
@@ -1899,7 +1916,7 @@ int fh_collins_create(fh_ctx* ctx, const fh_prop_desc* d, double dx_out, double 
                        d->dx, d->dy, lam, ax, ay, b, 0.0, nrm_im);
     if (hipGetLastError() != hipSuccess) { rc = fail(FH_E_HIP, "Collins sample kernel launch failed"); break; }
     if ((rc = fft2_d(ctx, dbuf, nx2, ny2, p->lx, p->ly))) break;
-    hipLaunchKernelGGL(k_tab_extract, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_tab_extract<float2>, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream,
                        p->tab + k * p->tabsize, dbuf, nx2, ny2, p->qx, p->qy, 0, scale);
     // chirps (collins_integral.jl:5-36) on the input grid, in cycles: A = cis(pi*x^2*(a -
     // alpha)/(b*lam))/lam, D = cis(pi*x^2*alpha*(d*alpha - 1)/(b*lam)); a tilted field whose
@@ -1972,7 +1989,7 @@ int fh_czt_create(fh_ctx* ctx, int nx, int ny, int batch, int dims, const double
   if (hipGetLastError() != hipSuccess) rc = fail(FH_E_HIP, "CZT sample kernel launch failed");
   if (rc == FH_OK) rc = fft2_d(ctx, dbuf, nx2, ny2, p->lx, p->ly);
   if (rc == FH_OK) {
-    hipLaunchKernelGGL(k_tab_extract, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream, p->tab, dbuf,
+    hipLaunchKernelGGL(k_tab_extract<float2>, dim3((p->qx + 255) / 256, p->qy), dim3(256), 0, ctx->stream, p->tab, dbuf,
                        nx2, ny2, p->qx, p->qy, 0, 1.0 / ((double)nx2 * (double)ny2));
     // chirp lines in the plan's A_x, A_y (input) and D_x, D_y (output) slots
     float2* cax = p->chirp;
@@ -2568,6 +2585,7 @@ struct fh_bpm {
   int generic = 0;
   fh_plan* pfull = nullptr;
   fh_plan* phalf = nullptr;
+  float* gpart = nullptr;    // [B][ny][nx] per-field dn gradient terms (batches of several fields)
   // the dn adjoint recomputes segment s-1 on a second stream while segment s's gradient steps
   // run on the caller's (double-buffered segments, event-ordered)
   float2* seg2 = nullptr;
@@ -2592,6 +2610,7 @@ static void bpm_free(fh_bpm* m) {
     if (q) (void)hipFree(q);
   plan_free(m->pfull);
   plan_free(m->phalf);
+  if (m->gpart) (void)hipFree(m->gpart);
   if (m->seg2) (void)hipFree(m->seg2);
   for (hipEvent_t e : {m->ev_fork, m->ev_rec[0], m->ev_rec[1], m->ev_used[0], m->ev_used[1]})
     if (e) (void)hipEventDestroy(e);
@@ -2616,27 +2635,39 @@ __global__ void k_bpm_phase(float2* __restrict__ u, const float* __restrict__ dn
   }
 }
 
-// Its adjoint with the dn gradient (bpm.jl:275-312): du_b *= cis(-s kl_b dn), then
-// ddn = s sum_b kl_b Im(du_b conj(saved_b)), summed over the batch in field order.
-__global__ void k_bpm_phase_grad(float2* __restrict__ du, const float* __restrict__ dn, const double* __restrict__ kl,
-                                 double s, const float2* __restrict__ saved, float* __restrict__ ddn, long npix,
-                                 int batch) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
-    const double d = dn[i];
-    double g = 0.0;
-    for (int b = 0; b < batch; ++b) {
-      const long k = b * npix + i;
-      const float2 v = du[k];
-      double sn, cs;
-      sincos(-s * kl[b] * d, &sn, &cs);
-      const float2 w = make_float2((float)(v.x * cs - v.y * sn), (float)(v.x * sn + v.y * cs));
-      du[k] = w;
-      if (ddn) {
-        const float2 sv = saved[k];
-        g += kl[b] * ((double)w.y * sv.x - (double)w.x * sv.y);  // Im(w conj(sv))
-      }
+// Its adjoint (bpm.jl:275-312), as the epilogue of the generic BPM's adjoint x IFFT: du_b *=
+// cis(-s kl_b dn), and with saved set, part[b] = s kl_b Im(du_b conj(saved_b)), the field's term of
+// the slice's dn gradient.  Both come from the FP64 transform values: the product cancels where
+// du and the slice input are nearly in phase, so it is formed before du is rounded to c64.
+struct DnPhaseGradOp {
+  const float* dn;      // [ny][nx] slice
+  const double* kl;     // [B]
+  const float2* saved;  // [B][ny][nx] slice inputs (nullable: no gradient)
+  float* part;          // [B][ny][nx] gradient terms (the slice's gradient itself for one field)
+  long npix;
+  double s;
+  int nx;
+  template <typename C>
+  __device__ __forceinline__ C operator()(C v, int b, int y, int x) const {
+    if (x >= nx) return v;
+    const long i = (long)y * nx + x;
+    double sn, cs;
+    sincos(-s * kl[b] * (double)dn[i], &sn, &cs);
+    const double wx = v.x * cs - v.y * sn, wy = v.x * sn + v.y * cs;
+    if (saved) {
+      const float2 sv = saved[b * npix + i];
+      part[b * npix + i] = (float)(s * kl[b] * (wy * sv.x - wx * sv.y));
     }
-    if (ddn) ddn[i] = (float)(s * g);
+    return cmake<C>(wx, wy);
+  }
+};
+
+// out[i] = sum over fields of part[b][i], in field order (the batch sum of the dn gradient)
+__global__ void k_sum_fields(const float* __restrict__ part, float* __restrict__ out, long npix, int batch) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
+    double g = 0.0;
+    for (int b = 0; b < batch; ++b) g += part[b * npix + i];
+    out[i] = (float)g;
   }
 }
 
@@ -2719,31 +2750,70 @@ static int bpm_phase(fh_bpm* m, float2* f, const float* dn_k, double sgn, float2
   return FH_OK;
 }
 
+// Slice propagator of the generic BPM: the padded convolution of a general ASProp plan (zero
+// padding to the plan's power-of-two lengths, FFT, transfer function, IFFT, crop) computed in FP64
+// with the plan's unrounded tables.  Hundreds of near-identity steps would repeat one FP32
+// rounding of the butterflies and tables coherently; the values stored between the three
+// launches are c64, as in the power-of-two slice steps.  p->scratch holds the x-spectrum rows,
+// [B][ny][nx2].  gop (nullable) is the epilogue of the last launch: a slice phase un-applied in
+// the adjoint, with its gradient terms.
+static int bpm_generic_prop(fh_bpm* m, float2* f, const fh_plan* p, int direction,
+                            const DnPhaseGradOp* gop = nullptr) {
+  const int nx2 = 1 << p->lx, ny2 = 1 << p->ly;
+  const double2 *twx = nullptr, *twy = nullptr;
+  int rc;
+  if ((rc = get_twiddles(m->ctx, nx2, &twx)) || (rc = get_twiddles(m->ctx, ny2, &twy))) return rc;
+  LineIO<float2> xf{};
+  xf.in = f; xf.out = p->scratch;
+  xf.in_f = (long)m->nx * m->ny; xf.out_f = (long)nx2 * m->ny;
+  xf.in_l = m->nx; xf.out_l = nx2; xf.in_e = xf.out_e = 1;
+  xf.nlines = m->ny; xf.in_valid = m->nx; xf.out_count = nx2;
+  FH_HIP((launch_fft<float2, double2, false, false>(p->lx, xf, m->batch, twx, NoOp{}, m->ctx->stream)));
+  LineIO<float2> yc{};
+  yc.in = p->scratch; yc.out = p->scratch;
+  yc.in_f = yc.out_f = (long)nx2 * m->ny;
+  yc.in_l = yc.out_l = 1; yc.in_e = yc.out_e = nx2;
+  yc.nlines = nx2; yc.in_valid = yc.out_count = m->ny;
+  QuadMulT<double2> q{p->tab_d, p->d_tabidx, p->tabsize, nx2, ny2, p->qy, direction < 0,
+                      p->lam_tables.size() == 1, p->full};
+  FH_HIP((launch_conv<float2, double2>(p->ly, yc, m->batch, twy, q, m->ctx->stream)));
+  LineIO<float2> xi{};
+  xi.in = p->scratch; xi.out = f;
+  xi.in_f = (long)nx2 * m->ny; xi.out_f = (long)m->nx * m->ny;
+  xi.in_l = nx2; xi.out_l = m->nx; xi.in_e = xi.out_e = 1;
+  xi.nlines = m->ny; xi.in_valid = nx2; xi.out_count = m->nx;
+  if (gop) FH_HIP((launch_fft<float2, double2, false, true>(p->lx, xi, m->batch, twx, *gop, m->ctx->stream)));
+  else FH_HIP((launch_fft<float2, double2, false, true>(p->lx, xi, m->batch, twx, NoOp{}, m->ctx->stream)));
+  return FH_OK;
+}
+
 // Generic forward (bpm.jl:244-260): P(dz/2); per slice [checkpoint], phase, P(dz) (P(dz/2) after
 // the last slice).  Checkpoint c holds the field entering slice c * every.
 static int bpm_generic_forward(fh_bpm* m, float2* f, const float* dn, int direction, int save) {
   const long npix = (long)m->nx * m->ny;
   const long fsz = npix * m->batch;
   int rc;
-  if ((rc = fh_propagate(m->phalf, f, direction))) return rc;
+  if ((rc = bpm_generic_prop(m, f, m->phalf, direction))) return rc;
   for (int k = 0; k < m->nslices; ++k) {
     if (save && k % m->every == 0)
       FH_HIP(hipMemcpyAsync(m->ckpt + (long)(k / m->every) * fsz, f, fsz * sizeof(float2), hipMemcpyDeviceToDevice,
                             m->ctx->stream));
     if ((rc = bpm_phase(m, f, dn + (long)k * npix, direction, nullptr))) return rc;
-    if ((rc = fh_propagate(k + 1 < m->nslices ? m->pfull : m->phalf, f, direction))) return rc;
+    if ((rc = bpm_generic_prop(m, f, k + 1 < m->nslices ? m->pfull : m->phalf, direction))) return rc;
   }
   m->saved_dir = save ? direction : 0;
   return FH_OK;
 }
 
-// Generic adjoint (bpm.jl:289-312) with the dn gradient; slice inputs are recomputed one segment
-// at a time from the checkpoints with the forward's own launches.
+// Generic adjoint (bpm.jl:289-312) with the dn gradient: the adjoint of the propagator before
+// slice k (P(dz/2) for the last slice, P(dz) otherwise) un-applies slice k's phase in its last
+// launch and writes the slice's gradient terms; a final P(dz/2) adjoint follows.  Slice inputs are
+// recomputed one segment at a time from the checkpoints with the forward's own launches, before
+// the propagation that needs them (one stream: the plans' scratch rows are reused in order).
 static int bpm_generic_backward(fh_bpm* m, float2* f, const float* dn, float* ddn, int direction) {
   const long npix = (long)m->nx * m->ny;
   const long fsz = npix * m->batch;
   int rc;
-  if ((rc = fh_propagate(m->phalf, f, -direction))) return rc;
   for (int k = m->nslices - 1; k >= 0; --k) {
     const float2* saved = nullptr;
     if (ddn) {
@@ -2754,18 +2824,22 @@ static int bpm_generic_backward(fh_bpm* m, float2* f, const float* dn, float* dd
                               hipMemcpyDeviceToDevice, m->ctx->stream));
         for (int j = k0; j < k1; ++j) {
           if ((rc = bpm_phase(m, m->work, dn + (long)j * npix, direction, m->seg + (long)(j - k0) * fsz))) return rc;
-          if (j + 1 < k1 && (rc = fh_propagate(m->pfull, m->work, direction))) return rc;
+          if (j + 1 < k1 && (rc = bpm_generic_prop(m, m->work, m->pfull, direction))) return rc;
         }
       }
       saved = m->seg + (long)(k - k0) * fsz;
     }
-    hipLaunchKernelGGL(k_bpm_phase_grad, dim3(elementwise_blocks(npix)), dim3(256), 0, m->ctx->stream, f,
-                       dn + (long)k * npix, (const double*)m->kl, (double)direction, saved,
-                       ddn ? ddn + (long)k * npix : nullptr, npix, m->batch);
-    FH_HIP(hipGetLastError());
-    if ((rc = fh_propagate(k > 0 ? m->pfull : m->phalf, f, -direction))) return rc;
+    float* ddn_k = ddn ? ddn + (long)k * npix : nullptr;
+    const DnPhaseGradOp gop{dn + (long)k * npix, m->kl, saved, m->batch > 1 ? m->gpart : ddn_k, npix,
+                            (double)direction, m->nx};
+    if ((rc = bpm_generic_prop(m, f, k == m->nslices - 1 ? m->phalf : m->pfull, -direction, &gop))) return rc;
+    if (ddn && m->batch > 1) {
+      hipLaunchKernelGGL(k_sum_fields, dim3(elementwise_blocks(npix)), dim3(256), 0, m->ctx->stream,
+                         (const float*)m->gpart, ddn_k, npix, m->batch);
+      FH_HIP(hipGetLastError());
+    }
   }
-  return FH_OK;
+  return bpm_generic_prop(m, f, m->phalf, -direction);
 }
 
 }  // namespace fh
@@ -2802,20 +2876,22 @@ int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* d, double thickness, int n
   m->lam_tables = tmp.lam_tables; m->tab_tx = tmp.tab_tx; m->tab_ty = tmp.tab_ty;
   m->field_table = tmp.field_table;
   if (!pow2) {
-    // other grids: general plans for P(dz) and P(dz/2) (the padded convolution form) and an
-    // elementwise slice phase
+    // other grids: general plans for P(dz) and P(dz/2) (the padded convolution form, run in
+    // FP64 from their unrounded tables by bpm_generic_prop) and an elementwise slice phase
     m->generic = 1;
     fh_prop_desc pd = *d;
     double zf = m->dz, zh = m->dz / 2;
     pd.n0 = n0; pd.nz = 1;
     pd.z = &zf;
-    if ((rc = fh_asprop_create_ex(ctx, &pd, &m->pfull))) { bpm_free(m); return rc; }
+    if ((rc = asprop_create(ctx, &pd, &m->pfull, true))) { bpm_free(m); return rc; }
     pd.z = &zh;
-    if ((rc = fh_asprop_create_ex(ctx, &pd, &m->phalf))) { bpm_free(m); return rc; }
+    if ((rc = asprop_create(ctx, &pd, &m->phalf, true))) { bpm_free(m); return rc; }
     m->nck = (nslices + m->every - 1) / m->every;
     const size_t fsz = (size_t)nx * ny * batch * sizeof(float2);
+    const size_t gbytes = batch > 1 ? (size_t)nx * ny * batch * sizeof(float) : 0;
     if (hipMalloc(&m->kl, batch * sizeof(double)) != hipSuccess || hipMalloc(&m->ckpt, m->nck * fsz) != hipSuccess ||
-        hipMalloc(&m->seg, m->every * fsz) != hipSuccess || hipMalloc(&m->work, fsz) != hipSuccess) {
+        hipMalloc(&m->seg, m->every * fsz) != hipSuccess || hipMalloc(&m->work, fsz) != hipSuccess ||
+        (gbytes && hipMalloc(&m->gpart, gbytes) != hipSuccess)) {
       bpm_free(m);
       return fail(FH_E_NOMEM, "BPM: out of device memory");
     }
@@ -2828,7 +2904,11 @@ int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* d, double thickness, int n
       bpm_free(m);
       return fail(FH_E_HIP, "BPM: set-up failed");
     }
-    m->bytes = (m->nck + m->every + 1) * fsz;
+    // checkpoints, segment, work field, and per plan its x-spectrum rows and c64 + FP64 tables
+    m->bytes = (m->nck + m->every + 1) * fsz + gbytes;
+    for (const fh_plan* q : {m->pfull, m->phalf})
+      m->bytes += ((size_t)1 << q->lx) * mid_rows(q) * batch * sizeof(float2) +
+                  q->tabsize * q->lam_tables.size() * (sizeof(float2) + sizeof(double2));
     *out = m;
     return FH_OK;
   }

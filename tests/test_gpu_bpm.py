@@ -14,13 +14,16 @@ from _util import c64, rel_l2, random_field, to_dev, to_host
 pytestmark = pytest.mark.gpu
 
 
-def smooth_dn(ns, n, sigma, peak, seed):
-    """Gaussian-filtered white noise (periodic, FFT filter) scaled to max |dn| = peak."""
-    w = inputs.normal(seed, (ns, n, n))
-    k = np.fft.fftfreq(n)
+def smooth_dn(ns, n, sigma, peak, seed, nx=None):
+    """Gaussian-filtered white noise (periodic, FFT filter) scaled to max |dn| = peak; the
+    grid is n x n, or n rows of nx points."""
+    nx = n if nx is None else nx
+    w = inputs.normal(seed, (ns, n, nx))
+    ky = np.fft.fftfreq(n)
+    kx = np.fft.fftfreq(nx)
     kz = np.fft.fftfreq(ns)
-    g = np.exp(-2 * (math.pi * sigma) ** 2 * (kz[:, None, None] ** 2 + k[None, :, None] ** 2
-                                              + k[None, None, :] ** 2))
+    g = np.exp(-2 * (math.pi * sigma) ** 2 * (kz[:, None, None] ** 2 + ky[None, :, None] ** 2
+                                              + kx[None, None, :] ** 2))
     f = np.real(np.fft.ifftn(np.fft.fftn(w) * g))
     return (peak * f / np.max(np.abs(f))).astype(np.float32)
 
@@ -180,3 +183,34 @@ def test_bpm_any_grid_size(kind, every):
     assert rel_l2(to_host(du.electric), duref) <= 1e-5
     torch.cuda.synchronize()
     assert rel_l2(bpm.grad.cpu().numpy(), ddnref) <= 1e-4
+
+
+def test_bpm_any_grid_size_long_stack():
+    """A 200-slice graded-index stack on a 240 x 180 grid (padded slice propagators computed in
+    FP64, c64 between launches): forward, adjoint and dn gradient against the FP64 oracle, with
+    the bars of the power-of-two volume (1e-5 field and adjoint, 1e-4 gradient).  FP32 slice
+    transforms repeat one rounding on every slice and end at 1.6e-5 here."""
+    nx, ny, ns, every = 240, 180, 200, 10
+    print(f"[bpm any size, long stack] {nx}x{ny} x {ns} slices every={every}", flush=True)
+    ds, lam, n0 = (0.5, 0.5), 1.0, 1.5
+    dn = smooth_dn(ns, ny, 3.0, 0.01, 8, nx=nx)
+    x = inputs.grid(nx, ds[0])[None, :]
+    y = inputs.grid(ny, ds[1])[:, None]
+    g = np.exp(-(x ** 2 + y ** 2) / 30.0 ** 2) * np.exp(2j * math.pi * math.sin(math.radians(2.0)) * x / lam)
+    u0 = c64(g[None])
+    t0 = c64(0.01 * random_field((1, ny, nx), 16))
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    bpm = fo.AS_BPM(u, float(ns), n0, dn, trainable=True, buffered=True, checkpoint_every=every)
+    bpm.propagate_and_save_(u, fo.Forward)
+    v = to_host(u.electric)
+    vref, saved = O.bpm_forward(u0, dn.astype(np.float64), float(ns), n0, lam, ds)
+    ev = rel_l2(v, vref)
+    dv0 = c64(v - t0)
+    du = fo.ScalarField(to_dev(dv0), ds, lam)
+    bpm.backpropagate_with_gradient_(du, None, fo.Forward)
+    duref, ddnref = O.bpm_backward(dv0, dn.astype(np.float64), float(ns), n0, lam, ds, saved)
+    edu = rel_l2(to_host(du.electric), duref)
+    torch.cuda.synchronize()
+    eg = rel_l2(bpm.grad.cpu().numpy(), ddnref)
+    print(f"{nx}x{ny} x {ns} slices: field {ev:.2e} adjoint {edu:.2e} ddn {eg:.2e}", flush=True)
+    assert ev <= 1e-5 and edu <= 1e-5 and eg <= 1e-4
