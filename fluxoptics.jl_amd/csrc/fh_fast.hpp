@@ -207,16 +207,32 @@ k_fast_fft(FastIO io, const float2* __restrict__ tab, TimeOp op) {
     }
   if (!INV && live) {
     if constexpr (TwoPhaseOp<TimeOp>::value) {
-      typename TimeOp::Pre pre[V];
+      // the phase factors and the saved input have the field's natural layout, so their loads
+      // and stores take the field's offsets (one base register, immediate offsets); the factor
+      // loads are all issued before the saved stores, which may alias them
+      const auto rf = make_rsrc(op.cis, io.in_bytes);
+      float2 pre[V];
 #pragma unroll
       for (int m = 0; m < V / RD::R0; ++m)
 #pragma unroll
-        for (int r = 0; r < RD::R0; ++r) pre[m * RD::R0 + r] = op.pre(b, li, t + P * m + r * (n / RD::R0));
+        for (int r = 0; r < RD::R0; ++r) pre[m * RD::R0 + r] = buf_load(rf, vin, (P * m + r * (n / RD::R0)) * kin);
+      if (op.saved) {
+        const auto rsv = make_rsrc(op.saved + b * op.fstride, io.in_bytes);
+#pragma unroll
+        for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+          for (int r = 0; r < RD::R0; ++r) {
+            const int c = P * m + r * (n / RD::R0);
+            if (t + c < op.nx) buf_store(rsv, vin, c * kin, v[m * RD::R0 + r]);
+          }
+      }
 #pragma unroll
       for (int m = 0; m < V / RD::R0; ++m)
 #pragma unroll
-        for (int r = 0; r < RD::R0; ++r)
-          v[m * RD::R0 + r] = op.apply(v[m * RD::R0 + r], pre[m * RD::R0 + r], b, li, t + P * m + r * (n / RD::R0));
+        for (int r = 0; r < RD::R0; ++r) {
+          const int c = P * m + r * (n / RD::R0);
+          if (t + c < op.nx) v[m * RD::R0 + r] = cmul(v[m * RD::R0 + r], pre[m * RD::R0 + r]);
+        }
     } else {
 #pragma unroll
       for (int m = 0; m < V / RD::R0; ++m)

@@ -46,34 +46,27 @@ struct FullMulD {
 // ---------------------------------------------------------------- time-side ops (x lines)
 
 // u * cis(s*phi[y, x]), optionally saving the input first.  The line index is the row y
-// and the line position is x; phi is one nx*ny mask shared by every field.
-// Two-phase form (pre, then apply; see TwoPhaseOp): the mask loads and phase factors come
-// first, for every point of a thread, so that they are in flight with the field's loads
-// rather than queued behind the stores of the saved input, which may alias them.
+// and the line position is x; the phase factors cis(s*phi) are one nx*ny c64 array shared by
+// every field, formed once per call (k_phase_cis): evaluating sincos per field would repeat it
+// batch times and its registers would halve the pass's resident waves.
+// In the fast x pass (TwoPhaseOp) the factor loads come first, for every point of a thread, so
+// that they are in flight with the field's loads rather than queued behind the stores of the saved
+// input, which may alias them; there the factors and the saved input use the field's own buffer
+// offsets, since they share its [ny][nx] layout.
 struct PhaseTimeOp {
-  const float* phi;
-  float2* saved;   // nullable: [B][ny][nx]
-  long fstride;    // nx*ny
+  const float2* cis;  // [ny][nx]
+  float2* saved;      // nullable: [B][ny][nx]
+  long fstride;       // nx*ny
   int nx;
-  float s;         // +1 Forward, -1 Backward
-  using Pre = float2;
-  __device__ __forceinline__ float2 pre(int, int y, int x) const {
-    if (x >= nx) return make_float2(1.f, 0.f);
-    float sn, cs;
-    sincosf(s * phi[(long)y * nx + x], &sn, &cs);
-    return make_float2(cs, sn);
-  }
-  __device__ __forceinline__ float2 apply(float2 v, float2 f, int b, int y, int x) const {
-    if (x >= nx) return v;
-    if (saved) saved[b * fstride + (long)y * nx + x] = v;
-    return cmul(v, f);
-  }
   __device__ __forceinline__ float2 operator()(float2 v, int b, int y, int x) const {
-    return apply(v, pre(b, y, x), b, y, x);
+    if (x >= nx) return v;
+    const long i = (long)y * nx + x;
+    if (saved) saved[b * fstride + i] = v;
+    return cmul(v, cis[i]);
   }
 };
 
-// Time-side ops with a separate load phase (pre) and apply phase.
+// Time-side ops that the fast x pass applies in a load phase and an apply phase (see above).
 template <typename Op> struct TwoPhaseOp : std::false_type {};
 template <> struct TwoPhaseOp<PhaseTimeOp> : std::true_type {};
 

@@ -508,6 +508,15 @@ __global__ void k_tab_extract(T* __restrict__ tab, const double2* __restrict__ d
 
 // ================================================================ elementwise kernels
 
+// out = cis(s*phi) (the phase factors of phasemask.jl:90-113, once for every field of a batch)
+__global__ void k_phase_cis(const float* __restrict__ phi, float s, float2* __restrict__ out, long npix) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
+    float sn, cs;
+    sincosf(s * phi[i], &sn, &cs);
+    out[i] = make_float2(cs, sn);
+  }
+}
+
 // u <- u * cis(s*phi), saving u first when saved != nullptr (phasemask.jl:90-113).
 __global__ void k_phase_apply(float2* __restrict__ u, const float* __restrict__ phi,
                               float2* __restrict__ saved, long npix, int batch, float s) {
@@ -964,6 +973,17 @@ __global__ void k_mul_full_d(double2* __restrict__ u, const double2* __restrict_
 // summed in registers, in field order, and written once; the phase factors are taken once
 // per position for the whole batch and the next field's line is loaded while this one is
 // transformed.  Rows of RS plans hold 2nx points of which the first nx are kept.
+// Where they fit beside the exchange areas (160 KB per CU), the phase factors are taken once per
+// position and kept in LDS, one slot per (value, thread), read back only by the thread that wrote
+// them (no barrier): that frees their registers and the per-field sincos; otherwise the mask
+// values stay in registers and cis(-s*phi) is re-taken per field.
+template <int n, int L, int P, int V> struct PhaseGradLds {
+  static constexpr size_t exch = (size_t)L * LdsLine<n, L>::size * sizeof(float2);
+  static constexpr size_t cis = (size_t)L * P * V * sizeof(float2);
+  static constexpr bool in_lds = exch + cis <= 160 * 1024;
+  static constexpr size_t bytes = exch + (in_lds ? cis : 0);
+};
+
 template <typename RDF, int L, bool GRAD>
 __global__ void __launch_bounds__(L * RDF::P)
 k_fast_phase_grad(FastIO io, int batch, const float2* __restrict__ tab, const float* __restrict__ phi,
@@ -982,15 +1002,23 @@ k_fast_phase_grad(FastIO io, int batch, const float2* __restrict__ tab, const fl
   const int kin = fast_k<false>(io.in_l, io.in_e, io.in_blk) * 8;
   const int vout = fast_off<false>(li, t, io.out_l, io.out_e, io.out_blk) * 8;
   const int kout = fast_k<false>(io.out_l, io.out_e, io.out_blk) * 8;
-  // the mask values stay in registers; cis(-s*phi) is re-taken per field (cheap next to the
-  // memory traffic, and it keeps the register count down)
+  using PL = PhaseGradLds<n, L, P, V>;
+  constexpr int LP = L * P;
+  float2* csl = smem + L * LdsLine<n, L>::size + threadIdx.x;  // the LDS factors (PL::in_lds)
   float ph[V], g[V];
 #pragma unroll
   for (int m = 0; m < V / RL; ++m)
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int x = t + P * m + r * (n / RL);
-      ph[m * RL + r] = (live && x < nx) ? -s * phi[(long)li * nx + x] : 0.f;
+      const float a = (live && x < nx) ? -s * phi[(long)li * nx + x] : 0.f;
+      if constexpr (PL::in_lds) {
+        float sn, cs;
+        sincosf(a, &sn, &cs);
+        csl[(m * RL + r) * LP] = make_float2(cs, sn);
+      } else {
+        ph[m * RL + r] = a;
+      }
       g[m * RL + r] = 0.f;
     }
   // the next field's spectrum, and this field's saved input, are in flight while this field
@@ -1031,9 +1059,15 @@ k_fast_phase_grad(FastIO io, int batch, const float2* __restrict__ tab, const fl
         for (int r = 0; r < RL; ++r) {
           const int c = P * m + r * (n / RL);
           if (t + c < nx) {
-            float sn, cs;
-            sincosf(ph[m * RL + r], &sn, &cs);
-            const float2 du = cmul(v[m * RL + r], make_float2(cs, sn));
+            float2 f;
+            if constexpr (PL::in_lds) {
+              f = csl[(m * RL + r) * LP];
+            } else {
+              float sn, cs;
+              sincosf(ph[m * RL + r], &sn, &cs);
+              f = make_float2(cs, sn);
+            }
+            const float2 du = cmul(v[m * RL + r], f);
             if constexpr (GRAD) g[m * RL + r] += cmulc(du, sv[m * RL + r]).y;
             buf_store(rout, vout, c * kout, du);
           }
@@ -1105,6 +1139,7 @@ struct fh_plan {
   float2* scratch = nullptr;       // RS: (2nx) x ny x batch
   double2* plain_tab = nullptr;    // plain FP64 comparator (AS): full tables
   double2* tab_d = nullptr;        // padded AS, on request: the tables before rounding to c64
+  float2* cis = nullptr;           // Phase factors cis(s*phi) of the fused Phase pass, [ny][nx]
   int conv_var = -1;               // y-pass variant (-1 = built-in default)
   int xf_var = -1, xi_var = -1;    // x-pass variants (forward, inverse)
   float2* chirp = nullptr;         // Collins: A_x, A_y, D_x, D_y chirp lines per table
@@ -1305,7 +1340,7 @@ static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float
   int rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &tab);
   if (rc) return rc;
   auto kern = dphi ? k_fast_phase_grad<RDF, L, true> : k_fast_phase_grad<RDF, L, false>;
-  const size_t lds = (size_t)L * LdsLine<RDF::n, L>::size * sizeof(float2);
+  const size_t lds = PhaseGradLds<RI::n, L, RI::P, RI::V>::bytes;
   FH_HIP(ensure_lds(kern, lds));
   hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L), dim3(L * RDF::P), lds, c->stream, io, batch, tab, phi,
                      saved, dphi, nx, s, acc);
@@ -1313,9 +1348,12 @@ static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float
   return FH_OK;
 }
 
+// A workgroup owns whole 4-row groups (L >= 4): on ASProp plans this pass writes natural rows over
+// the 4x4-blocked spectrum it reads, and the rows 4g..4g+3 share those bytes.
 template <int LOGN>
 static int fast_phase_grad_t(fh_ctx* c, const FastIO& io, int batch, const float* phi, const float2* saved,
                              float* dphi, int nx, float s, int acc) {
+  static_assert(Cfg<float, LOGN, false>::L >= 4, "the in-place pass needs whole 4-row groups");
   return fast_phase_grad_l<LOGN, Cfg<float, LOGN, false>::L>(c, io, batch, phi, saved, dphi, nx, s, acc);
 }
 
@@ -1508,6 +1546,7 @@ static void plan_free(fh_plan* p) {
   if (p->scratch) (void)hipFree(p->scratch);
   if (p->plain_tab) (void)hipFree(p->plain_tab);
   if (p->tab_d) (void)hipFree(p->tab_d);
+  if (p->cis) (void)hipFree(p->cis);
   if (p->chirp) (void)hipFree(p->chirp);
   delete p;
 }
@@ -1542,7 +1581,8 @@ static int fft2_d(fh_ctx* c, double2* d, int nx2, int ny2, int lx, int ly) {
 static long mid_rows(const fh_plan* p) { return p->kind == 0 ? p->ny : ((p->ny + 3) / 4) * 4; }
 
 template <typename XOp>
-static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1) {
+// src (nullable): the first pass reads the input there instead of in u (u is then only written)
+static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1, const float2* src = nullptr) {
   fh_ctx* c = p->ctx;
   const int nxl = 1 << p->lx;
   const bool rs = p->kind != 0;
@@ -1551,7 +1591,7 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
   int rc;
   FastIO io{};
   if (only < 0 || only == 0) {
-    io.in = u; io.out = mid; io.in_f = (long)p->nx * p->ny; io.out_f = midf;
+    io.in = src ? src : u; io.out = mid; io.in_f = (long)p->nx * p->ny; io.out_f = midf;
     io.in_l = p->nx; io.out_l = nxl; io.in_e = io.out_e = 1; io.in_blk = 0; io.out_blk = 1;
     io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = midf * 8;
     io.in_valid = p->nx;
@@ -2117,14 +2157,27 @@ int fh_autotune(fh_plan* p, void* u, int reps, char* report, int report_len) {
   return FH_OK;
 }
 
+// The Phase pass's operator: cis(s*phi) into the plan's factor array (allocated at the first call;
+// its size is fixed by the plan, so graphs recorded later keep a valid pointer), then the op.
+static int phase_time_op(fh_plan* p, const float* phi, void* saved, int direction, PhaseTimeOp* op) {
+  const long npix = (long)p->nx * p->ny;
+  if (!p->cis && hipMalloc(&p->cis, npix * sizeof(float2)) != hipSuccess)
+    return fail(FH_E_NOMEM, "Phase factors: out of device memory");
+  hipLaunchKernelGGL(k_phase_cis, dim3((int)std::min<long>((npix + 255) / 256, 4096)), dim3(256), 0, p->ctx->stream, phi,
+                     (float)direction, p->cis, npix);
+  FH_HIP(hipGetLastError());
+  *op = PhaseTimeOp{p->cis, reinterpret_cast<float2*>(saved), npix, p->nx};
+  return FH_OK;
+}
+
 int fh_phase_propagate(fh_plan* p, void* u, const float* phi, void* saved, int direction) {
   if (!p || !u || !phi) return fail(FH_E_ARG, "null plan, field or mask");
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(u);
   if (p->kind == 2) return fail(FH_E_UNSUPP, "Phase fuses with ASProp and RSProp plans only");
-  PhaseTimeOp op{phi, reinterpret_cast<float2*>(saved), (long)p->nx * p->ny, p->nx, (float)direction};
-  if ((rc = run_prop(p, f, direction, op))) return rc;
+  PhaseTimeOp op{};
+  if ((rc = phase_time_op(p, phi, saved, direction, &op)) || (rc = run_prop(p, f, direction, op))) return rc;
   return run_xifft(p, f, direction);
 }
 
@@ -2430,24 +2483,24 @@ int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const vo
   return phase_grad_pass(p, f, phi, saved, dphi, direction, accumulate);
 }
 
-int fh_sqint_block(fh_plan* p, void* u, const float* target, int target_batch, const double* dc, double* loss,
-                   int direction, const float* phi, void* saved, float* dphi, int accumulate) {
-  if (!p || !u || !target || !loss) return fail(FH_E_ARG, "null plan, field, target or loss");
+// The fused metric block; src (nullable) is the block's input when it is not u itself: the first
+// pass reads it there and the Phase gradient takes it as the saved input (nothing is saved).
+static int sqint_block(fh_plan* p, const float2* src, float2* f, const float* target, int target_batch,
+                       const double* dc, double* loss, int direction, const float* phi, void* saved, float* dphi,
+                       int accumulate) {
   if (p->kind == 2) return fail(FH_E_UNSUPP, "the fused metric block takes ASProp and RSProp plans");
   if (target_batch != p->batch && !(target_batch == 1 && p->batch == 1))
     return fail(FH_E_UNSUPP, "the fused metric block needs one target per field (target batch %d, batch %d)",
                 target_batch, p->batch);
-  if ((saved || dphi) && !phi) return fail(FH_E_ARG, "a saved input or mask gradient needs the mask");
-  if (dphi && !saved) return fail(FH_E_ARG, "the mask gradient needs the saved mask input");
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   fh_ctx* c = p->ctx;
-  float2* f = reinterpret_cast<float2*>(u);
   // forward: [Phase in the first pass] x FFT, y pass
   if (phi) {
-    PhaseTimeOp op{phi, reinterpret_cast<float2*>(saved), (long)p->nx * p->ny, p->nx, (float)direction};
-    if ((rc = run_prop(p, f, direction, op))) return rc;
-  } else if ((rc = run_prop(p, f, direction, NoOp{}))) {
+    PhaseTimeOp op{};
+    if ((rc = phase_time_op(p, phi, saved, direction, &op)) || (rc = run_prop(p, f, direction, op, -1, src)))
+      return rc;
+  } else if ((rc = run_prop(p, f, direction, NoOp{}, -1, src))) {
     return rc;
   }
   // fused x IFFT + metric + pullback + x FFT of the adjoint, in place on the spectrum
@@ -2469,7 +2522,28 @@ int fh_sqint_block(fh_plan* p, void* u, const float* target, int target_batch, c
   // adjoint: y pass with the conjugate kernel, then x IFFT [un-applying the Phase, its gradient]
   if ((rc = run_prop(p, f, -direction, NoOp{}, 1))) return rc;
   if (!phi) return run_xifft(p, f, -direction);
-  return phase_grad_pass(p, f, phi, saved, dphi, direction, accumulate);
+  return phase_grad_pass(p, f, phi, src ? (const void*)src : saved, dphi, direction, accumulate);
+}
+
+int fh_sqint_block(fh_plan* p, void* u, const float* target, int target_batch, const double* dc, double* loss,
+                   int direction, const float* phi, void* saved, float* dphi, int accumulate) {
+  if (!p || !u || !target || !loss) return fail(FH_E_ARG, "null plan, field, target or loss");
+  if ((saved || dphi) && !phi) return fail(FH_E_ARG, "a saved input or mask gradient needs the mask");
+  if (dphi && !saved) return fail(FH_E_ARG, "the mask gradient needs the saved mask input");
+  return sqint_block(p, nullptr, reinterpret_cast<float2*>(u), target, target_batch, dc, loss, direction, phi,
+                     saved, dphi, accumulate);
+}
+
+int fh_sqint_block_src(fh_plan* p, const void* src, void* u, const float* target, int target_batch,
+                       const double* dc, double* loss, int direction, const float* phi, float* dphi,
+                       int accumulate) {
+  if (!p || !src || !u || !target || !loss) return fail(FH_E_ARG, "null plan, source, field, target or loss");
+  if (dphi && !phi) return fail(FH_E_ARG, "a mask gradient needs the mask");
+  const size_t bytes = (size_t)p->nx * p->ny * p->batch * sizeof(float2);
+  const char *a = static_cast<const char*>(src), *b = static_cast<const char*>(u);
+  if (a < b + bytes && b < a + bytes) return fail(FH_E_ARG, "the source and the field must not overlap");
+  return sqint_block(p, reinterpret_cast<const float2*>(src), reinterpret_cast<float2*>(u), target, target_batch,
+                     dc, loss, direction, phi, nullptr, dphi, accumulate);
 }
 
 int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {

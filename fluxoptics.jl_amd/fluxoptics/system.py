@@ -208,16 +208,27 @@ class OpticalSystem:
 
     def _withgradient_launches(self, loss, x, weights=None, fuse: bool = True):
         """The device work of ``withgradient``: per-pair loss tensors and the gradients."""
-        u = self._start(x)
+        if (x is None) == (self.source is None):
+            raise ValueError("give an input field exactly when the system has no source")
+        start = self.source.u0 if x is None else x
         comps = list(self.components)
-        tail = _fusable_tail(comps, loss, u, self.direction) if fuse else None
+        tail = _fusable_tail(comps, loss, start, self.direction) if fuse else None
         grads: Dict = {}
-        if tail is not None:
+        if tail is not None and not tail[0] and (x is None or not self.inplace):
+            # the whole system is the fused block: it reads the start field itself (no copy of
+            # the source field, and the Phase gradient takes it as its saved input)
+            _, ph, prop = tail
+            u = start.set_field_data(torch.empty_like(start.electric))
+            losses = (_sqint_block(loss, u, ph, prop, [], grads, weights, src=start.electric),)
+            du = u
+        elif tail is not None:
+            u = self._start(x)
             head, ph, prop = tail
             tape = _run_forward(head, u, self.direction, save=True)
             losses = (_sqint_block(loss, u, ph, prop, tape, grads, weights),)
             du = _run_backward(head, tape, u, self.direction, grads)
         else:
+            u = self._start(x)
             tape = _run_forward(comps, u, self.direction, save=True)
             dcs = None if weights is None else [weights]
             losses, (du,) = loss.value_and_pullback((u,), dcs, inplace=True)
@@ -311,8 +322,10 @@ def _fusable_tail(comps: List, loss, u: ScalarField, direction):
     return comps[:-1], None, comps[-1]
 
 
-def _sqint_block(loss, u: ScalarField, ph, prop, tape, grads: Dict, weights):
-    """The fused tail (``fh_sqint_block``); u ends as the cotangent at the tail's input."""
+def _sqint_block(loss, u: ScalarField, ph, prop, tape, grads: Dict, weights, src=None):
+    """The fused tail (``fh_sqint_block``); u ends as the cotangent at the tail's input.  With
+    ``src`` (the tail's input, left unchanged) the first pass reads it and u is only written
+    (``fh_sqint_block_src``)."""
     t = loss.targets[0]
     tb = 1 if t.dim() == 2 else t.shape[0]
     if t.shape[-2:] != (u.ny, u.nx):
@@ -331,12 +344,17 @@ def _sqint_block(loss, u: ScalarField, ph, prop, tape, grads: Dict, weights):
     if ph is not None:
         phi = ph.phi
         if ph.trainable:
-            saved = ph.saved if (ph.saved is not None and not any(e[2] is ph.saved for e in tape)) \
-                else torch.empty_like(u.electric)
+            if src is None:
+                saved = ph.saved if (ph.saved is not None and not any(e[2] is ph.saved for e in tape)) \
+                    else torch.empty_like(u.electric)
             g, acc = _grad_buffer(grads, ph, ph.phi)
     plan = prop.plan(u)
-    _lib.call("fh_sqint_block", plan, _ptr(u.electric), _ptr(t), tb, _ptr(dc), _ptr(out), dir_sign(Forward),
-              _ptr(phi), _ptr(saved), _ptr(g), int(acc))
+    if src is None:
+        _lib.call("fh_sqint_block", plan, _ptr(u.electric), _ptr(t), tb, _ptr(dc), _ptr(out), dir_sign(Forward),
+                  _ptr(phi), _ptr(saved), _ptr(g), int(acc))
+    else:
+        _lib.call("fh_sqint_block_src", plan, _ptr(src), _ptr(u.electric), _ptr(t), tb, _ptr(dc), _ptr(out),
+                  dir_sign(Forward), _ptr(phi), _ptr(g), int(acc))
     return out
 
 

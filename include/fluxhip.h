@@ -209,6 +209,15 @@ int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx
 int fh_sqint_block(fh_plan* plan, void* u, const float* target, int target_batch, const double* dc,
                    double* loss, int direction, const float* phi, void* saved, float* dphi, int accumulate);
 
+/* fh_sqint_block on a block whose input is a system's source field: ScalarSource's propagate copies
+ * u0 into its output field before the first component runs (scalar_source.jl:68-71); here the first
+ * pass reads src itself and writes u (which need not hold anything), and the Phase gradient takes src
+ * as its saved input, so neither the copy nor the save is made.  src is left unchanged and must not
+ * overlap u; u ends as the cotangent at the block's input, as in fh_sqint_block. */
+int fh_sqint_block_src(fh_plan* plan, const void* src, void* u, const float* target, int target_batch,
+                       const double* dc, double* loss, int direction, const float* phi, float* dphi,
+                       int accumulate);
+
 /* AS_BPM: split-step beam propagation through nslices slices of index perturbation dn
  * (AS_BPM(u, thickness, n0, dn), bpm.jl:130-142 and :8-74): dz = thickness/nslices,
  * P(dz) and P(dz/2) are ASProps in index n0, the slice phase is

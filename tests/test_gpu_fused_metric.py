@@ -84,3 +84,43 @@ def test_captured_fused_tail_replays():
         assert float(v) == v0
         assert torch.equal(g[ph], g0)
         torch.zeros(1, device="cuda").sum()  # eager work between replays
+
+
+@pytest.mark.parametrize("kind", ["as", "rs"])
+def test_block_reading_the_source_equals_block_on_a_copy(kind):
+    """fh_sqint_block_src (the first pass reads the source field; no copy of it, no saved mask
+    input) against fh_sqint_block on a copy with a saved input: loss, cotangent and mask gradient
+    bit for bit.  The source stays unchanged, and a field overlapping it is refused."""
+    from fluxoptics import _lib
+    from fluxoptics.components import _ptr
+    n, b = (256, 3) if kind == "as" else (128, 2)
+    print(f"[fused metric from source] {kind} n={n} batch={b}", flush=True)
+    u0 = c64(speckle_batch(b, n, n, 1.0, 1.0, 0.1 * n, 171))
+    tgt = torch.from_numpy((np.abs(c64(speckle_batch(b, n, n, 1.0, 1.0, 0.1 * n, 173))) ** 2).astype(np.float32)).cuda()
+    src = fo.ScalarField(to_dev(u0), (1.0, 1.0), [0.8, 0.6, 0.5][:b])
+    prop = fo.ASProp(src, 300.0) if kind == "as" else fo.RSProp(src, 300.0)
+    plan = prop.plan(src)
+    phi = torch.from_numpy(inputs.uniform(172, (n, n), 0, 2 * math.pi).astype(np.float32)).cuda()
+    dc = torch.tensor([1.0, -0.5, 2.0][:b], dtype=torch.float64, device="cuda")
+
+    def run(from_src):
+        u = torch.full_like(src.electric, complex(float("nan"), 0.0)) if from_src else src.electric.clone()
+        loss = torch.empty(b, dtype=torch.float64, device="cuda")
+        dphi = torch.empty_like(phi)
+        if from_src:
+            _lib.call("fh_sqint_block_src", plan, _ptr(src.electric), _ptr(u), _ptr(tgt), b, _ptr(dc), _ptr(loss),
+                      1, _ptr(phi), _ptr(dphi), 0)
+        else:
+            saved = torch.empty_like(u)
+            _lib.call("fh_sqint_block", plan, _ptr(u), _ptr(tgt), b, _ptr(dc), _ptr(loss), 1, _ptr(phi),
+                      _ptr(saved), _ptr(dphi), 0)
+        return u, loss, dphi
+
+    u1, l1, g1 = run(True)
+    u0d, l0, g0 = run(False)
+    torch.cuda.synchronize()
+    assert torch.equal(l1, l0) and torch.equal(u1, u0d) and torch.equal(g1, g0)
+    assert torch.equal(src.electric.cpu(), torch.from_numpy(u0))
+    with pytest.raises(_lib.FluxHipError, match="overlap"):
+        _lib.call("fh_sqint_block_src", plan, _ptr(src.electric), _ptr(src.electric[1:]), _ptr(tgt), b, _ptr(dc),
+                  _ptr(torch.empty(b, dtype=torch.float64, device="cuda")), 1, _ptr(phi), None, 0)

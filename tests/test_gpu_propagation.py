@@ -115,7 +115,8 @@ def test_phase_apply_and_gradient():
     assert rel_l2(dphi.cpu().numpy(), g_ref) <= GRAD_TOL
 
 
-@pytest.mark.parametrize("kind,n", [("as", 256), ("as", 1024), ("as", 2048), ("rs", 128), ("rs", 512)])
+@pytest.mark.parametrize("kind,n", [("as", 256), ("as", 1024), ("as", 2048), ("as", 4096), ("rs", 128),
+                                    ("rs", 512)])
 def test_fused_phase_propagate_and_adjoint(kind, n):
     print(f"[fused phase+{kind}] n={n}", flush=True)
     batch, ds, lam, z = 3, (1.0, 1.0), [0.64, 0.538, 0.455], 800.0
