@@ -114,6 +114,72 @@ __device__ __forceinline__ void line_fft(C* v, C* s, int t, const C* __restrict_
   }
 }
 
+// Frequency-side ops with a separate load (pre) and multiply (apply); specialised where the op
+// is defined.
+template <typename Op> struct FreqPre : std::false_type {};
+
+// Stage twiddles of one thread, loaded up front: kernels that run one short transform per
+// launch (the AS_BPM slice steps) would otherwise wait for a global load at every stage.
+template <typename C, int n, int R, int NS, int V, int P, bool INV> struct StageTw {
+  static constexpr bool used = NS > 1 && R > 1;
+  C w[used ? V / R : 1][used ? R - 1 : 1];
+  __device__ __forceinline__ void load(const C* __restrict__ tw, int t) {
+    if constexpr (used) {
+#pragma unroll
+      for (int m = 0; m < V / R; ++m) {
+        const int jm = (t + P * m) % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[m][r - 1] = tw_load<INV>(tw, jm * r * (n / (NS * R)));
+      }
+    }
+  }
+  __device__ __forceinline__ void compute(C* v) const {
+#pragma unroll
+    for (int m = 0; m < V / R; ++m) {
+      if constexpr (used) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[m * R + r] = cmul(v[m * R + r], w[m][r - 1]);
+      }
+      dft<R, INV>(&v[m * R]);
+    }
+  }
+};
+
+template <typename C, typename RD, bool INV> struct LineTw {
+  StageTw<C, RD::n, RD::R1, RD::R0, RD::V, RD::P, INV> s1;
+  StageTw<C, RD::n, RD::R2, RD::R0 * RD::R1, RD::V, RD::P, INV> s2;
+  // registers (4 bytes) that the prefetched twiddles of one thread take
+  static constexpr int regs =
+      (int)(sizeof(C) / 4) * ((RD::NST >= 2 ? (RD::V / RD::R1) * (RD::R1 - 1) : 0) +
+                              (RD::NST >= 3 ? (RD::V / RD::R2) * (RD::R2 - 1) : 0));
+  __device__ __forceinline__ void load(const C* __restrict__ tw, int t) {
+    if constexpr (RD::NST >= 2) s1.load(tw, t);
+    if constexpr (RD::NST >= 3) s2.load(tw, t);
+  }
+};
+
+// line_fft with the stage twiddles already in registers
+template <typename C, typename RD, bool INV>
+__device__ __forceinline__ void line_fft_tw(C* v, C* s, int t, const LineTw<C, RD, INV>& lt) {
+  constexpr int n = RD::n, V = RD::V, P = RD::P;
+#pragma unroll
+  for (int m = 0; m < V / RD::R0; ++m) dft<RD::R0, INV>(&v[m * RD::R0]);
+  if constexpr (RD::NST >= 2) {
+    __syncthreads();
+    stage_write<C, n, RD::R0, 1, V, P>(s, v, t);
+    __syncthreads();
+    stage_read<C, n, RD::R1, V, P>(v, s, t);
+    lt.s1.compute(v);
+  }
+  if constexpr (RD::NST >= 3) {
+    __syncthreads();
+    stage_write<C, n, RD::R1, RD::R0, V, P>(s, v, t);
+    __syncthreads();
+    stage_read<C, n, RD::R2, V, P>(v, s, t);
+    lt.s2.compute(v);
+  }
+}
+
 // position held in register (m, r) when the stage with radix R is the first (input) or
 // last (output) stage: j + r*n/R with j = t + P*m
 template <int n, int R, int P>
@@ -243,16 +309,41 @@ k_line_conv(LineIO<S> io, const C* __restrict__ tw, FreqOp op, int G) {
       if (live && pos < io.in_valid) x = cvt<C>(src[elem_addr<LFAST>(li, pos, io.in_l, io.in_e, io.in_blk)]);
       v[m * RDF::R0 + r] = x;
     }
-  line_fft<C, RDF, false>(v, s, t, tw);
   constexpr int RL = RI::R0;  // last forward radix == first inverse radix
+  // short transforms (at most 256 threads a workgroup, e.g. one 512-point FP64 line per wave)
+  // load their stage twiddles and transfer-function values up front, with the line: the launch
+  // then waits for one round of global loads instead of one per stage
+  constexpr bool PRE = L * P <= 256 && LineTw<C, RDF, false>::regs + LineTw<C, RI, true>::regs <= 128;
+  constexpr bool HPRE = PRE && FreqPre<FreqOp>::value;
+  LineTw<C, RDF, false> twf;
+  LineTw<C, RI, true> twi;
+  C hv[HPRE ? V : 1];
+  if constexpr (PRE) {
+    twf.load(tw, t);
+    twi.load(tw, t);
+  }
+  if constexpr (HPRE) {
+#pragma unroll
+    for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+      for (int r = 0; r < RL; ++r)
+        hv[m * RL + r] = live ? op.template pre<C>(b, li, reg_pos<n, RL, P>(t, m, r)) : cmake<C>(1, 0);
+  }
+  if constexpr (PRE) line_fft_tw<C, RDF, false>(v, s, t, twf);
+  else line_fft<C, RDF, false>(v, s, t, tw);
 #pragma unroll
   for (int m = 0; m < V / RL; ++m)
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int pos = reg_pos<n, RL, P>(t, m, r);
-      if (live) v[m * RL + r] = op(v[m * RL + r], b, li, pos);
+      if constexpr (HPRE) {
+        v[m * RL + r] = op.apply(v[m * RL + r], hv[m * RL + r]);
+      } else {
+        if (live) v[m * RL + r] = op(v[m * RL + r], b, li, pos);
+      }
     }
-  line_fft<C, RI, true>(v, s, t, tw);
+  if constexpr (PRE) line_fft_tw<C, RI, true>(v, s, t, twi);
+  else line_fft<C, RI, true>(v, s, t, tw);
   constexpr int RE = (RI::NST == 3) ? RI::R2 : ((RI::NST == 2) ? RI::R1 : RI::R0);
   S* dst = io.out + b * io.out_f;
 #pragma unroll

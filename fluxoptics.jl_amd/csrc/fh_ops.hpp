@@ -22,15 +22,24 @@ template <typename T> struct QuadMulT {
   int conj;     // Backward direction multiplies by the conjugate
   int one_tab;  // all fields share one table
   int full = 0; // full tables: every column kx; column entries ky < n/2 at ky, the rest at n/2 + n - ky
+  // the value at (kx, ky) (pre) and the multiply (apply): kernels that know their positions up
+  // front issue the loads early
   template <typename C>
-  __device__ __forceinline__ C operator()(C v, int b, int kx, int ky) const {
+  __device__ __forceinline__ C pre(int b, int kx, int ky) const {
     const int qxi = full ? kx : (kx <= nx - kx ? kx : nx - kx);
     const int qyi = full ? (ky < ny / 2 ? ky : ny / 2 + ny - ky) : (ky <= ny - ky ? ky : ny - ky);
-    const C h = cvt<C>(tab[tabidx[b] * tabsize + (long)qxi * qcol + qyi]);
-    return conj ? cmulc(v, h) : cmul(v, h);
+    return cvt<C>(tab[tabidx[b] * tabsize + (long)qxi * qcol + qyi]);
+  }
+  template <typename C>
+  __device__ __forceinline__ C apply(C v, C h) const { return conj ? cmulc(v, h) : cmul(v, h); }
+  template <typename C>
+  __device__ __forceinline__ C operator()(C v, int b, int kx, int ky) const {
+    return apply(v, pre<C>(b, kx, ky));
   }
 };
 using QuadMul = QuadMulT<float2>;
+
+template <typename T> struct FreqPre<QuadMulT<T>> : std::true_type {};
 
 // Plain multiply by a full table (FP64 comparator path); the table is [ky][kx].
 struct FullMulD {

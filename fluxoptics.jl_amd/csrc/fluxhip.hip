@@ -1228,14 +1228,33 @@ static int fast_x_t(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
 // fewer than kFillWGs workgroups: single small fields otherwise leave most CUs idle.  An x
 // pass converts between the natural and the 4x4-blocked layout in place, which is only safe
 // when every workgroup owns whole 4-row groups, so L stays a multiple of 4.
+// Plans with 8 points a thread for rows of 512 and 1024 points, taken when even 4 rows per
+// workgroup leave fewer than kFillWGs workgroups (a 250 x 250 RS field has 63 4-row groups per
+// field): the 32-point plans then run one wave per workgroup with four times the serial work.
+template <int LOGN> struct FewRowsPlan { using RD = void; };
+template <> struct FewRowsPlan<9> { using RD = Radices<8, 8, 8>; };
+template <> struct FewRowsPlan<10> { using RD = Radices<16, 8, 8>; };
+
+template <typename RDF, int L, bool INV, bool IH, bool OH, typename Op>
+static int fast_x_launch(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes);
+
 template <int LOGN, bool INV, bool IH, bool OH, typename Op, int L>
 static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
   using RDF = typename Cfg<float, LOGN, false>::RD;
-  using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
+  const bool few = (long)((io.nlines + L - 1) / L) * batch < kFillWGs;
   if constexpr (L >= 8 && (L / 2) * RDF::P >= 64) {
-    if ((long)((io.nlines + L - 1) / L) * batch < kFillWGs)
-      return fast_x_l<LOGN, INV, IH, OH, Op, L / 2>(c, io, batch, op, lanes);
+    if (few) return fast_x_l<LOGN, INV, IH, OH, Op, L / 2>(c, io, batch, op, lanes);
   }
+  using FR = typename FewRowsPlan<LOGN>::RD;
+  if constexpr (!std::is_void<FR>::value && L == 4) {
+    if (few) return fast_x_launch<FR, 4, INV, IH, OH, Op>(c, io, batch, op, lanes);
+  }
+  return fast_x_launch<RDF, L, INV, IH, OH, Op>(c, io, batch, op, lanes);
+}
+
+template <typename RDF, int L, bool INV, bool IH, bool OH, typename Op>
+static int fast_x_launch(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
+  using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
   const float2* tab = nullptr;
   int rc = get_stage_tw(c, RD::n, RD::R0, RD::R1, RD::R2, INV, &tab);
   if (rc) return rc;
@@ -1327,15 +1346,10 @@ static int fast_sqint(fh_ctx* c, int logn, bool half, const FastIO& io, int batc
   FH_LOGN_SWITCH_INT(logn, (fast_sqint_t<LG, false>(c, io, batch, I, tstride, nx, dc, part, nparts)))
 }
 
-template <int LOGN, int L>
-static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float* phi, const float2* saved,
-                             float* dphi, int nx, float s, int acc) {
-  using RDF = typename Cfg<float, LOGN, false>::RD;
+template <typename RDF, int L>
+static int fast_phase_grad_launch(fh_ctx* c, const FastIO& io, int batch, const float* phi, const float2* saved,
+                                  float* dphi, int nx, float s, int acc) {
   using RI = typename RDF::Rev;
-  if constexpr (L >= 8 && (L / 2) * RDF::P >= 64) {
-    if ((long)((io.nlines + L - 1) / L) < kFillWGs)
-      return fast_phase_grad_l<LOGN, L / 2>(c, io, batch, phi, saved, dphi, nx, s, acc);
-  }
   const float2* tab = nullptr;
   int rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &tab);
   if (rc) return rc;
@@ -1346,6 +1360,22 @@ static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float
                      saved, dphi, nx, s, acc);
   FH_HIP(hipGetLastError());
   return FH_OK;
+}
+
+// (FewRowsPlan as for the x passes: this pass loops over the batch, so only the 4-row groups count)
+template <int LOGN, int L>
+static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float* phi, const float2* saved,
+                             float* dphi, int nx, float s, int acc) {
+  using RDF = typename Cfg<float, LOGN, false>::RD;
+  const bool few = (long)((io.nlines + L - 1) / L) < kFillWGs;
+  if constexpr (L >= 8 && (L / 2) * RDF::P >= 64) {
+    if (few) return fast_phase_grad_l<LOGN, L / 2>(c, io, batch, phi, saved, dphi, nx, s, acc);
+  }
+  using FR = typename FewRowsPlan<LOGN>::RD;
+  if constexpr (!std::is_void<FR>::value && L == 4) {
+    if (few) return fast_phase_grad_launch<FR, 4>(c, io, batch, phi, saved, dphi, nx, s, acc);
+  }
+  return fast_phase_grad_launch<RDF, L>(c, io, batch, phi, saved, dphi, nx, s, acc);
 }
 
 // A workgroup owns whole 4-row groups (L >= 4): on ASProp plans this pass writes natural rows over
