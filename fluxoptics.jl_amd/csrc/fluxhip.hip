@@ -1249,6 +1249,14 @@ static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
   if constexpr (!std::is_void<FR>::value && L == 4) {
     if (few) return fast_x_launch<FR, 4, INV, IH, OH, Op>(c, io, batch, op, lanes);
   }
+  // Out-of-place passes (RS, Collins and CZT plans: natural rows <-> the scratch spectrum, or a
+  // source read into a separate field) need no whole 4-row groups.  Rows of 4096 points then go
+  // two to a workgroup: a 4-row workgroup takes 139 KB of LDS, one per CU, which then alternates
+  // between loading, transforming and storing; two 2-row workgroups share a CU and overlap.
+  if constexpr (L == 4 && LOGN == 12) {
+    if (static_cast<const void*>(io.in) != static_cast<const void*>(io.out))
+      return fast_x_launch<RDF, 2, INV, IH, OH, Op>(c, io, batch, op, lanes);
+  }
   return fast_x_launch<RDF, L, INV, IH, OH, Op>(c, io, batch, op, lanes);
 }
 
@@ -1489,12 +1497,14 @@ static const FastConvVariant kFast11[] = {{"8x16x16 L4 f1", fast_conv_v<8, 16, 1
                                           {"pipe 16x16x8 L4 hlds", fast_pipe_v<16, 16, 8, 4, true>}};
 static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16, 16, 4, 1>},
                                           {"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
-                                          {"16x16x16 L2 f1", fast_conv_v<16, 16, 16, 2, 1>}};
+                                          {"16x16x16 L2 f1", fast_conv_v<16, 16, 16, 2, 1>},
+                                          {"pipe 16x16x16 L2", fast_pipe_v<16, 16, 16, 2>}};
 
 // default variant index for a y length and whether the batch shares one transfer function
-static int default_conv_variant(int logn, bool one_tab) {
-  if (one_tab && logn == 11) return 5;  // pipe 16x16x8 L4, transfer function in LDS
-  if (logn == 12) return 1;             // 16x16x16 L4
+static int default_conv_variant(int logn, bool one_tab, bool half) {
+  if (one_tab && logn == 11) return 5;           // pipe 16x16x8 L4, transfer function in LDS
+  if (logn == 12 && half && !one_tab) return 3;  // pipe 16x16x16 L2 (RS, a table per field)
+  if (logn == 12) return 1;                      // 16x16x16 L4
   return 0;
 }
 
@@ -1508,7 +1518,7 @@ static const FastConvVariant* fast_conv_variants(int logn, int* count) {
     case 9: *count = 3; return kFast9;
     case 10: *count = 4; return kFast10;
     case 11: *count = 6; return kFast11;
-    case 12: *count = 3; return kFast12;
+    case 12: *count = 4; return kFast12;
     default: *count = 0; return nullptr;
   }
 }
@@ -1639,7 +1649,7 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
   const FastConvVariant* vars = fast_conv_variants(p->ly, &nvar);
   if (!vars) return fail(FH_E_UNSUPP, "y length 2^%d unsupported", p->ly);
   const int v = (p->conv_var >= 0 && p->conv_var < nvar) ? p->conv_var
-                                                          : default_conv_variant(p->ly, q.one_tab);
+                                                          : default_conv_variant(p->ly, q.one_tab, rs);
   return vars[v].fn(c, io, p->batch, q, rs);
 }
 
