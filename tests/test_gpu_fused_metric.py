@@ -124,3 +124,36 @@ def test_block_reading_the_source_equals_block_on_a_copy(kind):
     with pytest.raises(_lib.FluxHipError, match="overlap"):
         _lib.call("fh_sqint_block_src", plan, _ptr(src.electric), _ptr(src.electric[1:]), _ptr(tgt), b, _ptr(dc),
                   _ptr(torch.empty(b, dtype=torch.float64, device="cuda")), 1, _ptr(phi), None, 0)
+
+
+def test_trainable_source_through_the_block_reading_it():
+    """A trainable ScalarSource right before the fused block (which reads the source field in
+    place of a copy): the source's gradient and the mask gradient equal the component-by-component
+    path, the source field is unchanged, and an input field given in place of a source (inplace off)
+    takes the same path."""
+    n, b = 256, 2
+    print("[fused metric] trainable source", flush=True)
+    u0 = c64(speckle_batch(b, n, n, 1.0, 1.0, 30.0, 181))
+    tgt = (np.abs(c64(speckle_batch(b, n, n, 1.0, 1.0, 30.0, 183))) ** 2).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), (1.0, 1.0), [0.8, 0.7])
+    src = fo.ScalarSource(u, trainable=True, buffered=True)
+    ph = fo.Phase(u, inputs.uniform(182, (n, n), 0, 2 * math.pi).astype(np.float32), trainable=True,
+                  buffered=True)
+    prop = fo.ASProp(u, 500.0)
+    metric = fo.SquaredIntensityDifference((u, tgt))
+    system = src | ph | prop
+    v1, g1 = system.withgradient(metric)
+    g1 = {c: g.clone() for c, g in g1.items()}
+    v0, g0 = system.withgradient(metric, fuse=False)
+    torch.cuda.synchronize()
+    assert abs(float(v1) - float(v0)) <= 1e-6 * abs(float(v0))
+    assert rel_l2(g1[src].cpu().numpy(), g0[src].cpu().numpy()) <= 1e-5
+    assert rel_l2(g1[ph].cpu().numpy(), g0[ph].cpu().numpy()) <= 1e-5
+    assert torch.equal(src.u0.electric.cpu(), torch.from_numpy(u0))
+    free = ph | prop
+    x = fo.ScalarField(to_dev(u0), (1.0, 1.0), [0.8, 0.7])
+    v2, g2 = free.withgradient(metric, x)
+    torch.cuda.synchronize()
+    assert abs(float(v2) - float(v1)) <= 1e-6 * abs(float(v1))
+    assert rel_l2(g2[ph].cpu().numpy(), g1[ph].cpu().numpy()) <= 1e-6
+    assert torch.equal(x.electric.cpu(), torch.from_numpy(u0))
