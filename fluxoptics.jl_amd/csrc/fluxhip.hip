@@ -1503,6 +1503,7 @@ static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16,
 // default variant index for a y length and whether the batch shares one transfer function
 static int default_conv_variant(int logn, bool one_tab, bool half) {
   if (one_tab && logn == 11) return 5;           // pipe 16x16x8 L4, transfer function in LDS
+  if (logn == 11) return 3;                      // 16x16x8 L4 f1 (a table per field)
   if (logn == 12 && half && !one_tab) return 3;  // pipe 16x16x16 L2 (RS, a table per field)
   if (logn == 12) return 1;                      // 16x16x16 L4
   return 0;

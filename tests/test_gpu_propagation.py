@@ -35,6 +35,8 @@ GRAD_TOL = 1e-4
     (64, 64, 1, (1.0, 1.0), 0.8, 40.0, 1.0),
     (2048, 1024, 1, (0.5, 1.0), 0.8, 2000.0, 1.0),
     (2048, 2048, 2, (0.5, 0.5), 0.8, 5000.0, 1.0),
+    # a table per field (three wavelengths) at 2048: the batched y pass without a shared table
+    (2048, 2048, 3, (0.5, 0.5), [0.64, 0.538, 0.455], 5000.0, 1.0),
     (4096, 4096, 1, (0.5, 0.5), 0.8, 5000.0, 1.0),
 ])
 def test_asprop_forward_backward_parity(nx, ny, batch, ds, lams, z, n0):
