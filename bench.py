@@ -271,11 +271,30 @@ def workload_published_rgb(torch, fo, inputs):
         losses.append(val)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
+    # the same warm-up and 200 iterations, each one replay of a recorded iteration (gradient +
+    # Fista step, its coefficients in device memory)
+    for d in does:
+        d.h.zero_()
+    opt = fo.setup(fo.ProxRule(fo.Fista(200), fo.ClampProx(-0.5, 0.5)), system)
+    _, grads = system.withgradient(metric)
+    fo.update_(opt, system, grads)
+    step = system.capture_train_step(metric, opt)
+    glosses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(200):
+        val, _ = step.replay()
+        glosses.append(val)
+    torch.cuda.synchronize()
+    gsec = time.perf_counter() - t0
     return {"seconds_200_iterations": sec, "ms_per_iter": sec * 5.0, "loss_first": float(losses[0]),
-            "loss_last": float(losses[-1]), "published_gpu_seconds": 6.196,
-            "published_hardware": "CUDA GPU, model not recorded", "speedup_vs_published": 6.196 / sec,
+            "loss_last": float(losses[-1]), "graph_seconds_200_iterations": gsec,
+            "graph_loss_first": float(glosses[0]), "graph_loss_last": float(glosses[-1]),
+            "published_gpu_seconds": 6.196, "published_hardware": "CUDA GPU, model not recorded",
+            "speedup_vs_published": 6.196 / sec, "graph_speedup_vs_published": 6.196 / gsec,
             "what": "800^2 x 3 lambda, 5 RSProp(3000) + 4 TeaDOE, intensity loss, Fista(200) + clamp, 200 "
-                    "iterations of withgradient + update_, host-driven (RS padded to 2048^2); synthetic targets"}
+                    "iterations of withgradient + update_, host-driven (RS padded to 2048^2), and as 200 replays "
+                    "of one recorded iteration; synthetic targets"}
 
 
 def workload_train_step(torch, fo, inputs):

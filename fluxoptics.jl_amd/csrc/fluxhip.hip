@@ -676,8 +676,11 @@ __global__ void k_tea_backward(float2* __restrict__ dv, const float* __restrict_
 // eta already squared: d = eta dx - beta (x - xk), xk = x, x -= d.  Prox
 // (proximal_operators/*.jl): 0 none, 1 clamp to [p0, p1], 2 soft threshold by p0 around p1,
 // 3 negatives to zero.
+// beta_d (nullable): Fista's beta from device memory (k_fista_coef) in place of the argument
 __global__ void k_optim_step(float* __restrict__ x, const float* __restrict__ dx, float* __restrict__ st,
-                             long n, int rule, float eta, float rho, float beta, int prox, float p0, float p1) {
+                             long n, int rule, float eta, float rho, float beta, int prox, float p0, float p1,
+                             const double* __restrict__ beta_d) {
+  if (beta_d) beta = (float)*beta_d;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     float xi = x[i];
     const float g = dx[i];
@@ -706,6 +709,18 @@ __global__ void k_optim_step(float* __restrict__ x, const float* __restrict__ dx
       xi = xi < 0.f ? 0.f : xi;
     }
     x[i] = xi;
+  }
+}
+
+// Fista's coefficients in device memory (OptimisersExt.jl:12-40), so that a recorded step
+// replays correctly: tb = {t_k, beta}; beta <- (t_k - 1) / t_{k+1}, t <- t_{k+1} =
+// (1 + sqrt(1 + 4 t_k^2)) / 2.  One thread.
+__global__ void k_fista_coef(double* tb) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double t = tb[0];
+    const double tn = (1.0 + sqrt(1.0 + 4.0 * t * t)) / 2.0;
+    tb[1] = (t - 1.0) / tn;
+    tb[0] = tn;
   }
 }
 
@@ -2465,7 +2480,23 @@ int fh_optim_step(fh_ctx* ctx, int rule, float* x, const float* dx, float* state
   if ((rc = set_device(ctx))) return rc;
   const int blocks = (int)std::min<long>((n + 255) / 256, 8192);
   hipLaunchKernelGGL(k_optim_step, dim3(blocks), dim3(256), 0, ctx->stream, x, dx, state, n, rule, (float)eta,
-                     (float)rho, (float)beta, prox, (float)p0, (float)p1);
+                     (float)rho, (float)beta, prox, (float)p0, (float)p1, (const double*)nullptr);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_fista_step(fh_ctx* ctx, float* x, const float* dx, float* state, long n, double eta, double* tb, int prox,
+                  double p0, double p1) {
+  if (!ctx || !x || !dx || !state || !tb) return fail(FH_E_ARG, "null context, parameter, gradient, state or coefficients");
+  if (prox < 0 || prox > 3) return fail(FH_E_ARG, "unknown proximal operator %d", prox);
+  if (n <= 0) return fail(FH_E_ARG, "n must be positive");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  hipLaunchKernelGGL(k_fista_coef, dim3(1), dim3(64), 0, ctx->stream, tb);
+  FH_HIP(hipGetLastError());
+  const int blocks = (int)std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_optim_step, dim3(blocks), dim3(256), 0, ctx->stream, x, dx, state, n, 3, (float)eta, 0.f,
+                     0.f, prox, (float)p0, (float)p1, (const double*)(tb + 1));
   FH_HIP(hipGetLastError());
   return FH_OK;
 }

@@ -9,7 +9,6 @@ device kernel (rule + proximal operator) over the parameter (``fh_optim_step``).
 
 from __future__ import annotations
 
-import math
 from typing import Dict, Iterable, Optional
 
 import torch
@@ -146,7 +145,6 @@ class _Leaf:
         self.component, self.x, self.rule = component, tensor, rule
         base = rule.rule if isinstance(rule, ProxRule) else rule
         self.base = base
-        self.t = 1.0  # Fista's t_k
         self.state = None
         if isinstance(base, (Momentum, Nesterov)):
             self.state = torch.zeros_like(tensor)
@@ -156,6 +154,8 @@ class _Leaf:
             self.bt = torch.tensor(base.beta, dtype=torch.float64, device=tensor.device)
         elif isinstance(base, Fista):
             self.state = tensor.detach().clone()
+            # {t_k, beta} on the device (each step advances them), t_1 = 1
+            self.tb = torch.tensor([1.0, 0.0], dtype=torch.float64, device=tensor.device)
 
 
 def setup(rules, model, default_rule=None):
@@ -218,14 +218,13 @@ def _step(leaf: _Leaf, grad: torch.Tensor) -> None:
         return
     eta = base.eta
     rho = getattr(base, "rho", 0.0)
-    beta = 0.0
-    if isinstance(base, Fista):
-        tn = (1.0 + math.sqrt(1.0 + 4.0 * leaf.t * leaf.t)) / 2.0
-        beta = (leaf.t - 1.0) / tn
-        leaf.t = tn
     first = proxes[0] if proxes else None
-    _lib.call("fh_optim_step", ctx, _RULE_IDS[type(base).__name__], _ptr(xr), _ptr(gr), _ptr(sr), n, eta, rho, beta,
-              first.code if first else 0, first.p0 if first else 0.0, first.p1 if first else 0.0)
+    if isinstance(base, Fista):
+        _lib.call("fh_fista_step", ctx, _ptr(xr), _ptr(gr), _ptr(sr), n, eta, _ptr(leaf.tb),
+                  first.code if first else 0, first.p0 if first else 0.0, first.p1 if first else 0.0)
+    else:
+        _lib.call("fh_optim_step", ctx, _RULE_IDS[type(base).__name__], _ptr(xr), _ptr(gr), _ptr(sr), n, eta, rho,
+                  0.0, first.code if first else 0, first.p0 if first else 0.0, first.p1 if first else 0.0)
     zero = torch.zeros_like(xr)
     for p in proxes[1:]:  # further operators of a composition: a prox-only pass each
         _lib.call("fh_optim_step", ctx, 0, _ptr(xr), _ptr(zero), None, n, 0.0, 0.0, 0.0, p.code, p.p0, p.p1)

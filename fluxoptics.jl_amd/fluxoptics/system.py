@@ -265,11 +265,7 @@ class OpticalSystem:
 
 
 def _capture_train_step(system, loss, opt_state, x=None, weights=None):
-    from .optim import Fista, update_
-    for leaf in opt_state:
-        if isinstance(leaf.base, Fista):
-            raise NotImplementedError("Fista's step coefficient is a host value per step; record steps with "
-                                      "Descent, Momentum, Nesterov or Adam")
+    from .optim import update_
     if weights is not None:
         weights = torch.as_tensor(weights, dtype=torch.float64).cuda()
     system._withgradient_launches(loss, x, weights)  # plans, tables, workspaces; parameters unchanged

@@ -299,6 +299,13 @@ int fh_tea_backward(fh_ctx* ctx, void* dv, const float* h, const void* saved, fl
 int fh_optim_step(fh_ctx* ctx, int rule, float* x, const float* dx, float* state, long n, double eta,
                   double rho, double beta, int prox, double p0, double p1);
 
+/* Fista (OptimisersExt.jl:12-40) with its coefficients in device memory, so that a recorded step
+ * replays correctly: tb holds {t_k, beta} (start with t_1 = 1); each call sets beta = (t_k - 1) /
+ * t_{k+1}, t_k <- t_{k+1} = (1 + sqrt(1 + 4 t_k^2)) / 2, then runs fh_optim_step's Fista update with
+ * that beta (eta = the squared rate the reference stores; state = the previous iterate). */
+int fh_fista_step(fh_ctx* ctx, float* x, const float* dx, float* state, long n, double eta, double* tb, int prox,
+                  double p0, double p1);
+
 /* Adam (Optimisers.jl 0.4.6, the reference's dependency; the rule of its published
  * beam-splitter example): m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, x -= eta m/(1 - bt1) /
  * (sqrt(v/(1 - bt2)) + eps), then the proximal operator as in fh_optim_step.  state holds 2n

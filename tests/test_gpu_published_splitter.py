@@ -55,11 +55,13 @@ def test_published_splitter_loss_trajectory():
     torch.cuda.synchronize()
 
 
-def test_recorded_train_step_matches_eager_loop():
-    """capture_train_step: replays of one recorded iteration (gradient + Adam step) follow the
-    eager loop's losses and masks bit for bit (eager work between the replays)."""
+@pytest.mark.parametrize("rule", ["adam", "fista"])
+def test_recorded_train_step_matches_eager_loop(rule):
+    """capture_train_step: replays of one recorded iteration (gradient + Adam step, or Fista with a
+    clamp, whose coefficients live in device memory) follow the eager loop's losses and masks bit
+    for bit (eager work between the replays)."""
     n, ds, lam = 128, 10.0, 0.7
-    print("[recorded train step] 128^2", flush=True)
+    print(f"[recorded train step] 128^2 {rule}", flush=True)
     g = inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 80.0), ds, ds).astype(np.complex64)
     t = inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 80.0, 200.0, -150.0), ds, ds).astype(np.complex64)
     losses = {}
@@ -73,7 +75,8 @@ def test_recorded_train_step_matches_eager_loop():
             prop = fo.RSProp(u, 5e4)
         system = fo.ScalarSource(u) | does[0] | prop | does[1] | prop
         metric = fo.PowerCoupling(v)
-        opt = fo.setup(fo.Adam(0.05), system)
+        opt = fo.setup(fo.Adam(0.05) if rule == "adam" else fo.ProxRule(fo.Fista(3.0), fo.ClampProx(-1.0, 1.0)),
+                       system)
         vals = []
         if mode == "eager":
             for _ in range(5):
