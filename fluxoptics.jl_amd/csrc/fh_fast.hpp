@@ -518,7 +518,6 @@ __device__ __forceinline__ void ypipe_field(float2* v, const float2* h, float2* 
                                             int kout) {
   using RI = typename RDF::Rev;
   constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
-  constexpr int RL = RI::R0;
   constexpr int RE = LastRadix<RI>::value;
   fast_fft_b<RDF, false>(v, s, t, pf);
 #pragma unroll
