@@ -508,6 +508,22 @@ __global__ void k_tab_extract(T* __restrict__ tab, const double2* __restrict__ d
 
 // ================================================================ elementwise kernels
 
+// dst(x, y) = src(x - x0, y - y0) where that lies inside src, else fill, per field: pad! with
+// x0 = offset >= 0 and the pad value as fill (pad_crop_operators.jl:1-16), crop! with x0 = -offset
+// (:61-79).  One thread per destination element (coalesced stores, row-contiguous loads).
+__global__ void k_window(const float2* __restrict__ src, int snx, int sny, float2* __restrict__ dst, int dnx,
+                         int dny, int batch, int x0, int y0, float2 fill) {
+  const long dn = (long)dnx * dny;
+  const long total = dn * batch;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long b = i / dn;
+    const long r = i - b * dn;
+    const int y = (int)(r / dnx), x = (int)(r - (long)y * dnx);
+    const int sx = x - x0, sy = y - y0;
+    dst[i] = (sx >= 0 && sx < snx && sy >= 0 && sy < sny) ? src[(b * sny + sy) * (long)snx + sx] : fill;
+  }
+}
+
 // out = cis(s*phi) (the phase factors of phasemask.jl:90-113, once for every field of a batch)
 __global__ void k_phase_cis(const float* __restrict__ phi, float s, float2* __restrict__ out, long npix) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix; i += (long)gridDim.x * blockDim.x) {
@@ -2235,6 +2251,24 @@ int fh_phase_propagate(fh_plan* p, void* u, const float* phi, void* saved, int d
   PhaseTimeOp op{};
   if ((rc = phase_time_op(p, phi, saved, direction, &op)) || (rc = run_prop(p, f, direction, op))) return rc;
   return run_xifft(p, f, direction);
+}
+
+int fh_window(fh_ctx* ctx, const void* src, int snx, int sny, void* dst, int dnx, int dny, int batch, int x0,
+              int y0, double fill_re, double fill_im) {
+  if (!ctx || !src || !dst) return fail(FH_E_ARG, "null context, source or destination");
+  if (snx <= 0 || sny <= 0 || dnx <= 0 || dny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  const size_t sb = (size_t)snx * sny * batch * sizeof(float2), db = (size_t)dnx * dny * batch * sizeof(float2);
+  const char *a = static_cast<const char*>(src), *d = static_cast<const char*>(dst);
+  if (a < d + db && d < a + sb) return fail(FH_E_ARG, "source and destination must not overlap");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const long total = (long)dnx * dny * batch;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_window, dim3(blocks), dim3(256), 0, ctx->stream, reinterpret_cast<const float2*>(src), snx,
+                     sny, reinterpret_cast<float2*>(dst), dnx, dny, batch, x0, y0,
+                     make_float2((float)fill_re, (float)fill_im));
+  FH_HIP(hipGetLastError());
+  return FH_OK;
 }
 
 int fh_phase_apply(fh_ctx* ctx, void* u, const float* phi, int nx, int ny, int batch, int direction,

@@ -13,12 +13,14 @@ from .components import (AS_BPM, BPM, ASProp, CollinsProp, FourierLens, Paraxial
                          propagate_phase_backward_)
 from .czt import CZTPlan, plan_czt, plan_czt_
 from .group import PropagationGroup
+from .padcrop import CropOperator, FieldProbe, PadCropOperator, PadOperator, crop, crop_, pad, pad_
 from .metrics import DotProduct, PowerCoupling, SquaredIntensityDifference
 from .system import OpticalSequence, OpticalSystem, ScalarSource, merge
 from .optim import (Adam, ClampProx, CompositeProx, Descent, Fista, IstaProx, Momentum, Nesterov, NoDescent,
                     PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
+    "CropOperator", "FieldProbe", "PadCropOperator", "PadOperator", "crop", "crop_", "pad", "pad_",
     "CZTPlan", "plan_czt", "plan_czt_", "AS_BPM", "BPM", "Shift_BPM", "TiltAnchor", "ASProp", "CollinsProp", "FourierLens", "ParaxialProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
     "RSProp", "ScalarField", "ShiftProp", "Mask", "TeaDOE", "TeaReflector", "FourierPhase", "FourierMask",
     "Adam", "ClampProx", "CompositeProx", "Descent", "Fista", "IstaProx", "Momentum", "Nesterov", "NoDescent",

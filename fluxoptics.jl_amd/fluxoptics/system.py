@@ -27,6 +27,7 @@ from .components import (BPM, CollinsProp, TiltAnchor, ASProp, FourierOperator, 
 from .components import phase_propagate_, propagate_phase_backward_
 from .fields import Backward, Forward, ScalarField, dir_sign
 from .metrics import SquaredIntensityDifference
+from .padcrop import FieldProbe
 from . import _lib
 
 
@@ -186,9 +187,12 @@ class OpticalSystem:
         return x if self.inplace else x.copy()
 
     def __call__(self, x: Optional[ScalarField] = None) -> SystemOutput:
+        """``system(x)`` (``optical_system.jl:189-214``): ``out`` and, for every ``FieldProbe``,
+        a copy of the field where it sits (``probes``)."""
         u = self._start(x)
-        _run_forward(list(self.components), u, self.direction, save=False)
-        return SystemOutput(u)
+        probes: Dict = {}
+        _run_forward(list(self.components), u, self.direction, save=False, probes=probes)
+        return SystemOutput(u, probes)
 
     # -- gradient
     def withgradient(self, loss, x: Optional[ScalarField] = None, weights=None, fuse: bool = True):
@@ -354,14 +358,21 @@ def _sqint_block(loss, u: ScalarField, ph, prop, tape, grads: Dict, weights, src
     return out
 
 
-def _run_forward(comps: List, u: ScalarField, direction, save: bool) -> list:
+def _run_forward(comps: List, u: ScalarField, direction, save: bool, probes: Optional[Dict] = None) -> list:
     """Run the components in order; returns the tape of (kind, components, saved buffer).
-    A Phase followed by a propagator runs fused with it."""
+    A Phase followed by a propagator runs fused with it; a FieldProbe stores a copy of the field
+    in ``probes`` when that is given."""
     tape = []
     seq = comps if dir_sign(direction) > 0 else comps[::-1]
     i = 0
     while i < len(seq):
         c = seq[i]
+        if isinstance(c, FieldProbe):
+            if probes is not None:
+                probes[c] = u.copy()
+            tape.append(("plain", (c,), None))
+            i += 1
+            continue
         if isinstance(c, Phase) and i + 1 < len(seq) and _is_prop(seq[i + 1]):
             p = seq[i + 1]
             saved = None

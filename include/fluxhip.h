@@ -299,6 +299,13 @@ int fh_tea_backward(fh_ctx* ctx, void* dv, const float* h, const void* saved, fl
 int fh_optim_step(fh_ctx* ctx, int rule, float* x, const float* dx, float* state, long n, double eta,
                   double rho, double beta, int prox, double p0, double p1);
 
+/* pad! / crop! (pad_crop_operators.jl:1-16,61-79) on c64 fields of batch fields: dst (dnx x dny per
+ * field) receives dst(x, y) = src(x - x0, y - y0) where that lies inside src (snx x sny per field),
+ * else fill.  pad!(u_pad, u; offset, pad_val) is x0, y0 = offset >= 0, fill = pad_val;
+ * crop!(u_crop, u; offset) is x0, y0 = -offset.  src and dst must not overlap. */
+int fh_window(fh_ctx* ctx, const void* src, int snx, int sny, void* dst, int dnx, int dny, int batch, int x0,
+              int y0, double fill_re, double fill_im);
+
 /* Fista (OptimisersExt.jl:12-40) with its coefficients in device memory, so that a recorded step
  * replays correctly: tb holds {t_k, beta} (start with t_1 = 1); each call sets beta = (t_k - 1) /
  * t_{k+1}, t_k <- t_{k+1} = (1 + sqrt(1 + 4 t_k^2)) / 2, then runs fh_optim_step's Fista update with

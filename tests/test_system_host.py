@@ -272,3 +272,23 @@ def test_host_dispatch_and_argument_parsing():
     # TiltAnchor keeps the template's tilts
     a = fo.TiltAnchor(u)
     assert a.tilts == ([0.01, 0.02], 0.0) and not a.trainable
+
+
+def test_pad_crop_host_checks():
+    """pad / crop (pad_crop_operators.jl:1-16,61-79) check offsets and sizes before any device
+    work, and take device tensors only; PadCropOperator's adjoint swaps pad and crop."""
+    import torch
+    u = torch.zeros(2, 8, 6, dtype=torch.complex64)
+    with pytest.raises(ValueError):
+        fo.pad(u, (10, 10), offset=(-1, 0))
+    with pytest.raises(ValueError):
+        fo.pad(u, (10, 10), offset=(5, 0))  # 6 + 5 > 10
+    with pytest.raises(ValueError):
+        fo.crop(u, (4, 4), offset=(3, 0))   # 4 + 3 > 6
+    with pytest.raises(ValueError):
+        fo.pad(u, (10,))
+    with pytest.raises(TypeError):
+        fo.pad(u, (10, 10))                 # no CPU path
+    op = fo.PadCropOperator(u, torch.zeros(2, 16, 12, dtype=torch.complex64))
+    assert op.ispad and not op.adjoint().ispad and op.adjoint().adjoint().ispad
+    assert op.p_pad.size_pad == (12, 16) and op.p_crop.size_crop == (6, 8)
