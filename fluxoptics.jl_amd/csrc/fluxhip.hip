@@ -836,6 +836,27 @@ k_sqint_partial(const float2* __restrict__ u, const float* __restrict__ I, long 
   if (threadIdx.x == 0) part[slot * (long)gridDim.x + blockIdx.x] = s;
 }
 
+// SquaredFieldDifference (squared_field_difference.jl): loss[b] = sum |u_b - v_b|^2 (formed in
+// FP64), pullback du_b = 2 dc[b] (u_b - v_b); the same fixed-shape sum as k_sqint_partial.
+__global__ void __launch_bounds__(kMetricThreads)
+k_sqfield_partial(const float2* __restrict__ u, const float2* __restrict__ v, long npix,
+                  const double* __restrict__ dc, float2* __restrict__ du, double* __restrict__ part) {
+  __shared__ double red[kMetricThreads / 64];
+  const int slot = blockIdx.y;
+  const double g = dc ? 2.0 * dc[slot] : 2.0;
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)kMetricThreads + threadIdx.x; i < npix;
+       i += (long)gridDim.x * kMetricThreads) {
+    const long k = slot * npix + i;
+    const float2 a = u[k], b = v[k];
+    const double dx = (double)a.x - b.x, dy = (double)a.y - b.y;
+    acc += dx * dx + dy * dy;
+    if (du) du[k] = make_float2((float)(g * dx), (float)(g * dy));
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) part[slot * (long)gridDim.x + blockIdx.x] = s;
+}
+
 // loss[slot] = ordered sum of the slot's partials (one workgroup per slot)
 __global__ void __launch_bounds__(kMetricThreads)
 k_sum_partials(const double* __restrict__ part, int nper, double* __restrict__ out) {
@@ -2571,6 +2592,23 @@ int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx
                      reinterpret_cast<float2*>(du), ctx->metric_part);
   FH_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_sum_partials, dim3(nout), dim3(kMetricThreads), 0, ctx->stream,
+                     (const double*)ctx->metric_part, kMetricBlocks, loss);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_sq_field_diff(fh_ctx* ctx, const void* u, const void* v, int nx, int ny, int batch, const double* dc,
+                     double* loss, void* du) {
+  if (!ctx || !u || !v || !loss) return fail(FH_E_ARG, "null context, field, target field or loss");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = metric_partials(ctx, (size_t)batch * kMetricBlocks * sizeof(double)))) return rc;
+  hipLaunchKernelGGL(k_sqfield_partial, dim3(kMetricBlocks, batch), dim3(kMetricThreads), 0, ctx->stream,
+                     reinterpret_cast<const float2*>(u), reinterpret_cast<const float2*>(v), (long)nx * ny, dc,
+                     reinterpret_cast<float2*>(du), ctx->metric_part);
+  FH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(kMetricThreads), 0, ctx->stream,
                      (const double*)ctx->metric_part, kMetricBlocks, loss);
   FH_HIP(hipGetLastError());
   return FH_OK;

@@ -14,12 +14,13 @@ from .components import (AS_BPM, BPM, ASProp, CollinsProp, FourierLens, Paraxial
 from .czt import CZTPlan, plan_czt, plan_czt_
 from .group import PropagationGroup
 from .padcrop import CropOperator, FieldProbe, PadCropOperator, PadOperator, crop, crop_, pad, pad_
-from .metrics import DotProduct, PowerCoupling, SquaredIntensityDifference
+from .metrics import DotProduct, PowerCoupling, SquaredFieldDifference, SquaredIntensityDifference
 from .system import OpticalSequence, OpticalSystem, ScalarSource, merge
 from .optim import (Adam, ClampProx, CompositeProx, Descent, Fista, IstaProx, Momentum, Nesterov, NoDescent,
                     PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
+    "SquaredFieldDifference",
     "CropOperator", "FieldProbe", "PadCropOperator", "PadOperator", "crop", "crop_", "pad", "pad_",
     "CZTPlan", "plan_czt", "plan_czt_", "AS_BPM", "BPM", "Shift_BPM", "TiltAnchor", "ASProp", "CollinsProp", "FourierLens", "ParaxialProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",
     "RSProp", "ScalarField", "ShiftProp", "Mask", "TeaDOE", "TeaReflector", "FourierPhase", "FourierMask",

@@ -93,6 +93,7 @@ SIGNATURES = {
                                c_void_p, c_void_p, c_int]),
     "fh_sqint_block_src": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int,
                                    c_void_p, c_void_p, c_int]),
+    "fh_sq_field_diff": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "fh_window": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int, c_int, c_int, c_double,
                           c_double]),
     "fh_fista_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_long, c_double, c_void_p, c_int,

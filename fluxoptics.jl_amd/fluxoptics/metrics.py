@@ -2,7 +2,8 @@
 
 ``SquaredIntensityDifference`` mirrors ``src/metrics/squared_intensity_difference.jl``
 (constructor from ``(field, target_intensity)`` pairs, ``compute_metric`` at :78-83 and
-``backpropagate_metric`` at :85-90); ``DotProduct`` and ``PowerCoupling`` mirror
+``backpropagate_metric`` at :85-90); ``SquaredFieldDifference`` mirrors
+``src/metrics/squared_field_difference.jl``; ``DotProduct`` and ``PowerCoupling`` mirror
 ``src/metrics/dot_product.jl`` and ``power_coupling.jl`` (mode-selective).  The sums run in
 libfluxhip (FP64 accumulation in a fixed order); nothing is computed on the host.
 """
@@ -85,6 +86,59 @@ class SquaredIntensityDifference:
     def value_and_pullback(self, fields: Sequence[ScalarField], dcs=None, inplace: bool = False):
         """Loss values and cotangents in one pass over each field (``inplace`` overwrites the
         fields with their cotangents)."""
+        losses, cots = [], []
+        for k, u in enumerate(fields):
+            du = u.electric if inplace else torch.empty_like(u.electric)
+            losses.append(self._run(k, u, None if dcs is None else dcs[k], du))
+            cots.append(u if inplace else u.set_field_data(du))
+        return tuple(losses), tuple(cots)
+
+
+class SquaredFieldDifference:
+    """``SquaredFieldDifference(fields...)`` (``metrics/squared_field_difference.jl``): per target
+    field v, one value per field of the batch, ``sum |u_b - v_b|^2`` (no area factor, as the
+    reference's ``sum!(abs2, c, u - v)``), and the pullback ``2 dc (u - v)``."""
+
+    def __init__(self, *fields: ScalarField):
+        if not fields:
+            raise ValueError("SquaredFieldDifference needs at least one target field")
+        for v in fields:
+            check_field(v.electric)
+        self.targets = [v.electric for v in fields]
+
+    def _run(self, k: int, u: ScalarField, dc, du: Optional[torch.Tensor]):
+        check_field(u.electric)
+        v = self.targets[k]
+        if tuple(v.shape) != tuple(u.electric.shape):
+            raise ValueError("field does not match the metric's target field")
+        loss = torch.empty(u.batch, dtype=torch.float64, device=u.electric.device)
+        if dc is not None:
+            dc = torch.as_tensor(dc, dtype=torch.float64, device=u.electric.device).reshape(-1).contiguous()
+            if dc.numel() == 1 and u.batch > 1:
+                dc = dc.expand(u.batch).contiguous()
+            if dc.numel() != u.batch:
+                raise ValueError(f"dc must hold {u.batch} values")
+        _lib.call("fh_sq_field_diff", context(u.electric.device), _ptr(u.electric), _ptr(v), u.nx, u.ny, u.batch,
+                  _ptr(dc), _ptr(loss), _ptr(du))
+        return loss
+
+    def compute_metric(self, *fields: ScalarField):
+        if len(fields) != len(self.targets):
+            raise ValueError(f"expected {len(self.targets)} fields")
+        return tuple(self._run(k, u, None, None) for k, u in enumerate(fields))
+
+    __call__ = compute_metric
+
+    def backpropagate_metric(self, fields: Sequence[ScalarField], dcs=None):
+        """Cotangent fields 2 dc (u - v), one per target (new buffers)."""
+        out = []
+        for k, u in enumerate(fields):
+            du = torch.empty_like(u.electric)
+            self._run(k, u, None if dcs is None else dcs[k], du)
+            out.append(u.set_field_data(du))
+        return tuple(out)
+
+    def value_and_pullback(self, fields: Sequence[ScalarField], dcs=None, inplace: bool = False):
         losses, cots = [], []
         for k, u in enumerate(fields):
             du = u.electric if inplace else torch.empty_like(u.electric)

@@ -197,6 +197,13 @@ int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double sc
 int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx, int ny, int batch,
                          int target_batch, const double* dc, double* loss, void* du);
 
+/* SquaredFieldDifference (squared_field_difference.jl, compute_metric and backpropagate_metric):
+ * loss[b] = sum |u_b - v_b|^2 over the pixels of field b (summed in FP64 in a fixed order); when du
+ * is not NULL the pullback du_b = 2 dc[b] (u_b - v_b) is written there (du may equal u); dc as in
+ * fh_sq_intensity_diff. */
+int fh_sq_field_diff(fh_ctx* ctx, const void* u, const void* v, int nx, int ny, int batch, const double* dc,
+                     double* loss, void* du);
+
 /* The tail of a differentiated system ending in [Phase ->] propagator -> SquaredIntensityDifference,
  * fused: propagate u (the Phase, when phi is given, rides on the first pass and saves its input to
  * saved), then one pass does the last x IFFT, the metric values (loss[b], one per field; the target

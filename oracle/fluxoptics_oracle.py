@@ -701,6 +701,17 @@ def half_sq_field_diff(v, t):
     return 0.5 * float(np.vdot(d, d).real), d
 
 
+def sq_field_diff_values(v, t, dc=None):
+    """``SquaredFieldDifference`` (``metrics/squared_field_difference.jl``, compute_metric and
+    backpropagate_metric): per field b, c_b = sum |v_b - t_b|^2 (no area factor) and the pullback
+    2 dc_b (v_b - t_b); returns (c, dv)."""
+    v, t = _as_batch(np.asarray(v, np.complex128)), _as_batch(np.asarray(t, np.complex128))
+    d = v - t
+    c = (np.abs(d) ** 2).sum(axis=(-2, -1))
+    g = 2.0 * (np.ones(len(c)) if dc is None else np.asarray(dc, np.float64).reshape(-1))
+    return c, g[:, None, None] * d
+
+
 def coupling_efficiency(u, v) -> np.ndarray:
     """|<u, v>|^2 / (<u,u><v,v>) per slice (``Fields.jl:748-756``)."""
     u, v = _as_batch(u), _as_batch(v)

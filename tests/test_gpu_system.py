@@ -228,3 +228,41 @@ def test_mode_coupling_system_gradient():
     assert abs(float(value) + p_ref.sum()) <= FIELD_TOL * p_ref.sum()
     torch.cuda.synchronize()
     assert rel_l2(doe.grad.cpu().numpy(), g_ref) <= GRAD_TOL
+
+
+def test_sq_field_difference_on_device():
+    """SquaredFieldDifference: the reference's testset values (test/metrics_test.jl:77-99) and its
+    "Metric Backpropagation" system (source | trainable Phase, :101-131), loss and mask gradient
+    against the oracle chain; values and weighted pullback of a batch of three against the oracle."""
+    print("[sq field difference]", flush=True)
+    n = 32
+    ones = fo.ScalarField(to_dev(np.ones((n, n))), (1.0, 1.0), 1.0)
+    twos = fo.ScalarField(to_dev(2 * np.ones((n, n))), (1.0, 1.0), 1.0)
+    assert float(fo.SquaredFieldDifference(twos)(ones)[0][0]) == 1024.0
+    assert float(fo.SquaredFieldDifference(ones)(ones)[0][0]) == 0.0
+    u0 = c64(np.stack([inputs.gaussian(48, 40, 1.0, 1.0, 8.0 + 2 * k) for k in range(3)]) * (1 + 0.5j))
+    t0 = c64(np.stack([inputs.gaussian(48, 40, 1.0, 1.0, 9.0, 3.0 - k, 1.0) for k in range(3)]))
+    u = fo.ScalarField(to_dev(u0), (1.0, 1.0), [0.8, 0.7, 0.6])
+    t = fo.ScalarField(to_dev(t0), (1.0, 1.0), [0.8, 0.7, 0.6])
+    m = fo.SquaredFieldDifference(t)
+    dc = [0.5, -1.0, 2.0]
+    (vals,), (du,) = m.value_and_pullback((u,), [dc])
+    c_ref, dv_ref = O.sq_field_diff_values(u0, t0, dc)
+    assert rel_l2(vals.cpu().numpy(), c_ref) <= 1e-6
+    assert rel_l2(to_host(du.electric), dv_ref) <= 1e-6
+    # the reference's backpropagation testset: source | Phase(zeros, trainable) against a shifted target
+    x = inputs.grid(n, 1.0)
+    g0 = c64(inputs.gaussian(n, n, 1.0, 1.0, 10.0))
+    tg = c64(inputs.gaussian(n, n, 1.0, 1.0, 10.0, 5.0, 0.0))
+    src = fo.ScalarField(to_dev(g0), (1.0, 1.0), 1.0)
+    phi = (0.3 * np.cos(2 * math.pi * x[None, :] / 16.0) * np.ones((n, 1))).astype(np.float32)
+    ph = fo.Phase(src, phi, trainable=True, buffered=True)
+    metric = fo.SquaredFieldDifference(fo.ScalarField(to_dev(tg), (1.0, 1.0), 1.0))
+    value, grads = (fo.ScalarSource(src) | ph).withgradient(metric)
+    torch.cuda.synchronize()
+    stages = [("phase", phi.astype(np.float64))]
+    v, saved = O.pipeline_forward(g0, stages, (1.0, 1.0), 1.0)
+    c, dv = O.sq_field_diff_values(v, tg)
+    _, (g_ref,) = O.pipeline_backward(dv, stages, (1.0, 1.0), 1.0, saved)
+    assert abs(float(value) - c.sum()) <= 1e-6 * c.sum()
+    assert rel_l2(grads[ph].cpu().numpy(), g_ref) <= 1e-5

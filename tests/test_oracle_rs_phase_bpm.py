@@ -245,3 +245,27 @@ def test_general_bpm_adjoint_and_dn_gradient(kind):
     _, ddn = O.bpm_backward(dv, dn, L, n0, lams, ds, saved, **kw)
     idx = [int(i) for i in inputs.splitmix64(99, 10) % np.uint64(ns * n * n)]
     _fd_check(loss, dn, idx, ddn, h=1e-5)
+
+
+def test_oracle_sq_field_diff_pins():
+    """SquaredFieldDifference: the reference's doctest (4x4 ones against zeros -> 16,
+    squared_field_difference.jl), its testset (32x32 ones against 2 ones -> 1024, a field against
+    itself -> 0; test/metrics_test.jl:77-99), and the pullback 2 dc (v - t) against central finite
+    differences of the loss in the real and imaginary parts of single pixels."""
+    c, _ = O.sq_field_diff_values(np.zeros((4, 4)), np.ones((4, 4)))
+    assert c.shape == (1,) and c[0] == 16.0
+    assert O.sq_field_diff_values(np.ones((32, 32)), 2 * np.ones((32, 32)))[0][0] == 1024.0
+    u = inputs.normal(61, (2, 6, 5)) + 1j * inputs.normal(62, (2, 6, 5))
+    assert np.all(O.sq_field_diff_values(u, u)[0] == 0.0)
+    t = inputs.normal(63, (2, 6, 5)) + 1j * inputs.normal(64, (2, 6, 5))
+    dc = np.array([0.7, -1.3])
+    _, dv = O.sq_field_diff_values(u, t, dc)
+    loss = lambda w: float(np.dot(dc, O.sq_field_diff_values(w, t)[0]))
+    h = 1e-6
+    for (b, y, x) in [(0, 1, 2), (1, 5, 0), (1, 3, 4)]:
+        for unit, part in ((1.0, "re"), (1j, "im")):
+            e = np.zeros_like(u)
+            e[b, y, x] = unit * h
+            fd = (loss(u + e) - loss(u - e)) / (2 * h)
+            an = dv[b, y, x].real if part == "re" else dv[b, y, x].imag
+            assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
