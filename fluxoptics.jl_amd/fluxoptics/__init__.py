@@ -5,8 +5,8 @@ ShiftProp), modulators, BPM volumes (AS_BPM, Shift_BPM), metrics and optimisers;
 the arithmetic is the HIP library's (``libfluxhip.so``, C ABI in include/fluxhip.h).
 """
 
-from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficiency,
-                     rs_valid_distance)
+from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficiency, intensity,
+                     normalize_power_, phase, power, rs_valid_distance)
 from .components import (AS_BPM, BPM, ASProp, CollinsProp, FourierLens, ParaxialProp, FourierMask, FourierOperator, FourierPhase, FourierWrapper,
                          Mask, Phase, RSProp, ShiftProp, Shift_BPM, TeaDOE, TeaReflector, TiltAnchor,
                          phase_propagate_,
@@ -20,6 +20,7 @@ from .optim import (Adam, ClampProx, CompositeProx, Descent, Fista, IstaProx, Mo
                     PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
+    "intensity", "normalize_power_", "phase", "power",
     "SquaredFieldDifference",
     "CropOperator", "FieldProbe", "PadCropOperator", "PadOperator", "crop", "crop_", "pad", "pad_",
     "CZTPlan", "plan_czt", "plan_czt_", "AS_BPM", "BPM", "Shift_BPM", "TiltAnchor", "ASProp", "CollinsProp", "FourierLens", "ParaxialProp", "Backward", "Direction", "FourierOperator", "FourierWrapper", "Forward", "Phase",

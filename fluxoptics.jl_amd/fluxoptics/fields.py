@@ -147,10 +147,67 @@ class ScalarField:
         self.electric *= scale.reshape((-1, 1, 1) if self.electric.dim() == 3 else (1, 1))
         return self
 
+    # -- arithmetic (the reference's test/fields_test.jl:74-93): elementwise on the data, the
+    # result keeps this field's grid, wavelengths and tilts
+    def _same_grid(self, other: "ScalarField") -> None:
+        if tuple(other.electric.shape) != tuple(self.electric.shape) or other.ds != self.ds:
+            raise ValueError("fields must share shape and sampling")
+        if other.lambda_list() != self.lambda_list():
+            raise ValueError("fields must share wavelengths")
+
+    def __add__(self, other):
+        if isinstance(other, ScalarField):
+            self._same_grid(other)
+            return self.set_field_data(self.electric + other.electric)
+        return NotImplemented
+
+    def __sub__(self, other):
+        if isinstance(other, ScalarField):
+            self._same_grid(other)
+            return self.set_field_data(self.electric - other.electric)
+        return NotImplemented
+
+    def __mul__(self, a):
+        if isinstance(a, (int, float, complex)):
+            return self.set_field_data(self.electric * a)
+        return NotImplemented
+
+    __rmul__ = __mul__
+
+    def __neg__(self):
+        return self.set_field_data(-self.electric)
+
+    def conj(self) -> "ScalarField":
+        """``conj(u)`` (``Fields.jl:899``): the complex conjugate field."""
+        return self.set_field_data(self.electric.conj().resolve_conj())
+
     def __repr__(self):
         return (f"ScalarField(shape={tuple(self.electric.shape)}, dtype={self.electric.dtype}, "
                 f"device={self.electric.device}, ds={self.ds}, lambdas={self.lambdas}, "
                 f"tilts={self.tilts})")
+
+
+def intensity(u) -> torch.Tensor:
+    """``intensity(u)`` (``Fields.jl:608-616``): sum over the fields of |u|^2, shape (ny, nx)."""
+    e = u.electric if isinstance(u, ScalarField) else u
+    i = e.abs() ** 2
+    return i if i.dim() == 2 else i.sum(dim=0)
+
+
+def phase(u) -> torch.Tensor:
+    """``phase(u)`` (``Fields.jl:667-673``): arg(u) in [-pi, pi], the data's shape."""
+    e = u.electric if isinstance(u, ScalarField) else u
+    return torch.angle(e)
+
+
+def power(u: ScalarField) -> torch.Tensor:
+    """``power(u)`` (``Fields.jl:843-846``)."""
+    return u.power()
+
+
+def normalize_power_(u: ScalarField, target: float = 1.0) -> ScalarField:
+    """``normalize_power!(u, P)`` (``Fields.jl:889-892``)."""
+    return u.normalize_power_(target)
 
 
 def coupling_efficiency(u: ScalarField, v: ScalarField) -> torch.Tensor:

@@ -266,3 +266,32 @@ def test_sq_field_difference_on_device():
     _, (g_ref,) = O.pipeline_backward(dv, stages, (1.0, 1.0), 1.0, saved)
     assert abs(float(value) - c.sum()) <= 1e-6 * c.sum()
     assert rel_l2(grads[ph].cpu().numpy(), g_ref) <= 1e-5
+
+
+def test_reference_field_testsets_on_device():
+    """test/fields_test.jl:49-130 on device fields: power and normalize_power! (a waist-20 Gaussian
+    on 128^2, ds 0.5), arithmetic, copy independence, intensity and phase of exp(i(x + y))."""
+    x = inputs.grid(128, 0.5)
+    u = fo.ScalarField(to_dev(inputs.gaussian(128, 128, 0.5, 0.5, 20.0)), (0.5, 0.5), 1.064)
+    assert float(fo.power(u)[0]) > 0
+    fo.normalize_power_(u, 1.0)
+    assert abs(float(fo.power(u)[0]) - 1.0) <= 1e-6
+    fo.normalize_power_(u, 2.5)
+    assert abs(float(fo.power(u)[0]) - 2.5) <= 2.5e-6
+    u1 = fo.ScalarField(to_dev(np.ones((32, 32))), (1.0, 1.0), 1.0)
+    u2 = fo.ScalarField(to_dev(2 * np.ones((32, 32))), (1.0, 1.0), 1.0)
+    assert np.all(to_host((u1 + u2).electric) == 3.0)
+    assert np.all(to_host((u2 - u1).electric) == 1.0)
+    assert np.all(to_host((3 * u1).electric) == 3.0)
+    u3 = u1.copy()
+    u3.electric[0, 0] = 2.0
+    assert to_host(u1.electric)[0, 0] == 1.0 and to_host(u3.electric)[0, 0] == 2.0
+    y = inputs.grid(64, 1.0)
+    xx = inputs.grid(64, 1.0)
+    f = fo.ScalarField(to_dev(np.exp(1j * (xx[None, :] + y[:, None]))), (1.0, 1.0), 1.064)
+    assert np.allclose(fo.intensity(f).cpu().numpy(), 1.0, atol=1e-6)
+    ph = fo.phase(f).cpu().numpy()
+    assert ph.shape == (64, 64) and np.all(np.abs(ph) <= math.pi)
+    assert np.allclose(to_host(f.conj().electric), np.conj(to_host(f.electric)))
+    with pytest.raises(ValueError):
+        u1 + fo.ScalarField(to_dev(np.ones((32, 32))), (1.0, 1.0), 0.8)
