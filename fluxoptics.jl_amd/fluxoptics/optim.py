@@ -135,9 +135,15 @@ def _trainable_components(model) -> list:
 
 
 def make_rules(*pairs) -> Dict:
-    """``make_rules(component => rule, ...)`` (``OptimisersExt.jl:160-190``): a dict keyed by
-    component."""
-    return {c: r for c, r in pairs}
+    """``make_rules(component => rule, ...)`` (``OptimisersExt.jl:160-190``): a dict keyed, as
+    the reference's, by each component's trainable parameter arrays (``rules[phase.phi] is
+    rule``), and by the component itself."""
+    out = {}
+    for c, r in pairs:
+        out[c] = r
+        for _, x in _params(c):
+            out[x] = r
+    return out
 
 
 class _Leaf:
@@ -158,6 +164,14 @@ class _Leaf:
             self.tb = torch.tensor([1.0, 0.0], dtype=torch.float64, device=tensor.device)
 
 
+def _rule_of(rules: Dict, x: torch.Tensor, c, default_rule):
+    # parameter keys first (tensors hash by identity), then the component
+    for k, r in rules.items():
+        if k is x:
+            return r
+    return rules.get(c, default_rule)
+
+
 def setup(rules, model, default_rule=None):
     """``setup(rules, [default_rule,] model)`` (``OptimisersExt.jl:95-140``): one optimiser state
     per trainable parameter; parameters without a rule get ``default_rule`` (NoDescent).
@@ -168,8 +182,8 @@ def setup(rules, model, default_rule=None):
     default_rule = NoDescent() if default_rule is None else default_rule
     leaves = []
     for c in _trainable_components(model):
-        rule = rules.get(c, default_rule)
         for _, x in _params(c):
+            rule = _rule_of(rules, x, c, default_rule)
             leaves.append(_Leaf(c, x, rule))
     if not leaves:
         import warnings

@@ -292,3 +292,19 @@ def test_pad_crop_host_checks():
     op = fo.PadCropOperator(u, torch.zeros(2, 16, 12, dtype=torch.complex64))
     assert op.ispad and not op.adjoint().ispad and op.adjoint().adjoint().ispad
     assert op.p_pad.size_pad == (12, 16) and op.p_crop.size_crop == (6, 8)
+
+
+def test_make_rules_keys_like_the_reference():
+    """test/optimisers_test.jl:87-106: make_rules(phase1 => rule1, phase2 => rule2) is keyed by
+    the parameter arrays (rules[phase1.phi] is rule1); setup takes each parameter's rule."""
+    import torch
+    u = fo.ScalarField(torch.ones(32, 32, dtype=torch.complex64), (1.0, 1.0), 1.064)
+    p1 = fo.Phase.__new__(fo.Phase)
+    p2 = fo.Phase.__new__(fo.Phase)
+    for p in (p1, p2):
+        p.phi, p.trainable = torch.zeros(32, 32), True
+    r1, r2 = fo.Fista(0.1), fo.Fista(0.05)
+    rules = fo.make_rules((p1, r1), (p2, r2))
+    assert p1.phi in rules and rules[p1.phi] is r1 and rules[p2.phi] is r2
+    leaves = fo.setup(rules, [p1, p2])
+    assert [leaf.rule for leaf in leaves] == [r1, r2]
