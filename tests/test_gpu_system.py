@@ -295,3 +295,13 @@ def test_reference_field_testsets_on_device():
     assert np.allclose(to_host(f.conj().electric), np.conj(to_host(f.electric)))
     with pytest.raises(ValueError):
         u1 + fo.ScalarField(to_dev(np.ones((32, 32))), (1.0, 1.0), 0.8)
+
+
+def test_reference_multiple_fields_metric():
+    """test/metrics_test.jl:133-152: a SquaredIntensityDifference of two (field, target) pairs
+    gives one value per pair, both zero for matching intensities."""
+    u1 = fo.ScalarField(to_dev(np.ones((32, 32))), (1.0, 1.0), 1.0)
+    u2 = fo.ScalarField(to_dev(2 * np.ones((32, 32))), (1.0, 1.0), 1.0)
+    metric = fo.SquaredIntensityDifference((u1, np.ones((32, 32))), (u2, 4 * np.ones((32, 32))))
+    result = metric(u1, u2)
+    assert len(result) == 2 and all(float(r.sum()) == 0.0 for r in result)
