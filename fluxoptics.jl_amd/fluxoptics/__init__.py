@@ -13,6 +13,7 @@ from .components import (AS_BPM, BPM, ASProp, CollinsProp, FourierLens, Paraxial
                          propagate_phase_backward_)
 from .czt import CZTPlan, plan_czt, plan_czt_
 from .group import PropagationGroup
+from .api import (FFTPlans, backpropagate, backpropagate_metric, compute_ft_, compute_ift_, compute_metric, dot, get_data, get_lambdas, get_lambdas_collection, get_source, get_tilts, get_tilts_collection, is_on_axis, isbuffered, istrainable, make_fft_plans, offset_tilts_, set_field_data, set_field_ds_, set_field_tilts, spatial_vectors, trainable)
 from .padcrop import CropOperator, FieldProbe, PadCropOperator, PadOperator, crop, crop_, pad, pad_
 from .metrics import DotProduct, PowerCoupling, SquaredFieldDifference, SquaredIntensityDifference
 from .system import OpticalSequence, OpticalSystem, ScalarSource, merge
@@ -20,6 +21,7 @@ from .optim import (Adam, ClampProx, CompositeProx, Descent, Fista, IstaProx, Mo
                     PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
+    "FFTPlans", "backpropagate", "backpropagate_metric", "compute_ft_", "compute_ift_", "compute_metric", "dot", "get_data", "get_lambdas", "get_lambdas_collection", "get_source", "get_tilts", "get_tilts_collection", "is_on_axis", "isbuffered", "istrainable", "make_fft_plans", "offset_tilts_", "set_field_data", "set_field_ds_", "set_field_tilts", "spatial_vectors", "trainable",
     "intensity", "normalize_power_", "phase", "power",
     "SquaredFieldDifference",
     "CropOperator", "FieldProbe", "PadCropOperator", "PadOperator", "crop", "crop_", "pad", "pad_",

@@ -305,3 +305,21 @@ def test_reference_multiple_fields_metric():
     metric = fo.SquaredIntensityDifference((u1, np.ones((32, 32))), (u2, 4 * np.ones((32, 32))))
     result = metric(u1, u2)
     assert len(result) == 2 and all(float(r.sum()) == 0.0 for r in result)
+
+
+def test_function_forms_of_field_and_fft_helpers():
+    """make_fft_plans / compute_ft! / compute_ift! (FFTutils.jl:13-28) against numpy's FFT and as a
+    round trip; dot (Fields.jl:810-814); field accessors and setters."""
+    u0 = c64(np.random.default_rng(3).normal(size=(2, 24, 40)) + 1j * np.random.default_rng(4).normal(size=(2, 24, 40)))
+    u = fo.ScalarField(to_dev(u0), (1.0, 0.5), [0.8, 0.7], tilts=([0.0, 0.01], 0.0))
+    plans = fo.make_fft_plans(u)
+    fo.compute_ft_(plans, u)
+    assert rel_l2(to_host(u.electric), np.fft.fft2(u0.astype(np.complex128))) <= 1e-6
+    fo.compute_ift_(plans, u)
+    assert rel_l2(to_host(u.electric), u0) <= 1e-6
+    d = fo.dot(u, u).cpu().numpy()
+    assert np.allclose(d, (np.abs(to_host(u.electric).astype(np.complex128)) ** 2).sum(axis=(1, 2)), rtol=1e-12)
+    assert fo.get_lambdas_collection(u) == [0.8, 0.7] and not fo.is_on_axis(u)
+    assert fo.get_tilts_collection(u)[0] == [0.0, 0.01]
+    fo.set_field_ds_(u, (2.0, 2.0))
+    assert u.ds == (2.0, 2.0) and fo.set_field_data(u, u.electric * 0).ds == (2.0, 2.0)

@@ -308,3 +308,16 @@ def test_make_rules_keys_like_the_reference():
     assert p1.phi in rules and rules[p1.phi] is r1 and rules[p2.phi] is r2
     leaves = fo.setup(rules, [p1, p2])
     assert [leaf.rule for leaf in leaves] == [r1, r2]
+
+
+def test_reference_spatial_vector_testsets():
+    """test/gridutils_test.jl:2-33: the 1-D, 2-D and tuple forms of spatial_vectors, with offsets."""
+    x, = fo.spatial_vectors(64, 2.0)
+    assert len(x) == 64 and np.allclose(np.diff(x), 2.0) and abs(x[31] + x[32]) < 1e-10
+    assert abs(fo.spatial_vectors(64, 2.0, xc=5.0)[0].mean() + 5.0) < 1e-10
+    xv, yv = fo.spatial_vectors(128, 128, 2.0, 2.0)
+    assert len(xv) == len(yv) == 128 and np.allclose(np.diff(yv), 2.0)
+    xo, yo = fo.spatial_vectors(128, 128, 2.0, 2.0, xc=-5.0, yc=30.5)
+    assert abs(xo.mean() - 5.0) < 1e-10 and abs(yo.mean() + 30.5) < 1e-10
+    x2, y2 = fo.spatial_vectors((128, 128), (2.0, 2.0), offset=(-5.0, 30.5))
+    assert np.array_equal(x2, xo) and np.array_equal(y2, yo)
