@@ -857,6 +857,34 @@ k_sqfield_partial(const float2* __restrict__ u, const float2* __restrict__ v, lo
   if (threadIdx.x == 0) part[slot * (long)gridDim.x + blockIdx.x] = s;
 }
 
+// dL/dz of an on-axis ASProp with a trainable distance (ASPropZ, angular_spectrum.jl:236-300):
+// partials of Re(conj(DV) * 2 pi i w~ * V) over one field's spectrum, w = sqrt((n0/lam)^2 - f^2)
+// (principal root), w~ = conj(w) for z < 0 (H(z) = conj(H(-z))), all in FP64.
+__global__ void __launch_bounds__(kMetricThreads)
+k_as_zgrad_partial(const float2* __restrict__ V, const float2* __restrict__ DV, int nx, int ny, double dx,
+                   double dy, double kk, int zneg, double* __restrict__ part) {
+  __shared__ double red[kMetricThreads / 64];
+  const long npix = (long)nx * ny;
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)kMetricThreads + threadIdx.x; i < npix; i += (long)gridDim.x * kMetricThreads) {
+    const int y = (int)(i / nx), x = (int)(i - (long)y * nx);
+    const double fx = (x <= (nx - 1) / 2 ? x : x - nx) / (nx * dx);
+    const double fy = (y <= (ny - 1) / 2 ? y : y - ny) / (ny * dy);
+    const double a = kk - fx * fx - fy * fy;  // (n0/lam)^2 - f^2
+    const float2 v = V[i], d = DV[i];
+    // P = conj(DV) V
+    const double pr = (double)d.x * v.x + (double)d.y * v.y, pi = (double)d.x * v.y - (double)d.y * v.x;
+    if (a >= 0.0) {
+      acc += -2.0 * M_PI * sqrt(a) * pi;            // Re(P * 2 pi i r), r = w real
+    } else {
+      const double s = zneg ? -sqrt(-a) : sqrt(-a);  // w~ = i s
+      acc += -2.0 * M_PI * s * pr;                  // Re(P * 2 pi i * i s)
+    }
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s / (double)npix;  // Parseval's 1/N
+}
+
 // loss[slot] = ordered sum of the slot's partials (one workgroup per slot)
 __global__ void __launch_bounds__(kMetricThreads)
 k_sum_partials(const double* __restrict__ part, int nper, double* __restrict__ out) {
@@ -2593,6 +2621,31 @@ int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx
   FH_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_sum_partials, dim3(nout), dim3(kMetricThreads), 0, ctx->stream,
                      (const double*)ctx->metric_part, kMetricBlocks, loss);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_as_z_gradient(fh_ctx* ctx, const void* V, const void* DV, int nx, int ny, int batch, double dx, double dy,
+                     const double* lambdas, int nlambdas, double n0, double z, double* out) {
+  if (!ctx || !V || !DV || !lambdas || !out) return fail(FH_E_ARG, "null context, spectra, wavelengths or output");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  if (nlambdas != 1 && nlambdas != batch) return fail(FH_E_ARG, "give 1 or batch wavelengths");
+  if (!(dx > 0) || !(dy > 0) || !(n0 > 0)) return fail(FH_E_ARG, "sampling and n0 must be positive");
+  for (int k = 0; k < nlambdas; ++k)
+    if (!(lambdas[k] > 0)) return fail(FH_E_ARG, "wavelengths must be positive");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = metric_partials(ctx, (size_t)batch * kMetricBlocks * sizeof(double)))) return rc;
+  const long npix = (long)nx * ny;
+  for (int b = 0; b < batch; ++b) {
+    const double k = n0 / lambdas[nlambdas == 1 ? 0 : b];
+    hipLaunchKernelGGL(k_as_zgrad_partial, dim3(kMetricBlocks), dim3(kMetricThreads), 0, ctx->stream,
+                       reinterpret_cast<const float2*>(V) + b * npix, reinterpret_cast<const float2*>(DV) + b * npix,
+                       nx, ny, dx, dy, k * k, z < 0 ? 1 : 0, ctx->metric_part + (long)b * kMetricBlocks);
+    FH_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_sum_partials, dim3(batch), dim3(kMetricThreads), 0, ctx->stream,
+                     (const double*)ctx->metric_part, kMetricBlocks, out);
   FH_HIP(hipGetLastError());
   return FH_OK;
 }

@@ -7,7 +7,7 @@ the arithmetic is the HIP library's (``libfluxhip.so``, C ABI in include/fluxhip
 
 from .fields import (Backward, Direction, Forward, ScalarField, coupling_efficiency, intensity,
                      normalize_power_, phase, power, rs_valid_distance)
-from .components import (AS_BPM, BPM, ASProp, CollinsProp, FourierLens, ParaxialProp, FourierMask, FourierOperator, FourierPhase, FourierWrapper,
+from .components import (AS_BPM, BPM, ASProp, ASPropZ, CollinsProp, FourierLens, ParaxialProp, FourierMask, FourierOperator, FourierPhase, FourierWrapper,
                          Mask, Phase, RSProp, ShiftProp, Shift_BPM, TeaDOE, TeaReflector, TiltAnchor,
                          phase_propagate_,
                          propagate_phase_backward_)
@@ -21,6 +21,7 @@ from .optim import (Adam, ClampProx, CompositeProx, Descent, Fista, IstaProx, Mo
                     PositiveProx, ProxRule, compose, make_rules, setup, update_)
 
 __all__ = [
+    "ASPropZ",
     "FFTPlans", "backpropagate", "backpropagate_metric", "compute_ft_", "compute_ift_", "compute_metric", "dot", "get_data", "get_lambdas", "get_lambdas_collection", "get_source", "get_tilts", "get_tilts_collection", "is_on_axis", "isbuffered", "istrainable", "make_fft_plans", "offset_tilts_", "set_field_data", "set_field_ds_", "set_field_tilts", "spatial_vectors", "trainable",
     "intensity", "normalize_power_", "phase", "power",
     "SquaredFieldDifference",

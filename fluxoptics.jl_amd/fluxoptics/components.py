@@ -207,6 +207,60 @@ class ASProp(_KernelPropagator):
         return self
 
 
+class ASPropZ(_PipeComponent):
+    """``ASPropZ(u, z; n0, trainable)`` (``angular_spectrum.jl:236-300``): an on-axis ASProp whose
+    distance z is a trainable parameter (a one-element FP32 device tensor, ``self.z``).  The
+    propagation is ASProp's (its plan is rebuilt when z changes, as the reference recomputes its
+    kernel each call); inside ``withgradient`` the distance gradient comes from the output and its
+    cotangent (``fh_as_z_gradient``).  Paraxial kernels, frequency filters and tilted fields are not
+    provided here; the gradient is taken in the Forward direction."""
+
+    def __init__(self, u: ScalarField, z: float, *, n0: float = 1.0, trainable: bool = False,
+                 buffered: bool = False, paraxial: bool = False, track_tilts: bool = False,
+                 filter: Optional[Callable] = None, double_precision_kernel: bool = False):
+        check_field(u.electric)
+        if paraxial or filter is not None:
+            raise NotImplementedError("ASPropZ: paraxial kernels and frequency filters are not provided")
+        if not u.is_on_axis():
+            raise NotImplementedError("ASPropZ: tilted fields are not provided")
+        self.n0 = float(n0)
+        self.z = torch.tensor([float(z)], dtype=torch.float32, device=u.electric.device)
+        self.trainable = bool(trainable)
+        self.buffered = self.trainable and buffered
+        self.grad = torch.zeros_like(self.z) if self.buffered else None
+        self._prop, self._zp = None, None
+
+    def _plan(self, u: ScalarField) -> ASProp:
+        zv = float(self.z.item())
+        if self._prop is None or zv != self._zp:
+            self._prop, self._zp = ASProp(u, zv, n0=self.n0), zv
+        return self._prop
+
+    def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self._plan(u).propagate_(u, direction)
+
+    def propagate(self, u: ScalarField, direction=Forward) -> ScalarField:
+        return self.propagate_(u.copy(), direction)
+
+    def backpropagate_(self, du: ScalarField, direction=Forward) -> ScalarField:
+        return self._plan(du).backpropagate_(du, direction)
+
+    def z_gradient(self, v: ScalarField, dv: ScalarField) -> torch.Tensor:
+        """dL/dz (summed over the fields, shape (1,), FP32) from the Forward output v and its
+        cotangent dv."""
+        V = v.electric.clone()
+        DV = dv.electric.clone()
+        ctx = context(v.electric.device)
+        for a in (V, DV):
+            _lib.call("fh_fft2", ctx, _ptr(a), v.nx, v.ny, v.batch, -1, 1.0)
+        lams = v.lambda_list()
+        arr = (ctypes.c_double * len(lams))(*lams)
+        out = torch.empty(v.batch, dtype=torch.float64, device=v.electric.device)
+        _lib.call("fh_as_z_gradient", ctx, _ptr(V), _ptr(DV), v.nx, v.ny, v.batch, v.ds[0], v.ds[1], arr, len(lams),
+                  self.n0, float(self._zp if self._zp is not None else self.z.item()), _ptr(out))
+        return out.sum().reshape(1).to(torch.float32)
+
+
 class ShiftProp(_KernelPropagator):
     """``ShiftProp(u, z)`` (``freespace_propagators/shift_kernel.jl:47-117``): the geometric
     shift of each field by z tan(theta) according to its tilts, no diffraction, applied as the

@@ -14,7 +14,7 @@ from typing import Dict, Iterable, Optional
 import torch
 
 from . import _lib
-from .components import BPM, _Modulator, _ptr, context, Phase
+from .components import ASPropZ, BPM, _Modulator, _ptr, context, Phase
 from .system import OpticalSystem, ScalarSource
 
 _RULE_IDS = {"Descent": 0, "Momentum": 1, "Nesterov": 2, "Fista": 3}
@@ -117,6 +117,8 @@ def _params(component):
         return [("dn", component.dn)]
     if isinstance(component, ScalarSource):
         return [("u0", component.u0.electric)]
+    if isinstance(component, ASPropZ):
+        return [("z", component.z)]
     return []
 
 

@@ -22,7 +22,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 
 import torch
 
-from .components import (BPM, CollinsProp, TiltAnchor, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
+from .components import (ASPropZ, BPM, CollinsProp, TiltAnchor, ASProp, FourierOperator, Phase, RSProp, _KernelPropagator,
                          _Modulator, _ptr, check_field)
 from .components import phase_propagate_, propagate_phase_backward_
 from .fields import Backward, Forward, ScalarField, dir_sign
@@ -393,6 +393,11 @@ def _run_forward(comps: List, u: ScalarField, direction, save: bool, probes: Opt
                 else torch.empty_like(u.electric)
             c.propagate_and_save_(u, saved, direction)
             tape.append(("mod", (c,), saved))
+        elif save and isinstance(c, ASPropZ) and c.trainable:
+            if dir_sign(direction) < 0:
+                raise NotImplementedError("ASPropZ's distance gradient is taken in the Forward direction")
+            c.propagate_(u, direction)
+            tape.append(("asz", (c,), u.copy()))
         elif save and isinstance(c, TiltAnchor):
             tape.append(("anchor", (c,), u.tilts))
             c.propagate_(u, direction)
@@ -420,6 +425,15 @@ def _grad_buffer(grads: Dict, c, like: torch.Tensor):
 
 def _run_backward(comps: List, tape: list, du: ScalarField, direction, grads: Dict) -> ScalarField:
     for kind, cs, saved in reversed(tape):
+        if kind == "asz":
+            (c,) = cs
+            gz = c.z_gradient(saved, du)
+            if c in grads:
+                grads[c] += gz
+            else:
+                grads[c] = c.grad.copy_(gz) if c.grad is not None else gz
+            c.backpropagate_(du, direction)
+            continue
         if kind == "phase+prop":
             ph, p = cs
             if ph.trainable:

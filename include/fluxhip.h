@@ -197,6 +197,14 @@ int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double sc
 int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx, int ny, int batch,
                          int target_batch, const double* dc, double* loss, void* du);
 
+/* ASPropZ (angular_spectrum.jl:236-300), an on-axis ASProp whose distance z is trainable: dL/dz per
+ * field (out: batch doubles, device) from V = FFT(v) of the output and DV = FFT(dv) of its cotangent
+ * (unnormalised 2-D FFTs, e.g. fh_fft2 with sign -1): Re (1/N) sum conj(DV) 2 pi i w~ V with
+ * w = sqrt((n0/lambda)^2 - fx^2 - fy^2) (principal root), w~ = w for z >= 0 and conj(w) for z < 0
+ * (the kernel's H(z) = conj(H(-z)), angular_spectrum.jl:1-12).  lambdas: 1 or batch values (host). */
+int fh_as_z_gradient(fh_ctx* ctx, const void* V, const void* DV, int nx, int ny, int batch, double dx, double dy,
+                     const double* lambdas, int nlambdas, double n0, double z, double* out);
+
 /* SquaredFieldDifference (squared_field_difference.jl, compute_metric and backpropagate_metric):
  * loss[b] = sum |u_b - v_b|^2 over the pixels of field b (summed in FP64 in a fixed order); when du
  * is not NULL the pullback du_b = 2 dc[b] (u_b - v_b) is written there (du may equal u); dc as in

@@ -196,6 +196,27 @@ def as_transfer(nx: int, ny: int, dx: float, dy: float, lam: float, n0: float, z
     return tilted_as_kernel(fx, fy, lam, tilts[0], tilts[1], track, n0, z)
 
 
+def as_z_gradient(v, dv, ds, lams, z: float, n0: float = 1.0):
+    """dL/dz of an on-axis ASProp with a trainable distance (``ASPropZ``,
+    ``angular_spectrum.jl:236-300``, kernel of ``:1-12``), per field, given its output v and the
+    cotangent dv at v: with V = FFT(v) = H FFT(u), DV = FFT(dv) and dH/dz = 2 pi i w~ H, where
+    w = sqrt((n0/lam)^2 - fx^2 - fy^2) (principal root) and w~ = w for z >= 0, conj(w) for z < 0
+    (H(z) = conj(H(-z))), Parseval gives Re (1/N) sum conj(DV) 2 pi i w~ V."""
+    v = _as_batch(np.asarray(v, np.complex128))
+    dv = _as_batch(np.asarray(dv, np.complex128))
+    b, ny, nx = v.shape
+    lam_b = _lambdas(lams, b)
+    fx = fftfreq(nx, ds[0])[None, :]
+    fy = fftfreq(ny, ds[1])[:, None]
+    V, DV = ft2(v), ft2(dv)
+    out = np.empty(b)
+    for k in range(b):
+        w = np.sqrt(np.asarray((n0 / lam_b[k]) ** 2 - fx ** 2 - fy ** 2, np.complex128))
+        wt = w if z >= 0 else np.conj(w)
+        out[k] = np.real(np.sum(np.conj(DV[k]) * (TWO_PI * 1j * wt) * V[k])) / (nx * ny)
+    return out
+
+
 def _per_field(v, b: int):
     a = np.atleast_1d(np.asarray(v, dtype=np.float64))
     return np.broadcast_to(a, (b,)) if a.size == 1 else a

@@ -186,3 +186,50 @@ def test_error_paths_on_device():
     u2 = fo.ScalarField(to_dev(np.zeros((64, 64))), (1.0, 1.0), 0.8)
     with pytest.raises(ValueError):
         fo.ASProp(u, 10.0).propagate_(u2)
+
+
+@pytest.mark.parametrize("nx,ny,z", [(64, 48, 30.0), (60, 40, -25.0)])
+def test_aspropz_distance_gradient(nx, ny, z):
+    """ASPropZ (angular_spectrum.jl:236-300): the forward equals ASProp at the current z; the
+    distance gradient from the output and a cotangent against the oracle (which finite
+    differences of the FP64 propagator pin); a grid with evanescent frequencies (ds 0.4 with
+    lambda 0.5, 0.7) and both signs of z; a non-power-of-two grid."""
+    print(f"[ASPropZ] {nx}x{ny} z={z}", flush=True)
+    ds, lams, n0 = (0.4, 0.4), [0.5, 0.7], 1.2
+    u0 = c64(random_field((2, ny, nx), 201))
+    c0 = c64(random_field((2, ny, nx), 202))
+    u = fo.ScalarField(to_dev(u0), ds, lams)
+    pz = fo.ASPropZ(u, z, n0=n0, trainable=True)
+    v = pz.propagate(u)
+    assert rel_l2(to_host(v.electric), O.asprop(u0, ds, lams, z, n0, 1)) <= FIELD_TOL
+    g = pz.z_gradient(v, fo.ScalarField(to_dev(c0), ds, lams))
+    ref = O.as_z_gradient(to_host(v.electric), c0, ds, lams, z, n0).sum()
+    assert abs(float(g[0]) - ref) <= 1e-4 * max(1.0, abs(ref))
+
+
+def test_aspropz_in_a_differentiated_system():
+    """source | ASPropZ(trainable) with SquaredFieldDifference against the field propagated to
+    z = 40: dL/dz against the oracle chain; one Descent step of 0.01 um against the gradient
+    sets z to z - eta g and lowers the loss (the field's carrier phase makes L(z) oscillate with
+    period lambda, so only small steps are descents)."""
+    n, ds, lam = 64, (1.0, 1.0), 0.8
+    g0 = c64(inputs.gaussian(n, n, 1.0, 1.0, 4.0))
+    t0 = c64(O.asprop(g0, ds, lam, 40.0, 1.0, 1)[0])
+    u = fo.ScalarField(to_dev(g0), ds, lam)
+    pz = fo.ASPropZ(u, 25.0, trainable=True, buffered=True)
+    system = fo.ScalarSource(u) | pz
+    metric = fo.SquaredFieldDifference(fo.ScalarField(to_dev(t0), ds, lam))
+    value, grads = system.withgradient(metric)
+    torch.cuda.synchronize()
+    v = O.asprop(g0, ds, lam, 25.0, 1.0, 1)
+    c, dv = O.sq_field_diff_values(v, t0)
+    ref = O.as_z_gradient(v, dv, ds, lam, 25.0, 1.0).sum()
+    assert abs(float(value) - c.sum()) <= 1e-5 * c.sum()
+    assert abs(float(grads[pz][0]) - ref) <= 1e-4 * abs(ref) and grads[pz] is pz.grad
+    eta = 0.01 / abs(ref)
+    opt = fo.setup(fo.Descent(eta), system)
+    fo.update_(opt, system, grads)
+    z1 = float(pz.z[0])
+    assert abs(z1 - (25.0 - eta * float(grads[pz][0]))) <= 1e-5
+    value1, _ = system.withgradient(metric)
+    assert float(value1) < float(value)

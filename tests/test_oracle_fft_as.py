@@ -159,3 +159,20 @@ def test_asprop_multi_lambda_slices_independent():
     v = O.asprop(u, ds, lams, 200.0)
     for b, lam in enumerate(lams):
         assert np.max(np.abs(v[b] - O.asprop(u[b], ds, lam, 200.0)[0])) < 1e-14
+
+
+@pytest.mark.parametrize("z", [35.0, -20.0])
+def test_oracle_as_z_gradient_finite_differences(z):
+    """ASPropZ's distance gradient against central finite differences of the FP64 propagator:
+    L(z) = Re sum conj(c) ASProp(u, z) per field, whose cotangent at the output is c.  The grid
+    (ds 0.4 with lambda 0.5 and 0.7) holds evanescent frequencies, so the decaying part and the
+    z < 0 conjugation are both exercised."""
+    u = inputs.normal(71, (2, 24, 20)) + 1j * inputs.normal(72, (2, 24, 20))
+    c = inputs.normal(73, (2, 24, 20)) + 1j * inputs.normal(74, (2, 24, 20))
+    ds, lams, n0 = (0.4, 0.4), [0.5, 0.7], 1.2
+    loss = lambda zz: np.real(np.sum(np.conj(c) * O.asprop(u, ds, lams, zz, n0), axis=(-2, -1)))
+    v = O.asprop(u, ds, lams, z, n0)
+    g = O.as_z_gradient(v, c, ds, lams, z, n0)
+    h = 1e-4
+    fd = (loss(z + h) - loss(z - h)) / (2 * h)
+    assert np.allclose(g, fd, rtol=1e-6, atol=1e-8)
