@@ -125,10 +125,12 @@ def test_new_entry_points_validate_arguments():
     # fused metric block and offset_tilts: null arguments
     assert lib.fh_sqint_block(None, None, None, 1, None, None, 1, None, None, None, 0) == -1
     assert lib.fh_sqint_block_src(None, None, None, None, 1, None, None, 1, None, None, 0) == -1
+    assert b"source" in lib.fh_last_error()
     assert lib.fh_window(None, None, 4, 4, None, 4, 4, 1, 0, 0, 0.0, 0.0) == -1
+    assert b"destination" in lib.fh_last_error()
     lam1 = (ctypes.c_double * 1)(0.8)
     assert lib.fh_as_z_gradient(None, None, None, 4, 4, 1, 1.0, 1.0, lam1, 1, 1.0, 10.0, None) == -1
-    assert b"source" in lib.fh_last_error()
+    assert b"spectra" in lib.fh_last_error()
     lam = (ctypes.c_double * 1)(0.8)
     t0 = (ctypes.c_double * 1)(0.0)
     assert lib.fh_offset_tilts(None, None, 4, 4, 1, 1.0, 1.0, lam, 1, t0, t0, t0, t0, 1) == -1
