@@ -221,15 +221,17 @@ class OpticalSystem:
         if tail is not None and not tail[0] and (x is None or not self.inplace):
             # the whole system is the fused block: it reads the start field itself (no copy of
             # the source field, and the Phase gradient takes it as its saved input)
-            _, ph, prop = tail
+            _, ph, prop, trail = tail
             u = start.set_field_data(torch.empty_like(start.electric))
             losses = (_sqint_block(loss, u, ph, prop, [], grads, weights, src=start.electric),)
+            _zero_grads(trail, grads)
             du = u
         elif tail is not None:
             u = self._start(x)
-            head, ph, prop = tail
+            head, ph, prop, trail = tail
             tape = _run_forward(head, u, self.direction, save=True)
             losses = (_sqint_block(loss, u, ph, prop, tape, grads, weights),)
+            _zero_grads(trail, grads)
             du = _run_backward(head, tape, u, self.direction, grads)
         else:
             u = self._start(x)
@@ -307,19 +309,36 @@ def _is_prop(c) -> bool:
 
 
 def _fusable_tail(comps: List, loss, u: ScalarField, direction):
-    """(head, Phase or None, propagator) when the system ends in [Phase ->] ASProp/RSProp and
-    the loss is a one-pair SquaredIntensityDifference with one target per field; else None."""
+    """(head, Phase or None, propagator, trailing Phases) when the system ends in [Phase ->]
+    ASProp/RSProp, possibly followed by Phases, and the loss is a one-pair
+    SquaredIntensityDifference with one target per field; else None.  A Phase leaves |u|^2 as it
+    is, so trailing Phases change neither the loss nor the pullback, and their mask gradients are
+    exactly zero."""
     if dir_sign(direction) < 0 or not comps or not isinstance(loss, SquaredIntensityDifference):
         return None
-    if len(loss.targets) != 1 or not _is_prop(comps[-1]):
+    k = len(comps)
+    while k > 0 and type(comps[k - 1]) is Phase:
+        k -= 1
+    trail, comps = comps[k:], comps[:k]
+    if len(loss.targets) != 1 or not comps or not _is_prop(comps[-1]):
         return None
     t = loss.targets[0]
     tb = 1 if t.dim() == 2 else t.shape[0]
     if not (tb == u.batch or (tb == 1 and u.batch == 1)):
         return None
     if len(comps) >= 2 and isinstance(comps[-2], Phase):
-        return comps[:-2], comps[-2], comps[-1]
-    return comps[:-1], None, comps[-1]
+        return comps[:-2], comps[-2], comps[-1], trail
+    return comps[:-1], None, comps[-1], trail
+
+
+def _zero_grads(phases, grads: Dict) -> None:
+    """Gradients of Phases that the loss does not see (their sum is unchanged where a Phase
+    already has one)."""
+    for ph in phases:
+        if ph.trainable:
+            g, acc = _grad_buffer(grads, ph, ph.phi)
+            if not acc:
+                g.zero_()
 
 
 def _sqint_block(loss, u: ScalarField, ph, prop, tape, grads: Dict, weights, src=None):

@@ -157,3 +157,29 @@ def test_trainable_source_through_the_block_reading_it():
     assert abs(float(v2) - float(v1)) <= 1e-6 * abs(float(v1))
     assert rel_l2(g2[ph].cpu().numpy(), g1[ph].cpu().numpy()) <= 1e-6
     assert torch.equal(x.electric.cpu(), torch.from_numpy(u0))
+
+
+def test_trailing_phase_is_skipped_by_the_fused_block():
+    """A system ending in a Phase (the RGB cascade's shape, Phi -> AS -> Phi -> AS -> Phi): the
+    trailing Phase leaves |u|^2 unchanged, so the fused run skips it; loss and the other mask
+    gradients equal the component-by-component run, and the trailing mask's gradient is exactly
+    zero (rounding-level in the unfused run)."""
+    n = 256
+    print("[fused metric] trailing phase", flush=True)
+    ds, lams = (1.0, 1.0), [0.64, 0.538, 0.455]
+    u0 = c64(speckle_batch(3, n, n, 1.0, 1.0, 30.0, 211))
+    tgt = (np.abs(c64(speckle_batch(3, n, n, 1.0, 1.0, 30.0, 212))) ** 2).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), ds, lams)
+    ph = [fo.Phase(u, inputs.uniform(213 + k, (n, n), 0, 2 * math.pi).astype(np.float32), trainable=True)
+          for k in range(3)]
+    system = fo.ScalarSource(u) | ph[0] | fo.ASProp(u, 600.0) | ph[1] | fo.ASProp(u, 600.0) | ph[2]
+    metric = fo.SquaredIntensityDifference((u, tgt))
+    v1, g1 = system.withgradient(metric)
+    g1 = {c: g.clone() for c, g in g1.items()}
+    v0, g0 = system.withgradient(metric, fuse=False)
+    torch.cuda.synchronize()
+    assert abs(float(v1) - float(v0)) <= 1e-6 * abs(float(v0))
+    for c in ph[:2]:
+        assert rel_l2(g1[c].cpu().numpy(), g0[c].cpu().numpy()) <= 1e-5
+    scale = float(g0[ph[0]].abs().max())
+    assert float(g1[ph[2]].abs().max()) == 0.0 and float(g0[ph[2]].abs().max()) <= 1e-4 * scale
