@@ -1463,6 +1463,13 @@ static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float
   if constexpr (!std::is_void<FR>::value && L == 4) {
     if (few) return fast_phase_grad_launch<FR, 4>(c, io, batch, phi, saved, dphi, nx, s, acc);
   }
+  // Out of place (RS, Collins and CZT plans read the scratch spectrum), rows of 4096 points go two
+  // to a workgroup: 512 threads may hold 256 VGPRs, and the phase factors then fit in LDS beside
+  // the exchange areas.  In place, a workgroup must own whole 4-row groups of the blocked input.
+  if constexpr (L == 4 && LOGN == 12) {
+    if (static_cast<const void*>(io.in) != static_cast<const void*>(io.out))
+      return fast_phase_grad_launch<RDF, 2>(c, io, batch, phi, saved, dphi, nx, s, acc);
+  }
   return fast_phase_grad_launch<RDF, L>(c, io, batch, phi, saved, dphi, nx, s, acc);
 }
 

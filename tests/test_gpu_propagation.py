@@ -118,7 +118,7 @@ def test_phase_apply_and_gradient():
 
 
 @pytest.mark.parametrize("kind,n", [("as", 256), ("as", 1024), ("as", 2048), ("as", 4096), ("rs", 128),
-                                    ("rs", 250), ("rs", 512)])
+                                    ("rs", 250), ("rs", 512), ("rs", 2048)])
 def test_fused_phase_propagate_and_adjoint(kind, n):
     print(f"[fused phase+{kind}] n={n}", flush=True)
     batch, ds, lam, z = 3, (1.0, 1.0), [0.64, 0.538, 0.455], 800.0
