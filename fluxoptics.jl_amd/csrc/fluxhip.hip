@@ -1196,6 +1196,9 @@ struct fh_ctx {
   std::mutex mu;
   double* metric_part = nullptr;  // metric block partials
   size_t metric_part_n = 0;
+  // partials buffers outgrown by a later call: graphs recorded earlier still hold their
+  // addresses, so they stay allocated until the context is destroyed
+  std::vector<double*> metric_part_retired;
   std::map<std::vector<int>, fh_plan*> dft_plans;  // (nx, ny, batch) -> DFT-as-CZT plan (fh_fft2)
 };
 
@@ -1839,6 +1842,7 @@ int fh_ctx_destroy(fh_ctx* ctx) {
   for (auto& kv : ctx->stage_tw) (void)hipFree(kv.second);
   for (auto& kv : ctx->dft_plans) plan_free(kv.second);
   if (ctx->metric_part) (void)hipFree(ctx->metric_part);
+  for (double* d : ctx->metric_part_retired) (void)hipFree(d);
   delete ctx;
   return FH_OK;
 }
@@ -2417,15 +2421,17 @@ int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double sc
 }
 
 static int metric_partials(fh_ctx* ctx, size_t bytes) {
-  // grown once to a generous size, so that graphs captured after the first calls keep a
-  // valid pointer
+  // Sized generously on first use.  A larger request allocates a new buffer and retires the
+  // old one without freeing it: a graph recorded before the growth (a captured training step)
+  // keeps the old address and must still find valid memory when it replays.  Work on the
+  // stream that still reads the old buffer is unaffected too.
   const size_t need = std::max<size_t>((bytes + sizeof(double) - 1) / sizeof(double), 1 << 16);
   if (ctx->metric_part_n >= need) return FH_OK;
-  if (ctx->metric_part) FH_HIP(hipFree(ctx->metric_part));
-  ctx->metric_part = nullptr;
-  ctx->metric_part_n = 0;
-  if (hipMalloc(&ctx->metric_part, need * sizeof(double)) != hipSuccess)
+  double* fresh = nullptr;
+  if (hipMalloc(&fresh, need * sizeof(double)) != hipSuccess)
     return fail(FH_E_NOMEM, "metric partials: out of device memory");
+  if (ctx->metric_part) ctx->metric_part_retired.push_back(ctx->metric_part);
+  ctx->metric_part = fresh;
   ctx->metric_part_n = need;
   return FH_OK;
 }
@@ -3368,6 +3374,7 @@ struct fh_group {
   std::vector<fh_plan*> plans;
   hipStream_t cap = nullptr;
   std::vector<hipStream_t> side;
+  std::vector<int> lane;       // side stream of plans[i]: (index of the plan's first listing) mod nside
   std::vector<hipEvent_t> ev;  // ev[0]: fork; ev[1 + s]: join of side stream s
   struct Graph {
     int direction;
@@ -3394,7 +3401,9 @@ static void group_free(fh_group* g) {
   delete g;
 }
 
-// Capture every plan's propagation of fields[i] (plan i on side stream i mod nside).
+// Capture every plan's propagation of fields[i] on side stream g->lane[i].  A plan listed more
+// than once (one propagator for several same-shaped fields) always gets the same side stream:
+// its launches share the plan's workspace and must run one after the other.
 static int group_capture(fh_group* g, void* const* fields, int direction, fh_group::Graph& out) {
   fh_ctx* c = g->ctx;
   hipStream_t user = c->stream;
@@ -3403,7 +3412,7 @@ static int group_capture(fh_group* g, void* const* fields, int direction, fh_gro
   hipError_t e = hipEventRecord(g->ev[0], g->cap);
   for (size_t s = 0; e == hipSuccess && s < g->side.size(); ++s) e = hipStreamWaitEvent(g->side[s], g->ev[0], 0);
   for (size_t i = 0; e == hipSuccess && rc == FH_OK && i < g->plans.size(); ++i) {
-    c->stream = g->side[i % g->side.size()];
+    c->stream = g->side[g->lane[i]];
     float2* f = reinterpret_cast<float2*>(fields[i]);
     rc = run_prop_plan(g->plans[i], f, direction);
     if (rc == FH_OK) rc = run_xifft(g->plans[i], f, direction);
@@ -3453,7 +3462,15 @@ int fh_group_create(fh_ctx* ctx, int nplans, fh_plan* const* plans, int autotune
   fh_group* g = new fh_group();
   g->ctx = ctx;
   g->plans.assign(plans, plans + nplans);
-  const int nside = nplans < 4 ? nplans : 4;
+  std::vector<int> first(nplans);  // distinct plans in order of first listing
+  int ndistinct = 0;
+  for (int i = 0; i < nplans; ++i) {
+    int j = 0;
+    while (j < i && plans[j] != plans[i]) ++j;
+    first[i] = j == i ? ndistinct++ : first[j];
+  }
+  const int nside = ndistinct < 4 ? ndistinct : 4;
+  for (int i = 0; i < nplans; ++i) g->lane.push_back(first[i] % nside);
   // streams and events at the default priority
   if (hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking) != hipSuccess) { group_free(g); return fail(FH_E_HIP, "stream creation failed"); }
   for (int s = 0; s < nside; ++s) {
@@ -3491,8 +3508,11 @@ int fh_group_propagate(fh_group* g, void* const* fields, int direction) {
   if (!g || !fields) return fail(FH_E_ARG, "null group or field list");
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(g->ctx))) return rc;
-  for (size_t i = 0; i < g->plans.size(); ++i)
+  for (size_t i = 0; i < g->plans.size(); ++i) {
     if (!fields[i]) return fail(FH_E_ARG, "field %zu is null", i);
+    for (size_t j = 0; j < i; ++j)  // side streams would propagate one buffer twice at once
+      if (fields[j] == fields[i]) return fail(FH_E_ARG, "fields %zu and %zu are the same buffer", j, i);
+  }
   fh_group::Graph* gr = nullptr;
   for (auto& x : g->graphs)
     if (x.direction == direction && std::equal(x.fields.begin(), x.fields.end(), fields)) { gr = &x; break; }

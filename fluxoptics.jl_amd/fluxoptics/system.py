@@ -260,6 +260,7 @@ class OpticalSystem:
         between replays.  One eager call runs first so that every plan, table and workspace
         exists before capture.
         """
+        _check_recordable(self)
         if weights is not None:  # a device copy that lives as long as the graph
             weights = torch.as_tensor(weights, dtype=torch.float64).cuda()
         self._withgradient_launches(loss, x, weights)
@@ -270,8 +271,19 @@ class OpticalSystem:
         return CapturedGradient(graph, losses, grads, weights)
 
 
+def _check_recordable(system):
+    """An ``ASPropZ`` reads its distance on the host to pick its transfer function, which a
+    recorded graph can neither do (a host read during capture) nor follow (the tables of the
+    recorded z would replay after z moved on), so systems holding one are not recorded."""
+    for c in system.components:
+        if isinstance(c, ASPropZ):
+            raise NotImplementedError("ASPropZ picks its transfer function from z on the host; a system "
+                                      "holding one runs eagerly (withgradient), not as a recorded graph")
+
+
 def _capture_train_step(system, loss, opt_state, x=None, weights=None):
     from .optim import update_
+    _check_recordable(system)
     if weights is not None:
         weights = torch.as_tensor(weights, dtype=torch.float64).cuda()
     system._withgradient_launches(loss, x, weights)  # plans, tables, workspaces; parameters unchanged

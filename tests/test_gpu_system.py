@@ -323,3 +323,35 @@ def test_function_forms_of_field_and_fft_helpers():
     assert fo.get_tilts_collection(u)[0] == [0.0, 0.01]
     fo.set_field_ds_(u, (2.0, 2.0))
     assert u.ds == (2.0, 2.0) and fo.set_field_data(u, u.electric * 0).ds == (2.0, 2.0)
+
+
+def test_recorded_step_survives_metric_partials_growth():
+    """A metric call with a large batch grows the context's partials buffer after a training
+    step was recorded; the recorded step keeps its own (retired, still allocated) buffer, so its
+    replays still equal the eager calls."""
+    n, ds, lam = 128, 1.0, 1.064
+    print("[captured withgradient] then DotProduct over 1000 fields", flush=True)
+    u0 = c64(inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 12.0), ds, ds))
+    tgt = (np.abs(_two_gaussians(n, ds, 12.0, 20.0)) ** 2).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), (ds, ds), lam)
+    doe = fo.Phase(u, inputs.uniform(63, (n, n), 0, 1).astype(np.float32), trainable=True, buffered=True)
+    p = fo.RSProp(u, 900.0)
+    system = fo.ScalarSource(u) | doe | p
+    metric = fo.SquaredIntensityDifference((u, tgt))
+    step = system.capture_withgradient(metric)
+    value, grads = step.replay()
+    torch.cuda.synchronize()
+    v_before, g_before = float(value), grads[doe].clone()
+    # 1000 fields: 1000 x 256 complex partials, more than any earlier call asked for
+    big0 = c64(inputs.normal(64, (1000, 16, 16)) + 1j * inputs.normal(65, (1000, 16, 16)))
+    big = fo.ScalarField(to_dev(big0), (ds, ds), lam)
+    (c,) = fo.DotProduct(big)(big)
+    assert np.allclose(c.cpu().numpy(), O.dot_product(big0, big0, (ds, ds)), rtol=1e-10, atol=0)
+    for rep in range(3):
+        _ = torch.randn(4096, device="cuda").sum()
+        value, grads = step.replay()
+        torch.cuda.synchronize()
+        assert float(value) == v_before and torch.equal(grads[doe], g_before), rep
+        ev, eg = system.withgradient(metric)
+        torch.cuda.synchronize()
+        assert float(ev) == v_before and torch.equal(eg[doe], g_before), rep

@@ -321,3 +321,19 @@ def test_reference_spatial_vector_testsets():
     assert abs(xo.mean() - 5.0) < 1e-10 and abs(yo.mean() + 30.5) < 1e-10
     x2, y2 = fo.spatial_vectors((128, 128), (2.0, 2.0), offset=(-5.0, 30.5))
     assert np.array_equal(x2, xo) and np.array_equal(y2, yo)
+
+
+def test_recording_refuses_systems_with_aspropz():
+    """ASPropZ chooses its transfer function from z on the host, so capture_withgradient and
+    capture_train_step refuse a system holding one before any capture begins."""
+    from types import SimpleNamespace
+    from fluxoptics.components import ASPropZ
+    from fluxoptics.system import OpticalSystem, _capture_train_step, _check_recordable
+    sysz = SimpleNamespace(components=(object(), ASPropZ.__new__(ASPropZ)))
+    with pytest.raises(NotImplementedError, match="ASPropZ"):
+        _check_recordable(sysz)
+    with pytest.raises(NotImplementedError, match="ASPropZ"):
+        _capture_train_step(sysz, None, None)
+    with pytest.raises(NotImplementedError, match="ASPropZ"):
+        OpticalSystem.capture_withgradient(sysz, None)
+    _check_recordable(SimpleNamespace(components=(object(),)))
