@@ -895,6 +895,18 @@ k_sum_partials(const double* __restrict__ part, int nper, double* __restrict__ o
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+}  // namespace fh
+
+// Block partials of the no-allocation metric entry point (fh_sq_intensity_diff_noalloc): a
+// fixed device array, one copy per GPU held until the process ends, with room for
+// kNoAllocSlots loss slots of kMetricBlocks partials each (2 MiB).  Host code never passes this
+// name to a kernel (on the host it is only a stand-in for the symbol): it uses the address that
+// hipGetSymbolAddress returned for the device, kept in g_scratch_addr.
+constexpr int kNoAllocSlots = 1024;
+static __device__ __align__(256) double scratch[kNoAllocSlots * fh::kMetricBlocks];
+
+namespace fh {
+
 // The last pass of a propagation fused with SquaredIntensityDifference
 // (squared_intensity_difference.jl:78-90) and the first pass of the adjoint propagation: per
 // x line of the 4x4-blocked spectrum, IFFT -> u; d = |u|^2 - I (FP64), loss partial += d^2,
@@ -1803,6 +1815,32 @@ static int phase_grad_pass(fh_plan* p, float2* u, const float* phi, const void* 
 
 // ================================================================ C ABI
 
+// Device address of `scratch` per device, looked up once per device (with that device
+// current) when the first context on it is created.
+static std::mutex g_scratch_mu;
+static std::map<int, double*> g_scratch_addr;
+
+static int scratch_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_addr.count(device)) return FH_OK;
+  int prev = 0;
+  FH_HIP(hipGetDevice(&prev));
+  FH_HIP(hipSetDevice(device));
+  void* addr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess || !addr)
+    return fail(FH_E_HIP, "device %d: address of the no-allocation scratch array: %s", device, hipGetErrorString(e));
+  g_scratch_addr[device] = static_cast<double*>(addr);
+  return FH_OK;
+}
+
+static double* scratch_addr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  return it == g_scratch_addr.end() ? nullptr : it->second;
+}
+
 extern "C" {
 
 int fh_version(void) { return 100; }
@@ -1816,6 +1854,8 @@ int fh_ctx_create(int device, void* stream, fh_ctx** out) {
   int ndev = 0;
   FH_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(FH_E_ARG, "device %d out of range (%d devices)", device, ndev);
+  int rc;
+  if ((rc = scratch_setup(device))) return rc;
   fh_ctx* c = new fh_ctx();
   c->device = device;
   c->stream = reinterpret_cast<hipStream_t>(stream);
@@ -2634,6 +2674,33 @@ int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx
   FH_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_sum_partials, dim3(nout), dim3(kMetricThreads), 0, ctx->stream,
                      (const double*)ctx->metric_part, kMetricBlocks, loss);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_sq_intensity_diff_noalloc(fh_ctx* ctx, const void* u, const float* target, int nx, int ny, int batch,
+                                 int target_batch, const double* dc, double* loss, void* du) {
+  if (!ctx || !u || !target || !loss) return fail(FH_E_ARG, "null context, field, target or loss");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  if (target_batch != batch && target_batch != 1)
+    return fail(FH_E_ARG, "the target holds %d intensities; expected %d (per field) or 1 (spatial only)",
+                target_batch, batch);
+  const int spatial = target_batch == 1 && batch > 1;
+  const int nout = spatial ? 1 : batch;
+  if (nout > kNoAllocSlots)
+    return fail(FH_E_ARG, "%d loss values exceed the no-allocation maximum of %d per call", nout, kNoAllocSlots);
+  double* part = scratch_addr(ctx->device);
+  if (!part) return fail(FH_E_HIP, "no scratch address for device %d (context not created by fh_ctx_create)", ctx->device);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  // every partial of the nout slots is written before k_sum_partials reads it: nothing to clear
+  const long npix = (long)nx * ny;
+  hipLaunchKernelGGL(k_sqint_partial, dim3(kMetricBlocks, nout), dim3(kMetricThreads), 0, ctx->stream,
+                     reinterpret_cast<const float2*>(u), target, npix, batch, spatial, dc,
+                     reinterpret_cast<float2*>(du), part);
+  FH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_partials, dim3(nout), dim3(kMetricThreads), 0, ctx->stream,
+                     (const double*)part, kMetricBlocks, loss);
   FH_HIP(hipGetLastError());
   return FH_OK;
 }

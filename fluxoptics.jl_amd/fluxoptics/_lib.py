@@ -83,6 +83,8 @@ SIGNATURES = {
     "fh_fft2": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double]),
     "fh_sq_intensity_diff": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p,
                                      c_void_p, c_void_p]),
+    "fh_sq_intensity_diff_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p,
+                                             c_void_p, c_void_p]),
     "fh_bpm_create": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_double_p, c_int,
                               c_double, c_double, c_int, c_int, ctypes.POINTER(c_void_p)]),
     "fh_collins_create": (c_int, [c_void_p, ctypes.POINTER(PropDesc), c_double, c_double, c_double, c_double,

@@ -197,6 +197,19 @@ int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double sc
 int fh_sq_intensity_diff(fh_ctx* ctx, const void* u, const float* target, int nx, int ny, int batch,
                          int target_batch, const double* dc, double* loss, void* du);
 
+/* fh_sq_intensity_diff without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  The block partials live in a fixed per-GPU array inside the library
+ * (2 MiB, held until the process ends; its address is looked up when the first context on the
+ * GPU is created), so a call writes at most 1024 loss values (batch fields with per-field
+ * targets, or one spatial-only value): more returns FH_E_ARG.  Results equal
+ * fh_sq_intensity_diff's bit for bit.  Every call on one GPU shares that array: calls that
+ * overlap in time on one GPU (other streams, other contexts, other host threads) race; calls in
+ * order on one stream do not.  No state carries over between calls (each call writes every
+ * partial it reads). */
+int fh_sq_intensity_diff_noalloc(fh_ctx* ctx, const void* u, const float* target, int nx, int ny, int batch,
+                                 int target_batch, const double* dc, double* loss, void* du);
+
 /* ASPropZ (angular_spectrum.jl:236-300), an on-axis ASProp whose distance z is trainable: dL/dz per
  * field (out: batch doubles, device) from V = FFT(v) of the output and DV = FFT(dv) of its cotangent
  * (unnormalised 2-D FFTs, e.g. fh_fft2 with sign -1): Re (1/N) sum conj(DV) 2 pi i w~ V with
