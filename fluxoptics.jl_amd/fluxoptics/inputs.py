@@ -85,3 +85,16 @@ def normalize_power(u: np.ndarray, dx: float, dy: float, target: float = 1.0) ->
     u = np.asarray(u)
     p = (np.abs(u) ** 2).sum(axis=(-2, -1), keepdims=True) * dx * dy
     return u * np.sqrt(target / p)
+
+
+def smooth_noise(ns: int, ny: int, nx: int, sigma: float, peak: float, seed: int) -> np.ndarray:
+    """Gaussian-filtered white noise on an (ns, ny, nx) grid (periodic filter of sigma grid
+    points along every axis, applied in the Fourier domain) scaled to max |value| = peak, FP32:
+    the graded-index volume of a beam-propagation workload."""
+    w = normal(seed, (ns, ny, nx))
+    kz = np.fft.fftfreq(ns)[:, None, None]
+    ky = np.fft.fftfreq(ny)[None, :, None]
+    kx = np.fft.rfftfreq(nx)[None, None, :]
+    g = np.exp(-2 * (math.pi * sigma) ** 2 * (kz ** 2 + ky ** 2 + kx ** 2))
+    f = np.fft.irfftn(np.fft.rfftn(w) * g, s=w.shape)
+    return (peak * f / np.max(np.abs(f))).astype(np.float32)

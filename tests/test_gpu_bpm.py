@@ -17,15 +17,7 @@ pytestmark = pytest.mark.gpu
 def smooth_dn(ns, n, sigma, peak, seed, nx=None):
     """Gaussian-filtered white noise (periodic, FFT filter) scaled to max |dn| = peak; the
     grid is n x n, or n rows of nx points."""
-    nx = n if nx is None else nx
-    w = inputs.normal(seed, (ns, n, nx))
-    ky = np.fft.fftfreq(n)
-    kx = np.fft.fftfreq(nx)
-    kz = np.fft.fftfreq(ns)
-    g = np.exp(-2 * (math.pi * sigma) ** 2 * (kz[:, None, None] ** 2 + ky[None, :, None] ** 2
-                                              + kx[None, None, :] ** 2))
-    f = np.real(np.fft.ifftn(np.fft.fftn(w) * g))
-    return (peak * f / np.max(np.abs(f))).astype(np.float32)
+    return inputs.smooth_noise(ns, n, n if nx is None else nx, sigma, peak, seed)
 
 
 def tilted_gaussian(n, ds, lam, w0, deg):
