@@ -245,10 +245,24 @@ __device__ inline void as_transfer(double fx, double fy, double lam, double n0, 
   }
 }
 
+// ASProp's frequency filter (angular_spectrum.jl:1-76: the kernel times Complex(filter(...))),
+// sampled on the host on the fftfreq grid [ny][nx] (FFT order) at the arguments the kernel
+// passes it; multiplied in FP64 before the table is rounded.  conjf: the reference conjugates
+// the whole kernel, filter included, when z <= 0 (z_pos = Val(false)) except in the paraxial one.
+__device__ inline void apply_filter(const double2* __restrict__ filt, int conjf, long k, double& re, double& im) {
+  if (!filt) return;
+  const double2 f = filt[k];
+  const double fi = conjf ? -f.y : f.y;
+  const double r = re * f.x - im * fi;
+  im = re * fi + im * f.x;
+  re = r;
+}
+
 template <typename T2>
 __global__ void k_as_table(T2* __restrict__ tab, int nx, int ny, int rows, int qcol, int full,
                            double dx, double dy, double lam, double n0, double z, double tx,
-                           double ty, int mode, int track, double scale) {
+                           double ty, int mode, int track, double scale,
+                           const double2* __restrict__ filt = nullptr, int conjf = 0) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
@@ -262,7 +276,8 @@ __global__ void k_as_table(T2* __restrict__ tab, int nx, int ny, int rows, int q
   const double r = t - rint(t);
   double sn, cs;
   sincospi(2.0 * r, &sn, &cs);
-  const double re = amp * cs, im = amp * sn;
+  double re = amp * cs, im = amp * sn;
+  apply_filter(filt, conjf, (long)(ky < 0 ? ky + ny : ky) * nx + (kx < 0 ? kx + nx : kx), re, im);
   store_tab(tab + (long)i * qcol + j, re * scale, im * scale);
 }
 
@@ -272,7 +287,8 @@ __global__ void k_as_table(T2* __restrict__ tab, int nx, int ny, int rows, int q
 // k_as_grid_d: H on the field's own n-point frequency grid [ny][nx] (FFT order, fftfreq for
 // any n), FP64, scaled by 1/(nx ny) (the inverse DFT's normalisation).
 __global__ void k_as_grid_d(double2* __restrict__ out, int nx, int ny, double dx, double dy, double lam, double n0,
-                            double z, double tx, double ty, int mode, int track) {
+                            double z, double tx, double ty, int mode, int track, const double2* __restrict__ filt,
+                            int conjf) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
@@ -285,7 +301,9 @@ __global__ void k_as_grid_d(double2* __restrict__ out, int nx, int ny, double dx
   double sn, cs;
   sincospi(2.0 * r, &sn, &cs);
   const double sc = 1.0 / ((double)nx * (double)ny);
-  out[(long)j * nx + i] = make_double2(amp * cs * sc, amp * sn * sc);
+  double re = amp * cs, im = amp * sn;
+  apply_filter(filt, conjf, (long)j * nx + i, re, im);
+  out[(long)j * nx + i] = make_double2(re * sc, im * sc);
 }
 
 // Direct inverse DFT along one axis (unnormalised, sign +), FP64: out[line][m] =
@@ -1227,6 +1245,7 @@ struct fh_plan {
   int mode = 0;                    // AS kernel: 0 on-axis, 1 paraxial, 2 tilted
   int track = 0;                   // tilts tracked
   int full = 0;                    // full (non-even) table layout
+  int filtered = 0;                // ASProp tables include a frequency filter
   float2* tab = nullptr;           // transfer-function tables, one per distinct key
   long tabsize = 0;
   int qx = 0, qy = 0;
@@ -1887,12 +1906,18 @@ int fh_ctx_destroy(fh_ctx* ctx) {
   return FH_OK;
 }
 
+// Whether table k takes the conjugated filter: the reference's on-axis and tilted kernels for
+// z <= 0 are conj(kernel(-z)) with the filter inside (angular_spectrum.jl:9-12,64-68); its
+// paraxial kernel multiplies the filter as given for either sign (:27-35).
+static int filter_conj(const fh_plan* p, long k) { return p->mode != 1 && !(p->tab_z[k] > 0.0); }
+
 // Tables of a padded ASProp plan (grids that are not powers of two): per key, H on the
 // n-point grid, its inverse 2-D DFT h (FP64, direct sums), h extended periodically onto the
 // padded grid, the padded FFT (FP64), rounded once; the plan then runs the padded passes.
 // fp64: also keep the tables unrounded in p->tab_d (FP64-compute users such as the general
 // AS_BPM slice steps, whose hundreds of steps would otherwise repeat the same rounding).
-static int as_padded_tables(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p, bool fp64) {
+static int as_padded_tables(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p, bool fp64,
+                            const double2* filt = nullptr, int nfilt = 0) {
   const int nx = d->nx, ny = d->ny;
   const int nx2 = 1 << p->lx, ny2 = 1 << p->ly;
   p->kind = 1;
@@ -1917,7 +1942,8 @@ static int as_padded_tables(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p, bool
   }
   for (long k = 0; k < nt && rc == FH_OK; ++k) {
     hipLaunchKernelGGL(k_as_grid_d, dim3((nx + 255) / 256, ny), dim3(256), 0, ctx->stream, hn, nx, ny, d->dx, d->dy,
-                       p->lam_tables[k], d->n0, p->tab_z[k], p->tab_tx[k], p->tab_ty[k], p->mode, p->track);
+                       p->lam_tables[k], d->n0, p->tab_z[k], p->tab_tx[k], p->tab_ty[k], p->mode, p->track,
+                       filt ? filt + (nfilt == 1 ? 0 : k) * (long)nx * ny : nullptr, filter_conj(p, k));
     hipLaunchKernelGGL(k_idft_lines_d, dim3((nx + 255) / 256, ny), dim3(256), 0, ctx->stream, (const double2*)hn, tmp,
                        nx, ny, (long)nx, 1L, (const double2*)wx);
     hipLaunchKernelGGL(k_idft_lines_d, dim3((ny + 255) / 256, nx), dim3(256), 0, ctx->stream, (const double2*)tmp, hn,
@@ -1942,7 +1968,19 @@ static int as_padded_tables(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p, bool
   return rc;
 }
 
-static int asprop_create(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out, bool fp64_tables) {
+// One table per field (per-field filter samples): the keys of plan_common are undone.
+static void plan_table_per_field(fh_plan* p) {
+  std::vector<double> l(p->batch), z(p->batch), tx(p->batch), ty(p->batch);
+  for (int b = 0; b < p->batch; ++b) {
+    const int k = p->field_table[b];
+    l[b] = p->lam_tables[k]; z[b] = p->tab_z[k]; tx[b] = p->tab_tx[k]; ty[b] = p->tab_ty[k];
+    p->field_table[b] = b;
+  }
+  p->lam_tables = l; p->tab_z = z; p->tab_tx = tx; p->tab_ty = ty;
+}
+
+static int asprop_create(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out, bool fp64_tables,
+                         const double* filter = nullptr, int nfilters = 0) {
   if (!out || !d) return fail(FH_E_ARG, "null descriptor or output pointer");
   *out = nullptr;
   if (d->flags & ~(FH_PROP_PARAXIAL | FH_PROP_TRACK_TILTS | FH_PROP_SHIFT)) return fail(FH_E_ARG, "unknown flags 0x%x", d->flags);
@@ -1974,8 +2012,34 @@ static int asprop_create(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out, bool
   p->mode = (d->flags & FH_PROP_SHIFT) ? 3 : ((d->flags & FH_PROP_PARAXIAL) ? 1 : (tilted ? 2 : 0));
   p->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
   p->full = tilted ? 1 : 0;
+  // frequency filter: samples are not even in fx, fy, so the tables take the full layout
+  double2* filt = nullptr;
+  if (filter) {
+    if (nfilters != 1 && nfilters != d->batch) {
+      plan_free(p);
+      return fail(FH_E_ARG, "need 1 or batch (=%d) filter grids, got %d", d->batch, nfilters);
+    }
+    if (p->mode == 3) { plan_free(p); return fail(FH_E_ARG, "ShiftProp takes no frequency filter"); }
+    const size_t nf = (size_t)nfilters * nx * ny;
+    for (size_t k = 0; k < 2 * nf; ++k)
+      if (!std::isfinite(filter[k])) { plan_free(p); return fail(FH_E_ARG, "filter samples must be finite"); }
+    if (nfilters > 1) plan_table_per_field(p);
+    p->full = 1;
+    p->filtered = 1;
+    if (hipMalloc(&filt, nf * sizeof(double2)) != hipSuccess) {
+      plan_free(p);
+      return fail(FH_E_NOMEM, "filter samples: out of device memory");
+    }
+    if (hipMemcpy(filt, filter, nf * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(filt);
+      plan_free(p);
+      return fail(FH_E_HIP, "filter upload failed");
+    }
+  }
   if (!pow2) {
-    if ((rc = as_padded_tables(ctx, d, p, fp64_tables))) { plan_free(p); return rc; }
+    rc = as_padded_tables(ctx, d, p, fp64_tables, filt, nfilters);
+    if (filt) (void)hipFree(filt);
+    if (rc) { plan_free(p); return rc; }
     *out = p;
     return FH_OK;
   }
@@ -1983,22 +2047,38 @@ static int asprop_create(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out, bool
   p->qy = p->full ? ny + 1 : ny / 2 + 1;
   p->tabsize = (long)p->qx * p->qy;
   const size_t bytes = p->tabsize * p->lam_tables.size() * sizeof(float2);
-  if (hipMalloc(&p->tab, bytes) != hipSuccess) { plan_free(p); return fail(FH_E_NOMEM, "ASProp table: out of device memory"); }
+  if (hipMalloc(&p->tab, bytes) != hipSuccess) {
+    if (filt) (void)hipFree(filt);
+    plan_free(p);
+    return fail(FH_E_NOMEM, "ASProp table: out of device memory");
+  }
   const double scale = 1.0 / ((double)nx * (double)ny);
-  for (size_t k = 0; k < p->lam_tables.size(); ++k) {
+  for (size_t k = 0; k < p->lam_tables.size() && rc == FH_OK; ++k) {
     dim3 grid((p->qx + 255) / 256, p->qy);
     hipLaunchKernelGGL(k_as_table<float2>, grid, dim3(256), 0, ctx->stream, p->tab + k * p->tabsize, nx, ny,
                        p->qx, p->qy, p->full, d->dx, d->dy, p->lam_tables[k], d->n0, p->tab_z[k],
-                       p->tab_tx[k], p->tab_ty[k], p->mode, p->track, scale);
-    if (hipGetLastError() != hipSuccess) { plan_free(p); return fail(FH_E_HIP, "ASProp table kernel launch failed"); }
+                       p->tab_tx[k], p->tab_ty[k], p->mode, p->track, scale,
+                       (const double2*)(filt ? filt + (nfilters == 1 ? 0 : k) * (size_t)nx * ny : nullptr),
+                       filter_conj(p, (long)k));
+    if (hipGetLastError() != hipSuccess) rc = fail(FH_E_HIP, "ASProp table kernel launch failed");
   }
-  if ((rc = upload_tabidx(p))) { plan_free(p); return rc; }
+  if (filt) {  // the table kernels read the samples: wait before freeing them
+    if (rc == FH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(FH_E_HIP, "ASProp table generation failed");
+    (void)hipFree(filt);
+  }
+  if (rc == FH_OK) rc = upload_tabidx(p);
+  if (rc) { plan_free(p); return rc; }
   *out = p;
   return FH_OK;
 }
 
 int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out) {
   return asprop_create(ctx, d, out, false);
+}
+
+int fh_asprop_create_filtered(fh_ctx* ctx, const fh_prop_desc* d, const double* filter, int nfilters, fh_plan** out) {
+  if (!filter) return fail(FH_E_ARG, "null filter samples");
+  return asprop_create(ctx, d, out, false, filter, nfilters);
 }
 
 int fh_asprop_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
@@ -2825,7 +2905,8 @@ int fh_sqint_block_src(fh_plan* p, const void* src, void* u, const float* target
 
 int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {
   if (!p || !u_c128) return fail(FH_E_ARG, "null plan or field");
-  if (p->kind != 0 || p->mode != 0) return fail(FH_E_UNSUPP, "the plain FP64 comparator covers on-axis ASProp only");
+  if (p->kind != 0 || p->mode != 0 || p->filtered)
+    return fail(FH_E_UNSUPP, "the plain FP64 comparator covers on-axis ASProp without a filter only");
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   fh_ctx* c = p->ctx;

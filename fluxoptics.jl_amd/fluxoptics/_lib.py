@@ -48,6 +48,8 @@ SIGNATURES = {
     "fh_rsprop_create": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_double_p,
                                  c_int, c_double, ctypes.POINTER(c_void_p)]),
     "fh_asprop_create_ex": (c_int, [c_void_p, ctypes.POINTER(PropDesc), ctypes.POINTER(c_void_p)]),
+    "fh_asprop_create_filtered": (c_int, [c_void_p, ctypes.POINTER(PropDesc), c_void_p, c_int,
+                                          ctypes.POINTER(c_void_p)]),
     "fh_rsprop_create_ex": (c_int, [c_void_p, ctypes.POINTER(PropDesc), ctypes.POINTER(c_void_p)]),
     "fh_group_create": (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_void_p)]),
     "fh_group_propagate": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_int]),

@@ -12,6 +12,7 @@ libfluxhip on the field's HIP device.
 from __future__ import annotations
 
 import ctypes
+import math
 import warnings
 from typing import Callable, Optional, Union
 
@@ -173,8 +174,14 @@ class ASProp(_KernelPropagator):
     reference does with ``use_cache=true`` (``double_precision_kernel=use_cache``).
     The kernel follows the reference's choice (``angular_spectrum.jl:126-139``): paraxial
     when ``paraxial=true``, else on-axis for untilted fields, else tilted (with the field's
-    tilts, ``track_tilts`` adding the walk-off phase).  ``filter`` (a Julia function of the
-    frequencies) is not on the accelerated path.
+    tilts, ``track_tilts`` adding the walk-off phase).  ``filter``, a function of the
+    frequencies (fx, fy) returning a (complex) transmission, multiplies the kernel as in the
+    reference (``angular_spectrum.jl:5,34,57``): it is sampled on the host in FP64 on the
+    fftfreq grid of ``ds`` -- at (fx, fy) for on-axis and paraxial kernels, at
+    (fx + f0x, fy + f0y) per field for the tilted one -- and folded into the FP64 transfer
+    function before rounding (``fh_asprop_create_filtered``).  It is called once on whole
+    frequency arrays; a filter written for scalars (``if`` on its arguments) is evaluated
+    element by element instead.
     """
 
     _create_ex = "fh_asprop_create_ex"
@@ -182,19 +189,44 @@ class ASProp(_KernelPropagator):
     def __init__(self, u: ScalarField, z: float, ds=None, *, n0: float = 1.0,
                  use_cache: bool = True, paraxial: bool = False, filter: Optional[Callable] = None,
                  track_tilts: bool = False, double_precision_kernel: Optional[bool] = None):
-        if filter is not None:
-            raise NotImplementedError("a frequency filter is not on the accelerated path")
         if double_precision_kernel is False:
             raise NotImplementedError("transfer functions are always computed in FP64")
+        if filter is not None and not callable(filter):
+            raise TypeError("filter must be a function of (fx, fy)")
         super().__init__(u, ds, z)
         self.n0 = float(n0)
         self.paraxial = bool(paraxial)
         self.track_tilts = bool(track_tilts)
+        self.filter = filter
         if not self.n0 > 0:
             raise ValueError("n0 must be positive")
 
     def _flags(self) -> int:
         return super()._flags() | (_lib.FH_PROP_PARAXIAL if self.paraxial else 0)
+
+    def _create(self, ctx, desc, p) -> None:
+        if self.filter is None:
+            return super()._create(ctx, desc, p)
+        grids = self.filter_grids(desc)
+        buf = np.ascontiguousarray(grids, dtype=np.complex128)
+        _lib.call("fh_asprop_create_filtered", ctx, ctypes.byref(desc), buf.ctypes.data_as(ctypes.c_void_p),
+                  len(grids), ctypes.byref(p))
+
+    def filter_grids(self, desc) -> np.ndarray:
+        """The filter's samples for a plan: (1 or batch, ny, nx) complex128 on the fftfreq grid
+        (FFT order), at the arguments the reference's kernel passes it."""
+        fx = np.fft.fftfreq(self.nx, self.ds[0])[None, :]
+        fy = np.fft.fftfreq(self.ny, self.ds[1])[:, None]
+        tx = [desc.tilts_x[b] for b in range(desc.batch)]
+        ty = [desc.tilts_y[b] for b in range(desc.batch)]
+        if self.paraxial or all(a == 0.0 for a in tx + ty):
+            return _sample_filter(self.filter, fx, fy)[None]
+        lams = [desc.lambdas[0 if desc.nlambdas == 1 else b] for b in range(desc.batch)]
+        out = []
+        for b in range(desc.batch):
+            le = lams[b] / self.n0
+            out.append(_sample_filter(self.filter, fx + math.sin(tx[b]) / le, fy + math.sin(ty[b]) / le))
+        return np.stack(out)
 
     @classmethod
     def _from_params(cls, nx: int, ny: int, ds, z: float, n0: float) -> "ASProp":
@@ -204,7 +236,25 @@ class ASProp(_KernelPropagator):
         self.n0 = float(n0)
         self.paraxial = False
         self.track_tilts = False
+        self.filter = None
         return self
+
+
+def _sample_filter(filt, fx, fy) -> np.ndarray:
+    """filt over the broadcast frequency grids as complex128: one call on whole arrays when the
+    filter accepts them, else one call per sample (as the reference broadcasts a scalar
+    function)."""
+    fx, fy = np.broadcast_arrays(np.asarray(fx, np.float64), np.asarray(fy, np.float64))
+    try:
+        with np.errstate(all="ignore"):
+            v = np.asarray(filt(fx, fy), dtype=np.complex128)
+        if v.shape == fx.shape:
+            return v
+        if v.ndim == 0:  # a constant filter
+            return np.full(fx.shape, complex(v))
+    except (ValueError, TypeError):
+        pass
+    return np.vectorize(lambda a, b: complex(filt(float(a), float(b))), otypes=[np.complex128])(fx, fy)
 
 
 class ASPropZ(_PipeComponent):

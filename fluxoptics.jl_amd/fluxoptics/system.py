@@ -77,6 +77,7 @@ def _merge_pair(a, b):
     if (type(a) is ASProp and type(b) is ASProp and a.n0 == b.n0 and a.ds == b.ds
             and (a.nx, a.ny) == (b.nx, b.ny) and not a.paraxial and not b.paraxial
             and not a.track_tilts and not b.track_tilts
+            and getattr(a, "filter", None) is None and getattr(b, "filter", None) is None
             and not isinstance(a.z, list) and not isinstance(b.z, list)
             and (a.z >= 0) == (b.z >= 0)):
         # H(z1) H(z2) = H(z1 + z2) for distances of one sign (evanescent terms decay with |z|)

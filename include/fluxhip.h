@@ -102,6 +102,20 @@ typedef struct fh_prop_desc {
  * z tan(theta) of each field, as a Fourier-domain linear phase. */
 int fh_asprop_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, fh_plan** out);
 
+/* ASProp with a frequency filter: ASProp(u, ds, z; filter) (angular_spectrum.jl:1-76,193-229),
+ * whose kernel multiplies the transfer function by Complex(filter(...)).  The filter is a Julia
+ * function, so the caller samples it on the host, in FP64, on the fftfreq grid of the plan:
+ * filter holds nfilters grids of ny*nx complex doubles (re, im interleaved), grid f element
+ * (i, j) at 2*(f*nx*ny + j*nx + i), sampled at the arguments the reference's kernel passes for
+ * frequency indices (i, j) in FFT order (fx_i = fftfreq(nx, 1/dx)[i], likewise fy_j):
+ * filter(fx_i, fy_j) for on-axis and paraxial kernels, filter(fx_i + f0x, fy_j + f0y) with
+ * f0 = sin(theta)/lambda' per field for the tilted kernel.  nfilters is 1 (shared) or batch (one
+ * grid per field, e.g. tilted fields).  The samples multiply the FP64 transfer function before
+ * it is rounded; as in the reference, the on-axis and tilted kernels conjugate them with the rest
+ * of the kernel when z <= 0, and Backward conjugates the whole kernel.  Not for FH_PROP_SHIFT. */
+int fh_asprop_create_filtered(fh_ctx* ctx, const fh_prop_desc* desc, const double* filter, int nfilters,
+                              fh_plan** out);
+
 /* RSProp with per-field z and tilted fields (rs_tilted_kernel, rayleigh_sommerfeld.jl:14-27,
  * selected at :90-96 when a tilt is nonzero).  flags: FH_PROP_TRACK_TILTS only. */
 int fh_rsprop_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, fh_plan** out);

@@ -178,22 +178,45 @@ def shiftprop(u, ds, z, tilts, direction: int = 1):
     return ift2(v)
 
 
+def filter_samples(filt, fx, fy):
+    """``Complex(filter(fx, fy))`` evaluated element by element over broadcast (fx, fy), as the
+    reference's kernel broadcast calls it (``angular_spectrum.jl:5,34,57``), complex FP64."""
+    fx, fy = np.broadcast_arrays(np.asarray(fx, np.float64), np.asarray(fy, np.float64))
+    out = np.empty(fx.shape, np.complex128)
+    for idx in np.ndindex(fx.shape):
+        out[idx] = complex(filt(float(fx[idx]), float(fy[idx])))
+    return out
+
+
 def as_transfer(nx: int, ny: int, dx: float, dy: float, lam: float, n0: float, z: float,
-                tilts=(0.0, 0.0), paraxial: bool = False, track: bool = False):
+                tilts=(0.0, 0.0), paraxial: bool = False, track: bool = False, filt=None):
     """H sampled on the FFT-order frequency grid, shape (ny, nx).
 
     ``fill_kernel_cache`` broadcasting ``as_kernel`` over ``f_vec = (x=fx, y=fy')``:
     ``abstract_kernel.jl:15-25``, ``fourier_kernel.jl:17-18``; the kernel choice of
     ``angular_spectrum.jl:126-139`` (paraxial, else on-axis when both tilts are zero, else
-    tilted).
+    tilted).  ``filt``: the optional frequency filter, a function of (fx, fy) multiplying the
+    kernel (``angular_spectrum.jl:5,34,57``): at (fx, fy) for the on-axis and paraxial kernels,
+    at (fx + f0x, fy + f0y) for the tilted one; the on-axis and tilted kernels for z <= 0 are
+    conj(kernel(-z)) with the filter inside (``:9-12,64-68``, chosen by ``Val(sign(z) > 0)``,
+    ``:119``), so there the filter enters conjugated; the paraxial kernel never conjugates it.
     """
     fx = fftfreq(nx, dx)[None, :]
     fy = fftfreq(ny, dy)[:, None]
     if paraxial:
-        return as_paraxial_kernel(fx, fy, lam, tilts[0], tilts[1], track, n0, z)
+        h = as_paraxial_kernel(fx, fy, lam, tilts[0], tilts[1], track, n0, z)
+        return h if filt is None else h * filter_samples(filt, fx, fy)
     if tilts[0] == 0 and tilts[1] == 0:
-        return as_kernel(fx, fy, lam, n0, z)
-    return tilted_as_kernel(fx, fy, lam, tilts[0], tilts[1], track, n0, z)
+        h = as_kernel(fx, fy, lam, n0, z)
+        gx, gy = fx, fy
+    else:
+        h = tilted_as_kernel(fx, fy, lam, tilts[0], tilts[1], track, n0, z)
+        le = lam / n0
+        gx, gy = fx + math.sin(tilts[0]) / le, fy + math.sin(tilts[1]) / le
+    if filt is None:
+        return h
+    f = filter_samples(filt, gx, gy)
+    return h * (f if z > 0 else np.conj(f))
 
 
 def as_z_gradient(v, dv, ds, lams, z: float, n0: float = 1.0):
@@ -223,13 +246,14 @@ def _per_field(v, b: int):
 
 
 def asprop(u, ds, lams, z, n0: float = 1.0, direction: int = 1, tilts=None, paraxial: bool = False,
-           track: bool = False):
+           track: bool = False, filt=None):
     """``ASProp`` propagation: FFT2 -> multiply by H (or conj H) per slice -> IFFT2.
 
     Sequence (ft, ASKernelProp, ift): ``angular_spectrum.jl:193-229``; per-slice kernel
     multiply ``abstract_kernel.jl:64-83``; Backward = reverse direction
     ``freespace.jl:61-63`` + ``fourier_operator.jl:65-79``.  ``z`` and each tilt angle
-    may be one value or one per field; ``tilts`` = (theta_x, theta_y) in radians.
+    may be one value or one per field; ``tilts`` = (theta_x, theta_y) in radians; ``filt`` the
+    optional frequency filter (see ``as_transfer``).
     """
     u = _as_batch(np.asarray(u, dtype=np.complex128))
     b, ny, nx = u.shape
@@ -240,7 +264,7 @@ def asprop(u, ds, lams, z, n0: float = 1.0, direction: int = 1, tilts=None, para
     ty_b = _per_field(0.0 if tilts is None else tilts[1], b)
     v = ft2(u)
     for k in range(b):
-        h = as_transfer(nx, ny, dx, dy, lam_b[k], n0, z_b[k], (tx_b[k], ty_b[k]), paraxial, track)
+        h = as_transfer(nx, ny, dx, dy, lam_b[k], n0, z_b[k], (tx_b[k], ty_b[k]), paraxial, track, filt)
         v[k] *= conj_direction(h, direction)
     return ift2(v)
 
