@@ -171,3 +171,120 @@ k_bpm_xstep_grad(LineIO<float2> io, int batch, const C* __restrict__ tw, DnPhase
 }
 
 }  // namespace fh
+
+namespace fh {
+
+// ---------------------------------------------------------------- 1-D fields
+// A 1-D field (Fields.jl:135-136) runs the whole split-step loop of one field line in one
+// workgroup: forward P(dz/2); per slice [save], phase, P(dz) (P(dz/2) after the last) (bpm.jl:
+// 244-260), or the adjoint's reverse loop with the phase un-applied and the slice's gradient
+// terms s * kl * Im(du * conj(saved)) (bpm.jl:275-312).  The line stays in FP64 registers across
+// every slice (nothing is rounded between slices); each slice propagator is FFT_N -> transfer
+// function -> IFFT_N on the line zero-padded to N points, cropped back to n after the inverse
+// transform.  The inverse plan ends in the register layout the forward plan starts from, so a
+// slice never leaves the registers except through the LDS exchanges of its transforms.
+struct BpmLine {
+  float2* u;             // [batch][n] fields, in place
+  const float* dn;       // [ns][n] slices
+  const double* kl;      // [batch] kdz / lambda_b (vacuum wavelength)
+  const double2* tfull;  // [tables][N] P(dz), 1/N folded in
+  const double2* thalf;  // [tables][N] P(dz/2)
+  const int* tabidx;     // [batch] table of each field
+  float2* save;          // forward: [ns][batch][n] slice inputs (nullable)
+  const float2* saved;   // adjoint: the forward's slice inputs (nullable: no gradient)
+  float* gpart;          // adjoint: [ns][batch][n] gradient terms (with saved)
+  int n, batch, ns;
+  int dir;               // direction of the forward propagation (+1 or -1)
+};
+
+template <typename RD>
+__device__ __forceinline__ void bpm_line_prop(double2* v, double2* s, int t, const double2* __restrict__ tw,
+                                              const double2* __restrict__ H, bool conj, int n) {
+  using RI = typename RD::Rev;
+  constexpr int N = RD::n, V = RD::V, P = RD::P, RL = RI::R0;
+  line_fft<double2, RD, false>(v, s, t, tw);
+#pragma unroll
+  for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const double2 h = H[reg_pos<N, RL, P>(t, m, r)];
+      v[m * RL + r] = conj ? cmulc(v[m * RL + r], h) : cmul(v[m * RL + r], h);
+    }
+  line_fft<double2, RI, true>(v, s, t, tw);
+  if (N != n) {  // the padded line: back to zero outside the field
+#pragma unroll
+    for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+      for (int r = 0; r < RD::R0; ++r)
+        if (reg_pos<N, RD::R0, P>(t, m, r) >= n) v[m * RD::R0 + r] = make_double2(0.0, 0.0);
+  }
+}
+
+template <typename RD, bool ADJ>
+__global__ void __launch_bounds__(RD::P) k_bpm_line(BpmLine a, const double2* __restrict__ tw) {
+  constexpr int N = RD::n, V = RD::V, P = RD::P, R0 = RD::R0;
+  extern __shared__ unsigned char smem_raw[];
+  double2* s = reinterpret_cast<double2*>(smem_raw);
+  const int t = threadIdx.x;
+  const int b = blockIdx.x;
+  const long fb = (long)b * a.n;
+  double2 v[V];
+#pragma unroll
+  for (int m = 0; m < V / R0; ++m)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int pos = reg_pos<N, R0, P>(t, m, r);
+      v[m * R0 + r] = pos < a.n ? cvt<double2>(a.u[fb + pos]) : make_double2(0.0, 0.0);
+    }
+  const double2* Hf = a.tfull + (long)a.tabidx[b] * N;
+  const double2* Hh = a.thalf + (long)a.tabidx[b] * N;
+  const double kl = a.kl[b];
+  const double sg = a.dir;
+  const bool conj = (ADJ ? -a.dir : a.dir) < 0;  // the adjoint runs every propagator reversed
+  bpm_line_prop<RD>(v, s, t, tw, Hh, conj, a.n);
+  for (int i = 0; i < a.ns; ++i) {
+    const int k = ADJ ? a.ns - 1 - i : i;
+    const float* dk = a.dn + (long)k * a.n;
+    const long sb = ((long)k * a.batch + b) * a.n;
+#pragma unroll
+    for (int m = 0; m < V / R0; ++m)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) {
+        const int pos = reg_pos<N, R0, P>(t, m, r);
+        if (pos < a.n) {
+          double2 x = v[m * R0 + r];
+          if (!ADJ && a.save) a.save[sb + pos] = cvt<float2>(x);
+          double sn, cs;
+          sincos((ADJ ? -sg : sg) * kl * (double)dk[pos], &sn, &cs);
+          x = make_double2(x.x * cs - x.y * sn, x.x * sn + x.y * cs);
+          if (ADJ && a.saved) {
+            const float2 sv = a.saved[sb + pos];
+            a.gpart[sb + pos] = (float)(sg * kl * (x.y * sv.x - x.x * sv.y));
+          }
+          v[m * R0 + r] = x;
+        }
+      }
+    const bool half = ADJ ? k == 0 : k == a.ns - 1;
+    bpm_line_prop<RD>(v, s, t, tw, half ? Hh : Hf, conj, a.n);
+  }
+#pragma unroll
+  for (int m = 0; m < V / R0; ++m)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int pos = reg_pos<N, R0, P>(t, m, r);
+      if (pos < a.n) a.u[fb + pos] = cvt<float2>(v[m * R0 + r]);
+    }
+}
+
+// ddn[k][x] = sum over fields of gpart[k][b][x], in field order
+__global__ void k_sum_line_grads(const float* __restrict__ gpart, float* __restrict__ ddn, int n, int batch, int ns) {
+  const long total = (long)ns * n;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long k = i / n, x = i - k * n;
+    double g = 0.0;
+    for (int b = 0; b < batch; ++b) g += gpart[(k * batch + b) * n + x];
+    ddn[i] = (float)g;
+  }
+}
+
+}  // namespace fh

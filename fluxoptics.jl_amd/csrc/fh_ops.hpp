@@ -41,6 +41,27 @@ using QuadMul = QuadMulT<float2>;
 
 template <typename T> struct FreqPre<QuadMulT<T>> : std::true_type {};
 
+// Multiply by the transfer function of a 1-D field (Fields.jl:135-136): one table of n entries
+// in FFT order per table index, the line index is the field (the batch runs as the lines of
+// one pass); 1/n of the inverse FFT folded in.
+template <typename T> struct LineTabMulT {
+  const T* tab;
+  const int* tabidx;
+  int n;
+  int conj;  // Backward direction multiplies by the conjugate
+  template <typename C>
+  __device__ __forceinline__ C pre(int, int field, int k) const {
+    return cvt<C>(tab[(long)tabidx[field] * n + k]);
+  }
+  template <typename C>
+  __device__ __forceinline__ C apply(C v, C h) const { return conj ? cmulc(v, h) : cmul(v, h); }
+  template <typename C>
+  __device__ __forceinline__ C operator()(C v, int b, int field, int k) const {
+    return apply(v, pre<C>(b, field, k));
+  }
+};
+template <typename T> struct FreqPre<LineTabMulT<T>> : std::true_type {};
+
 // Plain multiply by a full table (FP64 comparator path); the table is [ky][kx].
 struct FullMulD {
   const double2* tab;

@@ -173,6 +173,27 @@ static hipError_t launch_conv(int logn, const LineIO<S>& io, int batch, const C*
   FH_LOGN_SWITCH(logn, (launch_conv_t<S, C, LG, Op>(io, batch, tw, op, st)))
 }
 
+// FFT -> frequency op -> IFFT along contiguous lines (1-D fields: the batch's lines, one pass per
+// propagation), lines per workgroup halved while a launch has fewer than kFillWGs workgroups.
+template <typename S, typename C, typename RD, int L, typename Op>
+static hipError_t launch_line_conv_l(const LineIO<S>& io, const C* tw, Op op, hipStream_t st) {
+  if constexpr (L > 1) {
+    if ((long)((io.nlines + L - 1) / L) < kFillWGs) return launch_line_conv_l<S, C, RD, L / 2, Op>(io, tw, op, st);
+  }
+  auto kern = k_line_conv<S, C, RD, L, false, Op>;
+  const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n, L, sizeof(C)>::size * sizeof(C) : 0;
+  hipError_t e = ensure_lds(kern, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L, 1), dim3(L * RD::P), lds, st, io, tw, op, 1);
+  return hipGetLastError();
+}
+
+template <typename S, typename C, typename Op>
+static hipError_t launch_line_conv(int logn, const LineIO<S>& io, const C* tw, Op op, hipStream_t st) {
+  FH_LOGN_SWITCH(logn, (launch_line_conv_l<S, C, typename Cfg<typename RealOf<C>::type, LG, false>::RD,
+                                           Cfg<typename RealOf<C>::type, LG, false>::L, Op>(io, tw, op, st)))
+}
+
 // ================================================================ table kernels (FP64)
 
 // Transfer-function table layouts (column-major, one column per kx so the y pass reads a
@@ -279,6 +300,34 @@ __global__ void k_as_table(T2* __restrict__ tab, int nx, int ny, int rows, int q
   double re = amp * cs, im = amp * sn;
   apply_filter(filt, conjf, (long)(ky < 0 ? ky + ny : ky) * nx + (kx < 0 ? kx + nx : kx), re, im);
   store_tab(tab + (long)i * qcol + j, re * scale, im * scale);
+}
+
+// Transfer function of a 1-D field on an n-point power-of-two grid (FFT order): the reference's
+// 1-D kernels (angular_spectrum.jl:14-25,37-43,64-76; shift_kernel.jl:8-12) are its 2-D ones at
+// fy = 0 and theta_y = 0, so as_transfer is evaluated there; filter samples and scale as in
+// k_as_table.
+template <typename T2>
+__global__ void k_as_line_table(T2* __restrict__ tab, int n, double dx, double lam, double n0, double z, double tx,
+                                int mode, int track, double scale, const double2* __restrict__ filt, int conjf) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int kx = i < n / 2 ? i : i - n;
+  double amp, t;
+  as_transfer((double)kx / ((double)n * dx), 0.0, lam, n0, z, tx, 0.0, mode, track, amp, t);
+  const double r = t - rint(t);
+  double sn, cs;
+  sincospi(2.0 * r, &sn, &cs);
+  double re = amp * cs, im = amp * sn;
+  apply_filter(filt, conjf, i, re, im);
+  store_tab(tab + i, re * scale, im * scale);
+}
+
+// one padded FP64 line (the FFT of the periodically extended 1-D kernel) scaled into a table
+template <typename T2>
+__global__ void k_line_extract(T2* __restrict__ tab, const double2* __restrict__ src, int n, double scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) store_tab(tab + i, src[i].x * scale, src[i].y * scale);
 }
 
 // ASProp on grids that are not powers of two runs as a zero-padded linear convolution: the
@@ -1246,6 +1295,7 @@ struct fh_plan {
   int track = 0;                   // tilts tracked
   int full = 0;                    // full (non-even) table layout
   int filtered = 0;                // ASProp tables include a frequency filter
+  int oned = 0;                    // 1-D fields (ny == 1): lines of nx points, one pass per propagation
   float2* tab = nullptr;           // transfer-function tables, one per distinct key
   long tabsize = 0;
   int qx = 0, qy = 0;
@@ -1906,6 +1956,88 @@ int fh_ctx_destroy(fh_ctx* ctx) {
   return FH_OK;
 }
 
+// ---------------------------------------------------------------- 1-D fields
+// A 1-D ScalarField (Fields.jl:135-136: electric [nx, batch...], Nd = 1) is a batch of lines of
+// nx points; a propagation is one pass of FFT -> transfer function -> IFFT per line.  Power-of-two
+// nx transforms at length nx; any other nx as the n-point circular convolution written as a
+// zero-padded linear one at the power of two N >= 2 nx - 1 (as as_padded_tables does in 2-D).
+
+struct LineKey { double lam, z, tx; };
+
+// Tables of 1-D propagators, [keys][N] in FFT order with 1/N folded in: the n-point transfer
+// function when N = n, else FFT_N of h = IDFT_n(H) extended periodically over |d| < n; FP64
+// throughout, rounded once into tab (c64) and/or kept in tab_d (FP64).  filt: 1 or one per key
+// grid of n filter samples (nullable).
+static int line_tables(fh_ctx* ctx, int n, int lx, double dx, double n0, int mode, int track,
+                       const std::vector<LineKey>& keys, const double2* filt, int nfilt, float2* tab,
+                       double2* tab_d) {
+  const int N = 1 << lx;
+  const bool direct = N == n;
+  hipStream_t st = ctx->stream;
+  int rc = FH_OK;
+  double2 *hn = nullptr, *tmp = nullptr, *wx = nullptr, *dbuf = nullptr;
+  const double2* twN = nullptr;
+  if (!direct) {
+    if (hipMalloc(&hn, n * sizeof(double2)) != hipSuccess || hipMalloc(&tmp, n * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&wx, n * sizeof(double2)) != hipSuccess || hipMalloc(&dbuf, (size_t)N * sizeof(double2)) != hipSuccess)
+      rc = fail(FH_E_NOMEM, "1-D tables: out of device memory");
+    if (rc == FH_OK) rc = get_twiddles(ctx, N, &twN);
+    if (rc == FH_OK) hipLaunchKernelGGL(k_unit_roots_d, dim3((n + 255) / 256), dim3(256), 0, st, wx, n);
+  }
+  const dim3 gN((N + 255) / 256), gn((n + 255) / 256);
+  for (size_t k = 0; k < keys.size() && rc == FH_OK; ++k) {
+    const double2* fk = filt ? filt + (nfilt == 1 ? 0 : k) * (size_t)n : nullptr;
+    const int conjf = mode != 1 && !(keys[k].z > 0.0);
+    if (direct) {
+      if (tab)
+        hipLaunchKernelGGL(k_as_line_table<float2>, gN, dim3(256), 0, st, tab + k * N, N, dx, keys[k].lam, n0,
+                           keys[k].z, keys[k].tx, mode, track, 1.0 / N, fk, conjf);
+      if (tab_d)
+        hipLaunchKernelGGL(k_as_line_table<double2>, gN, dim3(256), 0, st, tab_d + k * N, N, dx, keys[k].lam, n0,
+                           keys[k].z, keys[k].tx, mode, track, 1.0 / N, fk, conjf);
+    } else {
+      hipLaunchKernelGGL(k_as_grid_d, gn, dim3(256), 0, st, hn, n, 1, dx, 1.0, keys[k].lam, n0, keys[k].z,
+                         keys[k].tx, 0.0, mode, track, fk, conjf);
+      hipLaunchKernelGGL(k_idft_lines_d, gn, dim3(256), 0, st, (const double2*)hn, tmp, n, 1, (long)n, 1L,
+                         (const double2*)wx);
+      hipLaunchKernelGGL(k_circ_embed_d, gN, dim3(256), 0, st, dbuf, (const double2*)tmp, n, 1, N, 1);
+      LineIO<double2> io{};
+      io.in = dbuf; io.out = dbuf; io.in_l = io.out_l = N; io.in_e = io.out_e = 1;
+      io.nlines = 1; io.in_valid = io.out_count = N;
+      if (launch_fft<double2, double2, false, false>(lx, io, 1, twN, NoOp{}, st) != hipSuccess) {
+        rc = fail(FH_E_HIP, "1-D table FFT failed to launch");
+        break;
+      }
+      if (tab) hipLaunchKernelGGL(k_line_extract<float2>, gN, dim3(256), 0, st, tab + k * N, (const double2*)dbuf, N, 1.0 / N);
+      if (tab_d)
+        hipLaunchKernelGGL(k_line_extract<double2>, gN, dim3(256), 0, st, tab_d + k * N, (const double2*)dbuf, N, 1.0 / N);
+    }
+    if (hipGetLastError() != hipSuccess) rc = fail(FH_E_HIP, "1-D table kernels failed to launch");
+  }
+  // the buffers below (and the caller's filter samples) are read by the kernels: wait first
+  if (hipStreamSynchronize(st) != hipSuccess && rc == FH_OK) rc = fail(FH_E_HIP, "1-D table generation failed");
+  for (void* q : {(void*)hn, (void*)tmp, (void*)wx, (void*)dbuf})
+    if (q) (void)hipFree(q);
+  return rc;
+}
+
+// Uploads nfilters grids of nx*ny filter samples after checking them (nullptr when filter is).
+static int upload_filter(const double* filter, int nfilters, int batch, size_t npix, double2** out) {
+  *out = nullptr;
+  if (!filter) return FH_OK;
+  if (nfilters != 1 && nfilters != batch) return fail(FH_E_ARG, "need 1 or batch (=%d) filter grids, got %d", batch, nfilters);
+  const size_t nf = (size_t)nfilters * npix;
+  for (size_t k = 0; k < 2 * nf; ++k)
+    if (!std::isfinite(filter[k])) return fail(FH_E_ARG, "filter samples must be finite");
+  if (hipMalloc(out, nf * sizeof(double2)) != hipSuccess) return fail(FH_E_NOMEM, "filter samples: out of device memory");
+  if (hipMemcpy(*out, filter, nf * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(*out);
+    *out = nullptr;
+    return fail(FH_E_HIP, "filter upload failed");
+  }
+  return FH_OK;
+}
+
 // Whether table k takes the conjugated filter: the reference's on-axis and tilted kernels for
 // z <= 0 are conj(kernel(-z)) with the filter inside (angular_spectrum.jl:9-12,64-68); its
 // paraxial kernel multiplies the filter as given for either sign (:27-35).
@@ -1979,6 +2111,60 @@ static void plan_table_per_field(fh_plan* p) {
   p->lam_tables = l; p->tab_z = z; p->tab_tx = tx; p->tab_ty = ty;
 }
 
+// A 1-D ASProp / ShiftProp plan (ny == 1): the reference's kernel choice with theta_y = 0.
+static int line_plan(fh_ctx* ctx, const fh_prop_desc* d, fh_plan* p, bool fp64, const double* filter, int nfilters) {
+  const int nx = d->nx;
+  for (double ty : p->tab_ty)
+    if (ty != 0.0) return fail(FH_E_ARG, "a 1-D field has no y tilt");
+  const bool pow2 = ilog2_exact(nx) >= 0;
+  p->lx = pow2 ? ilog2_exact(nx) : ilog2_ceil(2 * nx - 1);
+  p->ly = 0;
+  p->oned = 1;
+  if (nx < 2 || p->lx < kMinLog || p->lx > kMaxLog)
+    return fail(FH_E_UNSUPP, "1-D ASProp needs power-of-two nx in [%d, %d], or another nx with 2 nx - 1 <= %d "
+                "(got %d)", 1 << kMinLog, 1 << kMaxLog, 1 << kMaxLog, nx);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const bool tilted = plan_tilted(p);
+  p->mode = (d->flags & FH_PROP_SHIFT) ? 3 : ((d->flags & FH_PROP_PARAXIAL) ? 1 : (tilted ? 2 : 0));
+  p->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
+  p->full = 1;
+  if (filter && p->mode == 3) return fail(FH_E_ARG, "ShiftProp takes no frequency filter");
+  double2* filt = nullptr;
+  if ((rc = upload_filter(filter, nfilters, d->batch, (size_t)nx, &filt))) return rc;
+  if (filter) {
+    if (nfilters > 1) plan_table_per_field(p);
+    p->filtered = 1;
+  }
+  const int N = 1 << p->lx;
+  p->qx = N; p->qy = 1;
+  p->tabsize = N;
+  const size_t ntab = p->lam_tables.size();
+  if (hipMalloc(&p->tab, ntab * N * sizeof(float2)) != hipSuccess ||
+      (fp64 && hipMalloc(&p->tab_d, ntab * N * sizeof(double2)) != hipSuccess))
+    rc = fail(FH_E_NOMEM, "1-D ASProp tables: out of device memory");
+  std::vector<LineKey> keys;
+  for (size_t k = 0; k < ntab; ++k) keys.push_back({p->lam_tables[k], p->tab_z[k], p->tab_tx[k]});
+  if (rc == FH_OK) rc = line_tables(ctx, nx, p->lx, d->dx, d->n0, p->mode, p->track, keys, filt, nfilters, p->tab, p->tab_d);
+  if (filt) (void)hipFree(filt);
+  if (rc == FH_OK) rc = upload_tabidx(p);
+  return rc;
+}
+
+// One propagation of a 1-D plan: every field a line, one launch.
+static int run_line_prop(fh_plan* p, float2* u, int direction) {
+  const float2* tw = nullptr;
+  int rc;
+  if ((rc = get_twiddles(p->ctx, 1 << p->lx, &tw))) return rc;
+  LineIO<float2> io{};
+  io.in = u; io.out = u; io.in_f = io.out_f = 0;
+  io.in_l = io.out_l = p->nx; io.in_e = io.out_e = 1;
+  io.nlines = p->batch; io.in_valid = io.out_count = p->nx;
+  const LineTabMulT<float2> op{p->tab, p->d_tabidx, 1 << p->lx, direction < 0};
+  FH_HIP((launch_line_conv<float2, float2>(p->lx, io, tw, op, p->ctx->stream)));
+  return FH_OK;
+}
+
 static int asprop_create(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out, bool fp64_tables,
                          const double* filter = nullptr, int nfilters = 0) {
   if (!out || !d) return fail(FH_E_ARG, "null descriptor or output pointer");
@@ -1994,6 +2180,12 @@ static int asprop_create(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out, bool
   const int nx = d->nx, ny = d->ny;
   if (!(d->n0 > 0)) { plan_free(p); return fail(FH_E_ARG, "n0 must be positive"); }
   p->kind = 0; p->z = p->tab_z[0]; p->n0 = d->n0;
+  if (ny == 1) {
+    rc = line_plan(ctx, d, p, fp64_tables, filter, nfilters);
+    if (rc) { plan_free(p); return rc; }
+    *out = p;
+    return FH_OK;
+  }
   const bool pow2 = ilog2_exact(nx) >= 0 && ilog2_exact(ny) >= 0;
   if (pow2) {
     p->lx = ilog2_exact(nx); p->ly = ilog2_exact(ny);
@@ -2014,27 +2206,12 @@ static int asprop_create(fh_ctx* ctx, const fh_prop_desc* d, fh_plan** out, bool
   p->full = tilted ? 1 : 0;
   // frequency filter: samples are not even in fx, fy, so the tables take the full layout
   double2* filt = nullptr;
+  if (filter && p->mode == 3) { plan_free(p); return fail(FH_E_ARG, "ShiftProp takes no frequency filter"); }
+  if ((rc = upload_filter(filter, nfilters, d->batch, (size_t)nx * ny, &filt))) { plan_free(p); return rc; }
   if (filter) {
-    if (nfilters != 1 && nfilters != d->batch) {
-      plan_free(p);
-      return fail(FH_E_ARG, "need 1 or batch (=%d) filter grids, got %d", d->batch, nfilters);
-    }
-    if (p->mode == 3) { plan_free(p); return fail(FH_E_ARG, "ShiftProp takes no frequency filter"); }
-    const size_t nf = (size_t)nfilters * nx * ny;
-    for (size_t k = 0; k < 2 * nf; ++k)
-      if (!std::isfinite(filter[k])) { plan_free(p); return fail(FH_E_ARG, "filter samples must be finite"); }
     if (nfilters > 1) plan_table_per_field(p);
     p->full = 1;
     p->filtered = 1;
-    if (hipMalloc(&filt, nf * sizeof(double2)) != hipSuccess) {
-      plan_free(p);
-      return fail(FH_E_NOMEM, "filter samples: out of device memory");
-    }
-    if (hipMemcpy(filt, filter, nf * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
-      (void)hipFree(filt);
-      plan_free(p);
-      return fail(FH_E_HIP, "filter upload failed");
-    }
   }
   if (!pow2) {
     rc = as_padded_tables(ctx, d, p, fp64_tables, filt, nfilters);
@@ -2322,6 +2499,7 @@ int fh_propagate(fh_plan* p, void* u, int direction) {
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(u);
+  if (p->oned) return run_line_prop(p, f, direction);
   if ((rc = run_prop_plan(p, f, direction))) return rc;
   return run_xifft(p, f, direction);
 }
@@ -2332,6 +2510,7 @@ int fh_propagate_pass(fh_plan* p, void* u, int direction, int pass) {
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(u);
+  if (p->oned) return pass == 1 ? run_line_prop(p, f, direction) : FH_OK;  // one pass does it all
   if (pass < 2) return run_prop_plan(p, f, direction, pass);
   return run_xifft(p, f, direction);
 }
@@ -2341,6 +2520,7 @@ int fh_plan_variants(fh_plan* p, int pass, int* count, int* current) {
   if (pass < 0 || pass > 2) return fail(FH_E_ARG, "pass must be 0, 1 or 2, got %d", pass);
   int n = kNumXVariants;
   if (pass == 1) fast_conv_variants(p->ly, &n);
+  if (p->oned) n = 1;
   *count = n > 0 ? n : 1;
   *current = pass == 1 ? p->conv_var : (pass == 0 ? p->xf_var : p->xi_var);
   return FH_OK;
@@ -2351,6 +2531,7 @@ int fh_plan_set_variant(fh_plan* p, int pass, int variant) {
   if (pass < 0 || pass > 2) return fail(FH_E_ARG, "pass must be 0, 1 or 2, got %d", pass);
   int n = kNumXVariants;
   if (pass == 1) fast_conv_variants(p->ly, &n);
+  if (p->oned) n = 1;
   if (variant < -1 || variant >= std::max(n, 1)) return fail(FH_E_ARG, "variant %d out of range (%d)", variant, n);
   if (pass == 1) p->conv_var = variant;
   else if (pass == 0) p->xf_var = variant;
@@ -2363,6 +2544,13 @@ int fh_autotune(fh_plan* p, void* u, int reps, char* report, int report_len) {
   if (reps < 1) return fail(FH_E_ARG, "reps must be >= 1");
   int rc;
   if ((rc = set_device(p->ctx))) return rc;
+  if (p->oned) {  // one launch shape: nothing to choose
+    if (report && report_len > 0) {
+      strncpy(report, "{}", report_len - 1);
+      report[report_len - 1] = 0;
+    }
+    return FH_OK;
+  }
   float2* f = reinterpret_cast<float2*>(u);
   hipEvent_t e0, e1;
   FH_HIP(hipEventCreate(&e0));
@@ -2428,6 +2616,10 @@ int fh_phase_propagate(fh_plan* p, void* u, const float* phi, void* saved, int d
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(u);
   if (p->kind == 2) return fail(FH_E_UNSUPP, "Phase fuses with ASProp and RSProp plans only");
+  if (p->oned) {  // 1-D: the Phase, then the one-pass propagation
+    if ((rc = fh_phase_apply(p->ctx, u, phi, p->nx, 1, p->batch, direction, saved))) return rc;
+    return run_line_prop(p, f, direction);
+  }
   PhaseTimeOp op{};
   if ((rc = phase_time_op(p, phi, saved, direction, &op)) || (rc = run_prop(p, f, direction, op))) return rc;
   return run_xifft(p, f, direction);
@@ -2489,6 +2681,25 @@ int fh_fft2(fh_ctx* ctx, void* u, int nx, int ny, int batch, int sign, double sc
   if (!std::isfinite(scale)) return fail(FH_E_ARG, "scale must be finite");
   const int lx = ilog2_exact(nx), ly = ilog2_exact(ny);
   int rc;
+  if (ny == 1) {  // 1-D fields: line FFTs of the batch (power-of-two nx)
+    if (lx < kMinLog || lx > kMaxLog)
+      return fail(FH_E_UNSUPP, "1-D FFT needs a power-of-two nx in [%d, %d] (got %d)", 1 << kMinLog, 1 << kMaxLog, nx);
+    if ((rc = set_device(ctx))) return rc;
+    const float2* tw = nullptr;
+    if ((rc = get_twiddles(ctx, nx, &tw))) return rc;
+    LineIO<float2> io{};
+    io.in = io.out = reinterpret_cast<float2*>(u); io.in_f = io.out_f = 0;
+    io.in_l = io.out_l = nx; io.in_e = io.out_e = 1; io.nlines = batch; io.in_valid = io.out_count = nx;
+    const ScaleOp sc{(float)scale};
+    if (sign < 0) {
+      if (scale == 1.0) FH_HIP((launch_fft<float2, float2, false, false>(lx, io, 1, tw, NoOp{}, ctx->stream)));
+      else FH_HIP((launch_fft<float2, float2, false, false>(lx, io, 1, tw, sc, ctx->stream)));
+    } else {
+      if (scale == 1.0) FH_HIP((launch_fft<float2, float2, false, true>(lx, io, 1, tw, NoOp{}, ctx->stream)));
+      else FH_HIP((launch_fft<float2, float2, false, true>(lx, io, 1, tw, sc, ctx->stream)));
+    }
+    return FH_OK;
+  }
   if (lx < kMinLog || lx > kMaxLog || ly < kMinLog || ly > kMaxLog) {
     // other sizes: the DFT as a chirp z-transform (a = 1, w = e^{-2 pi i/n}; its adjoint plan is
     // the unnormalised inverse), one cached plan per (nx, ny, batch)
@@ -2835,6 +3046,10 @@ int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const vo
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(dv);
   if (p->kind == 2) return fail(FH_E_UNSUPP, "Phase fuses with ASProp and RSProp plans only");
+  if (p->oned) {  // 1-D: the one-pass adjoint propagation, then the Phase adjoint and gradient
+    if ((rc = run_line_prop(p, f, -direction))) return rc;
+    return fh_phase_backward(p->ctx, dv, phi, saved, dphi, p->nx, 1, p->batch, direction, accumulate);
+  }
   // adjoint propagation = the propagator in the reverse direction
   if ((rc = run_prop(p, f, -direction, NoOp{}))) return rc;
   return phase_grad_pass(p, f, phi, saved, dphi, direction, accumulate);
@@ -2845,7 +3060,7 @@ int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const vo
 static int sqint_block(fh_plan* p, const float2* src, float2* f, const float* target, int target_batch,
                        const double* dc, double* loss, int direction, const float* phi, void* saved, float* dphi,
                        int accumulate) {
-  if (p->kind == 2) return fail(FH_E_UNSUPP, "the fused metric block takes ASProp and RSProp plans");
+  if (p->kind == 2 || p->oned) return fail(FH_E_UNSUPP, "the fused metric block takes 2-D ASProp and RSProp plans");
   if (target_batch != p->batch && !(target_batch == 1 && p->batch == 1))
     return fail(FH_E_UNSUPP, "the fused metric block needs one target per field (target batch %d, batch %d)",
                 target_batch, p->batch);
@@ -2905,8 +3120,8 @@ int fh_sqint_block_src(fh_plan* p, const void* src, void* u, const float* target
 
 int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {
   if (!p || !u_c128) return fail(FH_E_ARG, "null plan or field");
-  if (p->kind != 0 || p->mode != 0 || p->filtered)
-    return fail(FH_E_UNSUPP, "the plain FP64 comparator covers on-axis ASProp without a filter only");
+  if (p->kind != 0 || p->mode != 0 || p->filtered || p->oned)
+    return fail(FH_E_UNSUPP, "the plain FP64 comparator covers on-axis 2-D ASProp without a filter only");
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
   fh_ctx* c = p->ctx;
@@ -3021,6 +3236,9 @@ struct fh_bpm {
   // the dn adjoint recomputes segment s-1 on a second stream while segment s's gradient steps
   // run on the caller's (double-buffered segments, event-ordered)
   float2* seg2 = nullptr;
+  // 1-D fields: one workgroup per field line runs every slice (k_bpm_line); seg keeps every slice
+  // input of a saving forward, [nslices][B][nx], and gpart the gradient terms [nslices][B][nx]
+  int oned = 0;
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
 };
@@ -3274,6 +3492,95 @@ static int bpm_generic_backward(fh_bpm* m, float2* f, const float* dn, float* dd
   return bpm_generic_prop(m, f, m->phalf, -direction);
 }
 
+template <int LOGN, bool ADJ>
+static hipError_t bpm_line_launch_t(const BpmLine& a, const double2* tw, hipStream_t st) {
+  using RD = typename Cfg<double, LOGN, false>::RD;
+  auto kern = k_bpm_line<RD, ADJ>;
+  const size_t lds = (size_t)LdsLine<RD::n, 1, sizeof(double2)>::size * sizeof(double2);
+  hipError_t e = ensure_lds(kern, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(a.batch), dim3(RD::P), lds, st, a, tw);
+  return hipGetLastError();
+}
+
+template <bool ADJ>
+static hipError_t bpm_line_launch(int logn, const BpmLine& a, const double2* tw, hipStream_t st) {
+  FH_LOGN_SWITCH(logn, (bpm_line_launch_t<LG, ADJ>(a, tw, st)))
+}
+
+// 1-D AS_BPM / Shift_BPM: one launch for the forward, one (plus the batch sum of the gradient
+// terms) for the adjoint.
+static int bpm_line_run(fh_bpm* m, float2* f, const float* dn, int direction, bool adj, bool with_save, float* ddn) {
+  const double2* tw = nullptr;
+  int rc;
+  if ((rc = get_twiddles(m->ctx, 1 << m->lx, &tw))) return rc;
+  BpmLine a{};
+  a.u = f; a.dn = dn; a.kl = m->kl; a.tfull = m->tfull; a.thalf = m->thalf; a.tabidx = m->d_tabidx;
+  a.n = m->nx; a.batch = m->batch; a.ns = m->nslices; a.dir = direction;
+  if (!adj && with_save) a.save = m->seg;
+  if (adj && ddn) {
+    a.saved = m->seg;
+    a.gpart = m->batch > 1 ? m->gpart : ddn;
+  }
+  FH_HIP(adj ? bpm_line_launch<true>(m->lx, a, tw, m->ctx->stream) : bpm_line_launch<false>(m->lx, a, tw, m->ctx->stream));
+  if (adj && ddn && m->batch > 1) {
+    const long total = (long)m->nslices * m->nx;
+    hipLaunchKernelGGL(k_sum_line_grads, dim3((int)std::min<long>((total + 255) / 256, 4096)), dim3(256), 0,
+                       m->ctx->stream, (const float*)m->gpart, ddn, m->nx, m->batch, m->nslices);
+    FH_HIP(hipGetLastError());
+  }
+  return FH_OK;
+}
+
+// 1-D set-up: FP64 line tables of P(dz) and P(dz/2) per (wavelength, tilt), kdz per field, the
+// slice-input stack and the gradient terms.
+static int bpm_line_create(fh_bpm* m, const fh_prop_desc* d, bool shift) {
+  const int nx = d->nx;
+  for (double ty : m->tab_ty)
+    if (ty != 0.0) return fail(FH_E_ARG, "a 1-D field has no y tilt");
+  const bool pow2 = ilog2_exact(nx) >= 0;
+  m->lx = pow2 ? ilog2_exact(nx) : ilog2_ceil(2 * nx - 1);
+  m->ly = 0;
+  m->oned = 1;
+  if (nx < 2 || m->lx < kMinLog || m->lx > kMaxLog)
+    return fail(FH_E_UNSUPP, "1-D BPM needs power-of-two nx in [%d, %d], or another nx with 2 nx - 1 <= %d (got %d)",
+                1 << kMinLog, 1 << kMaxLog, 1 << kMaxLog, nx);
+  bool tilted = false;
+  for (double tx : m->tab_tx) tilted = tilted || tx != 0.0;
+  m->mode = shift ? 3 : ((d->flags & FH_PROP_PARAXIAL) ? 1 : (tilted ? 2 : 0));
+  m->track = (d->flags & FH_PROP_TRACK_TILTS) ? 1 : 0;
+  const int N = 1 << m->lx;
+  m->tabsize = N;
+  const size_t ntab = m->lam_tables.size();
+  const size_t stack = (size_t)m->nslices * m->batch * nx;
+  if (hipMalloc(&m->tfull, ntab * N * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&m->thalf, ntab * N * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&m->d_tabidx, m->batch * sizeof(int)) != hipSuccess ||
+      hipMalloc(&m->kl, m->batch * sizeof(double)) != hipSuccess ||
+      hipMalloc(&m->seg, stack * sizeof(float2)) != hipSuccess ||
+      (m->batch > 1 && hipMalloc(&m->gpart, stack * sizeof(float)) != hipSuccess))
+    return fail(FH_E_NOMEM, "1-D BPM: out of device memory");
+  std::vector<LineKey> kf, kh;
+  for (size_t k = 0; k < ntab; ++k) {
+    kf.push_back({m->lam_tables[k], m->dz, m->tab_tx[k]});
+    kh.push_back({m->lam_tables[k], m->dz / 2, m->tab_tx[k]});
+  }
+  int rc;
+  if ((rc = line_tables(m->ctx, nx, m->lx, d->dx, m->n0, m->mode, m->track, kf, nullptr, 0, nullptr, m->tfull)) ||
+      (rc = line_tables(m->ctx, nx, m->lx, d->dx, m->n0, m->mode, m->track, kh, nullptr, 0, nullptr, m->thalf)))
+    return rc;
+  std::vector<double> kl(m->batch);
+  for (int b = 0; b < m->batch; ++b) {
+    const int t = m->field_table[b];
+    kl[b] = 2.0 * M_PI * m->dz / cos(m->tab_tx[t]) / m->lam_tables[t];
+  }
+  if (hipMemcpy(m->kl, kl.data(), m->batch * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_tabidx, m->field_table.data(), m->batch * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(FH_E_HIP, "1-D BPM: set-up failed");
+  m->bytes = 2 * ntab * N * sizeof(double2) + stack * (sizeof(float2) + (m->batch > 1 ? sizeof(float) : 0));
+  return FH_OK;
+}
+
 }  // namespace fh
 
 extern "C" {
@@ -3307,6 +3614,11 @@ int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* d, double thickness, int n
   m->thickness = thickness; m->n0 = n0; m->dz = thickness / nslices;
   m->lam_tables = tmp.lam_tables; m->tab_tx = tmp.tab_tx; m->tab_ty = tmp.tab_ty;
   m->field_table = tmp.field_table;
+  if (ny == 1) {  // 1-D fields
+    if ((rc = bpm_line_create(m, d, shift))) { bpm_free(m); return rc; }
+    *out = m;
+    return FH_OK;
+  }
   if (!pow2) {
     // other grids: general plans for P(dz) and P(dz/2) (the padded convolution form, run in
     // FP64 from their unrounded tables by bpm_generic_prop) and an elementwise slice phase
@@ -3429,6 +3741,11 @@ int fh_bpm_forward(fh_bpm* m, void* u, const float* dn, int direction, int save)
   int rc;
   if ((rc = check_dir(direction)) || (rc = set_device(m->ctx))) return rc;
   float2* f = reinterpret_cast<float2*>(u);
+  if (m->oned) {
+    if ((rc = bpm_line_run(m, f, dn, direction, false, save != 0, nullptr))) return rc;
+    m->saved_dir = save ? direction : 0;
+    return FH_OK;
+  }
   if (m->generic) return bpm_generic_forward(m, f, dn, direction, save);
   const long npix = (long)m->nx * m->ny;
   const long fsz = npix * m->batch;
@@ -3454,6 +3771,7 @@ int fh_bpm_backward(fh_bpm* m, void* du, const float* dn, float* ddn, int direct
   if (ddn && m->saved_dir != direction)
     return fail(FH_E_ARG, "the dn gradient needs a saving fh_bpm_forward in the same direction first");
   float2* f = reinterpret_cast<float2*>(du);
+  if (m->oned) return bpm_line_run(m, f, dn, direction, true, false, ddn);
   if (m->generic) return bpm_generic_backward(m, f, dn, ddn, direction);
   const long npix = (long)m->nx * m->ny;
   const long fsz = npix * m->batch;
@@ -3604,6 +3922,7 @@ int fh_group_create(fh_ctx* ctx, int nplans, fh_plan* const* plans, int autotune
   for (int i = 0; i < nplans; ++i) {
     if (!plans[i]) return fail(FH_E_ARG, "plan %d is null", i);
     if (plans[i]->ctx != ctx) return fail(FH_E_ARG, "plan %d belongs to another context", i);
+    if (plans[i]->oned) return fail(FH_E_UNSUPP, "plan %d is a 1-D plan: one launch already covers its batch", i);
   }
   int rc;
   if ((rc = set_device(ctx))) return rc;

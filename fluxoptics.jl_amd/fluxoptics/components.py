@@ -53,11 +53,12 @@ def check_field(e: torch.Tensor, what: str = "field") -> None:
         raise ValueError(f"{what} must be contiguous")
 
 
-def _check_mask(phi: torch.Tensor, ny: int, nx: int, device) -> None:
+def _check_mask(phi: torch.Tensor, ny: int, nx: int, device, nd: int = 2) -> None:
     if phi.dtype != torch.float32 or phi.device != device or not phi.is_contiguous():
         raise TypeError("mask must be a contiguous float32 tensor on the field's device")
-    if tuple(phi.shape) != (ny, nx):
-        raise ValueError(f"mask shape {tuple(phi.shape)} does not match the field ({ny}, {nx})")
+    want = (ny, nx) if nd == 2 else (nx,)
+    if tuple(phi.shape) != want:
+        raise ValueError(f"mask shape {tuple(phi.shape)} does not match the field {want}")
 
 
 # --------------------------------------------------------------------------- pipe
@@ -82,6 +83,8 @@ class _KernelPropagator(_PipeComponent):
 
     def __init__(self, u: ScalarField, ds, z):
         self.ds = tuple(float(d) for d in (ds if ds is not None else u.ds))
+        if len(self.ds) != getattr(u, "nd", 2):
+            raise ValueError("ds must have one sampling per spatial dimension of the field")
         # one distance, or one per field of the batch (an extension for uneven batches)
         self.z = float(z) if np.ndim(z) == 0 else [float(v) for v in np.asarray(z).ravel()]
         self.nx, self.ny = u.nx, u.ny
@@ -94,7 +97,7 @@ class _KernelPropagator(_PipeComponent):
         arr = lambda v: (ctypes.c_double * len(v))(*v)
         d = _lib.PropDesc()
         d.nx, d.ny, d.batch = self.nx, self.ny, batch
-        d.dx, d.dy = self.ds
+        d.dx, d.dy = self.ds if len(self.ds) == 2 else (self.ds[0], 1.0)  # a 1-D field's fy is 0
         zs = self.z if isinstance(self.z, list) else [self.z]
         if len(zs) not in (1, batch):
             raise ValueError(f"{len(zs)} distances for a batch of {batch} fields")
@@ -216,16 +219,17 @@ class ASProp(_KernelPropagator):
         """The filter's samples for a plan: (1 or batch, ny, nx) complex128 on the fftfreq grid
         (FFT order), at the arguments the reference's kernel passes it."""
         fx = np.fft.fftfreq(self.nx, self.ds[0])[None, :]
-        fy = np.fft.fftfreq(self.ny, self.ds[1])[:, None]
+        fy = np.fft.fftfreq(self.ny, self.ds[1])[:, None] if len(self.ds) == 2 else np.zeros((1, 1))
+        filt = self.filter if len(self.ds) == 2 else (lambda a, b: self.filter(a))  # 1-D: filter(fx)
         tx = [desc.tilts_x[b] for b in range(desc.batch)]
         ty = [desc.tilts_y[b] for b in range(desc.batch)]
         if self.paraxial or all(a == 0.0 for a in tx + ty):
-            return _sample_filter(self.filter, fx, fy)[None]
+            return _sample_filter(filt, fx, fy)[None]
         lams = [desc.lambdas[0 if desc.nlambdas == 1 else b] for b in range(desc.batch)]
         out = []
         for b in range(desc.batch):
             le = lams[b] / self.n0
-            out.append(_sample_filter(self.filter, fx + math.sin(tx[b]) / le, fy + math.sin(ty[b]) / le))
+            out.append(_sample_filter(filt, fx + math.sin(tx[b]) / le, fy + math.sin(ty[b]) / le))
         return np.stack(out)
 
     @classmethod
@@ -341,6 +345,8 @@ class RSProp(_KernelPropagator):
 
     def __init__(self, u: ScalarField, z: float, ds=None, *, use_cache: bool = True,
                  track_tilts: bool = False, double_precision_kernel: Optional[bool] = None):
+        if getattr(u, "nd", 2) == 1:
+            raise NotImplementedError("the reference's RSProp has no 1-D kernel (rs_kernel takes x and y)")
         super().__init__(u, ds, z)
         self.track_tilts = bool(track_tilts)
         zc = rs_valid_distance(self.nx, self.ny, self.ds[0], self.ds[1], u.lambda_min())
@@ -481,6 +487,7 @@ class _GridOnly:
 
     def __init__(self, nx, ny, ds):
         self.nx, self.ny, self.ds = int(nx), int(ny), tuple(float(d) for d in ds)
+        self.nd = len(self.ds)
 
 
 def _spatial(n: int, d: float) -> np.ndarray:
@@ -499,17 +506,22 @@ class Phase(_PipeComponent):
                  ds=None, *, trainable: bool = False, buffered: bool = False):
         ds = tuple(ds if ds is not None else u.ds)
         device = u.electric.device
+        nd = getattr(u, "nd", 2)
+        shape = (u.ny, u.nx) if nd == 2 else (u.nx,)
         if phi is None:
-            arr = np.zeros((u.ny, u.nx))
+            arr = np.zeros(shape)
         elif callable(phi):
             x = _spatial(u.nx, ds[0])[None, :]
-            y = _spatial(u.ny, ds[1])[:, None]
-            arr = np.broadcast_to(np.vectorize(phi)(x, y), (u.ny, u.nx))
+            if nd == 2:
+                y = _spatial(u.ny, ds[1])[:, None]
+                arr = np.broadcast_to(np.vectorize(phi)(x, y), shape)
+            else:
+                arr = np.broadcast_to(np.vectorize(phi)(x[0]), shape)
         else:
             arr = phi
         self.phi = torch.as_tensor(np.asarray(arr) if not isinstance(arr, torch.Tensor) else arr,
                                    dtype=torch.float32, device=device).contiguous()
-        _check_mask(self.phi, u.ny, u.nx, device)
+        _check_mask(self.phi, u.ny, u.nx, device, nd)
         self.trainable = trainable
         self.buffered = trainable and buffered
         self.grad = torch.zeros_like(self.phi) if self.buffered else None
@@ -528,7 +540,7 @@ class Phase(_PipeComponent):
 
     def propagate_(self, u: ScalarField, direction=Forward) -> ScalarField:
         check_field(u.electric)
-        _check_mask(self.phi, u.ny, u.nx, u.electric.device)
+        _check_mask(self.phi, u.ny, u.nx, u.electric.device, u.nd)
         _lib.call("fh_phase_apply", context(u.electric.device), _ptr(u.electric), _ptr(self.phi),
                   u.nx, u.ny, u.batch, dir_sign(direction), None)
         return u
@@ -798,13 +810,15 @@ class BPM(_PipeComponent):
         device = u.electric.device
         self.dn = torch.as_tensor(np.asarray(dn) if not isinstance(dn, torch.Tensor) else dn,
                                   dtype=torch.float32, device=device).contiguous()
-        if self.dn.dim() != 3 or tuple(self.dn.shape[1:]) != (u.ny, u.nx):
+        if u.nd == 2 and (self.dn.dim() != 3 or tuple(self.dn.shape[1:]) != (u.ny, u.nx)):
             raise ValueError("dn must have shape (nslices, ny, nx)")
+        if u.nd == 1 and (self.dn.dim() != 2 or self.dn.shape[1] != u.nx):
+            raise ValueError("a 1-D field's dn must have shape (nslices, nx)")
         if self.dn.shape[0] < 2:
             raise ValueError("BPM needs at least 2 slices (bpm.jl:40)")
         self.nslices = int(self.dn.shape[0])
         self.thickness, self.n0 = float(thickness), float(n0)
-        self.nx, self.ny, self.ds = u.nx, u.ny, u.ds
+        self.nx, self.ny, self.ds = u.nx, u.ny, u.ds2
         self.tilts = tuple(tuple(t) for t in u.tilt_lists())
         self.checkpoint_every = int(checkpoint_every)
         self.trainable = trainable

@@ -48,24 +48,36 @@ class ScalarField:
     ``lambdas`` is a scalar or one value per field (``src/Fields.jl:28-32``).  A
     ComplexF32 field in the reference stores lambda as Float32; pass
     ``float(np.float32(lam))`` to reproduce that exactly.
+
+    The number of spatial dimensions is ``len(ds)`` (``Fields.jl:135-136``: Nd in (1, 2)):
+    2-D fields are ``(ny, nx)`` or ``(B, ny, nx)`` with ``ds = (dx, dy)`` and
+    ``tilts = (theta_x, theta_y)``; 1-D fields are ``(nx,)`` or ``(B, nx)`` (the reference's
+    ``electric[nx, B]``) with ``ds = (dx,)`` and ``tilts = (theta_x,)``, and reach the library as
+    fields of one row (``ny = 1``).
     """
 
     def __init__(self, electric, ds: Sequence[float], lambdas: Union[float, Sequence[float]],
-                 tilts=(0.0, 0.0)):
+                 tilts=None):
         if not isinstance(electric, torch.Tensor):
             electric = torch.as_tensor(np.asarray(electric))
-        if electric.dim() not in (2, 3):
-            raise ValueError("electric must be (ny, nx) or (B, ny, nx)")
+        self.nd = len(ds)
+        if self.nd not in (1, 2):
+            raise ValueError("ds must hold one (1-D field) or two (2-D field) samplings")
+        if electric.dim() not in ((2, 3) if self.nd == 2 else (1, 2)):
+            raise ValueError("electric must be (ny, nx) or (B, ny, nx)" if self.nd == 2 else
+                             "a 1-D field's electric must be (nx,) or (B, nx)")
         self.electric = electric
-        self.ds = (float(ds[0]), float(ds[1]))
+        self.ds = tuple(float(d) for d in ds)
         lam = np.atleast_1d(np.asarray(lambdas, dtype=np.float64))
         if lam.size not in (1, self.batch):
             raise ValueError("lambdas must be a scalar or one value per field")
         if not np.all(lam > 0):
             raise ValueError("wavelengths must be positive")
         self.lambdas = float(lam[0]) if lam.size == 1 and np.ndim(lambdas) == 0 else [float(v) for v in lam]
-        if len(tilts) != 2:
-            raise ValueError("tilts must be (theta_x, theta_y)")
+        if tilts is None:
+            tilts = (0.0,) * self.nd
+        if len(tilts) != self.nd:
+            raise ValueError("tilts must be (theta_x, theta_y)" if self.nd == 2 else "a 1-D field's tilts are (theta_x,)")
         tl = []
         for t in tilts:
             a = np.atleast_1d(np.asarray(t, dtype=np.float64))
@@ -81,11 +93,20 @@ class ScalarField:
 
     @property
     def ny(self) -> int:
-        return self.electric.shape[-2]
+        return 1 if self.nd == 1 else self.electric.shape[-2]
 
     @property
     def batch(self) -> int:
-        return 1 if self.electric.dim() == 2 else self.electric.shape[0]
+        return 1 if self.electric.dim() == self.nd else self.electric.shape[0]
+
+    @property
+    def ds2(self):
+        """(dx, dy) as the library takes them (dy = 1 for a 1-D field: its fy is 0)."""
+        return self.ds if self.nd == 2 else (self.ds[0], 1.0)
+
+    def cell(self) -> float:
+        """The sampling cell dx dy (dx for a 1-D field)."""
+        return float(np.prod(self.ds))
 
     def lambda_list(self):
         lam = self.lambdas if isinstance(self.lambdas, list) else [self.lambdas]
@@ -95,11 +116,14 @@ class ScalarField:
         return min(self.lambda_list())
 
     def tilt_lists(self):
-        """(theta_x per field, theta_y per field) in radians (``Fields.jl:168-180``)."""
+        """(theta_x per field, theta_y per field) in radians (``Fields.jl:168-180``); theta_y is
+        0 for a 1-D field."""
         out = []
         for t in self.tilts:
             lst = t if isinstance(t, list) else [t]
             out.append(lst if len(lst) == self.batch else lst * self.batch)
+        if self.nd == 1:
+            out.append([0.0] * self.batch)
         return tuple(out)
 
     def is_on_axis(self) -> bool:
@@ -123,7 +147,8 @@ class ScalarField:
         on the device and takes the new tilts; nothing happens when they equal the current ones."""
         from . import _lib
         from .components import check_field, context
-        new = ScalarField(self.electric[..., :1, :1], self.ds, self.lambdas, tilts)  # parses the tilts
+        head = self.electric[..., :1] if self.nd == 1 else self.electric[..., :1, :1]
+        new = ScalarField(head, self.ds, self.lambdas, tilts)  # parses the tilts
         if tuple(new.tilts) == tuple(self.tilts):
             return self
         check_field(self.electric)
@@ -131,7 +156,7 @@ class ScalarField:
         (fx, fy), (tx, ty) = self.tilt_lists(), new.tilt_lists()
         lams = self.lambda_list()
         _lib.call("fh_offset_tilts", context(self.electric.device), ctypes.c_void_p(self.electric.data_ptr()),
-                  self.nx, self.ny, self.batch, self.ds[0], self.ds[1], arr(lams), len(lams), arr(fx), arr(fy),
+                  self.nx, self.ny, self.batch, self.ds2[0], self.ds2[1], arr(lams), len(lams), arr(fx), arr(fy),
                   arr(tx), arr(ty), self.batch)
         self.tilts = new.tilts
         return self
@@ -139,12 +164,13 @@ class ScalarField:
     def power(self) -> torch.Tensor:
         """Per-field power sum|u|^2 dx dy, shape (B,) (``Fields.jl:843-846``)."""
         e = self.electric.reshape(self.batch, self.ny, self.nx)
-        return (e.abs() ** 2).sum(dim=(-2, -1)) * (self.ds[0] * self.ds[1])
+        return (e.abs() ** 2).sum(dim=(-2, -1)) * self.cell()
 
     def normalize_power_(self, target: float = 1.0) -> "ScalarField":
         """``normalize_power!`` (``Fields.jl:889-892``)."""
         scale = torch.sqrt(target / self.power()).to(self.electric.dtype)
-        self.electric *= scale.reshape((-1, 1, 1) if self.electric.dim() == 3 else (1, 1))
+        per_field = self.electric.dim() > self.nd
+        self.electric *= scale.reshape((-1,) + (1,) * self.nd if per_field else (1,) * self.nd)
         return self
 
     # -- arithmetic (the reference's test/fields_test.jl:74-93): elementwise on the data, the
@@ -188,10 +214,12 @@ class ScalarField:
 
 
 def intensity(u) -> torch.Tensor:
-    """``intensity(u)`` (``Fields.jl:608-616``): sum over the fields of |u|^2, shape (ny, nx)."""
+    """``intensity(u)`` (``Fields.jl:608-616``): sum over the fields of |u|^2, shape (ny, nx)
+    ((nx,) for a 1-D field)."""
     e = u.electric if isinstance(u, ScalarField) else u
+    nd = u.nd if isinstance(u, ScalarField) else 2
     i = e.abs() ** 2
-    return i if i.dim() == 2 else i.sum(dim=0)
+    return i if i.dim() == nd else i.sum(dim=0)
 
 
 def phase(u) -> torch.Tensor:

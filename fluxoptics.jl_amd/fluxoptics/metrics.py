@@ -21,6 +21,14 @@ from .fields import ScalarField
 
 def _target(t, field: ScalarField) -> torch.Tensor:
     t = torch.as_tensor(t, dtype=torch.float32, device=field.electric.device).contiguous()
+    if field.nd == 1:  # a 1-D field's targets (nx,) or (B, nx) as rows of the library's layout
+        if t.dim() == 1 and t.shape[0] == field.nx:
+            t = t.reshape(1, field.nx)
+        elif t.dim() == 2 and tuple(t.shape) == (field.batch, field.nx) and field.batch > 1:
+            t = t.reshape(field.batch, 1, field.nx)
+        else:
+            raise ValueError(f"target intensity shape {tuple(t.shape)} matches neither the 1-D field's "
+                             f"spatial dims ({field.nx},) nor its full dims {(field.batch, field.nx)}")
     spatial = (field.ny, field.nx)
     if tuple(t.shape) == spatial:
         return t
@@ -168,7 +176,7 @@ class DotProduct:
             raise ValueError("field does not match the metric's reference field")
         c = torch.empty(u.batch, dtype=torch.complex128, device=u.electric.device)
         _lib.call("fh_dot_product", context(u.electric.device), _ptr(u.electric), _ptr(v.electric), u.nx, u.ny,
-                  u.batch, u.ds[0] * u.ds[1], _ptr(c), _ptr(p))
+                  u.batch, u.cell(), _ptr(c), _ptr(p))
         return c
 
     def compute_metric(self, *fields: ScalarField):
@@ -213,7 +221,7 @@ class PowerCoupling:
         c = torch.empty(u.batch, dtype=torch.complex128, device=u.electric.device)
         p = torch.empty(u.batch, dtype=torch.float64, device=u.electric.device)
         _lib.call("fh_dot_product", context(u.electric.device), _ptr(u.electric), _ptr(v.electric), u.nx, u.ny,
-                  u.batch, u.ds[0] * u.ds[1], _ptr(c), _ptr(p))
+                  u.batch, u.cell(), _ptr(c), _ptr(p))
         self._c[k] = c
         return p
 

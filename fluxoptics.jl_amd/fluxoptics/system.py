@@ -329,6 +329,8 @@ def _fusable_tail(comps: List, loss, u: ScalarField, direction):
     exactly zero."""
     if dir_sign(direction) < 0 or not comps or not isinstance(loss, SquaredIntensityDifference):
         return None
+    if getattr(u, "nd", 2) == 1:  # 1-D propagations are one pass already
+        return None
     k = len(comps)
     while k > 0 and type(comps[k - 1]) is Phase:
         k -= 1

@@ -14,7 +14,14 @@
  *    lambda as Float32 (src/Fields.jl:28-32), so a caller reproducing the reference
  *    passes (double)(float)lambda;
  *  - all work is asynchronous on the context's stream; the caller owns field, mask and
- *    gradient buffers; the library owns plans, tables and workspace.
+ *    gradient buffers; the library owns plans, tables and workspace;
+ *  - 1-D fields (a ScalarField with Nd = 1, electric[nx, batch], src/Fields.jl:135-136) are
+ *    passed with ny = 1 (their masks and dn slices are rows of nx values).  The reference's 1-D
+ *    kernels are its 2-D ones at fy = 0 and theta_y = 0 (angular_spectrum.jl:14-25,37-43,64-76,
+ *    shift_kernel.jl:8-12), so ASProp, ShiftProp, AS_BPM and Shift_BPM accept ny = 1 for any
+ *    power-of-two nx in [16, 4096] or any other nx with 2 nx - 1 <= 4096 (a propagation is one
+ *    launch: FFT, transfer function and IFFT per line); RSProp, CollinsProp and the CZT have no
+ *    1-D form in the reference and refuse them; fh_fft2 takes 1-D fields of power-of-two nx.
  */
 #ifndef FLUXHIP_H
 #define FLUXHIP_H
@@ -271,6 +278,9 @@ int fh_bpm_create(fh_ctx* ctx, int nx, int ny, int batch, double dx, double dy,
                   const double* lambdas, int nlambdas, double thickness, double n0,
                   int nslices, int checkpoint_every, fh_bpm** out);
 
+/* 1-D fields (ny = 1) run every slice of one field in one workgroup, in FP64 registers across
+ * the slices, and a saving forward keeps every slice input (nslices * batch * nx c64, as the
+ * reference's saved stack), so checkpoint_every does not apply to them. */
 /* The general BPM constructor (bpm.jl:54-73): the field template's tilts (per field or
  * shared) and the slice propagator from desc->flags -- ASProp(dz; n0) with the reference's
  * kernel choice and FH_PROP_PARAXIAL / FH_PROP_TRACK_TILTS (AS_BPM, bpm.jl:130-142), or
