@@ -338,6 +338,70 @@ def workload_published_rgb(torch, fo, inputs):
                     "of one recorded iteration; synthetic targets"}
 
 
+def tomography_setup(torch, fo, inputs):
+    """The reference's waveguide-tomography notebook (notebooks/04_waveguide_tomography.ipynb):
+    43 tilted 1-D fields of 1600 points (dx 0.0405 um), lambda 0.455 um, a 20 um sample of bulk
+    index 1.52 in 300 slices (AS_BPM, trainable dn from zero), then ASProp(-6 um; n0 1.52) with the
+    NA 1.4 diaphragm as its frequency filter; SquaredIntensityDifference against the measured
+    intensity; ProxRule(Fista(0.01), IstaProx(3e-5)) on dn.  The measured intensity (a file this
+    benchmark does not load) is replaced by the intensity the same system gives for a synthetic
+    waveguide dn, normalised per angle to a mean of 1 as the notebook does (cell 7); the angles
+    are synthetic, 43 of 49 spread over +-0.45 rad.  The notebook's trainable aberration
+    (a Fourier-domain phase on a polynomial basis) is not on the path and is left out."""
+    import numpy as np
+    nx, dx, lam, nb, na, nz, L = 1600, 0.0405, 0.455, 1.52, 1.4, 300, 20.0
+    th = list(np.linspace(-0.45, 0.45, 49)[5:48])
+    b = len(th)
+    dev = torch.device("cuda", 0)
+    u0 = fo.ScalarField(torch.ones((b, nx), dtype=torch.complex64, device=dev), (dx,), lam, tilts=(th,))
+    diaphragm = lambda fx: np.abs(fx) <= abs(na / lam)     # make_diaphragm(NA, lambda)
+    x = (np.arange(nx) - (nx - 1) / 2) * dx
+    zc = (np.arange(nz) + 0.5) * (L / nz)
+    guide = 0.02 * np.exp(-((x[None, :] - 1.5 * np.sin(2 * np.pi * zc[:, None] / L)) / 1.2) ** 2)
+    truth = fo.AS_BPM(u0, L, nb, guide.astype(np.float32))
+    p = fo.ASProp(u0, -6.0, n0=nb, filter=diaphragm)  # a 1-D filter takes fx only
+    meas = p.propagate(truth.propagate(u0))
+    I = (meas.electric.abs() ** 2)
+    I = (I * (nx / I.sum(dim=1, keepdim=True))).to(torch.float32).contiguous()
+    sample = fo.AS_BPM(u0, L, nb, np.zeros((nz, nx), np.float32), trainable=True, buffered=True)
+    system = fo.ScalarSource(u0) | sample | p
+    metric = fo.SquaredIntensityDifference((u0, I))
+    return system, sample, metric
+
+
+def workload_published_tomography(torch, fo, inputs):
+    """50 iterations of the tomography loop (withgradient + update_), host-driven as the notebook's
+    cell 29 (published: 2.481 s on an unrecorded CUDA GPU, with the aberration basis), and as
+    replays of one recorded iteration."""
+    res = {}
+    for mode in ("host", "graph"):
+        system, sample, metric = tomography_setup(torch, fo, inputs)
+        opt = fo.setup(fo.ProxRule(fo.Fista(0.01), fo.IstaProx(3e-5)), system)
+        _, grads = system.withgradient(metric)   # warm-up, as the notebook's cell 27
+        fo.update_(opt, system, grads)
+        step = system.capture_train_step(metric, opt) if mode == "graph" else None
+        losses = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(50):
+            if step is None:
+                val, grads = system.withgradient(metric)
+                fo.update_(opt, system, grads)
+            else:
+                val, _ = step.replay()
+            losses.append(val)
+        torch.cuda.synchronize()
+        res[mode] = (time.perf_counter() - t0, float(losses[0]), float(losses[-1]))
+    (sec, l0, l1), (gsec, g0, g1) = res["host"], res["graph"]
+    return {"seconds_50_iterations": sec, "ms_per_iter": sec * 20.0, "loss_first": l0, "loss_last": l1,
+            "graph_seconds_50_iterations": gsec, "graph_loss_first": g0, "graph_loss_last": g1,
+            "published_gpu_seconds": 2.481, "published_hardware": "CUDA GPU, model not recorded",
+            "speedup_vs_published": 2.481 / sec, "graph_speedup_vs_published": 2.481 / gsec,
+            "what": "43 tilted 1-D fields x 1600, AS_BPM 300 slices (trainable dn, Fista + Ista prox) -> NA-filtered "
+                    "ASProp(-6), intensity loss, 50 iterations; synthetic measured intensity and angles; the "
+                    "notebook's aberration basis left out"}
+
+
 def workload_train_step(torch, fo, inputs):
     """The headline grid as a training step: 8 fields of 2048x2048 (ds 1, lambda 0.8),
     Phase (trainable, shared) -> ASProp(5000), per-field intensity targets; value + mask
@@ -493,7 +557,7 @@ def main():
         from fluxoptics import inputs as inp
         fns = {"published_splitter_250": workload_published_splitter, "published_rgb_800x3": workload_published_rgb, "beam_splitter_512": workload_beam_splitter, "train_step_2048x8": workload_train_step,
                "rgb_cascade_1024x3": workload_rgb_cascade, "as_bpm_512x400": workload_bpm,
-               "uneven_batch_64": workload_uneven_batch}
+               "uneven_batch_64": workload_uneven_batch, "published_tomography_1600x43": workload_published_tomography}
         if args.workload not in fns:
             sys.exit(f"bench.py: unknown workload {args.workload!r} (one of {', '.join(fns)})")
         torch.cuda.set_device(0)
@@ -583,6 +647,7 @@ def main():
             "rgb_cascade_1024x3": workload_rgb_cascade(torch, fo, inp),
             "as_bpm_512x400": workload_bpm(torch, fo, inp),
             "uneven_batch_64": workload_uneven_batch(torch, fo, inp),
+            "published_tomography_1600x43": workload_published_tomography(torch, fo, inp),
         }
         torch.cuda.empty_cache()
 
