@@ -1116,6 +1116,111 @@ __global__ void k_dot_pullback(const float2* __restrict__ v, const double2* __re
   }
 }
 
+// DotProduct with mode_selective=false (dot_product.jl:53-66): the overlap matrix
+// c[i + n j] = ds * sum_x conj(v_i(x)) u_j(x) of the n modes (the reference's column-major c[i, j],
+// V' U), as FP64 block partials of pixel tiles staged in LDS (each thread keeps up to four (i, j)
+// sums), then an ordered sum per entry.  Tile rows are padded by one value so that the lanes that
+// read one pixel of different modes hit different banks.
+constexpr int kDotMaxModes = 32;
+constexpr int kDotTile = 32;
+__global__ void __launch_bounds__(256)
+k_dotm_partial(const float2* __restrict__ u, const float2* __restrict__ v, long npix, int n,
+               double2* __restrict__ part) {
+  __shared__ float2 su[kDotMaxModes][kDotTile + 1], sv[kDotMaxModes][kDotTile + 1];
+  const int t = threadIdx.x;
+  const int npair = n * n;
+  double ar[4] = {0.0, 0.0, 0.0, 0.0}, ai[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long base = (long)blockIdx.x * kDotTile; base < npix; base += (long)gridDim.x * kDotTile) {
+    __syncthreads();  // the previous tile's readers are done
+    for (int e = t; e < n * kDotTile; e += 256) {
+      const int m = e / kDotTile, x = e - m * kDotTile;
+      const long i = base + x;
+      su[m][x] = i < npix ? u[m * npix + i] : make_float2(0.f, 0.f);
+      sv[m][x] = i < npix ? v[m * npix + i] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = t + 256 * k;
+      if (q < npair) {
+        const int ii = q % n, jj = q / n;
+        for (int x = 0; x < kDotTile; ++x) {
+          const float2 a = sv[ii][x], b = su[jj][x];
+          ar[k] += (double)a.x * b.x + (double)a.y * b.y;  // Re(conj(a) b)
+          ai[k] += (double)a.x * b.y - (double)a.y * b.x;  // Im(conj(a) b)
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int q = t + 256 * k;
+    if (q < npair) part[(long)blockIdx.x * npair + q] = make_double2(ar[k], ai[k]);
+  }
+}
+
+// c[q] = ds * (ordered sum over the nblk block partials of entry q); p[q] = |c[q]|^2 when p is set
+__global__ void __launch_bounds__(kMetricThreads)
+k_dotm_final(const double2* __restrict__ part, int nblk, int npair, double ds, double2* __restrict__ c,
+             double* __restrict__ p) {
+  __shared__ double red[kMetricThreads / 64];
+  const int q = blockIdx.x;
+  double ar = 0.0, ai = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kMetricThreads) {
+    const double2 x = part[(long)b * npair + q];
+    ar += x.x;
+    ai += x.y;
+  }
+  const double sr = block_sum(ar, red);
+  __syncthreads();
+  const double si = block_sum(ai, red);
+  if (threadIdx.x == 0) {
+    const double2 r = make_double2(sr * ds, si * ds);
+    c[q] = r;
+    if (p) p[q] = r.x * r.x + r.y * r.y;
+  }
+}
+
+// Pullback of the overlap matrix (dot_product.jl:85-97, power_coupling.jl:66-69): du_j = sum_i v_i
+// G[i + n j] with G = dc (DotProduct) or 2 c o dp (PowerCoupling; dp null = ones), no ds factor as
+// in the reference; one thread per pixel holds the n outputs.
+__global__ void __launch_bounds__(256)
+k_dotm_pullback(const float2* __restrict__ v, const double2* __restrict__ dc, const double2* __restrict__ c,
+                const double* __restrict__ dp, long npix, int n, float2* __restrict__ du) {
+  __shared__ float2 G[kDotMaxModes * kDotMaxModes];
+  for (int q = threadIdx.x; q < n * n; q += 256) {
+    double gr, gi;
+    if (dc) {
+      gr = dc[q].x;
+      gi = dc[q].y;
+    } else {
+      const double w = 2.0 * (dp ? dp[q] : 1.0);
+      gr = w * c[q].x;
+      gi = w * c[q].y;
+    }
+    G[q] = make_float2((float)gr, (float)gi);
+  }
+  __syncthreads();
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < npix; x += (long)gridDim.x * 256) {
+    float2 acc[kDotMaxModes];
+#pragma unroll
+    for (int j = 0; j < kDotMaxModes; ++j) acc[j] = make_float2(0.f, 0.f);
+    for (int i = 0; i < n; ++i) {
+      const float2 a = v[i * npix + x];
+#pragma unroll
+      for (int j = 0; j < kDotMaxModes; ++j)
+        if (j < n) {
+          const float2 g = G[i + n * j];
+          acc[j].x += a.x * g.x - a.y * g.y;
+          acc[j].y += a.x * g.y + a.y * g.x;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kDotMaxModes; ++j)
+      if (j < n) du[j * npix + x] = acc[j];
+  }
+}
+
 __global__ void k_mul_full_d(double2* __restrict__ u, const double2* __restrict__ tab, long npix,
                              int conj) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npix;
@@ -2782,6 +2887,46 @@ int fh_dot_product(fh_ctx* ctx, const void* u, const void* v, int nx, int ny, in
   FH_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_dot_final, dim3(batch), dim3(kMetricThreads), 0, ctx->stream, (const double2*)part,
                      kMetricBlocks, ds, reinterpret_cast<double2*>(c), p);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_dot_matrix(fh_ctx* ctx, const void* u, const void* v, int nx, int ny, int batch, double ds, void* c,
+                  double* p) {
+  if (!ctx || !u || !v || !c) return fail(FH_E_ARG, "null context, fields or output");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  if (batch > kDotMaxModes) return fail(FH_E_UNSUPP, "the overlap matrix takes up to %d modes (got %d)", kDotMaxModes, batch);
+  if (!std::isfinite(ds)) return fail(FH_E_ARG, "ds must be finite");
+  const int npair = batch * batch;
+  int rc;
+  if ((rc = set_device(ctx)) || (rc = metric_partials(ctx, (size_t)kMetricBlocks * npair * sizeof(double2)))) return rc;
+  double2* part = reinterpret_cast<double2*>(ctx->metric_part);
+  const long npix = (long)nx * ny;
+  hipLaunchKernelGGL(k_dotm_partial, dim3(kMetricBlocks), dim3(256), 0, ctx->stream, reinterpret_cast<const float2*>(u),
+                     reinterpret_cast<const float2*>(v), npix, batch, part);
+  FH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_dotm_final, dim3(npair), dim3(kMetricThreads), 0, ctx->stream, (const double2*)part,
+                     kMetricBlocks, npair, ds, reinterpret_cast<double2*>(c), p);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+int fh_dot_matrix_pullback(fh_ctx* ctx, const void* v, const void* dc, const void* c, const double* dp, int nx, int ny,
+                           int batch, void* du) {
+  if (!ctx || !v || !du) return fail(FH_E_ARG, "null context, reference fields or output");
+  if (!dc && !c) return fail(FH_E_ARG, "give the cotangent dc (DotProduct) or the overlaps c (PowerCoupling)");
+  if (nx <= 0 || ny <= 0 || batch <= 0) return fail(FH_E_ARG, "sizes must be positive");
+  if (batch > kDotMaxModes) return fail(FH_E_UNSUPP, "the overlap matrix takes up to %d modes (got %d)", kDotMaxModes, batch);
+  const long npix = (long)nx * ny;
+  const size_t bytes = (size_t)npix * batch * sizeof(float2);
+  const char *a = static_cast<const char*>(v), *d = static_cast<const char*>(du);
+  if (a < d + bytes && d < a + bytes) return fail(FH_E_ARG, "du must not overlap the reference fields");
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const int blocks = (int)std::min<long>((npix + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_dotm_pullback, dim3(blocks), dim3(256), 0, ctx->stream, reinterpret_cast<const float2*>(v),
+                     reinterpret_cast<const double2*>(dc), reinterpret_cast<const double2*>(c), dp, npix, batch,
+                     reinterpret_cast<float2*>(du));
   FH_HIP(hipGetLastError());
   return FH_OK;
 }

@@ -155,44 +155,68 @@ class SquaredFieldDifference:
         return tuple(losses), tuple(cots)
 
 
+def _matrix(c: torch.Tensor, n: int) -> torch.Tensor:
+    """The library's column-major n x n buffer as the matrix M[i, j] (a view)."""
+    return c.view(n, n).t()
+
+
+def _colmajor(m, n: int, dtype, device) -> torch.Tensor:
+    """An (n, n) matrix M[i, j] as the library's column-major buffer (contiguous copy)."""
+    m = torch.as_tensor(m, dtype=dtype, device=device).reshape(n, n)
+    return m.t().contiguous().reshape(-1)
+
+
 class DotProduct:
     """``DotProduct(fields...; mode_selective=true)`` (``metrics/dot_product.jl``): one complex
-    value per mode, c_b = ds * sum u_b conj(v_b) (complex128 device tensors), and the pullback
-    du_b = v_b dc_b.  The non-selective overlap matrix is not on the accelerated path."""
+    value per mode, c_b = ds * sum u_b conj(v_b), with the pullback du_b = v_b dc_b; with
+    ``mode_selective=False`` the overlap matrix c[i, j] = ds * sum conj(v_i) u_j of the n <= 32
+    modes (an (n, n) complex128 device tensor) with the pullback du_j = sum_i v_i dc[i, j]."""
 
     def __init__(self, *fields: ScalarField, mode_selective: bool = True):
-        if not mode_selective:
-            raise NotImplementedError("the non-selective overlap matrix is not on the accelerated path")
         if not fields:
             raise ValueError("DotProduct needs at least one reference field")
         for v in fields:
             check_field(v.electric)
         self.v = [v.copy() for v in fields]
+        self.mode_selective = bool(mode_selective)
 
     def _dot(self, k, u, p=None):
         v = self.v[k]
         check_field(u.electric)
         if u.electric.shape != v.electric.shape:
             raise ValueError("field does not match the metric's reference field")
-        c = torch.empty(u.batch, dtype=torch.complex128, device=u.electric.device)
-        _lib.call("fh_dot_product", context(u.electric.device), _ptr(u.electric), _ptr(v.electric), u.nx, u.ny,
-                  u.batch, u.cell(), _ptr(c), _ptr(p))
+        dev = u.electric.device
+        if self.mode_selective:
+            c = torch.empty(u.batch, dtype=torch.complex128, device=dev)
+            _lib.call("fh_dot_product", context(dev), _ptr(u.electric), _ptr(v.electric), u.nx, u.ny, u.batch,
+                      u.cell(), _ptr(c), _ptr(p))
+            return c
+        c = torch.empty(u.batch * u.batch, dtype=torch.complex128, device=dev)
+        _lib.call("fh_dot_matrix", context(dev), _ptr(u.electric), _ptr(v.electric), u.nx, u.ny, u.batch, u.cell(),
+                  _ptr(c), _ptr(p))
         return c
 
     def compute_metric(self, *fields: ScalarField):
-        return tuple(self._dot(k, u) for k, u in enumerate(fields))
+        out = tuple(self._dot(k, u) for k, u in enumerate(fields))
+        return out if self.mode_selective else tuple(_matrix(c, u.batch) for c, u in zip(out, fields))
 
     __call__ = compute_metric
 
     def backpropagate_metric(self, fields, dcs):
         out = []
         for k, (u, dc) in enumerate(zip(fields, dcs)):
-            dc = torch.as_tensor(dc, dtype=torch.complex128, device=u.electric.device).reshape(-1)
-            dc = (dc.expand(u.batch) if dc.numel() == 1 else dc).contiguous()
+            dev = u.electric.device
             du = torch.empty_like(u.electric)
             v = self.v[k]
-            _lib.call("fh_dot_product_pullback", context(u.electric.device), _ptr(v.electric), _ptr(dc), None, None,
-                      u.nx, u.ny, u.batch, _ptr(du))
+            if self.mode_selective:
+                dc = torch.as_tensor(dc, dtype=torch.complex128, device=dev).reshape(-1)
+                dc = (dc.expand(u.batch) if dc.numel() == 1 else dc).contiguous()
+                _lib.call("fh_dot_product_pullback", context(dev), _ptr(v.electric), _ptr(dc), None, None,
+                          u.nx, u.ny, u.batch, _ptr(du))
+            else:
+                dcm = _colmajor(dc, u.batch, torch.complex128, dev)
+                _lib.call("fh_dot_matrix_pullback", context(dev), _ptr(v.electric), _ptr(dcm), None, None,
+                          u.nx, u.ny, u.batch, _ptr(du))
             out.append(u.set_field_data(du))
         return tuple(out)
 
@@ -200,17 +224,17 @@ class DotProduct:
 class PowerCoupling:
     """``PowerCoupling(fields...; mode_selective=true)`` (``metrics/power_coupling.jl``): the
     targets are copied and normalised to unit power; one value per mode, p_b = |<u_b, v_b>|^2,
-    with the pullback 2 c_b dp_b v_b."""
+    with the pullback 2 c_b dp_b v_b; with ``mode_selective=False`` the (n, n) matrix
+    p[i, j] = |c[i, j]|^2 of the overlap matrix, with the pullback du_j = sum_i v_i 2 c[i, j] dp[i, j]."""
 
     def __init__(self, *fields: ScalarField, mode_selective: bool = True):
-        if not mode_selective:
-            raise NotImplementedError("the non-selective coupling matrix is not on the accelerated path")
         if not fields:
             raise ValueError("PowerCoupling needs at least one target field")
         self.v = []
         for v in fields:
             check_field(v.electric)
             self.v.append(v.copy().normalize_power_())
+        self.mode_selective = bool(mode_selective)
         self._c = [None] * len(self.v)
 
     def _run(self, k, u):
@@ -218,12 +242,14 @@ class PowerCoupling:
         check_field(u.electric)
         if u.electric.shape != v.electric.shape:
             raise ValueError("field does not match the metric's target field")
-        c = torch.empty(u.batch, dtype=torch.complex128, device=u.electric.device)
-        p = torch.empty(u.batch, dtype=torch.float64, device=u.electric.device)
-        _lib.call("fh_dot_product", context(u.electric.device), _ptr(u.electric), _ptr(v.electric), u.nx, u.ny,
-                  u.batch, u.cell(), _ptr(c), _ptr(p))
+        dev = u.electric.device
+        n = u.batch if self.mode_selective else u.batch * u.batch
+        c = torch.empty(n, dtype=torch.complex128, device=dev)
+        p = torch.empty(n, dtype=torch.float64, device=dev)
+        _lib.call("fh_dot_product" if self.mode_selective else "fh_dot_matrix", context(dev), _ptr(u.electric),
+                  _ptr(v.electric), u.nx, u.ny, u.batch, u.cell(), _ptr(c), _ptr(p))
         self._c[k] = c
-        return p
+        return p if self.mode_selective else _matrix(p, u.batch)
 
     def compute_metric(self, *fields: ScalarField):
         return tuple(self._run(k, u) for k, u in enumerate(fields))
@@ -237,13 +263,26 @@ class PowerCoupling:
             c = self._c[k]
             if c is None:
                 raise RuntimeError("backpropagate_metric needs compute_metric first")
+            dev = u.electric.device
             dp = None
-            if dps is not None:
-                dp = torch.as_tensor(dps[k], dtype=torch.float64, device=u.electric.device).reshape(-1)
-                dp = (dp.expand(u.batch) if dp.numel() == 1 else dp).contiguous()
-            du = u.electric if inplace else torch.empty_like(u.electric)
-            _lib.call("fh_dot_product_pullback", context(u.electric.device), _ptr(self.v[k].electric), None, _ptr(c),
-                      _ptr(dp), u.nx, u.ny, u.batch, _ptr(du))
+            if self.mode_selective:
+                if dps is not None:
+                    dp = torch.as_tensor(dps[k], dtype=torch.float64, device=dev).reshape(-1)
+                    dp = (dp.expand(u.batch) if dp.numel() == 1 else dp).contiguous()
+                du = u.electric if inplace else torch.empty_like(u.electric)
+                _lib.call("fh_dot_product_pullback", context(dev), _ptr(self.v[k].electric), None, _ptr(c),
+                          _ptr(dp), u.nx, u.ny, u.batch, _ptr(du))
+            else:
+                if dps is not None:
+                    dpk = torch.as_tensor(dps[k], dtype=torch.float64, device=dev)
+                    dp = (dpk.expand(u.batch, u.batch) if dpk.numel() == 1 else dpk)
+                    dp = _colmajor(dp, u.batch, torch.float64, dev)
+                du = torch.empty_like(u.electric)  # the matrix pullback reads every target per pixel
+                _lib.call("fh_dot_matrix_pullback", context(dev), _ptr(self.v[k].electric), None, _ptr(c), _ptr(dp),
+                          u.nx, u.ny, u.batch, _ptr(du))
+                if inplace:
+                    u.electric.copy_(du)
+                    du = u.electric
             out.append(u if inplace else u.set_field_data(du))
         return tuple(out)
 

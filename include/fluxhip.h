@@ -385,6 +385,18 @@ int fh_dot_product(fh_ctx* ctx, const void* u, const void* v, int nx, int ny, in
 int fh_dot_product_pullback(fh_ctx* ctx, const void* v, const void* dc, const void* c, const double* dp,
                             int nx, int ny, int batch, void* du);
 
+/* DotProduct and PowerCoupling with mode_selective=false (metrics/dot_product.jl:53-66,85-97,
+ * power_coupling.jl:55-69): the overlap matrix of the batch's n <= 32 modes,
+ * c[i + n*j] = ds * sum_x conj(v_i(x)) u_j(x) (the reference's column-major c[i, j] = V'U, complex128,
+ * FP64 sums in a fixed order), and p[i + n*j] = |c[i + n*j]|^2 when p is not NULL.  The pullback
+ * writes du_j = sum_i v_i G[i + n*j] with G = dc (DotProduct; complex128, same layout) or, when dc is
+ * NULL, G = 2 c o dp (PowerCoupling; dp doubles, NULL = ones); no ds factor, as in the reference.
+ * du must not overlap v.  More than 32 modes return FH_E_UNSUPP. */
+int fh_dot_matrix(fh_ctx* ctx, const void* u, const void* v, int nx, int ny, int batch, double ds, void* c,
+                  double* p);
+int fh_dot_matrix_pullback(fh_ctx* ctx, const void* v, const void* dc, const void* c, const double* dp, int nx, int ny,
+                           int batch, void* du);
+
 #ifdef __cplusplus
 }
 #endif

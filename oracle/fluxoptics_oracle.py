@@ -740,6 +740,38 @@ def power_coupling_pullback(vh, c, dp=None):
     return dot_product_pullback(vh, 2.0 * c * dp)
 
 
+def dot_product_matrix(u, v, ds):
+    """Non-selective ``DotProduct`` (``metrics/dot_product.jl:73-82``): c = V' U * prod(ds) with
+    the fields reshaped to (pixels, modes), i.e. c[i, j] = sum conj(v_i) u_j dx dy; returned as an
+    (n, n) array indexed [i, j]."""
+    u, v = _as_batch(np.asarray(u, np.complex128)), _as_batch(np.asarray(v, np.complex128))
+    U = u.reshape(u.shape[0], -1).T
+    V = v.reshape(v.shape[0], -1).T
+    return (V.conj().T @ U) * (ds[0] * ds[1])
+
+
+def dot_product_matrix_pullback(v, dc):
+    """``backpropagate_metric`` of the non-selective DotProduct (``dot_product.jl:92-96``):
+    dU = V dc with the fields reshaped to (pixels, modes); no ds factor."""
+    v = _as_batch(np.asarray(v, np.complex128))
+    V = v.reshape(v.shape[0], -1).T
+    return (V @ np.asarray(dc, np.complex128)).T.reshape(v.shape)
+
+
+def power_coupling_matrix(u, v, ds):
+    """Non-selective ``PowerCoupling`` (``power_coupling.jl:55-64``): |c|^2 entrywise with c the
+    overlap matrix against the power-normalised targets.  Returns (p, c, v_normalised)."""
+    vh = normalize_power(v, ds)
+    c = dot_product_matrix(u, vh, ds)
+    return np.abs(c) ** 2, c, vh
+
+
+def power_coupling_matrix_pullback(vh, c, dp=None):
+    """``power_coupling.jl:66-69``: dc = 2 c o dp entrywise, then the matrix pullback."""
+    dp = np.ones(c.shape) if dp is None else np.asarray(dp, np.float64)
+    return dot_product_matrix_pullback(vh, 2.0 * c * dp)
+
+
 def half_sq_field_diff(v, t):
     """L = 1/2 sum|v - t|^2 and its cotangent v - t (harness loss for the fwd+adjoint unit)."""
     d = np.asarray(v) - np.asarray(t)

@@ -337,3 +337,42 @@ def test_recording_refuses_systems_with_aspropz():
     with pytest.raises(NotImplementedError, match="ASPropZ"):
         OpticalSystem.capture_withgradient(sysz, None)
     _check_recordable(SimpleNamespace(components=(object(),)))
+
+
+def test_overlap_matrix_reference_doctest_and_diagonal():
+    """The non-selective DotProduct doctest (dot_product.jl:35-39): the two normalised modes give the
+    2x2 identity; the diagonal equals the mode-selective values for any fields; PowerCoupling's
+    matrix is |c|^2 entrywise against normalised targets."""
+    u = _doc_modes()
+    assert np.allclose(O.dot_product_matrix(u, u, (1.0, 1.0)), np.eye(2), atol=1e-15)
+    a = inputs.normal(1, (3, 5, 6)) + 1j * inputs.normal(2, (3, 5, 6))
+    b = inputs.normal(3, (3, 5, 6)) + 1j * inputs.normal(4, (3, 5, 6))
+    c = O.dot_product_matrix(a, b, (0.5, 0.7))
+    assert np.allclose(np.diag(c), O.dot_product(a, b, (0.5, 0.7)), rtol=1e-13)
+    # entry (i, j) pairs target i with field j
+    assert np.isclose(c[0, 2], O.dot_product(a[2], b[0], (0.5, 0.7))[0], rtol=1e-13)
+    p, cm, vh = O.power_coupling_matrix(a, b, (0.5, 0.7))
+    assert np.allclose(p, np.abs(O.dot_product_matrix(a, vh, (0.5, 0.7))) ** 2, rtol=1e-13)
+
+
+def test_overlap_matrix_pullbacks_by_finite_differences():
+    """At ds = 1 the reference's pullbacks are the exact cotangents dL/dRe + i dL/dIm of
+    L = Re sum conj(dc) o c (DotProduct) and L = sum dp o |c|^2 (PowerCoupling)."""
+    n, ny, nx = 3, 4, 5
+    u = inputs.normal(5, (n, ny, nx)) + 1j * inputs.normal(6, (n, ny, nx))
+    v = inputs.normal(7, (n, ny, nx)) + 1j * inputs.normal(8, (n, ny, nx))
+    dc = inputs.normal(9, (n, n)) + 1j * inputs.normal(10, (n, n))
+    dp = inputs.uniform(11, (n, n), 0.5, 2.0)
+    ds = (1.0, 1.0)
+    g_dot = O.dot_product_matrix_pullback(v, dc)
+    p, c, vh = O.power_coupling_matrix(u, v, ds)
+    g_pow = O.power_coupling_matrix_pullback(vh, c, dp)
+    f_dot = lambda w: float(np.real(np.sum(np.conj(dc) * O.dot_product_matrix(w, v, ds))))
+    f_pow = lambda w: float(np.sum(dp * np.abs(O.dot_product_matrix(w, vh, ds)) ** 2))
+    h = 1e-6
+    for (b, i, j) in [(0, 1, 2), (2, 3, 4), (1, 0, 0)]:
+        e = np.zeros_like(u)
+        e[b, i, j] = h
+        for f, g in ((f_dot, g_dot), (f_pow, g_pow)):
+            fd = (f(u + e) - f(u - e)) / (2 * h) + 1j * (f(u + 1j * e) - f(u - 1j * e)) / (2 * h)
+            assert abs(fd - g[b, i, j]) <= 1e-6 * max(1.0, abs(fd)), (b, i, j, fd, g[b, i, j])
