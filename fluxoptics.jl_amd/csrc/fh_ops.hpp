@@ -119,6 +119,29 @@ struct ChirpOp {
   }
 };
 
+// The DotProduct / PowerCoupling pullback as the prologue of the adjoint's first x pass, which reads
+// the metric's reference fields v: v_b * g_b with g_b = dc[b] (DotProduct, dot_product.jl:97-106)
+// or 2 c[b] dp[b] (PowerCoupling, power_coupling.jl:66-69; dp null = ones), g rounded to c64 first
+// as the standalone pullback kernel does.  Padding positions hold zeros and stay zero.
+struct DotGainOp {
+  const double2* dc;
+  const double2* c;
+  const double* dp;
+  __device__ __forceinline__ float2 operator()(float2 v, int b, int, int) const {
+    double gr, gi;
+    if (dc) {
+      gr = dc[b].x;
+      gi = dc[b].y;
+    } else {
+      const double w = 2.0 * (dp ? dp[b] : 1.0);
+      gr = w * c[b].x;
+      gi = w * c[b].y;
+    }
+    const float fr = (float)gr, fi = (float)gi;
+    return make_float2(v.x * fr - v.y * fi, v.x * fi + v.y * fr);
+  }
+};
+
 // Multiply by a real constant (the 1/(nx*ny) of an inverse 2-D FFT, in the last pass).
 struct ScaleOp {
   float s;

@@ -2931,6 +2931,35 @@ int fh_dot_matrix_pullback(fh_ctx* ctx, const void* v, const void* dc, const voi
   return FH_OK;
 }
 
+int fh_dot_pullback_propagate(fh_plan* p, const void* v, const void* dc, const void* c, const double* dp, void* du,
+                              int direction, const float* phi, const void* saved, float* dphi, int accumulate) {
+  if (!p || !v || !du) return fail(FH_E_ARG, "null plan, reference fields or output");
+  if (!dc && !c) return fail(FH_E_ARG, "give the cotangent dc (DotProduct) or the overlaps c (PowerCoupling)");
+  if ((saved || dphi) && !phi) return fail(FH_E_ARG, "a saved input or mask gradient needs the mask");
+  if (dphi && !saved) return fail(FH_E_ARG, "the mask gradient needs the saved mask input");
+  if (p->kind == 2) return fail(FH_E_UNSUPP, "the fused pullback takes ASProp and RSProp plans");
+  const size_t bytes = (size_t)p->nx * p->ny * p->batch * sizeof(float2);
+  const char *a = static_cast<const char*>(v), *d = static_cast<const char*>(du);
+  if (a < d + bytes && d < a + bytes) return fail(FH_E_ARG, "du must not overlap the reference fields");
+  int rc;
+  if ((rc = check_dir(direction)) || (rc = set_device(p->ctx))) return rc;
+  float2* f = reinterpret_cast<float2*>(du);
+  if (p->oned) {  // 1-D: the pullback, then the one-pass adjoint propagation (and the Phase adjoint)
+    const long npix = p->nx;
+    const int blocks = (int)std::min<long>((npix + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_dot_pullback, dim3(blocks, p->batch), dim3(256), 0, p->ctx->stream,
+                       reinterpret_cast<const float2*>(v), reinterpret_cast<const double2*>(dc),
+                       reinterpret_cast<const double2*>(c), dp, npix, f);
+    FH_HIP(hipGetLastError());
+    if ((rc = run_line_prop(p, f, -direction))) return rc;
+    return phi ? fh_phase_backward(p->ctx, du, phi, saved, dphi, p->nx, 1, p->batch, direction, accumulate) : FH_OK;
+  }
+  const DotGainOp op{reinterpret_cast<const double2*>(dc), reinterpret_cast<const double2*>(c), dp};
+  if ((rc = run_prop(p, f, -direction, op, -1, reinterpret_cast<const float2*>(v)))) return rc;
+  if (phi) return phase_grad_pass(p, f, phi, saved, dphi, direction, accumulate);
+  return run_xifft(p, f, -direction);
+}
+
 int fh_dot_product_pullback(fh_ctx* ctx, const void* v, const void* dc, const void* c, const double* dp,
                             int nx, int ny, int batch, void* du) {
   if (!ctx || !v || !du) return fail(FH_E_ARG, "null context, field or output");

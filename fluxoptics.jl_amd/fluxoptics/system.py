@@ -26,7 +26,7 @@ from .components import (ASPropZ, BPM, CollinsProp, TiltAnchor, ASProp, FourierO
                          _Modulator, _ptr, check_field)
 from .components import phase_propagate_, propagate_phase_backward_
 from .fields import Backward, Forward, ScalarField, dir_sign
-from .metrics import SquaredIntensityDifference
+from .metrics import PowerCoupling, SquaredIntensityDifference
 from .padcrop import FieldProbe
 from . import _lib
 
@@ -227,6 +227,13 @@ class OpticalSystem:
             losses = (_sqint_block(loss, u, ph, prop, [], grads, weights, src=start.electric),)
             _zero_grads(trail, grads)
             du = u
+        elif fuse and (dtail := _dot_tail(comps, loss, start, self.direction)) is not None:
+            # [Phase ->] propagator -> PowerCoupling: the pullback rides on the adjoint's first pass
+            u = self._start(x)
+            head, ph, prop = dtail
+            tape = _run_forward(head, u, self.direction, save=True)
+            losses = (_dot_block(loss, u, ph, prop, tape, grads, weights),)
+            du = _run_backward(head, tape, u, self.direction, grads)
         elif tail is not None:
             u = self._start(x)
             head, ph, prop, trail = tail
@@ -344,6 +351,46 @@ def _fusable_tail(comps: List, loss, u: ScalarField, direction):
     if len(comps) >= 2 and isinstance(comps[-2], Phase):
         return comps[:-2], comps[-2], comps[-1], trail
     return comps[:-1], None, comps[-1], trail
+
+
+def _dot_tail(comps: List, loss, u: ScalarField, direction):
+    """(head, Phase or None, propagator) when the system ends in [Phase ->] ASProp/RSProp and the
+    loss is a mode-selective PowerCoupling with one target field; else None."""
+    if dir_sign(direction) < 0 or not comps or not isinstance(loss, PowerCoupling):
+        return None
+    if not loss.mode_selective or len(loss.v) != 1 or not _is_prop(comps[-1]):
+        return None
+    if tuple(loss.v[0].electric.shape) != tuple(u.electric.shape):
+        return None
+    if len(comps) >= 2 and isinstance(comps[-2], Phase):
+        return comps[:-2], comps[-2], comps[-1]
+    return comps[:-1], None, comps[-1]
+
+
+def _dot_block(loss, u: ScalarField, ph, prop, tape, grads: Dict, weights):
+    """[Phase ->] propagation, the PowerCoupling values, then the pullback fused with the adjoint
+    propagation and [the Phase adjoint with its gradient] (``fh_dot_pullback_propagate``): u ends as
+    the cotangent at the block's input."""
+    plan = prop.plan(u)
+    phi = saved = g = None
+    acc = False
+    if ph is not None:
+        phi = ph.phi
+        if ph.trainable:
+            saved = ph.saved if (ph.saved is not None and not any(e[2] is ph.saved for e in tape)) \
+                else torch.empty_like(u.electric)
+            g, acc = _grad_buffer(grads, ph, ph.phi)
+        _lib.call("fh_phase_propagate", plan, _ptr(u.electric), _ptr(phi), _ptr(saved), dir_sign(Forward))
+    else:
+        prop.propagate_(u, Forward)
+    (vals,) = loss.compute_metric(u)
+    dp = None
+    if weights is not None:
+        dp = torch.as_tensor(weights, dtype=torch.float64, device=u.electric.device).reshape(-1)
+        dp = (dp.expand(u.batch) if dp.numel() == 1 else dp).contiguous()
+    _lib.call("fh_dot_pullback_propagate", plan, _ptr(loss.v[0].electric), None, _ptr(loss._c[0]), _ptr(dp),
+              _ptr(u.electric), dir_sign(Forward), _ptr(phi), _ptr(saved), _ptr(g), int(acc))
+    return vals
 
 
 def _zero_grads(phases, grads: Dict) -> None:

@@ -385,6 +385,17 @@ int fh_dot_product(fh_ctx* ctx, const void* u, const void* v, int nx, int ny, in
 int fh_dot_product_pullback(fh_ctx* ctx, const void* v, const void* dc, const void* c, const double* dp,
                             int nx, int ny, int batch, void* du);
 
+/* The mode-selective DotProduct / PowerCoupling pullback fused with the adjoint propagation before
+ * it in a differentiated system ([Phase ->] propagator -> metric, ChainRulesCoreExt.jl:21-111):
+ * du <- P^H(v_b g_b) with g_b = dc[b] (DotProduct) or 2 c[b] dp[b] (PowerCoupling; dp NULL = ones),
+ * the first pass of the adjoint reading the reference fields v itself, so the cotangent v g is never
+ * stored; with phi the Phase adjoint and its gradient follow in the last pass, as in
+ * fh_propagate_phase_backward.  direction is the forward propagation's.  Equals
+ * fh_dot_product_pullback then fh_propagate(plan, du, -direction) [then fh_phase_backward] with one
+ * pass fewer; du must not overlap v.  ASProp and RSProp plans. */
+int fh_dot_pullback_propagate(fh_plan* plan, const void* v, const void* dc, const void* c, const double* dp, void* du,
+                              int direction, const float* phi, const void* saved, float* dphi, int accumulate);
+
 /* DotProduct and PowerCoupling with mode_selective=false (metrics/dot_product.jl:53-66,85-97,
  * power_coupling.jl:55-69): the overlap matrix of the batch's n <= 32 modes,
  * c[i + n*j] = ds * sum_x conj(v_i(x)) u_j(x) (the reference's column-major c[i, j] = V'U, complex128,

@@ -78,3 +78,52 @@ def test_system_through_power_coupling_matrix():
     dphi = np.imag(g * np.conj(a)).sum(axis=0)
     assert abs(float(value) - pr.sum()) <= 1e-5 * pr.sum()
     assert rel_l2(grads[doe].cpu().numpy(), dphi) <= 1e-4
+
+
+@pytest.mark.parametrize("kind", ["as", "rs"])
+def test_fused_power_coupling_tail_matches_unfused(kind):
+    """Phase -> ASProp/RSProp -> PowerCoupling: the fused tail (the pullback read by the adjoint's
+    first pass, fh_dot_pullback_propagate) against the component-by-component run and the oracle,
+    with weights; and the C entry point alone against pullback + adjoint propagation."""
+    n, m, ds, lam, z = 128, 3, (1.0, 1.0), 1.064, 900.0
+    print(f"[fused PowerCoupling] {kind}", flush=True)
+    u0 = c64(np.stack([inputs.laguerre_gaussian(n, n, 1.0, 1.0, 12.0, p, l) for p, l in [(0, 0), (0, 1), (1, -1)]]))
+    tg = c64(np.stack([inputs.gaussian(n, n, 1.0, 1.0, 9.0, 10.0 * k - 10.0, 5.0) for k in range(m)]))
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    t = fo.ScalarField(to_dev(tg), ds, lam)
+    phi = inputs.uniform(12, (n, n), 0, 2 * math.pi).astype(np.float32)
+    doe = fo.Phase(u, phi, trainable=True, buffered=True)
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        p = fo.RSProp(u, z) if kind == "rs" else fo.ASProp(u, z)
+    system = fo.ScalarSource(u) | doe | p
+    metric = fo.PowerCoupling(t)
+    w = [-1.0, 0.5, 2.0]
+    vf, gf = system.withgradient(metric, weights=w)
+    gf = gf[doe].clone()
+    vu, gu = system.withgradient(metric, weights=w, fuse=False)
+    assert abs(float(vf) - float(vu)) <= 1e-12 * abs(float(vu))
+    assert rel_l2(gf.cpu().numpy(), gu[doe].cpu().numpy()) <= 1e-6
+    a = u0 * np.exp(1j * phi.astype(np.float64))
+    prop = (lambda x, d: O.rsprop(x, ds, lam, z, d)) if kind == "rs" else (lambda x, d: O.asprop(x, ds, lam, z, 1.0, d))
+    v = prop(a, 1)
+    pr, cr, vh = O.power_coupling(v, tg, ds)
+    dv = O.power_coupling_pullback(vh, cr, np.array(w))
+    dphi = np.imag(prop(dv, -1) * np.conj(a)).sum(axis=0)
+    assert abs(float(vf) - float((np.array(w) * pr).sum())) <= 1e-5 * abs(float((np.array(w) * pr).sum()))
+    assert rel_l2(gf.cpu().numpy(), dphi) <= 1e-4
+    # the entry point alone: P^H(v g) equals the pullback kernel followed by the adjoint propagation
+    from fluxoptics import _lib
+    from fluxoptics.components import _ptr, context
+    plan = p.plan(u)
+    (pv,) = metric(u)
+    c = metric._c[0]
+    out = torch.empty_like(u.electric)
+    _lib.call("fh_dot_pullback_propagate", plan, _ptr(metric.v[0].electric), None, _ptr(c), None, _ptr(out), 1,
+              None, None, None, 0)
+    ref = torch.empty_like(u.electric)
+    _lib.call("fh_dot_product_pullback", context(u.electric.device), _ptr(metric.v[0].electric), None, _ptr(c), None,
+              n, n, m, _ptr(ref))
+    _lib.call("fh_propagate", plan, _ptr(ref), -1)
+    assert rel_l2(to_host(out), to_host(ref)) <= 1e-6
