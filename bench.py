@@ -167,6 +167,34 @@ def cpu_baseline(u0, dv0, v_dev0, du_dev0, budget_s):
              "oracle": "FP64 numpy restatement, field 0"})
 
 
+def cpu_baseline_beam_splitter(budget_s):
+    """The beam-splitter configuration on the CPU in Float64, the reference's own precision there:
+    512^2 Gaussian (w 30 um), RSProp(1500) -> Phase -> RSProp(2000) -> Phase -> RSProp(1500) with the
+    reference's 2n-1 padding, intensity loss against two Gaussians at +-60 um, both mask gradients;
+    pocketfft threads on all host cores; median of at least 5 iterations."""
+    import numpy as np
+    from fluxoptics import inputs
+    from oracle import fluxoptics_oracle as O
+    cores = len(os.sched_getaffinity(0))
+    n, ds, lam = 512, 1.0, 1.064
+    u0 = inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 30.0), ds, ds)
+    t = inputs.normalize_power(inputs.gaussian(n, n, ds, ds, 30.0, -60.0, 0.0)
+                               + inputs.gaussian(n, n, ds, ds, 30.0, 60.0, 0.0), ds, ds)
+    cpu = O.CpuRSPipelineF64(u0, np.abs(t) ** 2, (ds, ds), lam, [1500.0, 2000.0, 1500.0], workers=cores)
+    phis = [np.zeros((n, n)), np.zeros((n, n))]
+    cpu.iteration(phis)  # warm-up (FFT plans), excluded as FFTW planning is
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while len(times) < 5 or (time.perf_counter() < t_end and len(times) < 50):
+        t0 = time.perf_counter()
+        cpu.iteration(phis)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": 1.0 / med, "unit": "iterations/s", "cores": cores, "kind": "port",
+            "sample": f"beam splitter 512^2 (3 RSProp with 2n-1 padding, 2 Phases, loss + 2 mask gradients), "
+                      f"complex FP64, {len(times)} iterations (median), engine scipy.fft pocketfft workers={cores}"}
+
+
 def _timed_host(fn, reps, torch):
     """Average ms of fn() over reps (host-driven sequences; synchronised on both sides)."""
     torch.cuda.synchronize()
@@ -655,6 +683,7 @@ def main():
         cb, err = cpu_baseline(u_np, dv0, v0, du0, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["error_vs_fp64"] = err
+        out["cpu_baseline_beam_splitter_fp64"] = cpu_baseline_beam_splitter(min(args.cpu_seconds, 8.0))
 
     print(json.dumps(out))
 

@@ -188,10 +188,15 @@ static hipError_t launch_line_conv_l(const LineIO<S>& io, const C* tw, Op op, hi
   return hipGetLastError();
 }
 
+// at most 512 threads a workgroup, so a thread may hold 256 VGPRs: FFT, table values and IFFT of a
+// 4096-point line stay in registers (1024 threads would cap them at 128 and spill)
+template <typename RD, int L> constexpr int line_conv_L() { return (L * RD::P > 512 && L > 1) ? (512 / RD::P > 0 ? 512 / RD::P : 1) : L; }
+
 template <typename S, typename C, typename Op>
 static hipError_t launch_line_conv(int logn, const LineIO<S>& io, const C* tw, Op op, hipStream_t st) {
   FH_LOGN_SWITCH(logn, (launch_line_conv_l<S, C, typename Cfg<typename RealOf<C>::type, LG, false>::RD,
-                                           Cfg<typename RealOf<C>::type, LG, false>::L, Op>(io, tw, op, st)))
+                                           line_conv_L<typename Cfg<typename RealOf<C>::type, LG, false>::RD,
+                                                       Cfg<typename RealOf<C>::type, LG, false>::L>(), Op>(io, tw, op, st)))
 }
 
 // ================================================================ table kernels (FP64)
@@ -1656,10 +1661,14 @@ static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float
   // to a workgroup: 512 threads may hold 256 VGPRs, and the phase factors then fit in LDS beside
   // the exchange areas.  In place, a workgroup must own whole 4-row groups of the blocked input.
   if constexpr (L == 4 && LOGN == 12) {
-    if (static_cast<const void*>(io.in) != static_cast<const void*>(io.out))
-      return fast_phase_grad_launch<RDF, 2>(c, io, batch, phi, saved, dphi, nx, s, acc);
+    // every caller hands rows of 4096 points over out of place (grad_split): four such rows and
+    // the gradient accumulators do not fit one 1024-thread workgroup's registers
+    if (static_cast<const void*>(io.in) == static_cast<const void*>(io.out))
+      return fail(FH_E_UNSUPP, "the mask-gradient pass of 4096-point rows reads another buffer than it writes");
+    return fast_phase_grad_launch<RDF, 2>(c, io, batch, phi, saved, dphi, nx, s, acc);
+  } else {
+    return fast_phase_grad_launch<RDF, L>(c, io, batch, phi, saved, dphi, nx, s, acc);
   }
-  return fast_phase_grad_launch<RDF, L>(c, io, batch, phi, saved, dphi, nx, s, acc);
 }
 
 // A workgroup owns whole 4-row groups (L >= 4): on ASProp plans this pass writes natural rows over
@@ -1897,9 +1906,23 @@ static int fft2_d(fh_ctx* c, double2* d, int nx2, int ny2, int lx, int ly) {
 // groups) and only its first ny rows are ever stored.
 static long mid_rows(const fh_plan* p) { return p->kind == 0 ? p->ny : ((p->ny + 3) / 4) * 4; }
 
+// ASProp plans with rows of 4096 points end an adjoint-with-gradient in the out-of-place
+// mask-gradient pass (two rows per workgroup, no spill): their y pass then writes the spectrum into
+// the plan's scratch (fields-sized, allocated at the first such call) instead of in place.
+static bool grad_split(const fh_plan* p) { return p->kind == 0 && !p->oned && p->lx == 12; }
+
+static int grad_scratch(fh_plan* p) {
+  if (p->scratch) return FH_OK;
+  if (hipMalloc(&p->scratch, (size_t)p->nx * p->ny * p->batch * sizeof(float2)) != hipSuccess)
+    return fail(FH_E_NOMEM, "mask-gradient spectrum: out of device memory");
+  return FH_OK;
+}
+
 template <typename XOp>
-// src (nullable): the first pass reads the input there instead of in u (u is then only written)
-static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1, const float2* src = nullptr) {
+// src (nullable): the first pass reads the input there instead of in u (u is then only written);
+// y_to_scratch: an ASProp y pass writes the plan's scratch (see grad_split)
+static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1, const float2* src = nullptr,
+                    bool y_to_scratch = false) {
   fh_ctx* c = p->ctx;
   const int nxl = 1 << p->lx;
   const bool rs = p->kind != 0;
@@ -1918,7 +1941,7 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
   if (only == 0) return FH_OK;
   QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, 1 << p->ly, p->qy, direction < 0,
             p->lam_tables.size() == 1, p->full};
-  io.in = mid; io.out = mid; io.in_f = io.out_f = midf;
+  io.in = mid; io.out = (y_to_scratch && !rs) ? p->scratch : mid; io.in_f = io.out_f = midf;
   io.in_l = io.out_l = 1; io.in_e = io.out_e = nxl; io.in_blk = io.out_blk = 1;
   io.nlines = nxl; io.in_bytes = io.out_bytes = midf * 8;
   io.in_valid = io.out_valid = p->ny;
@@ -1977,7 +2000,7 @@ static int phase_grad_pass(fh_plan* p, float2* u, const float* phi, const void* 
                            int accumulate) {
   const int nxl = 1 << p->lx;
   FastIO io{};
-  io.in = p->kind == 0 ? u : p->scratch; io.out = u;
+  io.in = (p->kind == 0 && !grad_split(p)) ? u : p->scratch; io.out = u;
   io.in_f = (long)nxl * mid_rows(p); io.out_f = (long)p->nx * p->ny;
   io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1; io.in_blk = 1; io.out_blk = 0;
   io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
@@ -2955,7 +2978,9 @@ int fh_dot_pullback_propagate(fh_plan* p, const void* v, const void* dc, const v
     return phi ? fh_phase_backward(p->ctx, du, phi, saved, dphi, p->nx, 1, p->batch, direction, accumulate) : FH_OK;
   }
   const DotGainOp op{reinterpret_cast<const double2*>(dc), reinterpret_cast<const double2*>(c), dp};
-  if ((rc = run_prop(p, f, -direction, op, -1, reinterpret_cast<const float2*>(v)))) return rc;
+  const bool split = phi && grad_split(p);
+  if (split && (rc = grad_scratch(p))) return rc;
+  if ((rc = run_prop(p, f, -direction, op, -1, reinterpret_cast<const float2*>(v), split))) return rc;
   if (phi) return phase_grad_pass(p, f, phi, saved, dphi, direction, accumulate);
   return run_xifft(p, f, -direction);
 }
@@ -3225,7 +3250,8 @@ int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const vo
     return fh_phase_backward(p->ctx, dv, phi, saved, dphi, p->nx, 1, p->batch, direction, accumulate);
   }
   // adjoint propagation = the propagator in the reverse direction
-  if ((rc = run_prop(p, f, -direction, NoOp{}))) return rc;
+  if (grad_split(p) && (rc = grad_scratch(p))) return rc;
+  if ((rc = run_prop(p, f, -direction, NoOp{}, -1, nullptr, grad_split(p)))) return rc;
   return phase_grad_pass(p, f, phi, saved, dphi, direction, accumulate);
 }
 
@@ -3266,7 +3292,9 @@ static int sqint_block(fh_plan* p, const float2* src, float2* f, const float* ta
                      (const double*)c->metric_part, (int)nparts, loss);
   FH_HIP(hipGetLastError());
   // adjoint: y pass with the conjugate kernel, then x IFFT [un-applying the Phase, its gradient]
-  if ((rc = run_prop(p, f, -direction, NoOp{}, 1))) return rc;
+  const bool split = phi && grad_split(p);
+  if (split && (rc = grad_scratch(p))) return rc;
+  if ((rc = run_prop(p, f, -direction, NoOp{}, 1, nullptr, split))) return rc;
   if (!phi) return run_xifft(p, f, -direction);
   return phase_grad_pass(p, f, phi, src ? (const void*)src : saved, dphi, direction, accumulate);
 }

@@ -957,3 +957,48 @@ class CpuASPropC64:
 
     def adjoint(self, u):
         return self._apply(u, self.hc)
+
+
+class CpuRSPipelineF64:
+    """The reference's CPU path for the beam-splitter pipeline in Float64, for the timed CPU
+    baseline: RSProp(z1) -> Phase -> RSProp(z2) -> Phase -> RSProp(z3) with the intensity loss and
+    the gradients of both masks (the reference's config run through ``Zygote.withgradient``).
+
+    Each RSProp is the reference's (pad to 2n-1, FFT2, multiply by the cached kernel spectrum, IFFT2,
+    crop; ``rayleigh_sommerfeld.jl:135-156``, ``convolution_kernel.jl:47-55``) with pocketfft
+    (``scipy.fft``, ``workers`` threads) in place of FFTW; the Phases save their inputs and give
+    s * Im(du conj(u_saved)) (``phasemask.jl:100-150``); the loss and its pullback are
+    ``squared_intensity_difference.jl:78-90``.  ``iteration(phis)`` returns (loss, [dphi1, dphi2]).
+    """
+
+    def __init__(self, u0, target_i, ds, lam, zs, workers=1):
+        import scipy.fft as sfft
+        self._fft = sfft
+        self.workers = int(workers)
+        self.u0 = np.asarray(u0, np.complex128)
+        self.t = np.asarray(target_i, np.float64)
+        ny, nx = self.u0.shape
+        self.n = (ny, nx)
+        self.h = {z: rs_transfer(nx, ny, ds[0], ds[1], lam, z, "2n-1") for z in set(zs)}
+        self.zs = list(zs)
+
+    def _rs(self, u, z, direction):
+        ny, nx = self.n
+        f = self._fft.fft2(u, s=(2 * ny - 1, 2 * nx - 1), workers=self.workers)
+        f *= self.h[z] if direction > 0 else np.conj(self.h[z])
+        return self._fft.ifft2(f, workers=self.workers, overwrite_x=True)[:ny, :nx]
+
+    def iteration(self, phis):
+        saved = []
+        u = self._rs(self.u0, self.zs[0], 1)
+        for z, phi in zip(self.zs[1:], phis):
+            saved.append(u)
+            u = self._rs(u * np.exp(1j * phi), z, 1)
+        d = np.abs(u) ** 2 - self.t
+        loss = float((d * d).sum())
+        du = 4.0 * d * u
+        grads = []
+        for z, phi, s in zip(self.zs[:0:-1], phis[::-1], saved[::-1]):
+            du = self._rs(du, z, -1) * np.exp(-1j * phi)
+            grads.append(np.imag(du * np.conj(s)))
+        return loss, grads[::-1]

@@ -269,3 +269,21 @@ def test_oracle_sq_field_diff_pins():
             fd = (loss(u + e) - loss(u - e)) / (2 * h)
             an = dv[b, y, x].real if part == "re" else dv[b, y, x].imag
             assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
+
+
+def test_cpu_beam_splitter_baseline_equals_the_oracle_chain():
+    """The timed FP64 CPU restatement of the beam-splitter pipeline (threaded FFTs, cached kernel
+    spectra) gives the oracle chain's loss and mask gradients."""
+    n, ds, lam = 64, (1.0, 1.0), 1.064
+    u0 = inputs.normalize_power(inputs.gaussian(n, n, 1.0, 1.0, 8.0), 1.0, 1.0)
+    t = np.abs(inputs.gaussian(n, n, 1.0, 1.0, 8.0, 10.0, 0.0)) ** 2
+    phis = [inputs.uniform(1, (n, n), 0, 6.28), inputs.uniform(2, (n, n), 0, 6.28)]
+    cpu = O.CpuRSPipelineF64(u0, t, ds, lam, [300.0, 400.0, 300.0], workers=2)
+    loss, grads = cpu.iteration(phis)
+    st = [("rs", 300.0), ("phase", phis[0]), ("rs", 400.0), ("phase", phis[1]), ("rs", 300.0)]
+    v, sv = O.pipeline_forward(u0, st, ds, lam)
+    lr, dv = O.sq_intensity_diff(v, t)
+    _, gr = O.pipeline_backward(dv, st, ds, lam, sv)
+    assert abs(loss - lr) <= 1e-12 * lr
+    for a, b in zip(grads, gr):
+        assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b)
