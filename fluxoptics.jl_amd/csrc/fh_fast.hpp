@@ -616,3 +616,248 @@ k_fast_ypipe(FastIO io, int batch, const float2* __restrict__ tabf, const float2
 }
 
 }  // namespace fh
+
+namespace fh {
+
+// ---------------------------------------------------------------- wave-per-column y pass
+// A workgroup of four waves owns a tile of four adjacent spectrum columns of 2048 points and walks
+// the batch.  Per field the tile moves between HBM and LDS cooperatively, as whole 128-byte block
+// rows of the 4x4-blocked layout (16-byte loads and stores, every line read and written in full),
+// and each wave transforms its own column (32 points a lane, radices 32 x 8 x 8) with exchanges in
+// its own LDS column, so those exchanges need no workgroup barrier: LDS operations of one wave
+// complete in order, and a compiler fence keeps the accesses where they are.  Two barriers a field
+// remain: after the tile is written to LDS and after the columns are written back.  The thread
+// that writes a pair of tile values into LDS is the one that reads it back out for the store, so
+// the next field's tile may be written without a barrier in between.  The transfer function is
+// loaded per field (from L2) into the registers that then receive the next field's tile.
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stage twiddle bases (as TwBases) kept in LDS instead of registers, one slot per (base, lane):
+// base i of lane t at tw[i * 64 + t], so a wave reads 64 consecutive slots.  A stage whose
+// butterflies all have the same j mod NS (P a multiple of NS) stores one set per lane.
+template <int R, int NS, int V, int P>
+struct TwSlots {
+  static constexpr int NB = ilog2c(R);
+  static constexpr bool SHARED = P % NS == 0;
+  static constexpr int NBF = NS > 1 ? (SHARED ? 1 : V / R) : 0;
+  static constexpr int count = NBF * NB;
+};
+
+template <int R, int NS, int V, int P, int OFF>
+__device__ __forceinline__ void tw_slots_store(float2* tw, const float2* __restrict__ tab, int t) {
+  using S = TwSlots<R, NS, V, P>;
+#pragma unroll
+  for (int m = 0; m < S::NBF; ++m) {
+    const int jm = (t + P * m) & (NS - 1);
+#pragma unroll
+    for (int k = 0; k < S::NB; ++k) tw[(OFF + m * S::NB + k) * 64 + t] = tab[jm * (R - 1) + ((1 << k) - 1)];
+  }
+}
+
+template <int R, int NS, int V, int P, int OFF>
+__device__ __forceinline__ void tw_slots_apply(float2* v, const float2* tw, int t) {
+  using S = TwSlots<R, NS, V, P>;
+  if constexpr (NS > 1) {
+#pragma unroll
+    for (int m = 0; m < V / R; ++m) {
+      const int mb = S::SHARED ? 0 : m;
+      float2 b[S::NB];
+#pragma unroll
+      for (int k = 0; k < S::NB; ++k) b[k] = tw[(OFF + mb * S::NB + k) * 64 + t];
+      float2 w[R];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const int low = r & (-r);
+        const int k = ilog2c(low);
+        w[r] = (r == low) ? b[k] : cmul(w[r - low], b[k]);
+        v[m * R + r] = cmul(v[m * R + r], w[r]);
+      }
+    }
+  }
+}
+
+// the slots of a radix plan's two twiddled stages, starting at slot OFF
+template <typename RD, int OFF>
+struct PlanSlots {
+  using S1 = TwSlots<RD::R1, RD::R0, RD::V, RD::P>;
+  using S2 = TwSlots<RD::R2, RD::R0 * RD::R1, RD::V, RD::P>;
+  static constexpr int off1 = OFF, off2 = OFF + (RD::NST >= 2 ? S1::count : 0);
+  static constexpr int end = off2 + (RD::NST >= 3 ? S2::count : 0);
+  __device__ __forceinline__ static void store(float2* tw, const float2* __restrict__ tab, int t) {
+    if constexpr (RD::NST >= 2) tw_slots_store<RD::R1, RD::R0, RD::V, RD::P, off1>(tw, tab + FastTw<RD>::off1, t);
+    if constexpr (RD::NST >= 3) tw_slots_store<RD::R2, RD::R0 * RD::R1, RD::V, RD::P, off2>(tw, tab + FastTw<RD>::off2, t);
+  }
+};
+
+// one column's transform within one wave, exchanging through the wave's own LDS column s, with
+// the stage twiddles from the LDS slots at tw; after0() runs between the first stage and the
+// first exchange
+struct NoHook {
+  __device__ __forceinline__ void operator()() const {}
+};
+template <typename RD, int OFF, bool INV, typename F = NoHook>
+__device__ __forceinline__ void wave_fft(float2* v, float2* s, const float2* tw, int t, F after0 = F()) {
+  using C = float2;
+  using PS = PlanSlots<RD, OFF>;
+  constexpr int n = RD::n, V = RD::V, P = RD::P;
+  if (INV) fast_idfts<C, RD::R0, V>(v); else fast_dfts<C, RD::R0, V>(v);
+  after0();
+  if constexpr (RD::NST >= 2) {
+    wave_lds_sync();
+    fast_write<C, n, RD::R0, 1, V, P>(s, v, t);
+    wave_lds_sync();
+    fast_read<C, n, RD::R1, V, P>(v, s, t);
+    tw_slots_apply<RD::R1, RD::R0, V, P, PS::off1>(v, tw, t);
+    if (INV) fast_idfts<C, RD::R1, V>(v); else fast_dfts<C, RD::R1, V>(v);
+  }
+  if constexpr (RD::NST >= 3) {
+    wave_lds_sync();
+    fast_write<C, n, RD::R1, RD::R0, V, P>(s, v, t);
+    wave_lds_sync();
+    fast_read<C, n, RD::R2, V, P>(v, s, t);
+    tw_slots_apply<RD::R2, RD::R0 * RD::R1, V, P, PS::off2>(v, tw, t);
+    if (INV) fast_idfts<C, RD::R2, V>(v); else fast_dfts<C, RD::R2, V>(v);
+  }
+}
+
+__device__ __forceinline__ float4 buf_load4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_store4(__amdgpu_buffer_rsrc_t r, int voff, int soff, float4 v) {
+  using V4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, voff, soff, 0);
+}
+
+constexpr int kWaveCols = 4;   // columns (waves) of a tile
+constexpr int kWaveN = 2048;   // column length
+constexpr int kWavePairs = kWaveN * kWaveCols / 2 / (kWaveCols * 64);  // 16-byte pairs a thread
+
+// the thread's 16 pairs of field b's tile: pair k is block row 32 k + (T >> 3) of the tile,
+// elements 2 (T & 7) and 2 (T & 7) + 1
+__device__ __forceinline__ void ywave_tile_load(float2* q, const FastIO& io, int b, int voff, int kstep) {
+  const auto r = make_rsrc(io.in + b * io.in_f, io.in_bytes);
+#pragma unroll
+  for (int k = 0; k < kWavePairs; ++k) {
+    const float4 x = buf_load4(r, voff, k * kstep);
+    q[2 * k] = make_float2(x.x, x.y);
+    q[2 * k + 1] = make_float2(x.z, x.w);
+  }
+}
+
+// the transfer-function column values of field b at the positions the forward transform leaves
+// in registers (as ypipe_loadh); the table index is read once per wave into a scalar register,
+// since a descriptor the compiler cannot prove uniform turns every load into a loop over lanes
+template <typename RDF>
+__device__ __forceinline__ void ywave_loadh(float2* h, const QuadTab& qt, int b, int qoff_d, int qoff_m) {
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  constexpr int RL = RDF::Rev::R0;
+  const int ti = __builtin_amdgcn_readfirstlane(qt.tabidx[b]);
+  const auto rt = make_rsrc(qt.tab + (long)ti * qt.tabsize, qt.tabsize * 8);
+#pragma unroll
+  for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int c = P * m + r * (n / RL);
+      h[m * RL + r] = (c < n / 2) ? buf_load(rt, qoff_d, c * 8) : buf_load(rt, qoff_m, (n - c - P) * 8);
+    }
+}
+
+using WaveRDF = Radices<8, 8, 32>;  // forward radices (the inverse runs them reversed)
+constexpr int kWaveSlots = PlanSlots<WaveRDF::Rev, PlanSlots<WaveRDF, 0>::end>::end;
+// LDS of a workgroup: four columns and the twiddle slots (two workgroups fit one CU's 160 KiB)
+constexpr size_t kWaveLds = ((size_t)kWaveCols * LdsLine<kWaveN, kWaveCols>::size + (size_t)kWaveSlots * 64) * 8;
+static_assert(2 * kWaveLds <= 160 * 1024, "two workgroups per CU");
+
+// Two waves per SIMD (two workgroups per CU): at most 256 registers a lane.
+__global__ void __launch_bounds__(kWaveCols * 64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_fast_ywave(FastIO io, int batch, const float2* __restrict__ tabf, const float2* __restrict__ tabi, QuadTab qt) {
+  using RDF = WaveRDF;
+  using RI = typename RDF::Rev;
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  constexpr int RE = LastRadix<RI>::value;
+  static_assert(n == kWaveN && P == 64 && V == 32, "one wave per 2048-point column");
+  static_assert((n / RDF::R0) % 16 == 0 && (n / RE) % 16 == 0, "register positions 16 slots apart");
+  constexpr int CS = LdsLine<n, kWaveCols>::size;  // one column's LDS slots
+  extern __shared__ unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  const int T = threadIdx.x;
+  const int t = T & 63;
+  const int li = blockIdx.x * kWaveCols + (T >> 6);  // this wave's column
+  float2* s = smem + (T >> 6) * CS;
+  // the thread's tile pairs: rows 128 k + prow, columns pc and pc + 1 of the tile
+  const int pw = T & 7;
+  const int prow = 4 * (T >> 3) + (pw >> 1);
+  float2* ps = smem + 2 * (pw & 1) * CS + lds_pos(prow);
+  const int rowlen = io.in_e;  // spectrum columns (elements between rows)
+  const int vin = ((((T >> 3) * (rowlen >> 2)) + (int)blockIdx.x) * 16 + 2 * pw) * 8;
+  const int kin = rowlen * 1024;  // 32 block rows of the tile
+  const int vout = ((((T >> 3) * (io.out_e >> 2)) + (int)blockIdx.x) * 16 + 2 * pw) * 8;
+  const int kout = io.out_e * 1024;
+  const int qxi = qt.full ? li : (li <= qt.nx - li ? li : qt.nx - li);
+  const int qoff_d = (qxi * qt.qcol + t) * 8;
+  const int qoff_m = (qxi * qt.qcol + qt.moff + P - t) * 8;
+  // the stage twiddle bases of both transforms, after the columns (one wave writes them; the
+  // first barrier below orders the writes before any use)
+  using PF = PlanSlots<RDF, 0>;
+  using PI = PlanSlots<RI, PF::end>;
+  float2* tw = smem + kWaveCols * CS;
+  if (T < 64) {
+    PF::store(tw, tabf, t);
+    PI::store(tw, tabi, t);
+  }
+  float2 q[V];  // the tile pairs, or the transfer-function values
+  ywave_tile_load(q, io, 0, vin, kin);
+  for (int b = 0; b < batch; ++b) {
+#pragma unroll
+    for (int k = 0; k < kWavePairs; ++k) {  // lds_pos(128 k + prow) = 136 k + lds_pos(prow)
+      ps[136 * k] = q[2 * k];
+      ps[CS + 136 * k] = q[2 * k + 1];
+    }
+    __syncthreads();
+    float2 v[V];
+#pragma unroll
+    for (int m = 0; m < V / RDF::R0; ++m)  // the first stage's input positions
+#pragma unroll
+      for (int r = 0; r < RDF::R0; ++r) {
+        const int c = P * m + r * (n / RDF::R0);  // a multiple of 16: one pad slot per 16
+        v[m * RDF::R0 + r] = s[lds_pos(t) + c + (c >> 4)];
+      }
+    wave_fft<RDF, PF::off1, false>(v, s, tw, t);
+    // the transfer function at the positions the transform leaves in registers, from L2
+    ywave_loadh<RDF>(q, qt, b, qoff_d, qoff_m);
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k] = qt.conj ? cmulc(v[k], q[k]) : cmul(v[k], q[k]);
+    // the next field's tile (or this one again after the last) goes out after the inverse's
+    // first stage, the one with the most registers live, and is in flight during the rest
+    wave_fft<RI, PI::off1, true>(v, s, tw, t, [&] {
+      asm volatile("" ::: "memory");
+      ywave_tile_load(q, io, b + 1 < batch ? b + 1 : b, vin, kin);
+    });
+    wave_lds_sync();
+#pragma unroll
+    for (int m = 0; m < V / RE; ++m)  // the last stage's output positions
+#pragma unroll
+      for (int r = 0; r < RE; ++r) {
+        const int c = P * m + r * (n / RE);
+        s[lds_pos(t) + c + (c >> 4)] = v[m * RE + r];
+      }
+    // wait for the prefetch here, while the only other outstanding operations are the previous
+    // field's stores: a wait for a load issued after stores drains the stores too
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    __syncthreads();
+    const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
+#pragma unroll
+    for (int k = 0; k < kWavePairs; ++k) {
+      const float2 a = ps[136 * k], c = ps[CS + 136 * k];
+      buf_store4(rout, vout, k * kout, make_float4(a.x, a.y, c.x, c.y));
+    }
+  }
+}
+
+}  // namespace fh
+

@@ -1766,6 +1766,33 @@ static int fast_pipe_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool 
   return FH_OK;
 }
 
+// Wave-per-column y pass (2048-point columns of the blocked layout, whole lines only); RS
+// half lines, the natural layout and a ragged last tile take the LDS-table pipelined pass.
+static int fast_wave_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool half) {
+  if (half || !io.in_blk || !io.out_blk || io.nlines % kWaveCols || io.in_e % 4 || io.out_e % 4 ||
+      io.in_valid < kWaveN || io.out_valid < kWaveN)
+    return fast_pipe_v<16, 16, 8, 4, true>(c, io, batch, op, half);
+  using RDF = WaveRDF;
+  using RI = typename RDF::Rev;
+  const float2 *tf = nullptr, *ti = nullptr;
+  int rc;
+  if ((rc = get_stage_tw(c, RDF::n, RDF::R0, RDF::R1, RDF::R2, false, &tf)) ||
+      (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
+    return rc;
+  // the tile's offsets: 16-byte pairs k * rowlen * 1024 bytes apart within one field
+  if ((long)io.in_e * 1024 * (kWavePairs - 1) + (long)io.in_e * 128 > 0x7fffffffL ||
+      (long)io.out_e * 1024 * (kWavePairs - 1) + (long)io.out_e * 128 > 0x7fffffffL)
+    return fast_pipe_v<16, 16, 8, 4, true>(c, io, batch, op, half);
+  const size_t lds = kWaveLds;
+  QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab, op.full,
+             op.full ? op.ny / 2 : 0};
+  FH_HIP(ensure_lds(k_fast_ywave, lds));
+  hipLaunchKernelGGL(k_fast_ywave, dim3(io.nlines / kWaveCols), dim3(kWaveCols * 64), lds, c->stream, io,
+                     batch, tf, ti, qt);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
 // Variant lists per length; default_conv_variant picks the fastest measured on MI355X.
 static const FastConvVariant kFast9[] = {{"8x8x8 L4 f1", fast_conv_v<8, 8, 8, 4, 1>},
                                          {"8x8x8 L4", fast_conv_v<8, 8, 8, 4>},
@@ -1779,7 +1806,8 @@ static const FastConvVariant kFast11[] = {{"8x16x16 L4 f1", fast_conv_v<8, 16, 1
                                           {"pipe 16x16x8 L4", fast_pipe_v<16, 16, 8, 4>},
                                           {"16x16x8 L4 f1", fast_conv_v<16, 16, 8, 4, 1>},
                                           {"16x16x8 L4 f2", fast_conv_v<16, 16, 8, 4, 2>},
-                                          {"pipe 16x16x8 L4 hlds", fast_pipe_v<16, 16, 8, 4, true>}};
+                                          {"pipe 16x16x8 L4 hlds", fast_pipe_v<16, 16, 8, 4, true>},
+                                          {"wave 8x8x32 L4", fast_wave_v}};
 static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16, 16, 4, 1>},
                                           {"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
                                           {"16x16x16 L2 f1", fast_conv_v<16, 16, 16, 2, 1>},
@@ -1788,7 +1816,8 @@ static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16,
 // default variant index for a y length and whether the batch shares one transfer function
 static int default_conv_variant(int logn, bool one_tab, bool half) {
   if (one_tab && logn == 11) return 5;           // pipe 16x16x8 L4, transfer function in LDS
-  if (logn == 11) return 3;                      // 16x16x8 L4 f1 (a table per field)
+  if (logn == 11 && !half) return 6;             // wave per column (a table per field)
+  if (logn == 11) return 3;                      // 16x16x8 L4 f1 (RS, a table per field)
   if (logn == 12 && half && !one_tab) return 3;  // pipe 16x16x16 L2 (RS, a table per field)
   if (logn == 12) return 1;                      // 16x16x16 L4
   return 0;
@@ -1803,7 +1832,7 @@ static const FastConvVariant* fast_conv_variants(int logn, int* count) {
     case 8: *count = 2; return kFast8;
     case 9: *count = 3; return kFast9;
     case 10: *count = 4; return kFast10;
-    case 11: *count = 6; return kFast11;
+    case 11: *count = 7; return kFast11;
     case 12: *count = 4; return kFast12;
     default: *count = 0; return nullptr;
   }

@@ -1,0 +1,65 @@
+"""Every launch variant of the three passes of a propagation (the candidates the autotuner may
+keep, fh_plan_variants / fh_plan_set_variant) against the FP64 oracle, forward and backward:
+ASProp at 2048 x 2048 (the y-pass length with the most variants, the wave-per-column pass among
+them) with one wavelength, with a wavelength per field and with tilted fields (full
+transfer-function tables), and RSProp on a 1024 grid (zero-padded 2048-point lines, of which the
+y pass reads and writes half)."""
+
+import ctypes
+import warnings
+
+import numpy as np
+import pytest
+
+import fluxoptics as fo
+from fluxoptics import _lib
+from oracle import fluxoptics_oracle as O
+
+from _util import c64, rel_l2, speckle_batch, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-5
+
+
+def _variants(plan, pid):
+    cnt, cur = ctypes.c_int(), ctypes.c_int()
+    _lib.call("fh_plan_variants", plan, pid, ctypes.byref(cnt), ctypes.byref(cur))
+    return cnt.value
+
+
+def _sweep(prop, u, ref, tag):
+    plan = prop.plan(u)
+    for pid in (0, 1, 2):
+        for k in range(_variants(plan, pid)):
+            _lib.call("fh_plan_set_variant", plan, pid, k)
+            print(f"[variants] {tag} pass {pid} variant {k}", flush=True)
+            for d, s in ((fo.Forward, 1), (fo.Backward, -1)):
+                v = to_host(prop.propagate(u, d).electric)
+                err = rel_l2(v, ref[s])
+                assert err <= FIELD_TOL, (tag, pid, k, s, err)
+        _lib.call("fh_plan_set_variant", plan, pid, -1)
+
+
+@pytest.mark.parametrize("case", ["one_lambda", "lambda_per_field", "tilted"])
+def test_asprop_2048_every_variant(case):
+    n, ds, z, n0 = 2048, (0.5, 0.5), 3000.0, 1.0
+    lams = {"one_lambda": 0.8, "lambda_per_field": [0.8, 0.65], "tilted": [0.8, 0.8]}[case]
+    tilts = ([0.01, -0.02], [0.015, 0.0]) if case == "tilted" else None
+    u0 = c64(speckle_batch(2, n, n, ds[0], ds[1], 150.0, 41))
+    u = fo.ScalarField(to_dev(u0), ds, lams, tilts)
+    p = fo.ASProp(u, z, n0=n0)
+    ref = {s: O.asprop(u0, ds, lams, z, n0, s, tilts=tilts) for s in (1, -1)}
+    _sweep(p, u, ref, f"as {case}")
+
+
+def test_rsprop_1024_every_variant():
+    n, ds, lam, z = 1024, (0.5, 0.5), 0.8, 600.0
+    u0 = c64(speckle_batch(2, n, n, ds[0], ds[1], 80.0, 43))
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")  # RSProp's sampling advice
+        p = fo.RSProp(u, z)
+    ref = {s: O.rsprop(u0, ds, lam, z, s, "2n-1") for s in (1, -1)}
+    _sweep(p, u, ref, "rs")
+
