@@ -460,6 +460,34 @@ def workload_train_step(torch, fo, inputs):
                     "epilogue) vs component by component (3 + metric + 3 passes), each one recorded graph"}
 
 
+def workload_spectral(torch, fo, inputs):
+    """The headline grid with a wavelength per field: 8 x 2048x2048 c64, lambda 0.45..0.80 um
+    (one transfer function per field, read by the y pass for every field), ds 0.5 um, z 5000 um;
+    one step = forward of the batch + adjoint of a cotangent batch, as the headline."""
+    import numpy as np
+    from fluxoptics import _lib
+    from fluxoptics.components import _ptr
+    lams = [float(x) for x in np.linspace(0.45, 0.80, BATCH)]
+    u_np, t_np = make_inputs()
+    u = fo.ScalarField(torch.from_numpy(u_np).cuda(), (DS, DS), lams)
+    g = fo.ScalarField(torch.from_numpy(t_np).cuda(), (DS, DS), lams)
+    prop = fo.ASProp(u, Z)
+    plan = prop.plan(u)
+
+    def step():
+        prop.propagate_(u, fo.Forward)
+        prop.propagate_(g, fo.Backward)
+
+    for _ in range(3):
+        step()
+    ms = timed(step, 20, torch)
+    passes = pass_times_in_step(torch, _lib, _ptr, plan, u, g, 10)
+    return {"units_per_s": BATCH / (ms * 1e-3), "ms_per_step": ms,
+            "pass_ms": {k: round(v, 4) for k, v in passes.items()},
+            "y_pass_GBps": 16.0 * N * N * BATCH / (passes["y_fft_mul_ifft"] * 1e-3) / 1e9,
+            "what": "8 x 2048^2 ASProp fwd+adj with 8 wavelengths (a transfer function per field)"}
+
+
 def workload_rgb_cascade(torch, fo, inputs):
     """1024x1024 x 3 wavelengths, Phase -> ASProp(1e4) -> Phase -> ASProp(1e4) -> Phase,
     intensity loss, three mask gradients."""
@@ -585,7 +613,8 @@ def main():
         from fluxoptics import inputs as inp
         fns = {"published_splitter_250": workload_published_splitter, "published_rgb_800x3": workload_published_rgb, "beam_splitter_512": workload_beam_splitter, "train_step_2048x8": workload_train_step,
                "rgb_cascade_1024x3": workload_rgb_cascade, "as_bpm_512x400": workload_bpm,
-               "uneven_batch_64": workload_uneven_batch, "published_tomography_1600x43": workload_published_tomography}
+               "uneven_batch_64": workload_uneven_batch, "published_tomography_1600x43": workload_published_tomography,
+               "spectral_2048x8": workload_spectral}
         if args.workload not in fns:
             sys.exit(f"bench.py: unknown workload {args.workload!r} (one of {', '.join(fns)})")
         torch.cuda.set_device(0)
@@ -676,6 +705,7 @@ def main():
             "as_bpm_512x400": workload_bpm(torch, fo, inp),
             "uneven_batch_64": workload_uneven_batch(torch, fo, inp),
             "published_tomography_1600x43": workload_published_tomography(torch, fo, inp),
+            "spectral_2048x8": workload_spectral(torch, fo, inp),
         }
         torch.cuda.empty_cache()
 
