@@ -33,7 +33,8 @@ def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps first (the clocks settle over the first ~20 steps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparator")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -79,14 +80,22 @@ PASS_KERNELS = {"x_fft": "k_fast_fft<Radices<16, 16, 8>, 4, false, false,",
 
 def read_profiled_ms(pass_name):
     """Average ms per call of a pass's kernel in the committed rocprofv3 kernel-trace summary
-    of this benchmark (or None)."""
+    of this benchmark: the steady-state column (the calls of the window in which this script
+    times its passes) where the summary has one, else the average over all calls (or None)."""
     if not os.path.exists(KERNELS_FILE):
         return None
+    cols = None
     with open(KERNELS_FILE) as f:
         for line in f:
-            parts = line.split(None, 3)
-            if len(parts) == 4 and not line.startswith("#") and PASS_KERNELS[pass_name] in parts[3]:
-                return float(parts[1])
+            if line.startswith("#") or not line.strip():
+                continue
+            if cols is None:  # the header: calls avg_ms [steady_ms] share% kernel
+                cols = line.split()
+                continue
+            parts = line.split(None, len(cols) - 1)
+            if len(parts) == len(cols) and PASS_KERNELS[pass_name] in parts[-1]:
+                col = "steady_ms" if "steady_ms" in cols and parts[cols.index("steady_ms")] != "-" else "avg_ms"
+                return float(parts[cols.index(col)])
     return None
 
 
