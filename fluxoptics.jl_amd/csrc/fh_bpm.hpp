@@ -32,6 +32,37 @@ struct DnPhaseOp {
   }
 };
 
+// The slice phases of `nsl` consecutive slices, cis(sgn * kl[b] * dn) in FP64 with DnPhaseOp's
+// expression, into out[slice][b][ny*nx]: computed ahead of the slice loop (every element at once,
+// not one wave's dependent chain per row), so the slice steps read their phase instead of
+// evaluating it, with the same values.
+__global__ void k_bpm_cis(const float* __restrict__ dn, const double* __restrict__ kl, double sgn,
+                          double2* __restrict__ out, long npix, int batch, int nsl) {
+  const long total = npix * batch * nsl;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long p = i % npix;
+    const long jb = i / npix;
+    const int b = (int)(jb % batch);
+    const long j = jb / batch;
+    double sn, cs;
+    sincos(sgn * kl[b] * (double)dn[j * npix + p], &sn, &cs);
+    out[i] = make_double2(cs, sn);
+  }
+}
+
+// Slice phase read from a k_bpm_cis table ([B][ny][nx] of one slice).
+struct TabPhaseOp {
+  const double2* cis;
+  long fstride;  // ny * nx
+  int nx;
+  template <typename C>
+  __device__ __forceinline__ C operator()(C v, int b, int y, int x) const {
+    if (x >= nx) return v;
+    const double2 c = cis[b * fstride + (long)y * nx + x];
+    return cmul(v, cmake<C>(c.x, c.y));
+  }
+};
+
 // Stores the time-domain value as c64 (epilogue of an inverse x FFT).
 struct SaveOp {
   float2* dst;   // [B][ny][nx]
@@ -48,9 +79,9 @@ struct SaveOp {
 // multiply by the slice phase -> FFT_x.  The inverse plan ends in the register layout
 // the forward plan starts from, so the three steps share registers.  `ckspec`, when
 // set, receives the loaded x-spectrum rows (a checkpoint that replays this very step).
-template <typename C, typename RDF, int L>
+template <typename C, typename RDF, int L, typename PH = DnPhaseOp>
 __global__ void __launch_bounds__(L * RDF::P)
-k_bpm_xstep(LineIO<float2> io, const C* __restrict__ tw, DnPhaseOp ph, float2* __restrict__ save,
+k_bpm_xstep(LineIO<float2> io, const C* __restrict__ tw, PH ph, float2* __restrict__ save,
             float2* __restrict__ ckspec) {
   using RI = typename RDF::Rev;
   constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
@@ -105,10 +136,12 @@ k_bpm_xstep(LineIO<float2> io, const C* __restrict__ tw, DnPhaseOp ph, float2* _
 // cotangent: IFFT_x -> du = dv * cis(-sgn*kl[b]*dn) -> g += sgn*kl[b]*Im(du*conj(u_k))
 // -> FFT_x.  One workgroup owns L rows and loops over the batch, so the gradient
 // (compute_dn_gradient!, bpm.jl:275-287) is summed over fields in order and written once.
-template <typename C, typename RDF, int L>
+// TAB: cis(-sgn*kl[b]*dn) is the conjugate of the forward's k_bpm_cis value (the arguments are
+// exact negatives), read from `cis` ([B][ny][nx] of this slice) instead of evaluated.
+template <typename C, typename RDF, int L, bool TAB = false>
 __global__ void __launch_bounds__(L * RDF::P)
 k_bpm_xstep_grad(LineIO<float2> io, int batch, const C* __restrict__ tw, DnPhaseOp ph,
-                 const float2* __restrict__ saved, float* __restrict__ ddn) {
+                 const float2* __restrict__ saved, float* __restrict__ ddn, const double2* __restrict__ cis) {
   using R = decltype(C{}.x);
   using RI = typename RDF::Rev;
   constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
@@ -144,7 +177,13 @@ k_bpm_xstep_grad(LineIO<float2> io, int batch, const C* __restrict__ tw, DnPhase
         const int pos = reg_pos<n, RDF::R0, P>(t, m, r);
         if (live && pos < nx) {
           R sn, cs;
-          sincos_any<R>(-coef * (R)ph.dn[(long)li * nx + pos], &sn, &cs);
+          if constexpr (TAB) {
+            const double2 c = cis[b * io.in_f + (long)li * nx + pos];
+            cs = (R)c.x;
+            sn = -(R)c.y;
+          } else {
+            sincos_any<R>(-coef * (R)ph.dn[(long)li * nx + pos], &sn, &cs);
+          }
           const C du = cmul(v[m * RDF::R0 + r], cmake<C>(cs, sn));
           v[m * RDF::R0 + r] = du;
           g[m * RDF::R0 + r] += coef * cmulc(du, cvt<C>(sv[pos])).y;

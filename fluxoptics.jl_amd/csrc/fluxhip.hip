@@ -125,10 +125,14 @@ static hipError_t launch_fft_t(const LineIO<S>& io, int batch, const C* tw, Op o
 // least kFillWGs workgroups (or L = 1).
 constexpr long kFillWGs = 512;
 
+// The strided FP64 convolution (the AS_BPM slice propagator) keeps two columns per workgroup down
+// to kFillWGs / 2 workgroups: a 512 x 512 slice then runs in 6.6 us with 256 two-column workgroups
+// against 7.2 us with 512 one-column ones (a wave's column loads touch a 128-byte line per lane;
+// two columns per workgroup share those lines), and 8.8 us with four columns.
 template <typename S, typename C, typename RD, int L, typename Op>
 static hipError_t launch_conv_l(const LineIO<S>& io, int batch, const C* tw, Op op, hipStream_t st) {
-  if constexpr (L > 1) {
-    if ((long)((io.nlines + L - 1) / L) * batch < kFillWGs)
+  if constexpr (L > 2) {
+    if ((long)((io.nlines + L - 1) / L) * batch < kFillWGs / 2)
       return launch_conv_l<S, C, RD, L / 2, Op>(io, batch, tw, op, st);
   }
   auto kern = k_line_conv<S, C, RD, L, true, Op>;
@@ -3396,29 +3400,33 @@ int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {
 
 namespace fh {
 
+// cis (nullable): this slice's phases from k_bpm_cis ([B][ny][nx]), read instead of evaluated
 template <typename RD, int L>
 static hipError_t launch_bpm_step_l(const LineIO<float2>& io, int batch, const double2* tw,
                                     const DnPhaseOp& ph, float2* save, float2* ckspec, float* ddn,
-                                    const float2* saved, hipStream_t st) {
+                                    const float2* saved, const double2* cis, hipStream_t st) {
   if constexpr (L > 1) {
     // the gradient kernel loops over the batch inside a workgroup
     const long wgs = (long)((io.nlines + L - 1) / L) * (ddn ? 1 : batch);
-    if (wgs < kFillWGs) return launch_bpm_step_l<RD, L / 2>(io, batch, tw, ph, save, ckspec, ddn, saved, st);
+    if (wgs < kFillWGs) return launch_bpm_step_l<RD, L / 2>(io, batch, tw, ph, save, ckspec, ddn, saved, cis, st);
   }
   const size_t lds = (RD::NST > 1) ? (size_t)L * LdsLine<RD::n, L, sizeof(double2)>::size * sizeof(double2) : 0;
   const unsigned threads = L * RD::P;
+  const dim3 grid((io.nlines + L - 1) / L, ddn ? 1 : batch);
+  hipError_t e;
   if (ddn) {
-    auto kern = k_bpm_xstep_grad<double2, RD, L>;
-    hipError_t e = ensure_lds(kern, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L, 1), dim3(threads), lds, st, io, batch, tw, ph,
-                       saved, ddn);
+    auto kern = cis ? k_bpm_xstep_grad<double2, RD, L, true> : k_bpm_xstep_grad<double2, RD, L, false>;
+    if ((e = ensure_lds(kern, lds)) != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, grid, dim3(threads), lds, st, io, batch, tw, ph, saved, ddn, cis);
+  } else if (cis) {
+    auto kern = k_bpm_xstep<double2, RD, L, TabPhaseOp>;
+    if ((e = ensure_lds(kern, lds)) != hipSuccess) return e;
+    const TabPhaseOp tp{cis, io.in_f, ph.nx};
+    hipLaunchKernelGGL(kern, grid, dim3(threads), lds, st, io, tw, tp, save, ckspec);
   } else {
-    auto kern = k_bpm_xstep<double2, RD, L>;
-    hipError_t e = ensure_lds(kern, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L, batch), dim3(threads), lds, st, io, tw, ph, save,
-                       ckspec);
+    auto kern = k_bpm_xstep<double2, RD, L, DnPhaseOp>;
+    if ((e = ensure_lds(kern, lds)) != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, grid, dim3(threads), lds, st, io, tw, ph, save, ckspec);
   }
   return hipGetLastError();
 }
@@ -3426,18 +3434,22 @@ static hipError_t launch_bpm_step_l(const LineIO<float2>& io, int batch, const d
 template <int LOGN>
 static hipError_t launch_bpm_step_t(const LineIO<float2>& io, int batch, const double2* tw,
                                     const DnPhaseOp& ph, float2* save, float2* ckspec, float* ddn,
-                                    const float2* saved, hipStream_t st) {
+                                    const float2* saved, const double2* cis, hipStream_t st) {
   using CF = Cfg<double, LOGN, false>;
-  return launch_bpm_step_l<typename CF::RD, CF::L>(io, batch, tw, ph, save, ckspec, ddn, saved, st);
+  return launch_bpm_step_l<typename CF::RD, CF::L>(io, batch, tw, ph, save, ckspec, ddn, saved, cis, st);
 }
 
 static hipError_t launch_bpm_step(int logn, const LineIO<float2>& io, int batch, const double2* tw,
                                   const DnPhaseOp& ph, float2* save, float2* ckspec, float* ddn,
-                                  const float2* saved, hipStream_t st) {
-  FH_LOGN_SWITCH(logn, (launch_bpm_step_t<LG>(io, batch, tw, ph, save, ckspec, ddn, saved, st)))
+                                  const float2* saved, const double2* cis, hipStream_t st) {
+  FH_LOGN_SWITCH(logn, (launch_bpm_step_t<LG>(io, batch, tw, ph, save, ckspec, ddn, saved, cis, st)))
 }
 
 }  // namespace fh
+
+// Bound on the two segments' FP64 slice-phase tables of a BPM (512 x 512, 20 slices a segment:
+// 2 x 84 MB)
+constexpr size_t kBpmPhaseTabBytes = (size_t)512 << 20;
 
 struct fh_bpm {
   fh_ctx* ctx = nullptr;
@@ -3472,6 +3484,10 @@ struct fh_bpm {
   int oned = 0;
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
+  // slice phases of one segment, cis(s kl_b dn) in FP64 ([every][B][ny][nx], two buffers), filled on
+  // the side stream ahead of the slices that read them (null: the slice steps evaluate the phase)
+  double2* cis[2] = {nullptr, nullptr};
+  hipEvent_t ev_tab[2] = {nullptr, nullptr}, ev_tused[2] = {nullptr, nullptr};
 };
 
 namespace fh {
@@ -3493,7 +3509,10 @@ static void bpm_free(fh_bpm* m) {
   plan_free(m->phalf);
   if (m->gpart) (void)hipFree(m->gpart);
   if (m->seg2) (void)hipFree(m->seg2);
-  for (hipEvent_t e : {m->ev_fork, m->ev_rec[0], m->ev_rec[1], m->ev_used[0], m->ev_used[1]})
+  for (double2* q : m->cis)
+    if (q) (void)hipFree(q);
+  for (hipEvent_t e : {m->ev_fork, m->ev_rec[0], m->ev_rec[1], m->ev_used[0], m->ev_used[1], m->ev_tab[0],
+                       m->ev_tab[1], m->ev_tused[0], m->ev_tused[1]})
     if (e) (void)hipEventDestroy(e);
   if (m->side) (void)hipStreamDestroy(m->side);
   delete m;
@@ -3589,35 +3608,53 @@ static int bpm_ypass(fh_bpm* m, float2* f, const double2* tab, int direction) {
   return FH_OK;
 }
 
-static int bpm_step(fh_bpm* m, float2* f, const float* dn_k, double sgn, float2* save, float2* ckspec) {
+// cis_k (nullable): slice k's phases from k_bpm_cis, [B][ny][nx]
+static int bpm_step(fh_bpm* m, float2* f, const float* dn_k, double sgn, float2* save, float2* ckspec,
+                    const double2* cis_k = nullptr) {
   const double2* tw = nullptr;
   int rc;
   if ((rc = get_twiddles(m->ctx, m->nx, &tw))) return rc;
   DnPhaseOp ph{dn_k, m->kl, sgn, m->nx};
-  FH_HIP(launch_bpm_step(m->lx, xrows(m, f), m->batch, tw, ph, save, ckspec, nullptr, nullptr, m->ctx->stream));
+  FH_HIP(launch_bpm_step(m->lx, xrows(m, f), m->batch, tw, ph, save, ckspec, nullptr, nullptr, cis_k,
+                         m->ctx->stream));
   return FH_OK;
 }
 
 static int bpm_step_grad(fh_bpm* m, float2* f, const float* dn_k, double sgn, const float2* saved,
-                         float* ddn_k) {
+                         float* ddn_k, const double2* cis_k = nullptr) {
   const double2* tw = nullptr;
   int rc;
   if ((rc = get_twiddles(m->ctx, m->nx, &tw))) return rc;
   DnPhaseOp ph{dn_k, m->kl, sgn, m->nx};
-  FH_HIP(launch_bpm_step(m->lx, xrows(m, f), m->batch, tw, ph, nullptr, nullptr, ddn_k, saved, m->ctx->stream));
+  FH_HIP(launch_bpm_step(m->lx, xrows(m, f), m->batch, tw, ph, nullptr, nullptr, ddn_k, saved, cis_k,
+                         m->ctx->stream));
+  return FH_OK;
+}
+
+// The phases of slices k0 .. k1-1 (sign sgn) into out, on the context's current stream.
+static int bpm_cis(fh_bpm* m, const float* dn, int k0, int k1, double sgn, double2* out) {
+  const long npix = (long)m->nx * m->ny;
+  const long total = npix * m->batch * (k1 - k0);
+  const int blocks = (int)std::min<long>((total + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_bpm_cis, dim3(blocks), dim3(256), 0, m->ctx->stream, dn + (long)k0 * npix, m->kl, sgn, out,
+                     npix, m->batch, k1 - k0);
+  FH_HIP(hipGetLastError());
   return FH_OK;
 }
 
 // Slice inputs k0 .. k1-1 into seg[0 ..], replayed from the x-spectrum checkpoint of slice
 // k0 with the forward's own kernels, so they equal the forward's values bit for bit.
-static int bpm_recompute(fh_bpm* m, const float* dn, int k0, int k1, double sgn, int direction, float2* seg) {
+static int bpm_recompute(fh_bpm* m, const float* dn, int k0, int k1, double sgn, int direction, float2* seg,
+                         const double2* cis = nullptr) {
   const long npix = (long)m->nx * m->ny;
   const long fsz = npix * m->batch;
   FH_HIP(hipMemcpyAsync(m->work, m->ckpt + (long)(k0 / m->every) * fsz, fsz * sizeof(float2),
                         hipMemcpyDeviceToDevice, m->ctx->stream));
   int rc;
   for (int k = k0; k < k1; ++k) {
-    if ((rc = bpm_step(m, m->work, dn + (long)k * npix, sgn, seg + (long)(k - k0) * fsz, nullptr))) return rc;
+    if ((rc = bpm_step(m, m->work, dn + (long)k * npix, sgn, seg + (long)(k - k0) * fsz, nullptr,
+                       cis ? cis + (long)(k - k0) * fsz : nullptr)))
+      return rc;
     if (k + 1 < k1 && (rc = bpm_ypass(m, m->work, m->tfull, direction))) return rc;
   }
   return FH_OK;
@@ -3913,9 +3950,22 @@ int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* d, double thickness, int n
       hipEventCreateWithFlags(&m->ev_rec[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&m->ev_rec[1], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&m->ev_used[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&m->ev_used[1], hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&m->ev_used[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev_tab[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev_tab[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev_tused[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev_tused[1], hipEventDisableTiming) != hipSuccess) {
     bpm_free(m);
     return fail(FH_E_HIP, "BPM: stream or event creation failed");
+  }
+  // FP64 phase tables of two segments, when they fit kBpmPhaseTabBytes (otherwise the slice steps
+  // evaluate their phases)
+  const size_t cbytes = (size_t)m->every * nx * ny * batch * sizeof(double2);
+  if (2 * cbytes <= kBpmPhaseTabBytes) {
+    if (hipMalloc(&m->cis[0], cbytes) != hipSuccess || hipMalloc(&m->cis[1], cbytes) != hipSuccess) {
+      bpm_free(m);
+      return fail(FH_E_NOMEM, "BPM: out of device memory");
+    }
   }
   const double scale = 1.0 / ((double)nx * (double)ny);
   for (size_t k = 0; k < ntab; ++k) {
@@ -3940,7 +3990,7 @@ int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* d, double thickness, int n
     bpm_free(m);
     return fail(FH_E_HIP, "BPM: table set-up failed");
   }
-  m->bytes = (m->nck + 2 * m->every + 1) * fsz + 2 * tbytes;
+  m->bytes = (m->nck + 2 * m->every + 1) * fsz + 2 * tbytes + (m->cis[0] ? 2 * cbytes : 0);
   *out = m;
   return FH_OK;
 }
@@ -3985,10 +4035,41 @@ int fh_bpm_forward(fh_bpm* m, void* u, const float* dn, int direction, int save)
   // last slice); IFFT_x (bpm.jl:244-260)
   if ((rc = bpm_xfft(m, f, nullptr, sgn))) return rc;
   if ((rc = bpm_ypass(m, f, m->thalf, direction))) return rc;
-  for (int k = 0; k < m->nslices; ++k) {
-    float2* ck = (save && k % m->every == 0) ? m->ckpt + (long)(k / m->every) * fsz : nullptr;
-    if ((rc = bpm_step(m, f, dn + (long)k * npix, sgn, nullptr, ck))) return rc;
-    if ((rc = bpm_ypass(m, f, k + 1 < m->nslices ? m->tfull : m->thalf, direction))) return rc;
+  // segment by segment (checkpoint spacing); with phase tables the side stream fills segment
+  // sg + 1's table (buffer (sg + 1) % 2, once segment sg - 1 has finished reading it) while the
+  // caller's stream runs segment sg, which waits for its own table.  Every side-stream operation is
+  // waited on by the caller's stream, so the work also records into one graph.
+  const bool tab = m->cis[0] != nullptr;
+  hipStream_t main = m->ctx->stream;
+  const int nseg = m->nck;
+  auto tables = [&](int sg) -> int {
+    const int k0 = sg * m->every, k1 = std::min(k0 + m->every, m->nslices);
+    if (sg >= 2) FH_HIP(hipStreamWaitEvent(m->side, m->ev_tused[sg % 2], 0));
+    m->ctx->stream = m->side;
+    const int r = bpm_cis(m, dn, k0, k1, sgn, m->cis[sg % 2]);
+    m->ctx->stream = main;
+    if (r) return r;
+    FH_HIP(hipEventRecord(m->ev_tab[sg % 2], m->side));
+    return FH_OK;
+  };
+  if (tab) {
+    FH_HIP(hipEventRecord(m->ev_fork, main));
+    FH_HIP(hipStreamWaitEvent(m->side, m->ev_fork, 0));
+    if ((rc = tables(0))) return rc;
+  }
+  for (int sg = 0; sg < nseg; ++sg) {
+    const int k0 = sg * m->every, k1 = std::min(k0 + m->every, m->nslices);
+    if (tab) {
+      if (sg + 1 < nseg && (rc = tables(sg + 1))) return rc;
+      FH_HIP(hipStreamWaitEvent(main, m->ev_tab[sg % 2], 0));
+    }
+    for (int k = k0; k < k1; ++k) {
+      float2* ck = (save && k == k0) ? m->ckpt + (long)sg * fsz : nullptr;
+      const double2* ci = tab ? m->cis[sg % 2] + (long)(k - k0) * fsz : nullptr;
+      if ((rc = bpm_step(m, f, dn + (long)k * npix, sgn, nullptr, ck, ci))) return rc;
+      if ((rc = bpm_ypass(m, f, k + 1 < m->nslices ? m->tfull : m->thalf, direction))) return rc;
+    }
+    if (tab) FH_HIP(hipEventRecord(m->ev_tused[sg % 2], main));
   }
   if ((rc = bpm_xifft(m, f))) return rc;
   m->saved_dir = save ? direction : 0;
@@ -4028,10 +4109,14 @@ int fh_bpm_backward(fh_bpm* m, void* du, const float* dn, float* ddn, int direct
   const int nseg = m->nck;
   FH_HIP(hipEventRecord(m->ev_fork, main));
   FH_HIP(hipStreamWaitEvent(m->side, m->ev_fork, 0));
+  // with phase tables, the side stream fills segment sg's table (buffer sg % 2, ordered like the
+  // segment buffer) before recomputing the segment from it; its gradient steps read the same table
   auto recompute = [&](int sg) -> int {
     const int k0 = sg * m->every, k1 = std::min(k0 + m->every, m->nslices);
     m->ctx->stream = m->side;
-    const int r = bpm_recompute(m, dn, k0, k1, sgn, direction, bufs[sg % 2]);
+    const double2* ci = m->cis[sg % 2];
+    int r = ci ? bpm_cis(m, dn, k0, k1, sgn, m->cis[sg % 2]) : FH_OK;
+    if (!r) r = bpm_recompute(m, dn, k0, k1, sgn, direction, bufs[sg % 2], ci);
     m->ctx->stream = main;
     if (r) return r;
     FH_HIP(hipEventRecord(m->ev_rec[sg % 2], m->side));
@@ -4048,8 +4133,9 @@ int fh_bpm_backward(fh_bpm* m, void* du, const float* dn, float* ddn, int direct
     FH_HIP(hipStreamWaitEvent(main, m->ev_rec[sg % 2], 0));
     const int k0 = sg * m->every, k1 = std::min(k0 + m->every, m->nslices);
     for (int k = k1 - 1; k >= k0; --k) {
+      const double2* ci = m->cis[sg % 2] ? m->cis[sg % 2] + (long)(k - k0) * fsz : nullptr;
       if ((rc = bpm_step_grad(m, f, dn + (long)k * npix, sgn, bufs[sg % 2] + (long)(k - k0) * fsz,
-                              ddn + (long)k * npix)))
+                              ddn + (long)k * npix, ci)))
         return rc;
       if ((rc = bpm_ypass(m, f, k > 0 ? m->tfull : m->thalf, -direction))) return rc;
     }

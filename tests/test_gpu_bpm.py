@@ -206,3 +206,39 @@ def test_bpm_any_grid_size_long_stack():
     eg = rel_l2(bpm.grad.cpu().numpy(), ddnref)
     print(f"{nx}x{ny} x {ns} slices: field {ev:.2e} adjoint {edu:.2e} ddn {eg:.2e}", flush=True)
     assert ev <= 1e-5 and edu <= 1e-5 and eg <= 1e-4
+
+
+def test_bpm_phase_tables_match_evaluated_phases():
+    """The slice phases come from per-segment FP64 tables when two segments' tables fit the
+    library's 512 MiB bound (512^2, 8 or 16 slices a segment) and are evaluated inside the slice
+    steps otherwise (70 or 80 slices a segment: 2 x 280 MiB and more).  Both paths meet the parity
+    bars against the FP64 oracle and each is bitwise reproducible across checkpoint spacings; the
+    two differ from each other only at their own rounding level (measured: fields 2.6e-7 apart,
+    3.5e-7 and 3.3e-7 from the oracle; the adjoint fields are bitwise equal)."""
+    n, ns, ds, lam, n0 = 512, 80, (0.5, 0.5), 1.0, 1.5
+    print(f"[bpm phase tables] {n}^2 x {ns} slices, every 8, 16 (tables) and 70, 80 (evaluated)", flush=True)
+    dn = smooth_dn(ns, n, 3.0, 0.01, 21)
+    u0 = c64(tilted_gaussian(n, 0.5, lam, 60.0, 2.0))
+    dv0 = c64(0.01 * random_field((n, n), 22))
+    vref, saved = O.bpm_forward(u0, dn.astype(np.float64), float(ns), n0, lam, ds)
+    duref, ddnref = O.bpm_backward(dv0, dn.astype(np.float64), float(ns), n0, lam, ds, saved)
+    out = {}
+    for every in (8, 16, 70, 80):
+        u = fo.ScalarField(to_dev(u0), ds, lam)
+        bpm = fo.AS_BPM(u, float(ns), n0, dn, trainable=True, buffered=True, checkpoint_every=every)
+        bpm.propagate_and_save_(u, fo.Forward)
+        v = to_host(u.electric)
+        du = fo.ScalarField(to_dev(dv0), ds, lam)
+        bpm.backpropagate_with_gradient_(du, None, fo.Forward)
+        torch.cuda.synchronize()
+        out[every] = (v, to_host(du.electric), bpm.grad.cpu().numpy())
+        ev, ed, eg = rel_l2(v, vref), rel_l2(out[every][1], duref), rel_l2(out[every][2], ddnref)
+        print(f"every {every}: field {ev:.2e} adjoint {ed:.2e} ddn {eg:.2e}", flush=True)
+        assert ev <= 1e-5 and ed <= 1e-5 and eg <= 1e-4
+    for a, b in ((8, 16), (70, 80)):
+        for x, y in zip(out[a], out[b]):
+            assert np.array_equal(x, y)
+    v1, d1, g1 = out[8]
+    v2, d2, g2 = out[80]
+    print(f"tables vs evaluated phases: field {rel_l2(v1, v2):.2e} adjoint {rel_l2(d1, d2):.2e} "
+          f"ddn {rel_l2(g1, g2):.2e}")

@@ -75,6 +75,52 @@ def test_asprop_headline_forward_adjoint():
     assert err_v <= FIELD_TOL and err_du <= FIELD_TOL
 
 
+def test_asprop_headline_adjoint_central_differences():
+    """The 8 x 2048^2 adjoint against central finite differences on 64 random pixels.
+
+    L(u) = 1/2 sum |P u - t|^2 is evaluated with the library's plain complex-FP64 propagator and
+    summed in FP64, so the differences see no FP32 rounding.  L is quadratic in each pixel's real
+    and imaginary parts, so the central difference (L(u + eps e_j) - L(u - eps e_j)) / (2 eps) is
+    exact for any step.  It must equal dL/d(Re u) + i dL/d(Im u) = P^H (P u - t): the fast
+    complex-FP32 adjoint applied to v - t, with v from the fast forward."""
+    print("[asprop headline FD] 8 x 2048^2, 64 pixels", flush=True)
+    n, ds, lam, z, batch = 2048, (0.5, 0.5), 0.8, 5000.0, 8
+    u0 = np.asarray(speckle_batch(batch, n, n, 0.5, 0.5, 150.0, 2000), np.complex64)
+    t0 = np.asarray(0.01 * random_field((batch, n, n), 77), np.complex64)
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    p = fo.ASProp(u, z)
+    r = p.propagate(u, fo.Forward)
+    r.electric.sub_(to_dev(t0))
+    p.backpropagate_(r, fo.Forward)
+    g = r.electric
+    rng = np.random.default_rng(64)
+    pix = [(int(rng.integers(batch)), int(rng.integers(n)), int(rng.integers(n))) for _ in range(64)]
+    # the perturbed losses: a one-field plan of the same geometry, complex FP64 propagations
+    one = fo.ScalarField(to_dev(u0[:1]), ds, lam)
+    p1 = fo.ASProp(one, z)
+    eps = 0.25
+    got, fd = [], []
+    for b, y, x in pix:
+        base = torch.from_numpy(u0[b:b + 1].astype(np.complex128)).cuda()
+        tb = torch.from_numpy(t0[b:b + 1].astype(np.complex128)).cuda()
+
+        def loss(d):
+            w = base.clone()
+            w[0, y, x] += d
+            p1.propagate_plain_fp64_(w, lam, fo.Forward)
+            return 0.5 * float(((w - tb).abs() ** 2).sum().item())
+
+        gre = (loss(eps) - loss(-eps)) / (2 * eps)
+        gim = (loss(1j * eps) - loss(-1j * eps)) / (2 * eps)
+        fd.append(gre + 1j * gim)
+        got.append(complex(g[b, y, x].item()))
+    got, fd = np.array(got), np.array(fd)
+    err = rel_l2(got, fd)
+    worst = float(np.max(np.abs(got - fd) / np.abs(fd)))
+    print(f"rel-L2 over 64 pixels {err:.3e}, worst pixel {worst:.3e}")
+    assert err <= GRAD_TOL
+
+
 @pytest.mark.parametrize("nx,ny,batch,ds,lam,z", [
     (16, 16, 1, (1.0, 1.0), 1.064, 30.0),
     (64, 32, 2, (1.0, 1.0), 1.064, 300.0),
