@@ -291,13 +291,16 @@ int fh_bpm_create_ex(fh_ctx* ctx, const fh_prop_desc* desc, double thickness, in
                      int checkpoint_every, fh_bpm** out);
 int fh_bpm_destroy(fh_bpm* bpm);
 
-/* Device memory owned by the BPM object (checkpoints, segment, tables), in bytes. */
+/* Device memory owned by the BPM object (checkpoints, two segment buffers, transfer-function
+ * tables and, when two segments' FP64 slice phases fit 512 MiB, the two phase tables), in bytes. */
 int fh_bpm_workspace_bytes(fh_bpm* bpm, size_t* bytes);
 
 /* propagate!(u, bpm, direction) (bpm.jl:244-260) in place; dn is float
  * [nslices][ny][nx] (the reference's dn[nx, ny, nslices]).  save != 0 keeps the
  * checkpoints that fh_bpm_backward needs for the dn gradient
- * (propagate_and_save!, bpm.jl:262-273). */
+ * (propagate_and_save!, bpm.jl:262-273).  Work the BPM runs on its own second stream (the
+ * next segment's slice phases; in fh_bpm_backward the recomputed segments) is ordered after
+ * the context's stream and joined back into it before the call returns its last launch. */
 int fh_bpm_forward(fh_bpm* bpm, void* u, const float* dn, int direction, int save);
 
 /* backpropagate!(du, bpm, direction; u_saved, dp) (bpm.jl:289-312): du <- adjoint(du)
