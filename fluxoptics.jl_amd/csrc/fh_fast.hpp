@@ -629,7 +629,9 @@ namespace fh {
 // remain: after the tile is written to LDS and after the columns are written back.  The thread
 // that writes a pair of tile values into LDS is the one that reads it back out for the store, so
 // the next field's tile may be written without a barrier in between.  The transfer function is
-// loaded per field (from L2) into the registers that then receive the next field's tile.
+// loaded per field (from L2) into the tile registers as soon as the tile is in LDS, so the loads
+// are in flight during the forward transform; after the multiply the same registers receive the
+// next field's tile.
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -818,6 +820,9 @@ k_fast_ywave(FastIO io, int batch, const float2* __restrict__ tabf, const float2
       ps[136 * k] = q[2 * k];
       ps[CS + 136 * k] = q[2 * k + 1];
     }
+    // the transfer function at the positions the forward transform leaves in registers, from L2,
+    // in flight during the forward transform (the tile registers are free once written to LDS)
+    ywave_loadh<RDF>(q, qt, b, qoff_d, qoff_m);
     __syncthreads();
     float2 v[V];
 #pragma unroll
@@ -828,8 +833,6 @@ k_fast_ywave(FastIO io, int batch, const float2* __restrict__ tabf, const float2
         v[m * RDF::R0 + r] = s[lds_pos(t) + c + (c >> 4)];
       }
     wave_fft<RDF, PF::off1, false>(v, s, tw, t);
-    // the transfer function at the positions the transform leaves in registers, from L2
-    ywave_loadh<RDF>(q, qt, b, qoff_d, qoff_m);
 #pragma unroll
     for (int k = 0; k < V; ++k) v[k] = qt.conj ? cmulc(v[k], q[k]) : cmul(v[k], q[k]);
     // the next field's tile (or this one again after the last) goes out after the inverse's
