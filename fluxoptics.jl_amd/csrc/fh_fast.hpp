@@ -38,6 +38,14 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, int voff, in
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, voff, soff, 0);
 }
 
+__device__ __forceinline__ float4 buf_load4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_store4(__amdgpu_buffer_rsrc_t r, int voff, int soff, float4 v) {
+  using V4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, voff, soff, 0);
+}
+
 // Pass geometry of the fast kernels (offsets in elements; fields are < 2 GiB).
 struct FastIO {
   const float2* in;
@@ -255,6 +263,94 @@ k_fast_fft(FastIO io, const float2* __restrict__ tab, TimeOp op) {
         if (INV) x = op(x, b, li, t + c);
         if (!OUT_HALF || (c < n / 2 && t + c < io.out_valid)) buf_store(rout, vout, c * kout, x);
       }
+  }
+}
+
+// Line FFT of whole 4-row groups whose blocked side (the input of an inverse pass, the output of
+// a forward one) moves through LDS: the workgroup's rows are one contiguous run of L*n elements in
+// the 4x4-blocked layout, read or written with 16-byte accesses, a wave covering 1 KB of whole
+// 128-byte lines per instruction (the direct form covers 32 bytes of each of 16 lines, the rest of
+// each line coming from the three other rows' waves).  Chunk k*L*P + tid of the run is elements
+// 2c, 2c+1: block (2c)/16 of the run, row (2c/4) % 4 of that block, columns 2c % 4 and + 1.  The
+// two 8-byte LDS accesses of a chunk stay separate instructions: merged into one ds_read2_b64 of
+// adjacent slots, the first value came back stale for a few lanes per launch (measured on MI355X,
+// forward pass, 0.15% of the chunks, varying from run to run), with or without an extra barrier.
+template <typename RDF, int L, bool INV>
+__global__ void __launch_bounds__(L * RDF::P)
+k_fast_fft_stg(FastIO io, const float2* __restrict__ tab) {
+  using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
+  constexpr int n = RD::n, V = RD::V, P = RD::P;
+  constexpr int RL = LastRadix<RD>::value;
+  constexpr int LS = LdsLine<n, L>::size;
+  constexpr int NCH = n / (2 * P);  // 16-byte chunks a thread
+  static_assert(L % 4 == 0, "whole 4-row groups");
+  extern __shared__ unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int l = tid / P, t = tid % P;
+  const int b = blockIdx.y;
+  float2* s = smem + l * LS;
+  // the chunk's row and column within the workgroup's run
+  int crow[NCH], ccol[NCH];
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) {
+    const int e = 2 * (k * L * P + tid);
+    const int blk = e >> 4;
+    crow[k] = 4 * (blk / (n / 4)) + ((e >> 2) & 3);
+    ccol[k] = 4 * (blk % (n / 4)) + (e & 3);
+  }
+  float2 v[V];
+  if (INV) {
+    const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
+    const int vin = (blockIdx.x * L * n + 2 * tid) * 8;
+    float4 q[NCH];
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) q[k] = buf_load4(rin, vin, k * L * P * 16);
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+      float2* d = smem + crow[k] * LS + lds_pos(ccol[k]);
+      d[0] = make_float2(q[k].x, q[k].y);
+      asm volatile("" ::: "memory");
+      d[1] = make_float2(q[k].z, q[k].w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+      for (int r = 0; r < RD::R0; ++r) v[m * RD::R0 + r] = s[lds_pos(t + P * m + r * (n / RD::R0))];
+  } else {
+    const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
+    const int vin = ((blockIdx.x * L + l) * io.in_l + t) * 8;
+#pragma unroll
+    for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+      for (int r = 0; r < RD::R0; ++r) v[m * RD::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RD::R0)) * 8);
+  }
+  fast_fft<float2, RD, INV>(v, s, t, tab);
+  if (INV) {
+    const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
+    const int vout = ((blockIdx.x * L + l) * io.out_l + t) * 8;
+#pragma unroll
+    for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) buf_store(rout, vout, (P * m + r * (n / RL)) * 8, v[m * RL + r]);
+  } else {
+    __syncthreads();  // every thread's last exchange read is done
+#pragma unroll
+    for (int m = 0; m < V / RL; ++m)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) s[lds_pos(t + P * m + r * (n / RL))] = v[m * RL + r];
+    __syncthreads();
+    const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
+    const int vout = (blockIdx.x * L * n + 2 * tid) * 8;
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+      const float2* d = smem + crow[k] * LS + lds_pos(ccol[k]);
+      const float2 a0 = d[0];
+      asm volatile("" ::: "memory");
+      const float2 a1 = d[1];
+      buf_store4(rout, vout, k * L * P * 16, make_float4(a0.x, a0.y, a1.x, a1.y));
+    }
   }
 }
 
@@ -727,13 +823,6 @@ __device__ __forceinline__ void wave_fft(float2* v, float2* s, const float2* tw,
   }
 }
 
-__device__ __forceinline__ float4 buf_load4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
-}
-__device__ __forceinline__ void buf_store4(__amdgpu_buffer_rsrc_t r, int voff, int soff, float4 v) {
-  using V4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, voff, soff, 0);
-}
 
 constexpr int kWaveCols = 4;   // columns (waves) of a tile
 constexpr int kWaveN = 2048;   // column length

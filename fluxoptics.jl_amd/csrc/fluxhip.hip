@@ -1494,12 +1494,14 @@ static int get_stage_tw(fh_ctx* c, int n, int R0, int R1, int R2, bool inv, cons
 
 // ---------------------------------------------------------------- fast c64 launches
 
+// x-pass variants: 0 = lanes over positions of a line, 1 = lanes over the workgroup's lines,
+// 2 = the blocked side staged through LDS (plain transforms of whole 4-row groups; otherwise 0)
 template <int LOGN, bool INV, bool IH, bool OH, typename Op, int L>
-static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes);
+static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar);
 
 template <int LOGN, bool INV, bool IH, bool OH, typename Op>
-static int fast_x_t(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
-  return fast_x_l<LOGN, INV, IH, OH, Op, Cfg<float, LOGN, false>::L>(c, io, batch, op, lanes);
+static int fast_x_t(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar) {
+  return fast_x_l<LOGN, INV, IH, OH, Op, Cfg<float, LOGN, false>::L>(c, io, batch, op, xvar);
 }
 
 // Lines per workgroup halve (down to one full wave per workgroup) while a launch would have
@@ -1514,18 +1516,18 @@ template <> struct FewRowsPlan<9> { using RD = Radices<8, 8, 8>; };
 template <> struct FewRowsPlan<10> { using RD = Radices<16, 8, 8>; };
 
 template <typename RDF, int L, bool INV, bool IH, bool OH, typename Op>
-static int fast_x_launch(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes);
+static int fast_x_launch(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar);
 
 template <int LOGN, bool INV, bool IH, bool OH, typename Op, int L>
-static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
+static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar) {
   using RDF = typename Cfg<float, LOGN, false>::RD;
   const bool few = (long)((io.nlines + L - 1) / L) * batch < kFillWGs;
   if constexpr (L >= 8 && (L / 2) * RDF::P >= 64) {
-    if (few) return fast_x_l<LOGN, INV, IH, OH, Op, L / 2>(c, io, batch, op, lanes);
+    if (few) return fast_x_l<LOGN, INV, IH, OH, Op, L / 2>(c, io, batch, op, xvar);
   }
   using FR = typename FewRowsPlan<LOGN>::RD;
   if constexpr (!std::is_void<FR>::value && L == 4) {
-    if (few) return fast_x_launch<FR, 4, INV, IH, OH, Op>(c, io, batch, op, lanes);
+    if (few) return fast_x_launch<FR, 4, INV, IH, OH, Op>(c, io, batch, op, xvar);
   }
   // Out-of-place passes (RS, Collins and CZT plans: natural rows <-> the scratch spectrum, or a
   // source read into a separate field) need no whole 4-row groups.  Rows of 4096 points then go
@@ -1533,17 +1535,31 @@ static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
   // between loading, transforming and storing; two 2-row workgroups share a CU and overlap.
   if constexpr (L == 4 && LOGN == 12) {
     if (static_cast<const void*>(io.in) != static_cast<const void*>(io.out))
-      return fast_x_launch<RDF, 2, INV, IH, OH, Op>(c, io, batch, op, lanes);
+      return fast_x_launch<RDF, 2, INV, IH, OH, Op>(c, io, batch, op, xvar);
   }
-  return fast_x_launch<RDF, L, INV, IH, OH, Op>(c, io, batch, op, lanes);
+  return fast_x_launch<RDF, L, INV, IH, OH, Op>(c, io, batch, op, xvar);
 }
 
 template <typename RDF, int L, bool INV, bool IH, bool OH, typename Op>
-static int fast_x_launch(fh_ctx* c, const FastIO& io, int batch, Op op, bool lanes) {
+static int fast_x_launch(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar) {
   using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
   const float2* tab = nullptr;
   int rc = get_stage_tw(c, RD::n, RD::R0, RD::R1, RD::R2, INV, &tab);
   if (rc) return rc;
+  if constexpr (std::is_same<Op, NoOp>::value && !IH && !OH && L % 4 == 0) {
+    // the blocked side (input of an inverse pass, output of a forward one) as one contiguous run
+    // of whole 4-row groups of n-point rows
+    const bool blk_side = INV ? (io.in_blk && !io.out_blk && io.in_l == RD::n) : (io.out_blk && !io.in_blk && io.out_l == RD::n);
+    if (xvar == 2 && blk_side && io.nlines % L == 0 && (INV ? io.out_valid : io.in_valid) >= RD::n) {
+      auto kern = k_fast_fft_stg<RDF, L, INV>;
+      const size_t lds = (size_t)L * LdsLine<RD::n, L>::size * sizeof(float2);
+      FH_HIP(ensure_lds(kern, lds));
+      hipLaunchKernelGGL(kern, dim3(io.nlines / L, batch), dim3(L * RDF::P), lds, c->stream, io, tab);
+      FH_HIP(hipGetLastError());
+      return FH_OK;
+    }
+  }
+  const bool lanes = xvar == 1;
   auto kern = lanes ? k_fast_fft<RDF, L, false, INV, IH, OH, Op, true> : k_fast_fft<RDF, L, false, INV, IH, OH, Op, false>;
   const size_t lds = (size_t)L * LdsLine<RD::n, L>::size * sizeof(float2);
   FH_HIP(ensure_lds(kern, lds));
@@ -1568,24 +1584,24 @@ static int fast_x_launch(fh_ctx* c, const FastIO& io, int batch, Op op, bool lan
 
 // forward x FFT; half_in: rows hold n/2 points (zero-padded RS input)
 template <typename Op>
-static int fast_xfwd(fh_ctx* c, int logn, bool half_in, const FastIO& io, int batch, Op op, bool lanes) {
-  if (half_in) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, true, false, Op>(c, io, batch, op, lanes))) }
+static int fast_xfwd(fh_ctx* c, int logn, bool half_in, const FastIO& io, int batch, Op op, int xvar) {
+  if (half_in) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, true, false, Op>(c, io, batch, op, xvar))) }
   if constexpr (std::is_same<Op, ChirpOp>::value) {
     return fail(FH_E_UNSUPP, "chirp prologue needs a padded plan");
   } else {
-    FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, false, false, Op>(c, io, batch, op, lanes)))
+    FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, false, false, false, Op>(c, io, batch, op, xvar)))
   }
 }
 
 // inverse x FFT; half_out: only n/2 points are stored (RS crop).  The chirp op only runs on
 // padded (Collins) plans, so its unpadded variants are not instantiated.
 template <typename Op>
-static int fast_xinv(fh_ctx* c, int logn, bool half_out, const FastIO& io, int batch, Op op, bool lanes) {
-  if (half_out) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, true, Op>(c, io, batch, op, lanes))) }
+static int fast_xinv(fh_ctx* c, int logn, bool half_out, const FastIO& io, int batch, Op op, int xvar) {
+  if (half_out) { FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, true, Op>(c, io, batch, op, xvar))) }
   if constexpr (std::is_same<Op, ChirpOp>::value) {
     return fail(FH_E_UNSUPP, "chirp epilogue needs a padded plan");
   } else {
-    FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, false, Op>(c, io, batch, op, lanes)))
+    FH_LOGN_SWITCH_INT(logn, (fast_x_t<LG, true, false, false, Op>(c, io, batch, op, xvar)))
   }
 }
 
@@ -1691,10 +1707,12 @@ static int fast_phase_grad(fh_ctx* c, int logn, const FastIO& io, int batch, con
   FH_LOGN_SWITCH_INT(logn, (fast_phase_grad_t<LG>(c, io, batch, phi, saved, dphi, nx, s, acc)))
 }
 
-// x-pass variants: 0 = lanes over positions of a line, 1 = lanes over the workgroup's lines
-static const char* const kXVariants[] = {"lanes over x", "lanes over rows"};
-constexpr int kNumXVariants = 2;
-static int default_x_variant(int logn) { (void)logn; return 0; }
+// x-pass variants (fast_x_l): names for the autotuner's report; the staged form is the default for
+// rows of 2048 points (8 x 2048^2: 0.086 / 0.088 ms against 0.096-0.103 / 0.101 ms for the
+// forward / inverse pass)
+static const char* const kXVariants[] = {"lanes over x", "lanes over rows", "blocked side via LDS"};
+constexpr int kNumXVariants = 3;
+static int default_x_variant(int logn) { return logn == 11 ? 2 : 0; }
 
 // y-pass variants (radices, columns per workgroup) for the autotuner
 using FastConvLaunch = int (*)(fh_ctx*, const FastIO&, int, QuadMul, bool);
@@ -1969,7 +1987,7 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
     io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = midf * 8;
     io.in_valid = p->nx;
     const int xv = p->xf_var >= 0 ? p->xf_var : default_x_variant(p->lx);
-    if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop, xv == 1))) return rc;
+    if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop, xv))) return rc;
   }
   if (only == 0) return FH_OK;
   QuadMul q{p->tab, p->d_tabidx, p->tabsize, nxl, 1 << p->ly, p->qy, direction < 0,
@@ -1997,7 +2015,7 @@ static int run_xifft_op(fh_plan* p, float2* u, XOp xop) {
   io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
   io.out_valid = p->nx;
   const int xv = p->xi_var >= 0 ? p->xi_var : default_x_variant(p->lx);
-  return fast_xinv(p->ctx, p->lx, rs, io, p->batch, xop, xv == 1);
+  return fast_xinv(p->ctx, p->lx, rs, io, p->batch, xop, xv);
 }
 
 // CollinsProp's chirps for one direction (collins_integral.jl:155-189, reversed for

@@ -63,3 +63,29 @@ def test_rsprop_1024_every_variant():
     ref = {s: O.rsprop(u0, ds, lam, z, s, "2n-1") for s in (1, -1)}
     _sweep(p, u, ref, "rs")
 
+
+
+def test_staged_x_passes_match_direct_bitwise():
+    """The x passes whose blocked side moves through LDS (variant 2, the default at 2048) against
+    the direct passes (variant 0) on the same inputs, bit for bit, repeated: the staging keeps each
+    chunk's two 8-byte LDS accesses separate (merged into one instruction of adjacent slots, a few
+    values per launch came back stale), so every repetition must agree exactly."""
+    n, ds, lam, z, batch = 2048, (0.5, 0.5), 0.8, 3000.0, 8
+    u0 = c64(speckle_batch(batch, n, n, ds[0], ds[1], 150.0, 47))
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    p = fo.ASProp(u, z)
+    plan = p.plan(u)
+    ptr = lambda f: ctypes.c_void_p(f.electric.data_ptr())
+    for pid in (0, 2):
+        outs = {}
+        for k in (0, 2, 0, 2):
+            _lib.call("fh_plan_set_variant", plan, pid, k)
+            f = fo.ScalarField(to_dev(u0), ds, lam)
+            _lib.call("fh_propagate_pass", plan, ptr(f), 1, pid)
+            out = to_host(f.electric)
+            if k in outs:
+                assert np.array_equal(out, outs[k]), (pid, k)
+            outs[k] = out
+        _lib.call("fh_plan_set_variant", plan, pid, -1)
+        print(f"[staged x pass] pass {pid}: variants 0 and 2 bitwise equal", flush=True)
+        assert np.array_equal(outs[0], outs[2]), pid
