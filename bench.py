@@ -74,8 +74,8 @@ def read_traffic(pass_name):
 
 
 KERNELS_FILE = os.path.join(ROOT, "profiles", "asprop_2048x2048_b8_kernels.txt")
-PASS_KERNELS = {"x_fft": "k_fast_fft<Radices<16, 16, 8>, 4, false, false,",
-                "y_fft_mul_ifft": "k_fast_ypipe<", "x_ifft": "k_fast_fft<Radices<16, 16, 8>, 4, false, true,"}
+PASS_KERNELS = {"x_fft": "k_fast_fft_stg<Radices<16, 16, 8>, 4, false>",
+                "y_fft_mul_ifft": "k_fast_ypipe<", "x_ifft": "k_fast_fft_stg<Radices<16, 16, 8>, 4, true>"}
 
 
 def read_profiled_ms(pass_name):
