@@ -1707,12 +1707,13 @@ static int fast_phase_grad(fh_ctx* c, int logn, const FastIO& io, int batch, con
   FH_LOGN_SWITCH_INT(logn, (fast_phase_grad_t<LG>(c, io, batch, phi, saved, dphi, nx, s, acc)))
 }
 
-// x-pass variants (fast_x_l): names for the autotuner's report; the staged form is the default for
-// rows of 2048 points (8 x 2048^2: 0.086 / 0.088 ms against 0.096-0.103 / 0.101 ms for the
-// forward / inverse pass)
+// x-pass variants (fast_x_l): names for the autotuner's report.  The staged form is the default for
+// rows of 2048 and 4096 points (forward + adjoint step: 8 x 2048^2 0.607 ms against 0.656 ms,
+// 2 x 4096^2 0.885 against 0.977 ms) and not for shorter rows (32 x 512^2: 0.160 against 0.158 ms,
+// 16 x 1024^2: 0.325 against 0.314 ms).
 static const char* const kXVariants[] = {"lanes over x", "lanes over rows", "blocked side via LDS"};
 constexpr int kNumXVariants = 3;
-static int default_x_variant(int logn) { return logn == 11 ? 2 : 0; }
+static int default_x_variant(int logn) { return logn >= 11 ? 2 : 0; }
 
 // y-pass variants (radices, columns per workgroup) for the autotuner
 using FastConvLaunch = int (*)(fh_ctx*, const FastIO&, int, QuadMul, bool);
