@@ -275,9 +275,11 @@ k_fast_fft(FastIO io, const float2* __restrict__ tab, TimeOp op) {
 // two 8-byte LDS accesses of a chunk stay separate instructions: merged into one ds_read2_b64 of
 // adjacent slots, the first value came back stale for a few lanes per launch (measured on MI355X,
 // forward pass, 0.15% of the chunks, varying from run to run), with or without an extra barrier.
-template <typename RDF, int L, bool INV>
+// The natural side takes the same time-side op as k_fast_fft (prologue of a forward transform,
+// epilogue of an inverse one).
+template <typename RDF, int L, bool INV, typename TimeOp = NoOp>
 __global__ void __launch_bounds__(L * RDF::P)
-k_fast_fft_stg(FastIO io, const float2* __restrict__ tab) {
+k_fast_fft_stg(FastIO io, const float2* __restrict__ tab, TimeOp op) {
   using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
   constexpr int n = RD::n, V = RD::V, P = RD::P;
   constexpr int RL = LastRadix<RD>::value;
@@ -320,20 +322,57 @@ k_fast_fft_stg(FastIO io, const float2* __restrict__ tab) {
       for (int r = 0; r < RD::R0; ++r) v[m * RD::R0 + r] = s[lds_pos(t + P * m + r * (n / RD::R0))];
   } else {
     const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
-    const int vin = ((blockIdx.x * L + l) * io.in_l + t) * 8;
+    const int li = blockIdx.x * L + l;
+    const int vin = (li * io.in_l + t) * 8;
 #pragma unroll
     for (int m = 0; m < V / RD::R0; ++m)
 #pragma unroll
       for (int r = 0; r < RD::R0; ++r) v[m * RD::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RD::R0)) * 8);
+    if constexpr (TwoPhaseOp<TimeOp>::value) {
+      // as in k_fast_fft: the factor loads first, then the saved input, then the multiply
+      const auto rf = make_rsrc(op.cis, io.in_bytes);
+      float2 pre[V];
+#pragma unroll
+      for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+        for (int r = 0; r < RD::R0; ++r) pre[m * RD::R0 + r] = buf_load(rf, vin, (P * m + r * (n / RD::R0)) * 8);
+      if (op.saved) {
+        const auto rsv = make_rsrc(op.saved + b * op.fstride, io.in_bytes);
+#pragma unroll
+        for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+          for (int r = 0; r < RD::R0; ++r) {
+            const int c = P * m + r * (n / RD::R0);
+            if (t + c < op.nx) buf_store(rsv, vin, c * 8, v[m * RD::R0 + r]);
+          }
+      }
+#pragma unroll
+      for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+        for (int r = 0; r < RD::R0; ++r) {
+          const int c = P * m + r * (n / RD::R0);
+          if (t + c < op.nx) v[m * RD::R0 + r] = cmul(v[m * RD::R0 + r], pre[m * RD::R0 + r]);
+        }
+    } else {
+#pragma unroll
+      for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+        for (int r = 0; r < RD::R0; ++r)
+          v[m * RD::R0 + r] = op(v[m * RD::R0 + r], b, li, t + P * m + r * (n / RD::R0));
+    }
   }
   fast_fft<float2, RD, INV>(v, s, t, tab);
   if (INV) {
     const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
-    const int vout = ((blockIdx.x * L + l) * io.out_l + t) * 8;
+    const int li = blockIdx.x * L + l;
+    const int vout = (li * io.out_l + t) * 8;
 #pragma unroll
     for (int m = 0; m < V / RL; ++m)
 #pragma unroll
-      for (int r = 0; r < RL; ++r) buf_store(rout, vout, (P * m + r * (n / RL)) * 8, v[m * RL + r]);
+      for (int r = 0; r < RL; ++r) {
+        const int c = P * m + r * (n / RL);
+        buf_store(rout, vout, c * 8, op(v[m * RL + r], b, li, t + c));
+      }
   } else {
     __syncthreads();  // every thread's last exchange read is done
 #pragma unroll

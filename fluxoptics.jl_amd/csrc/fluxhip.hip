@@ -1495,7 +1495,7 @@ static int get_stage_tw(fh_ctx* c, int n, int R0, int R1, int R2, bool inv, cons
 // ---------------------------------------------------------------- fast c64 launches
 
 // x-pass variants: 0 = lanes over positions of a line, 1 = lanes over the workgroup's lines,
-// 2 = the blocked side staged through LDS (plain transforms of whole 4-row groups; otherwise 0)
+// 2 = the blocked side staged through LDS (whole 4-row groups of unpadded rows; otherwise 0)
 template <int LOGN, bool INV, bool IH, bool OH, typename Op, int L>
 static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar);
 
@@ -1546,15 +1546,15 @@ static int fast_x_launch(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar
   const float2* tab = nullptr;
   int rc = get_stage_tw(c, RD::n, RD::R0, RD::R1, RD::R2, INV, &tab);
   if (rc) return rc;
-  if constexpr (std::is_same<Op, NoOp>::value && !IH && !OH && L % 4 == 0) {
+  if constexpr (!IH && !OH && L % 4 == 0) {
     // the blocked side (input of an inverse pass, output of a forward one) as one contiguous run
     // of whole 4-row groups of n-point rows
     const bool blk_side = INV ? (io.in_blk && !io.out_blk && io.in_l == RD::n) : (io.out_blk && !io.in_blk && io.out_l == RD::n);
     if (xvar == 2 && blk_side && io.nlines % L == 0 && (INV ? io.out_valid : io.in_valid) >= RD::n) {
-      auto kern = k_fast_fft_stg<RDF, L, INV>;
+      auto kern = k_fast_fft_stg<RDF, L, INV, Op>;
       const size_t lds = (size_t)L * LdsLine<RD::n, L>::size * sizeof(float2);
       FH_HIP(ensure_lds(kern, lds));
-      hipLaunchKernelGGL(kern, dim3(io.nlines / L, batch), dim3(L * RDF::P), lds, c->stream, io, tab);
+      hipLaunchKernelGGL(kern, dim3(io.nlines / L, batch), dim3(L * RDF::P), lds, c->stream, io, tab, op);
       FH_HIP(hipGetLastError());
       return FH_OK;
     }

@@ -89,3 +89,31 @@ def test_staged_x_passes_match_direct_bitwise():
         _lib.call("fh_plan_set_variant", plan, pid, -1)
         print(f"[staged x pass] pass {pid}: variants 0 and 2 bitwise equal", flush=True)
         assert np.array_equal(outs[0], outs[2]), pid
+
+
+def test_staged_x_pass_with_phase_prologue_bitwise():
+    """A Phase -> ASProp -> SquaredIntensityDifference system at 2048^2 (the fused block: the first
+    x pass carries the Phase as its prologue and stores the saved input) with the plan's x passes
+    direct (variant 0) and staged (variant 2): loss and mask gradient bit for bit."""
+    import math
+    from fluxoptics import inputs
+    n, ds, lam, z = 2048, (0.5, 0.5), 0.8, 3000.0
+    u0 = c64(speckle_batch(2, n, n, ds[0], ds[1], 150.0, 49))
+    tgt = (np.abs(c64(speckle_batch(2, n, n, ds[0], ds[1], 150.0, 50))) ** 2).astype(np.float32)
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    ph = fo.Phase(u, inputs.uniform(51, (n, n), 0, 2 * math.pi).astype(np.float32), trainable=True)
+    p = fo.ASProp(u, z)
+    system = fo.ScalarSource(u) | ph | p
+    metric = fo.SquaredIntensityDifference((u, tgt))
+    plan = p.plan(u)
+    res = {}
+    for k in (0, 2):
+        for pid in (0, 2):
+            _lib.call("fh_plan_set_variant", plan, pid, k)
+        val, grads = system.withgradient(metric)
+        res[k] = (float(val), grads[ph].cpu().numpy().copy())
+    for pid in (0, 2):
+        _lib.call("fh_plan_set_variant", plan, pid, -1)
+    print(f"[staged x pass, Phase prologue] loss {res[2][0]:.9e}", flush=True)
+    assert res[0][0] == res[2][0]
+    assert np.array_equal(res[0][1], res[2][1])
