@@ -990,5 +990,120 @@ k_fast_ywave(FastIO io, int batch, const float2* __restrict__ tabf, const float2
   }
 }
 
-}  // namespace fh
+// ---------------------------------------------------------------- staged-tile pipelined y pass
+// The pipelined pass's eight-wave transform (L = 4 columns, one workgroup per CU) with the
+// column tile moved between HBM and LDS as whole 128-byte block rows of the 4x4-blocked layout
+// (16-byte loads and stores, a wave touching 8 lines per instruction) instead of 8-byte loads of
+// each thread's first-stage positions (4 lines per instruction).  Per field: the tile registers
+// go to the exchange area in column order, the next field's tile is loaded into the same
+// registers (in flight during both transforms), the forward transform starts from LDS, and the
+// inverse's outputs return through LDS to the tile registers' owners for the store.  The thread
+// that stores a tile pair is the one that writes the next field's pair to the same slots, so no
+// barrier separates them.  ONE_TAB: the shared transfer function sits in LDS after the exchange
+// area (conjugated there once for the adjoint); otherwise each field's values are loaded into
+// registers when the field starts and used after the forward transform.
+template <int NQ>
+__device__ __forceinline__ void ystg_tile_load(float4* q, const FastIO& io, int b, int voff, int kstep) {
+  const auto r = make_rsrc(io.in + b * io.in_f, io.in_bytes);
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) q[k] = buf_load4(r, voff, k * kstep);
+}
 
+template <typename RDF, bool ONE_TAB>
+__global__ void __launch_bounds__(4 * RDF::P)
+k_fast_ystg(FastIO io, int batch, const float2* __restrict__ tabf, const float2* __restrict__ tabi, QuadTab qt) {
+  constexpr int L = 4;  // one column of 4x4 blocks
+  constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
+  using RI = typename RDF::Rev;
+  constexpr int RE = LastRadix<RI>::value;
+  constexpr int LS = LdsLine<n, L>::size;
+  constexpr int NQ = n / (2 * P);          // 16-byte tile pairs a thread
+  constexpr int KS = 2 * P + 2 * P / 16;   // lds_pos step between a thread's pairs (2P rows)
+  constexpr int LP = L * P;
+  static_assert(P % 16 == 0 && (n / RDF::R0) % 16 == 0 && (n / RE) % 16 == 0, "slots 16 apart");
+  extern __shared__ unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  const int T = threadIdx.x;
+  int l, t;
+  thread_map<L, P, true>(T, l, t);
+  const int li = blockIdx.x * L + l;
+  float2* s = smem + l * LS;
+  // tile pair k of the thread: block row (T >> 3) + k P / 2, elements 2 (T & 7) and 2 (T & 7) + 1,
+  // i.e. row 4 (T >> 3) + ((T & 7) >> 1) + 2 P k of columns 2 (T & 1) and 2 (T & 1) + 1
+  const int pw = T & 7;
+  float2* ps = smem + 2 * (pw & 1) * LS + lds_pos(4 * (T >> 3) + (pw >> 1));
+  const int vin = ((((T >> 3) * (io.in_e >> 2)) + (int)blockIdx.x) * 16 + 2 * pw) * 8;
+  const int kin = io.in_e * (2 * P) * 8;
+  const int vout = ((((T >> 3) * (io.out_e >> 2)) + (int)blockIdx.x) * 16 + 2 * pw) * 8;
+  const int kout = io.out_e * (2 * P) * 8;
+  const int qxi = qt.full ? li : (li <= qt.nx - li ? li : qt.nx - li);
+  const int qoff_d = (qxi * qt.qcol + t) * 8;
+  const int qoff_m = (qxi * qt.qcol + qt.moff + P - t) * 8;
+  PlanBases<RDF> pf;
+  PlanBases<RI> pi;
+  pf.load(tabf, t);
+  pi.load(tabi, t);
+  float4 q[NQ];
+  ystg_tile_load<NQ>(q, io, 0, vin, kin);
+  float2* hs = smem + L * LS + T;
+  if constexpr (ONE_TAB) {
+    float2 ha[V];
+    ywave_loadh<RDF>(ha, qt, 0, qoff_d, qoff_m);
+#pragma unroll
+    for (int k = 0; k < V; ++k) hs[k * LP] = qt.conj ? make_float2(ha[k].x, -ha[k].y) : ha[k];
+  }
+  const float hsg = qt.conj ? -1.f : 1.f;
+  // nothing outstanding at the loop head: inside the loop the only wait is the one before the
+  // stores (a wait for a load issued before pending stores would drain the stores as well)
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  for (int b = 0; b < batch; ++b) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      ps[KS * k] = make_float2(q[k].x, q[k].y);
+      ps[LS + KS * k] = make_float2(q[k].z, q[k].w);
+    }
+    asm volatile("" ::: "memory");
+    float2 h[ONE_TAB ? 1 : V];
+    if constexpr (!ONE_TAB) ywave_loadh<RDF>(h, qt, b, qoff_d, qoff_m);
+    // the next field's tile (this one again after the last), in flight during both transforms
+    ystg_tile_load<NQ>(q, io, b + 1 < batch ? b + 1 : b, vin, kin);
+    __syncthreads();
+    float2 v[V];
+#pragma unroll
+    for (int m = 0; m < V / RDF::R0; ++m)  // the first stage's input positions
+#pragma unroll
+      for (int r = 0; r < RDF::R0; ++r) {
+        const int c = P * m + r * (n / RDF::R0);
+        v[m * RDF::R0 + r] = s[lds_pos(t) + c + (c >> 4)];
+      }
+    fast_fft_b<RDF, false>(v, s, t, pf);
+    if constexpr (ONE_TAB) {
+#pragma unroll
+      for (int k = 0; k < V; ++k) v[k] = cmul(v[k], hs[k * LP]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < V; ++k) v[k] = cmul(v[k], make_float2(h[k].x, hsg * h[k].y));
+    }
+    fast_fft_b<RI, true>(v, s, t, pi);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < V / RE; ++m)  // the last stage's output positions
+#pragma unroll
+      for (int r = 0; r < RE; ++r) {
+        const int c = P * m + r * (n / RE);
+        s[lds_pos(t) + c + (c >> 4)] = v[m * RE + r];
+      }
+    // the prefetch has had both transforms to arrive; the previous field's stores are the only
+    // other outstanding operations
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    __syncthreads();
+    const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const float2 a = ps[KS * k], c = ps[LS + KS * k];
+      buf_store4(rout, vout, k * kout, make_float4(a.x, a.y, c.x, c.y));
+    }
+  }
+}
+
+}  // namespace fh

@@ -1816,6 +1816,35 @@ static int fast_wave_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool 
   return FH_OK;
 }
 
+// Staged-tile pipelined y pass (whole 4-column tiles of the blocked layout on both sides, full
+// lines); otherwise the LDS-table pipelined pass.
+template <int A, int B, int C_>
+static int fast_ystg_v(fh_ctx* c, const FastIO& io, int batch, QuadMul op, bool half) {
+  using RDF = Radices<A, B, C_>;
+  using RI = typename RDF::Rev;
+  constexpr int n = RDF::n, P = RDF::P, NQ = n / (2 * P);
+  if (half || !io.in_blk || !io.out_blk || io.nlines % 4 || io.in_e % 4 || io.out_e % 4 ||
+      io.in_valid < n || io.out_valid < n ||
+      (long)io.in_e * 2 * P * 8 * NQ > 0x7fffffffL || (long)io.out_e * 2 * P * 8 * NQ > 0x7fffffffL)
+    return fast_pipe_v<A, B, C_, 4, true>(c, io, batch, op, half);
+  const float2 *tf = nullptr, *ti = nullptr;
+  int rc;
+  if ((rc = get_stage_tw(c, RDF::n, RDF::R0, RDF::R1, RDF::R2, false, &tf)) ||
+      (rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &ti)))
+    return rc;
+  QuadTab qt{op.tab, op.tabidx, op.tabsize, op.nx, op.qcol, op.conj, op.one_tab, op.full,
+             op.full ? op.ny / 2 : 0};
+  const size_t lds0 = (size_t)4 * LdsLine<n, 4>::size * sizeof(float2);
+  auto launch = [&](auto kern, size_t lds) -> int {
+    FH_HIP(ensure_lds(kern, lds));
+    hipLaunchKernelGGL(kern, dim3(io.nlines / 4), dim3(4 * P), lds, c->stream, io, batch, tf, ti, qt);
+    FH_HIP(hipGetLastError());
+    return FH_OK;
+  };
+  return qt.one_tab ? launch(k_fast_ystg<RDF, true>, lds0 + (size_t)4 * P * RDF::V * sizeof(float2))
+                    : launch(k_fast_ystg<RDF, false>, lds0);
+}
+
 // Variant lists per length; default_conv_variant picks the fastest measured on MI355X.
 static const FastConvVariant kFast9[] = {{"8x8x8 L4 f1", fast_conv_v<8, 8, 8, 4, 1>},
                                          {"8x8x8 L4", fast_conv_v<8, 8, 8, 4>},
@@ -1830,7 +1859,8 @@ static const FastConvVariant kFast11[] = {{"8x16x16 L4 f1", fast_conv_v<8, 16, 1
                                           {"16x16x8 L4 f1", fast_conv_v<16, 16, 8, 4, 1>},
                                           {"16x16x8 L4 f2", fast_conv_v<16, 16, 8, 4, 2>},
                                           {"pipe 16x16x8 L4 hlds", fast_pipe_v<16, 16, 8, 4, true>},
-                                          {"wave 8x8x32 L4", fast_wave_v}};
+                                          {"wave 8x8x32 L4", fast_wave_v},
+                                          {"stg 16x16x8 L4", fast_ystg_v<16, 16, 8>}};
 static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16, 16, 4, 1>},
                                           {"16x16x16 L4", fast_conv_v<16, 16, 16, 4>},
                                           {"16x16x16 L2 f1", fast_conv_v<16, 16, 16, 2, 1>},
@@ -1838,6 +1868,7 @@ static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16,
 
 // default variant index for a y length and whether the batch shares one transfer function
 static int default_conv_variant(int logn, bool one_tab, bool half) {
+  if (one_tab && logn == 11 && !half) return 7;  // staged-tile pipe 16x16x8 L4, table in LDS
   if (one_tab && logn == 11) return 5;           // pipe 16x16x8 L4, transfer function in LDS
   if (logn == 11 && !half) return 6;             // wave per column (a table per field)
   if (logn == 11) return 3;                      // 16x16x8 L4 f1 (RS, a table per field)
@@ -1846,17 +1877,23 @@ static int default_conv_variant(int logn, bool one_tab, bool half) {
   return 0;
 }
 
+template <size_t N>
+static const FastConvVariant* variant_list(const FastConvVariant (&v)[N], int* count) {
+  *count = (int)N;
+  return v;
+}
+
 static const FastConvVariant* fast_conv_variants(int logn, int* count) {
   switch (logn) {
-    case 4: *count = 1; return kFast4;
-    case 5: *count = 1; return kFast5;
-    case 6: *count = 1; return kFast6;
-    case 7: *count = 1; return kFast7;
-    case 8: *count = 2; return kFast8;
-    case 9: *count = 3; return kFast9;
-    case 10: *count = 4; return kFast10;
-    case 11: *count = 7; return kFast11;
-    case 12: *count = 4; return kFast12;
+    case 4: return variant_list(kFast4, count);
+    case 5: return variant_list(kFast5, count);
+    case 6: return variant_list(kFast6, count);
+    case 7: return variant_list(kFast7, count);
+    case 8: return variant_list(kFast8, count);
+    case 9: return variant_list(kFast9, count);
+    case 10: return variant_list(kFast10, count);
+    case 11: return variant_list(kFast11, count);
+    case 12: return variant_list(kFast12, count);
     default: *count = 0; return nullptr;
   }
 }
