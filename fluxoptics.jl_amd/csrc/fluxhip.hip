@@ -1246,6 +1246,10 @@ __global__ void k_mul_full_d(double2* __restrict__ u, const double2* __restrict_
 // summed in registers, in field order, and written once; the phase factors are taken once
 // per position for the whole batch and the next field's line is loaded while this one is
 // transformed.  Rows of RS plans hold 2nx points of which the first nx are kept.
+// STG: the workgroup's 4-row groups of the blocked spectrum arrive as whole 128-byte lines (16-byte
+// loads of its contiguous run, as in the staged x passes) and go to the exchange areas before the
+// transform.  store == 0: du is formed for the gradient but not written (a caller that does not need
+// the cotangent at the Phase's input).
 // Where they fit beside the exchange areas (160 KB per CU), the phase factors are taken once per
 // position and kept in LDS, one slot per (value, thread), read back only by the thread that wrote
 // them (no barrier): that frees their registers and the per-field sincos; otherwise the mask
@@ -1257,13 +1261,16 @@ template <int n, int L, int P, int V> struct PhaseGradLds {
   static constexpr size_t bytes = exch + (in_lds ? cis : 0);
 };
 
-template <typename RDF, int L, bool GRAD>
+template <typename RDF, int L, bool GRAD, bool STG = false>
 __global__ void __launch_bounds__(L * RDF::P)
 k_fast_phase_grad(FastIO io, int batch, const float2* __restrict__ tab, const float* __restrict__ phi,
-                  const float2* __restrict__ saved, float* __restrict__ dphi, int nx, float s, int acc) {
+                  const float2* __restrict__ saved, float* __restrict__ dphi, int nx, float s, int acc, int store) {
   using RI = typename RDF::Rev;
   constexpr int n = RI::n, V = RI::V, P = RI::P;
   constexpr int RL = LastRadix<RI>::value;
+  constexpr int LS = LdsLine<n, L>::size;
+  constexpr int NCH = n / (2 * P);  // 16-byte chunks a thread (STG)
+  static_assert(!STG || L % 4 == 0, "whole 4-row groups");
   extern __shared__ unsigned char smem_raw[];
   float2* smem = reinterpret_cast<float2*>(smem_raw);
   int l, t;
@@ -1297,19 +1304,41 @@ k_fast_phase_grad(FastIO io, int batch, const float2* __restrict__ tab, const fl
   // the next field's spectrum, and this field's saved input, are in flight while this field
   // is transformed; the load past the last field re-reads it (unconditional, so that no load
   // sits under a branch)
-  float2 nxt[V];
+  float2 nxt[STG ? 1 : V];
+  float4 q[STG ? NCH : 1];
+  const int vstg = (blockIdx.x * L * n + 2 * (int)threadIdx.x) * 8;  // STG: the thread's first chunk
   const auto load_spec = [&](int b) {
     const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
+    if constexpr (STG) {
 #pragma unroll
-    for (int m = 0; m < V / RI::R0; ++m)
+      for (int k = 0; k < NCH; ++k) q[k] = buf_load4(rin, vstg, k * L * P * 16);
+    } else {
 #pragma unroll
-      for (int r = 0; r < RI::R0; ++r) nxt[m * RI::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RI::R0)) * kin);
+      for (int m = 0; m < V / RI::R0; ++m)
+#pragma unroll
+        for (int r = 0; r < RI::R0; ++r) nxt[m * RI::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RI::R0)) * kin);
+    }
   };
   load_spec(0);
   for (int b = 0; b < batch; ++b) {
     float2 v[V], sv[V];
+    if constexpr (STG) {
+      // chunk k holds elements 2 (k L P + tid) and the next of the run: block row, then block,
+      // then row and column within the block (the previous field's last exchange is read first)
+      __syncthreads();
 #pragma unroll
-    for (int k = 0; k < V; ++k) v[k] = nxt[k];
+      for (int k = 0; k < NCH; ++k) {
+        const int e = 2 * (k * L * P + (int)threadIdx.x);
+        const int blk = e >> 4;
+        float2* d = smem + (4 * (blk / (n / 4)) + ((e >> 2) & 3)) * LS + lds_pos(4 * (blk % (n / 4)) + (e & 3));
+        d[0] = make_float2(q[k].x, q[k].y);
+        asm volatile("" ::: "memory");  // two accesses, not one instruction over adjacent slots
+        d[1] = make_float2(q[k].z, q[k].w);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < V; ++k) v[k] = nxt[k];
+    }
     if constexpr (GRAD) {
       const auto rsv = make_rsrc(saved + b * io.out_f, io.out_bytes);
 #pragma unroll
@@ -1321,6 +1350,13 @@ k_fast_phase_grad(FastIO io, int batch, const float2* __restrict__ tab, const fl
         }
     }
     load_spec(b + 1 < batch ? b + 1 : b);
+    if constexpr (STG) {
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < V / RI::R0; ++m)
+#pragma unroll
+        for (int r = 0; r < RI::R0; ++r) v[m * RI::R0 + r] = sl[lds_pos(t + P * m + r * (n / RI::R0))];
+    }
     fast_fft<float2, RI, true>(v, sl, t, tab);
     // loads and stores share one counter: wait while only old operations are outstanding
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
@@ -1342,7 +1378,7 @@ k_fast_phase_grad(FastIO io, int batch, const float2* __restrict__ tab, const fl
             }
             const float2 du = cmul(v[m * RL + r], f);
             if constexpr (GRAD) g[m * RL + r] += cmulc(du, sv[m * RL + r]).y;
-            buf_store(rout, vout, c * kout, du);
+            if (!GRAD || store) buf_store(rout, vout, c * kout, du);
           }
         }
     }
@@ -1648,18 +1684,18 @@ static int fast_sqint(fh_ctx* c, int logn, bool half, const FastIO& io, int batc
   FH_LOGN_SWITCH_INT(logn, (fast_sqint_t<LG, false>(c, io, batch, I, tstride, nx, dc, part, nparts)))
 }
 
-template <typename RDF, int L>
+template <typename RDF, int L, bool STG = false>
 static int fast_phase_grad_launch(fh_ctx* c, const FastIO& io, int batch, const float* phi, const float2* saved,
-                                  float* dphi, int nx, float s, int acc) {
+                                  float* dphi, int nx, float s, int acc, int store) {
   using RI = typename RDF::Rev;
   const float2* tab = nullptr;
   int rc = get_stage_tw(c, RI::n, RI::R0, RI::R1, RI::R2, true, &tab);
   if (rc) return rc;
-  auto kern = dphi ? k_fast_phase_grad<RDF, L, true> : k_fast_phase_grad<RDF, L, false>;
+  auto kern = dphi ? k_fast_phase_grad<RDF, L, true, STG> : k_fast_phase_grad<RDF, L, false, STG>;
   const size_t lds = PhaseGradLds<RI::n, L, RI::P, RI::V>::bytes;
   FH_HIP(ensure_lds(kern, lds));
   hipLaunchKernelGGL(kern, dim3((io.nlines + L - 1) / L), dim3(L * RDF::P), lds, c->stream, io, batch, tab, phi,
-                     saved, dphi, nx, s, acc);
+                     saved, dphi, nx, s, acc, store);
   FH_HIP(hipGetLastError());
   return FH_OK;
 }
@@ -1667,15 +1703,21 @@ static int fast_phase_grad_launch(fh_ctx* c, const FastIO& io, int batch, const 
 // (FewRowsPlan as for the x passes: this pass loops over the batch, so only the 4-row groups count)
 template <int LOGN, int L>
 static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float* phi, const float2* saved,
-                             float* dphi, int nx, float s, int acc) {
+                             float* dphi, int nx, float s, int acc, int store) {
   using RDF = typename Cfg<float, LOGN, false>::RD;
   const bool few = (long)((io.nlines + L - 1) / L) < kFillWGs;
   if constexpr (L >= 8 && (L / 2) * RDF::P >= 64) {
-    if (few) return fast_phase_grad_l<LOGN, L / 2>(c, io, batch, phi, saved, dphi, nx, s, acc);
+    if (few) return fast_phase_grad_l<LOGN, L / 2>(c, io, batch, phi, saved, dphi, nx, s, acc, store);
   }
   using FR = typename FewRowsPlan<LOGN>::RD;
   if constexpr (!std::is_void<FR>::value && L == 4) {
-    if (few) return fast_phase_grad_launch<FR, 4>(c, io, batch, phi, saved, dphi, nx, s, acc);
+    if (few) return fast_phase_grad_launch<FR, 4>(c, io, batch, phi, saved, dphi, nx, s, acc, store);
+  }
+  // rows of 2048 points: the blocked spectrum staged through LDS when the workgroups' 4-row groups
+  // are whole, unpadded runs (as the staged x passes take them)
+  if constexpr (LOGN == 11 && L == 4) {
+    if (io.in_blk && io.in_l == RDF::n && io.nlines % L == 0)
+      return fast_phase_grad_launch<RDF, L, true>(c, io, batch, phi, saved, dphi, nx, s, acc, store);
   }
   // Out of place (RS, Collins and CZT plans read the scratch spectrum), rows of 4096 points go two
   // to a workgroup: 512 threads may hold 256 VGPRs, and the phase factors then fit in LDS beside
@@ -1685,9 +1727,9 @@ static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float
     // the gradient accumulators do not fit one 1024-thread workgroup's registers
     if (static_cast<const void*>(io.in) == static_cast<const void*>(io.out))
       return fail(FH_E_UNSUPP, "the mask-gradient pass of 4096-point rows reads another buffer than it writes");
-    return fast_phase_grad_launch<RDF, 2>(c, io, batch, phi, saved, dphi, nx, s, acc);
+    return fast_phase_grad_launch<RDF, 2>(c, io, batch, phi, saved, dphi, nx, s, acc, store);
   } else {
-    return fast_phase_grad_launch<RDF, L>(c, io, batch, phi, saved, dphi, nx, s, acc);
+    return fast_phase_grad_launch<RDF, L>(c, io, batch, phi, saved, dphi, nx, s, acc, store);
   }
 }
 
@@ -1695,16 +1737,17 @@ static int fast_phase_grad_l(fh_ctx* c, const FastIO& io, int batch, const float
 // the 4x4-blocked spectrum it reads, and the rows 4g..4g+3 share those bytes.
 template <int LOGN>
 static int fast_phase_grad_t(fh_ctx* c, const FastIO& io, int batch, const float* phi, const float2* saved,
-                             float* dphi, int nx, float s, int acc) {
+                             float* dphi, int nx, float s, int acc, int store) {
   static_assert(Cfg<float, LOGN, false>::L >= 4, "the in-place pass needs whole 4-row groups");
-  return fast_phase_grad_l<LOGN, Cfg<float, LOGN, false>::L>(c, io, batch, phi, saved, dphi, nx, s, acc);
+  return fast_phase_grad_l<LOGN, Cfg<float, LOGN, false>::L>(c, io, batch, phi, saved, dphi, nx, s, acc, store);
 }
 
 // x IFFT of the blocked spectrum into natural rows, un-applying a Phase and accumulating its
-// gradient over the batch (the adjoint end of a Phase -> propagator pair)
+// gradient over the batch (the adjoint end of a Phase -> propagator pair); store == 0 (with a
+// gradient) leaves the un-applied field unwritten
 static int fast_phase_grad(fh_ctx* c, int logn, const FastIO& io, int batch, const float* phi, const float2* saved,
-                           float* dphi, int nx, float s, int acc) {
-  FH_LOGN_SWITCH_INT(logn, (fast_phase_grad_t<LG>(c, io, batch, phi, saved, dphi, nx, s, acc)))
+                           float* dphi, int nx, float s, int acc, int store = 1) {
+  FH_LOGN_SWITCH_INT(logn, (fast_phase_grad_t<LG>(c, io, batch, phi, saved, dphi, nx, s, acc, store)))
 }
 
 // x-pass variants (fast_x_l): names for the autotuner's report.  The staged form is the default for
@@ -2086,7 +2129,7 @@ static int run_xifft(fh_plan* p, float2* u, int direction) {
 // The adjoint's last pass for a Phase -> propagator pair: x IFFT of the blocked spectrum,
 // the Phase un-applied, its gradient summed over the batch (fh_propagate_phase_backward).
 static int phase_grad_pass(fh_plan* p, float2* u, const float* phi, const void* saved, float* dphi, int direction,
-                           int accumulate) {
+                           int accumulate, int store = 1) {
   const int nxl = 1 << p->lx;
   FastIO io{};
   io.in = (p->kind == 0 && !grad_split(p)) ? u : p->scratch; io.out = u;
@@ -2094,7 +2137,7 @@ static int phase_grad_pass(fh_plan* p, float2* u, const float* phi, const void* 
   io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1; io.in_blk = 1; io.out_blk = 0;
   io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
   return fast_phase_grad(p->ctx, p->lx, io, p->batch, phi, reinterpret_cast<const float2*>(saved), dphi, p->nx,
-                         (float)direction, accumulate);
+                         (float)direction, accumulate, store);
 }
 
 }  // namespace fh
@@ -3348,7 +3391,7 @@ int fh_propagate_phase_backward(fh_plan* p, void* dv, const float* phi, const vo
 // pass reads it there and the Phase gradient takes it as the saved input (nothing is saved).
 static int sqint_block(fh_plan* p, const float2* src, float2* f, const float* target, int target_batch,
                        const double* dc, double* loss, int direction, const float* phi, void* saved, float* dphi,
-                       int accumulate) {
+                       int accumulate, int cotangent = 1) {
   if (p->kind == 2 || p->oned) return fail(FH_E_UNSUPP, "the fused metric block takes 2-D ASProp and RSProp plans");
   if (target_batch != p->batch && !(target_batch == 1 && p->batch == 1))
     return fail(FH_E_UNSUPP, "the fused metric block needs one target per field (target batch %d, batch %d)",
@@ -3380,12 +3423,15 @@ static int sqint_block(fh_plan* p, const float2* src, float2* f, const float* ta
   hipLaunchKernelGGL(k_sum_partials, dim3(p->batch), dim3(kMetricThreads), 0, c->stream,
                      (const double*)c->metric_part, (int)nparts, loss);
   FH_HIP(hipGetLastError());
-  // adjoint: y pass with the conjugate kernel, then x IFFT [un-applying the Phase, its gradient]
+  // adjoint: y pass with the conjugate kernel, then x IFFT [un-applying the Phase, its gradient];
+  // none of it when neither the cotangent nor a mask gradient is asked for
+  if (!cotangent && !(phi && dphi)) return FH_OK;
   const bool split = phi && grad_split(p);
   if (split && (rc = grad_scratch(p))) return rc;
   if ((rc = run_prop(p, f, -direction, NoOp{}, 1, nullptr, split))) return rc;
   if (!phi) return run_xifft(p, f, -direction);
-  return phase_grad_pass(p, f, phi, src ? (const void*)src : saved, dphi, direction, accumulate);
+  return phase_grad_pass(p, f, phi, src ? (const void*)src : saved, dphi, direction, accumulate,
+                         cotangent || !dphi);
 }
 
 int fh_sqint_block(fh_plan* p, void* u, const float* target, int target_batch, const double* dc, double* loss,
@@ -3399,14 +3445,14 @@ int fh_sqint_block(fh_plan* p, void* u, const float* target, int target_batch, c
 
 int fh_sqint_block_src(fh_plan* p, const void* src, void* u, const float* target, int target_batch,
                        const double* dc, double* loss, int direction, const float* phi, float* dphi,
-                       int accumulate) {
+                       int accumulate, int cotangent) {
   if (!p || !src || !u || !target || !loss) return fail(FH_E_ARG, "null plan, source, field, target or loss");
   if (dphi && !phi) return fail(FH_E_ARG, "a mask gradient needs the mask");
   const size_t bytes = (size_t)p->nx * p->ny * p->batch * sizeof(float2);
   const char *a = static_cast<const char*>(src), *b = static_cast<const char*>(u);
   if (a < b + bytes && b < a + bytes) return fail(FH_E_ARG, "the source and the field must not overlap");
   return sqint_block(p, reinterpret_cast<const float2*>(src), reinterpret_cast<float2*>(u), target, target_batch,
-                     dc, loss, direction, phi, nullptr, dphi, accumulate);
+                     dc, loss, direction, phi, nullptr, dphi, accumulate, cotangent);
 }
 
 int fh_propagate_plain_fp64(fh_plan* p, void* u_c128, int direction) {

@@ -101,7 +101,7 @@ SIGNATURES = {
     "fh_sqint_block": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                c_void_p, c_void_p, c_int]),
     "fh_sqint_block_src": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int,
-                                   c_void_p, c_void_p, c_int]),
+                                   c_void_p, c_void_p, c_int, c_int]),
     "fh_as_z_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, c_void_p,
                                  c_int, c_double, c_double, c_void_p]),
     "fh_sq_field_diff": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),

@@ -224,7 +224,10 @@ class OpticalSystem:
             # the source field, and the Phase gradient takes it as its saved input)
             _, ph, prop, trail = tail
             u = start.set_field_data(torch.empty_like(start.electric))
-            losses = (_sqint_block(loss, u, ph, prop, [], grads, weights, src=start.electric),)
+            # the cotangent at the source is needed only for a trainable source's gradient
+            need_du = self.source is not None and self.source.trainable
+            losses = (_sqint_block(loss, u, ph, prop, [], grads, weights, src=start.electric,
+                                   cotangent=need_du),)
             _zero_grads(trail, grads)
             du = u
         elif fuse and (dtail := _dot_tail(comps, loss, start, self.direction)) is not None:
@@ -403,10 +406,10 @@ def _zero_grads(phases, grads: Dict) -> None:
                 g.zero_()
 
 
-def _sqint_block(loss, u: ScalarField, ph, prop, tape, grads: Dict, weights, src=None):
+def _sqint_block(loss, u: ScalarField, ph, prop, tape, grads: Dict, weights, src=None, cotangent=True):
     """The fused tail (``fh_sqint_block``); u ends as the cotangent at the tail's input.  With
     ``src`` (the tail's input, left unchanged) the first pass reads it and u is only written
-    (``fh_sqint_block_src``)."""
+    (``fh_sqint_block_src``); there ``cotangent=False`` leaves u unspecified and skips writing it."""
     t = loss.targets[0]
     tb = 1 if t.dim() == 2 else t.shape[0]
     if t.shape[-2:] != (u.ny, u.nx):
@@ -435,7 +438,7 @@ def _sqint_block(loss, u: ScalarField, ph, prop, tape, grads: Dict, weights, src
                   _ptr(phi), _ptr(saved), _ptr(g), int(acc))
     else:
         _lib.call("fh_sqint_block_src", plan, _ptr(src), _ptr(u.electric), _ptr(t), tb, _ptr(dc), _ptr(out),
-                  dir_sign(Forward), _ptr(phi), _ptr(g), int(acc))
+                  dir_sign(Forward), _ptr(phi), _ptr(g), int(acc), int(cotangent))
     return out
 
 

@@ -262,10 +262,12 @@ int fh_sqint_block(fh_plan* plan, void* u, const float* target, int target_batch
  * u0 into its output field before the first component runs (scalar_source.jl:68-71); here the first
  * pass reads src itself and writes u (which need not hold anything), and the Phase gradient takes src
  * as its saved input, so neither the copy nor the save is made.  src is left unchanged and must not
- * overlap u; u ends as the cotangent at the block's input, as in fh_sqint_block. */
+ * overlap u.  cotangent != 0: u ends as the cotangent at the block's input, as in fh_sqint_block (the
+ * gradient of a trainable source, scalar_source.jl:76-79); cotangent == 0: u ends unspecified, and
+ * the last pass writes only the mask gradient (no adjoint at all without a mask gradient). */
 int fh_sqint_block_src(fh_plan* plan, const void* src, void* u, const float* target, int target_batch,
                        const double* dc, double* loss, int direction, const float* phi, float* dphi,
-                       int accumulate);
+                       int accumulate, int cotangent);
 
 /* AS_BPM: split-step beam propagation through nslices slices of index perturbation dn
  * (AS_BPM(u, thickness, n0, dn), bpm.jl:130-142 and :8-74): dz = thickness/nslices,

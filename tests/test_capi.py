@@ -124,7 +124,7 @@ def test_new_entry_points_validate_arguments():
     assert lib.fh_asprop_create_ex(None, ref(_desc(flags=4 | 1, z=1.0)), ref(out)) == -1
     # fused metric block and offset_tilts: null arguments
     assert lib.fh_sqint_block(None, None, None, 1, None, None, 1, None, None, None, 0) == -1
-    assert lib.fh_sqint_block_src(None, None, None, None, 1, None, None, 1, None, None, 0) == -1
+    assert lib.fh_sqint_block_src(None, None, None, None, 1, None, None, 1, None, None, 0, 1) == -1
     assert b"source" in lib.fh_last_error()
     assert lib.fh_window(None, None, 4, 4, None, 4, 4, 1, 0, 0, 0.0, 0.0) == -1
     assert b"destination" in lib.fh_last_error()

@@ -90,7 +90,8 @@ def test_captured_fused_tail_replays():
 def test_block_reading_the_source_equals_block_on_a_copy(kind):
     """fh_sqint_block_src (the first pass reads the source field; no copy of it, no saved mask
     input) against fh_sqint_block on a copy with a saved input: loss, cotangent and mask gradient
-    bit for bit.  The source stays unchanged, and a field overlapping it is refused."""
+    bit for bit; with the cotangent switched off, the same loss and mask gradient.  The source stays
+    unchanged, and a field overlapping it is refused."""
     from fluxoptics import _lib
     from fluxoptics.components import _ptr
     n, b = (256, 3) if kind == "as" else (128, 2)
@@ -103,13 +104,13 @@ def test_block_reading_the_source_equals_block_on_a_copy(kind):
     phi = torch.from_numpy(inputs.uniform(172, (n, n), 0, 2 * math.pi).astype(np.float32)).cuda()
     dc = torch.tensor([1.0, -0.5, 2.0][:b], dtype=torch.float64, device="cuda")
 
-    def run(from_src):
+    def run(from_src, cotangent=1):
         u = torch.full_like(src.electric, complex(float("nan"), 0.0)) if from_src else src.electric.clone()
         loss = torch.empty(b, dtype=torch.float64, device="cuda")
         dphi = torch.empty_like(phi)
         if from_src:
             _lib.call("fh_sqint_block_src", plan, _ptr(src.electric), _ptr(u), _ptr(tgt), b, _ptr(dc), _ptr(loss),
-                      1, _ptr(phi), _ptr(dphi), 0)
+                      1, _ptr(phi), _ptr(dphi), 0, cotangent)
         else:
             saved = torch.empty_like(u)
             _lib.call("fh_sqint_block", plan, _ptr(u), _ptr(tgt), b, _ptr(dc), _ptr(loss), 1, _ptr(phi),
@@ -118,12 +119,54 @@ def test_block_reading_the_source_equals_block_on_a_copy(kind):
 
     u1, l1, g1 = run(True)
     u0d, l0, g0 = run(False)
+    _, l2, g2 = run(True, cotangent=0)
     torch.cuda.synchronize()
     assert torch.equal(l1, l0) and torch.equal(u1, u0d) and torch.equal(g1, g0)
+    assert torch.equal(l2, l0) and torch.equal(g2, g0)
     assert torch.equal(src.electric.cpu(), torch.from_numpy(u0))
     with pytest.raises(_lib.FluxHipError, match="overlap"):
         _lib.call("fh_sqint_block_src", plan, _ptr(src.electric), _ptr(src.electric[1:]), _ptr(tgt), b, _ptr(dc),
-                  _ptr(torch.empty(b, dtype=torch.float64, device="cuda")), 1, _ptr(phi), None, 0)
+                  _ptr(torch.empty(b, dtype=torch.float64, device="cuda")), 1, _ptr(phi), None, 0, 1)
+
+
+def test_block_reading_the_source_2048_against_oracle():
+    """The block right after the source on 2 x 2048^2 (its mask-gradient pass takes the blocked
+    spectrum as whole 128-byte lines through LDS): per-field losses, mask gradient and cotangent
+    against the FP64 oracle chain; with the cotangent switched off (the pass then writes no field),
+    the same losses and mask gradient bit for bit."""
+    from fluxoptics import _lib
+    from fluxoptics.components import _ptr
+    n, b, ds, lam, z = 2048, 2, (0.5, 0.5), 0.8, 3000.0
+    print(f"[fused metric from source] as n={n} batch={b} vs oracle", flush=True)
+    u0 = c64(speckle_batch(b, n, n, 0.5, 0.5, 150.0, 191))
+    tgt = (np.abs(c64(speckle_batch(b, n, n, 0.5, 0.5, 150.0, 193))) ** 2).astype(np.float32)
+    phi = inputs.uniform(192, (n, n), 0, 2 * math.pi).astype(np.float32)
+    src = fo.ScalarField(to_dev(u0), ds, lam)
+    prop = fo.ASProp(src, z)  # owns the plan
+    plan = prop.plan(src)
+    tg, ph_d = torch.from_numpy(tgt).cuda(), torch.from_numpy(phi).cuda()
+
+    def run(cotangent):
+        u = torch.empty_like(src.electric)
+        loss = torch.empty(b, dtype=torch.float64, device="cuda")
+        dphi = torch.empty_like(ph_d)
+        _lib.call("fh_sqint_block_src", plan, _ptr(src.electric), _ptr(u), _ptr(tg), b, None, _ptr(loss), 1,
+                  _ptr(ph_d), _ptr(dphi), 0, cotangent)
+        return u, loss, dphi
+
+    u1, l1, g1 = run(1)
+    _, l0, g0 = run(0)
+    torch.cuda.synchronize()
+    assert torch.equal(l1, l0) and torch.equal(g1, g0)
+    stages = [("phase", phi.astype(np.float64)), ("as", z)]
+    v_ref, saved = O.pipeline_forward(u0, stages, ds, lam)
+    vals_ref, dv_ref = O.sq_intensity_diff_values(v_ref, tgt.astype(np.float64))
+    du_ref, (g_ref,) = O.pipeline_backward(dv_ref, stages, ds, lam, saved)
+    errs = (np.abs(l1.cpu().numpy() - vals_ref).max() / vals_ref.max(), rel_l2(g1.cpu().numpy(), g_ref),
+            rel_l2(u1.cpu().numpy(), du_ref))
+    print(f"[fused metric from source] loss {errs[0]:.2e} mask gradient {errs[1]:.2e} cotangent {errs[2]:.2e}",
+          flush=True)
+    assert errs[0] <= 1e-5 and errs[1] <= 1e-4 and errs[2] <= 1e-4
 
 
 def test_trainable_source_through_the_block_reading_it():
