@@ -26,7 +26,7 @@ def _compare(system, metric, comps, weights=None):
 
 
 @pytest.mark.parametrize("kind,n,tail_phase", [("as", 1024, True), ("as", 256, False), ("rs", 256, True),
-                                                ("rs", 128, False), ("as", 4096, True)])
+                                                ("rs", 128, False), ("as", 4096, True), ("rs", 1024, True)])
 def test_fused_tail_matches_unfused(kind, n, tail_phase):
     print(f"[fused metric] {kind} n={n} phase={tail_phase}", flush=True)
     ds, lams = (1.0, 1.0), [0.8, 0.6, 0.5]
