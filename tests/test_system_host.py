@@ -215,6 +215,28 @@ def test_rule_invariants():
         assert 0.5 * np.dot(xx, xx) <= 2 * np.dot(x, x) / (eta * (k + 1) ** 2) + 1e-15
 
 
+def test_momentum_nesterov_constant_gradient_closed_forms():
+    """Momentum and Nesterov at rho != 0 on a constant gradient g, from the textbook definitions
+    rather than the oracle's update formulas.  Heavy-ball momentum (v_k = rho v_(k-1) + eta g,
+    x_k = x_(k-1) - v_k) sums two geometric series: x_k = x_0 - eta g / (1 - rho) * (k - rho (1 -
+    rho^k) / (1 - rho)).  Sutskever's Nesterov step takes the gradient at x + rho v, which for a
+    constant gradient gives the heavy-ball velocities; Optimisers.jl tracks the look-ahead point
+    x_k - rho v_k, so its iterate is the heavy-ball one minus rho eta g (1 - rho^k) / (1 - rho).  Both
+    step sizes tend to eta g / (1 - rho)."""
+    x0 = inputs.normal(53, (5,))
+    g = inputs.normal(54, (5,))
+    for rho, eta in ((0.9, 0.1), (0.5, 0.3)):
+        xm, sm = x0.copy(), np.zeros(5)
+        xn, sn = x0.copy(), np.zeros(5)
+        for k in range(1, 40):
+            xm, sm, _ = O.optim_step("momentum", xm, g, sm, eta=eta, rho=rho)
+            xn, sn, _ = O.optim_step("nesterov", xn, g, sn, eta=eta, rho=rho)
+            geo = (1 - rho ** k) / (1 - rho)
+            heavy = x0 - eta * g / (1 - rho) * (k - rho * geo)
+            assert np.allclose(xm, heavy, rtol=1e-12, atol=1e-12)
+            assert np.allclose(xn, heavy - rho * eta * g * geo, rtol=1e-12, atol=1e-12)
+
+
 def test_oracle_adam_closed_forms():
     """Adam's first step is eta * g / (|g| + eps) (bias corrections undo (1 - b)); with b1 = b2 = 0
     every step is that sign-like step; the state stays FP32."""
