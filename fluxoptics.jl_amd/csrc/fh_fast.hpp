@@ -917,16 +917,25 @@ k_fast_ywave(FastIO io, int batch, const float2* __restrict__ tabf, const float2
   float2* smem = reinterpret_cast<float2*>(smem_raw);
   const int T = threadIdx.x;
   const int t = T & 63;
-  const int li = blockIdx.x * kWaveCols + (T >> 6);  // this wave's column
+  // tiles j and n/4 - 1 - j (whose columns kx and n - kx read 3 of the same 4 quadrant-table rows)
+  // go to workgroups 8 apart, which the dispatcher places on the same XCD one after the other, so
+  // the second finds the shared rows in that XCD's L2 (a bijection when the tile count is a
+  // multiple of 16; otherwise tiles follow the workgroups)
+  int tile = blockIdx.x;
+  if (gridDim.x % 16 == 0) {
+    const int b = blockIdx.x, s8 = b >> 3, pair = ((s8 >> 1) << 3) + (b & 7);
+    tile = (s8 & 1) ? (int)gridDim.x - 1 - pair : pair;
+  }
+  const int li = tile * kWaveCols + (T >> 6);  // this wave's column
   float2* s = smem + (T >> 6) * CS;
   // the thread's tile pairs: rows 128 k + prow, columns pc and pc + 1 of the tile
   const int pw = T & 7;
   const int prow = 4 * (T >> 3) + (pw >> 1);
   float2* ps = smem + 2 * (pw & 1) * CS + lds_pos(prow);
   const int rowlen = io.in_e;  // spectrum columns (elements between rows)
-  const int vin = ((((T >> 3) * (rowlen >> 2)) + (int)blockIdx.x) * 16 + 2 * pw) * 8;
+  const int vin = ((((T >> 3) * (rowlen >> 2)) + tile) * 16 + 2 * pw) * 8;
   const int kin = rowlen * 1024;  // 32 block rows of the tile
-  const int vout = ((((T >> 3) * (io.out_e >> 2)) + (int)blockIdx.x) * 16 + 2 * pw) * 8;
+  const int vout = ((((T >> 3) * (io.out_e >> 2)) + tile) * 16 + 2 * pw) * 8;
   const int kout = io.out_e * 1024;
   const int qxi = qt.full ? li : (li <= qt.nx - li ? li : qt.nx - li);
   const int qoff_d = (qxi * qt.qcol + t) * 8;
@@ -1026,15 +1035,16 @@ k_fast_ystg(FastIO io, int batch, const float2* __restrict__ tabf, const float2*
   const int T = threadIdx.x;
   int l, t;
   thread_map<L, P, true>(T, l, t);
-  const int li = blockIdx.x * L + l;
+  const int tile = blockIdx.x;  // (pairing mirrored tiles as k_fast_ywave does: 0.147 against 0.140 ms)
+  const int li = tile * L + l;
   float2* s = smem + l * LS;
   // tile pair k of the thread: block row (T >> 3) + k P / 2, elements 2 (T & 7) and 2 (T & 7) + 1,
   // i.e. row 4 (T >> 3) + ((T & 7) >> 1) + 2 P k of columns 2 (T & 1) and 2 (T & 1) + 1
   const int pw = T & 7;
   float2* ps = smem + 2 * (pw & 1) * LS + lds_pos(4 * (T >> 3) + (pw >> 1));
-  const int vin = ((((T >> 3) * (io.in_e >> 2)) + (int)blockIdx.x) * 16 + 2 * pw) * 8;
+  const int vin = ((((T >> 3) * (io.in_e >> 2)) + tile) * 16 + 2 * pw) * 8;
   const int kin = io.in_e * (2 * P) * 8;
-  const int vout = ((((T >> 3) * (io.out_e >> 2)) + (int)blockIdx.x) * 16 + 2 * pw) * 8;
+  const int vout = ((((T >> 3) * (io.out_e >> 2)) + tile) * 16 + 2 * pw) * 8;
   const int kout = io.out_e * (2 * P) * 8;
   const int qxi = qt.full ? li : (li <= qt.nx - li ? li : qt.nx - li);
   const int qoff_d = (qxi * qt.qcol + t) * 8;
