@@ -290,7 +290,17 @@ k_fast_fft_stg(FastIO io, const float2* __restrict__ tab, TimeOp op) {
   float2* smem = reinterpret_cast<float2*>(smem_raw);
   const int tid = threadIdx.x;
   const int l = tid / P, t = tid % P;
-  const int b = blockIdx.y;
+  // workgroup -> (4-row group, field); with the Phase prologue the fields of one row group run on
+  // one XCD one after the other (8 workgroups apart), so its factor rows come from that XCD's L2
+  // after the first field instead of once per field from HBM
+  int rg = blockIdx.x, b = blockIdx.y;
+  if constexpr (TwoPhaseOp<TimeOp>::value) {
+    if (gridDim.x % 8 == 0) {
+      const int lin = blockIdx.y * gridDim.x + blockIdx.x, s8 = lin >> 3;
+      b = s8 % (int)gridDim.y;
+      rg = ((s8 / (int)gridDim.y) << 3) + (lin & 7);
+    }
+  }
   float2* s = smem + l * LS;
   // the chunk's row and column within the workgroup's run
   int crow[NCH], ccol[NCH];
@@ -304,7 +314,7 @@ k_fast_fft_stg(FastIO io, const float2* __restrict__ tab, TimeOp op) {
   float2 v[V];
   if (INV) {
     const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
-    const int vin = (blockIdx.x * L * n + 2 * tid) * 8;
+    const int vin = (rg * L * n + 2 * tid) * 8;
     float4 q[NCH];
 #pragma unroll
     for (int k = 0; k < NCH; ++k) q[k] = buf_load4(rin, vin, k * L * P * 16);
@@ -322,7 +332,7 @@ k_fast_fft_stg(FastIO io, const float2* __restrict__ tab, TimeOp op) {
       for (int r = 0; r < RD::R0; ++r) v[m * RD::R0 + r] = s[lds_pos(t + P * m + r * (n / RD::R0))];
   } else {
     const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
-    const int li = blockIdx.x * L + l;
+    const int li = rg * L + l;
     const int vin = (li * io.in_l + t) * 8;
 #pragma unroll
     for (int m = 0; m < V / RD::R0; ++m)
@@ -364,7 +374,7 @@ k_fast_fft_stg(FastIO io, const float2* __restrict__ tab, TimeOp op) {
   fast_fft<float2, RD, INV>(v, s, t, tab);
   if (INV) {
     const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
-    const int li = blockIdx.x * L + l;
+    const int li = rg * L + l;
     const int vout = (li * io.out_l + t) * 8;
 #pragma unroll
     for (int m = 0; m < V / RL; ++m)
@@ -381,7 +391,7 @@ k_fast_fft_stg(FastIO io, const float2* __restrict__ tab, TimeOp op) {
       for (int r = 0; r < RL; ++r) s[lds_pos(t + P * m + r * (n / RL))] = v[m * RL + r];
     __syncthreads();
     const auto rout = make_rsrc(io.out + b * io.out_f, io.out_bytes);
-    const int vout = (blockIdx.x * L * n + 2 * tid) * 8;
+    const int vout = (rg * L * n + 2 * tid) * 8;
 #pragma unroll
     for (int k = 0; k < NCH; ++k) {
       const float2* d = smem + crow[k] * LS + lds_pos(ccol[k]);
