@@ -384,7 +384,8 @@ k_fast_fft_stg(FastIO io, const float2* __restrict__ tab, TimeOp op) {
         buf_store(rout, vout, c * 8, op(v[m * RL + r], b, li, t + c));
       }
   } else {
-    __syncthreads();  // every thread's last exchange read is done
+    // the outputs go to the slots this thread read in the last exchange (its radix is RL), which
+    // no other thread reads: no barrier before the writes
 #pragma unroll
     for (int m = 0; m < V / RL; ++m)
 #pragma unroll
@@ -1105,9 +1106,10 @@ k_fast_ystg(FastIO io, int batch, const float2* __restrict__ tabf, const float2*
       for (int k = 0; k < V; ++k) v[k] = cmul(v[k], make_float2(h[k].x, hsg * h[k].y));
     }
     fast_fft_b<RI, true>(v, s, t, pi);
-    __syncthreads();
+    // the last stage's output positions are the slots this thread read in the last exchange (the
+    // same radix on both sides), which no other thread reads: no barrier before the writes
 #pragma unroll
-    for (int m = 0; m < V / RE; ++m)  // the last stage's output positions
+    for (int m = 0; m < V / RE; ++m)
 #pragma unroll
       for (int r = 0; r < RE; ++r) {
         const int c = P * m + r * (n / RE);
