@@ -276,8 +276,10 @@ k_fast_fft(FastIO io, const float2* __restrict__ tab, TimeOp op) {
 // adjacent slots, the first value came back stale for a few lanes per launch (measured on MI355X,
 // forward pass, 0.15% of the chunks, varying from run to run), with or without an extra barrier.
 // The natural side takes the same time-side op as k_fast_fft (prologue of a forward transform,
-// epilogue of an inverse one).
-template <typename RDF, int L, bool INV, typename TimeOp = NoOp>
+// epilogue of an inverse one).  HALF: the natural rows hold in_valid (forward) or out_valid
+// (inverse) points of each n-point line, the rest zero padding (RS plans), as IN_HALF / OUT_HALF
+// of k_fast_fft.
+template <typename RDF, int L, bool INV, typename TimeOp = NoOp, bool HALF = false>
 __global__ void __launch_bounds__(L * RDF::P)
 k_fast_fft_stg(FastIO io, const float2* __restrict__ tab, TimeOp op) {
   using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
@@ -370,6 +372,15 @@ k_fast_fft_stg(FastIO io, const float2* __restrict__ tab, TimeOp op) {
         for (int r = 0; r < RD::R0; ++r)
           v[m * RD::R0 + r] = op(v[m * RD::R0 + r], b, li, t + P * m + r * (n / RD::R0));
     }
+    if constexpr (HALF) {
+      // the loads past the valid points (the next row, or past the field: zero from the descriptor)
+      // are replaced by the zero padding
+#pragma unroll
+      for (int m = 0; m < V / RD::R0; ++m)
+#pragma unroll
+        for (int r = 0; r < RD::R0; ++r)
+          if (t + P * m + r * (n / RD::R0) >= io.in_valid) v[m * RD::R0 + r] = make_float2(0.f, 0.f);
+    }
   }
   fast_fft<float2, RD, INV>(v, s, t, tab);
   if (INV) {
@@ -381,7 +392,7 @@ k_fast_fft_stg(FastIO io, const float2* __restrict__ tab, TimeOp op) {
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
         const int c = P * m + r * (n / RL);
-        buf_store(rout, vout, c * 8, op(v[m * RL + r], b, li, t + c));
+        if (!HALF || t + c < io.out_valid) buf_store(rout, vout, c * 8, op(v[m * RL + r], b, li, t + c));
       }
   } else {
     // the outputs go to the slots this thread read in the last exchange (its radix is RL), which

@@ -1554,6 +1554,18 @@ template <> struct FewRowsPlan<10> { using RD = Radices<16, 8, 8>; };
 template <typename RDF, int L, bool INV, bool IH, bool OH, typename Op>
 static int fast_x_launch(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar);
 
+// whether an x pass of 4-row workgroups takes the staged form: the blocked side (input of an
+// inverse pass, output of a forward one) one contiguous run of whole 4-row groups of n-point rows;
+// the natural side may hold fewer points a row (RS plans: IH of a forward pass, OH of an inverse one)
+template <typename RDF, bool INV, bool IH, bool OH>
+static bool x_staged(const FastIO& io, int xvar) {
+  using RD = typename std::conditional<INV, typename RDF::Rev, RDF>::type;
+  if ((IH && INV) || (OH && !INV)) return false;
+  const bool blk_side = INV ? (io.in_blk && !io.out_blk && io.in_l == RD::n) : (io.out_blk && !io.in_blk && io.out_l == RD::n);
+  const bool full = IH || OH || (INV ? io.out_valid : io.in_valid) >= RD::n;
+  return xvar == 2 && blk_side && full && io.nlines % 4 == 0;
+}
+
 template <int LOGN, bool INV, bool IH, bool OH, typename Op, int L>
 static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar) {
   using RDF = typename Cfg<float, LOGN, false>::RD;
@@ -1569,8 +1581,9 @@ static int fast_x_l(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar) {
   // source read into a separate field) need no whole 4-row groups.  Rows of 4096 points then go
   // two to a workgroup: a 4-row workgroup takes 139 KB of LDS, one per CU, which then alternates
   // between loading, transforming and storing; two 2-row workgroups share a CU and overlap.
+  // The staged form keeps whole 4-row groups (one 16-wave workgroup per CU).
   if constexpr (L == 4 && LOGN == 12) {
-    if (static_cast<const void*>(io.in) != static_cast<const void*>(io.out))
+    if (static_cast<const void*>(io.in) != static_cast<const void*>(io.out) && !x_staged<RDF, INV, IH, OH>(io, xvar))
       return fast_x_launch<RDF, 2, INV, IH, OH, Op>(c, io, batch, op, xvar);
   }
   return fast_x_launch<RDF, L, INV, IH, OH, Op>(c, io, batch, op, xvar);
@@ -1582,12 +1595,9 @@ static int fast_x_launch(fh_ctx* c, const FastIO& io, int batch, Op op, int xvar
   const float2* tab = nullptr;
   int rc = get_stage_tw(c, RD::n, RD::R0, RD::R1, RD::R2, INV, &tab);
   if (rc) return rc;
-  if constexpr (!IH && !OH && L % 4 == 0) {
-    // the blocked side (input of an inverse pass, output of a forward one) as one contiguous run
-    // of whole 4-row groups of n-point rows
-    const bool blk_side = INV ? (io.in_blk && !io.out_blk && io.in_l == RD::n) : (io.out_blk && !io.in_blk && io.out_l == RD::n);
-    if (xvar == 2 && blk_side && io.nlines % L == 0 && (INV ? io.out_valid : io.in_valid) >= RD::n) {
-      auto kern = k_fast_fft_stg<RDF, L, INV, Op>;
+  if constexpr (L % 4 == 0 && !(IH && INV) && !(OH && !INV)) {
+    if (x_staged<RDF, INV, IH, OH>(io, xvar) && io.nlines % L == 0) {
+      auto kern = k_fast_fft_stg<RDF, L, INV, Op, IH || OH>;
       const size_t lds = (size_t)L * LdsLine<RD::n, L>::size * sizeof(float2);
       FH_HIP(ensure_lds(kern, lds));
       hipLaunchKernelGGL(kern, dim3(io.nlines / L, batch), dim3(L * RDF::P), lds, c->stream, io, tab, op);
