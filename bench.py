@@ -74,8 +74,8 @@ def read_traffic(pass_name):
 
 
 KERNELS_FILE = os.path.join(ROOT, "profiles", "asprop_2048x2048_b8_kernels.txt")
-PASS_KERNELS = {"x_fft": "k_fast_fft_stg<Radices<16, 16, 8>, 4, false, NoOp>",
-                "y_fft_mul_ifft": "k_fast_ystg<Radices<16, 16, 8>, true>", "x_ifft": "k_fast_fft_stg<Radices<16, 16, 8>, 4, true, NoOp>"}
+PASS_KERNELS = {"x_fft": "k_fast_fft_stg<Radices<16, 16, 8>, 4, false, NoOp, false>",
+                "y_fft_mul_ifft": "k_fast_ystg<Radices<16, 16, 8>, true>", "x_ifft": "k_fast_fft_stg<Radices<16, 16, 8>, 4, true, NoOp, false>"}
 
 
 def read_profiled_ms(pass_name):
