@@ -1761,12 +1761,15 @@ static int fast_phase_grad(fh_ctx* c, int logn, const FastIO& io, int batch, con
 }
 
 // x-pass variants (fast_x_l): names for the autotuner's report.  The staged form is the default for
-// rows of 2048 and 4096 points (forward + adjoint step: 8 x 2048^2 0.607 ms against 0.656 ms,
-// 2 x 4096^2 0.885 against 0.977 ms) and not for shorter rows (32 x 512^2: 0.160 against 0.158 ms,
-// 16 x 1024^2: 0.325 against 0.314 ms).
+// rows of 2048 and 4096 points of ASProp plans (forward + adjoint step: 8 x 2048^2 0.607 ms against
+// 0.656 ms, 2 x 4096^2 0.885 against 0.977 ms) and not for shorter rows (32 x 512^2: 0.160 against
+// 0.158 ms, 16 x 1024^2: 0.325 against 0.314 ms).  Plans padded to 2n (RS, Collins: half rows on
+// the natural side, out of place) take lanes over rows there: 4096-point forward x pass of one
+// field 0.0235 ms against 0.0316 staged and 0.0324 lanes over x, of 16 fields 0.355 against 0.467
+// and 0.633 ms; 2048-point rows tie.
 static const char* const kXVariants[] = {"lanes over x", "lanes over rows", "blocked side via LDS"};
 constexpr int kNumXVariants = 3;
-static int default_x_variant(int logn) { return logn >= 11 ? 2 : 0; }
+static int default_x_variant(int logn, bool padded) { return logn >= 11 ? (padded ? 1 : 2) : 0; }
 
 // y-pass variants (radices, columns per workgroup) for the autotuner
 using FastConvLaunch = int (*)(fh_ctx*, const FastIO&, int, QuadMul, bool);
@@ -2077,7 +2080,7 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
     io.in_l = p->nx; io.out_l = nxl; io.in_e = io.out_e = 1; io.in_blk = 0; io.out_blk = 1;
     io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = midf * 8;
     io.in_valid = p->nx;
-    const int xv = p->xf_var >= 0 ? p->xf_var : default_x_variant(p->lx);
+    const int xv = p->xf_var >= 0 ? p->xf_var : default_x_variant(p->lx, rs);
     if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop, xv))) return rc;
   }
   if (only == 0) return FH_OK;
@@ -2105,7 +2108,7 @@ static int run_xifft_op(fh_plan* p, float2* u, XOp xop) {
   io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1; io.in_blk = 1; io.out_blk = 0;
   io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
   io.out_valid = p->nx;
-  const int xv = p->xi_var >= 0 ? p->xi_var : default_x_variant(p->lx);
+  const int xv = p->xi_var >= 0 ? p->xi_var : default_x_variant(p->lx, rs);
   return fast_xinv(p->ctx, p->lx, rs, io, p->batch, xop, xv);
 }
 
