@@ -1923,13 +1923,15 @@ static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16,
                                           {"pipe 16x16x16 L2", fast_pipe_v<16, 16, 16, 2>}};
 
 // default variant index for a y length and whether the batch shares one transfer function
-static int default_conv_variant(int logn, bool one_tab, bool half) {
+static int default_conv_variant(int logn, bool one_tab, bool half, int batch) {
   if (one_tab && logn == 11 && !half) return 7;  // staged-tile pipe 16x16x8 L4, table in LDS
   if (one_tab && logn == 11) return 5;           // pipe 16x16x8 L4, transfer function in LDS
   if (logn == 11 && !half) return 6;             // wave per column (a table per field)
   if (logn == 11) return 3;                      // 16x16x8 L4 f1 (RS, a table per field)
-  if (logn == 12 && half && !one_tab) return 3;  // pipe 16x16x16 L2 (RS, a table per field)
+  // RS, one field: 16x16x16 L2 f1 (0.0758 against 0.0836 ms pipelined); more fields: pipe L2
+  if (logn == 12 && half && !one_tab) return batch == 1 ? 2 : 3;
   if (logn == 12) return 1;                      // 16x16x16 L4
+  if (logn == 10 && half && batch == 1) return 1;  // pipe 16x8x8 L4 (RS, one field: 0.0078 against 0.0089 ms)
   return 0;
 }
 
@@ -2094,7 +2096,7 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
   const FastConvVariant* vars = fast_conv_variants(p->ly, &nvar);
   if (!vars) return fail(FH_E_UNSUPP, "y length 2^%d unsupported", p->ly);
   const int v = (p->conv_var >= 0 && p->conv_var < nvar) ? p->conv_var
-                                                          : default_conv_variant(p->ly, q.one_tab, rs);
+                                                          : default_conv_variant(p->ly, q.one_tab, rs, p->batch);
   return vars[v].fn(c, io, p->batch, q, rs);
 }
 
