@@ -1924,14 +1924,25 @@ static const FastConvVariant kFast12[] = {{"16x16x16 L4 f1", fast_conv_v<16, 16,
 
 // default variant index for a y length and whether the batch shares one transfer function
 static int default_conv_variant(int logn, bool one_tab, bool half, int batch) {
-  if (one_tab && logn == 11 && !half) return 7;  // staged-tile pipe 16x16x8 L4, table in LDS
+  // (times: autotuner, 10 reps, per call)
+  if (logn == 11 && !half) {
+    if (!one_tab) return 6;  // wave per column (a table per field)
+    // one table: wave per column for one or two fields (1 field 0.0197 ms against 0.0286 staged),
+    // pipe 16x16x8 L4 with the table in LDS up to four (3 fields 0.0542 against 0.0568), then the
+    // staged-tile pipe 16x16x8 L4 (8 fields 0.138 ms, the fastest)
+    return batch <= 2 ? 6 : (batch <= 4 ? 5 : 7);
+  }
   if (one_tab && logn == 11) return 5;           // pipe 16x16x8 L4, transfer function in LDS
-  if (logn == 11 && !half) return 6;             // wave per column (a table per field)
   if (logn == 11) return 3;                      // 16x16x8 L4 f1 (RS, a table per field)
   // RS, one field: 16x16x16 L2 f1 (0.0758 against 0.0836 ms pipelined); more fields: pipe L2
   if (logn == 12 && half && !one_tab) return batch == 1 ? 2 : 3;
+  // ASProp 4096: one table over 3 or more fields pipe 16x16x16 L2 (8 fields 0.790 against 0.933 ms
+  // for 16x16x16 L4); a table per field over 8 or more 16x16x16 L4 f1 (0.961 against 0.993)
+  if (logn == 12 && !half) return one_tab ? (batch >= 3 ? 3 : 1) : (batch >= 8 ? 0 : 1);
   if (logn == 12) return 1;                      // 16x16x16 L4
-  if (logn == 10 && half && batch == 1) return 1;  // pipe 16x8x8 L4 (RS, one field: 0.0078 against 0.0089 ms)
+  // RS, one field of 1024-point columns: pipe 16x8x8 L4 (0.0078 against 0.0089 ms; ASProp plans of
+  // that length measure within noise of one another)
+  if (logn == 10 && half && batch == 1) return 1;
   return 0;
 }
 
