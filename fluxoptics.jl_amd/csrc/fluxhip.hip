@@ -991,7 +991,10 @@ namespace fh {
 // transforms meet in registers.  HALF (zero-padded RS plans): points x >= nx of the IFFT are
 // the discarded wrap of the linear convolution; du is zero there, as the adjoint's padding
 // wants.  One FP64 partial per workgroup, summed in a fixed order afterwards.
-template <typename RDF, int L, bool HALF>
+// STG: the workgroup's 4-row groups of the blocked spectrum arrive as whole 128-byte lines (16-byte
+// loads of its contiguous run, through the exchange areas, as in the staged x IFFT); the output
+// side keeps the direct stores.
+template <typename RDF, int L, bool HALF, bool STG = false>
 __global__ void __launch_bounds__(L * RDF::P)
 k_fast_sqint(FastIO io, const float2* __restrict__ tabi, const float2* __restrict__ tabf,
              const float* __restrict__ I, long tstride, int nx, const double* __restrict__ dc,
@@ -1000,7 +1003,10 @@ k_fast_sqint(FastIO io, const float2* __restrict__ tabi, const float2* __restric
   constexpr int n = RDF::n, V = RDF::V, P = RDF::P;
   constexpr int RL = LastRadix<RI>::value;
   constexpr int RE = LastRadix<RDF>::value;
+  constexpr int LS = LdsLine<n, L>::size;
+  constexpr int NCH = n / (2 * P);  // 16-byte chunks a thread (STG)
   static_assert(RL == RDF::R0, "the inverse transform must end in the forward transform's input order");
+  static_assert(!STG || L % 4 == 0, "whole 4-row groups");
   __shared__ double red[(L * P + 63) / 64];
   extern __shared__ unsigned char smem_raw[];
   float2* smem = reinterpret_cast<float2*>(smem_raw);
@@ -1011,14 +1017,21 @@ k_fast_sqint(FastIO io, const float2* __restrict__ tabi, const float2* __restric
   const bool live = li < io.nlines;
   float2* s = smem + l * LdsLine<n, L>::size;
   const auto rin = make_rsrc(io.in + b * io.in_f, io.in_bytes);
-  const int vin = fast_off<false>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
-  const int kin = fast_k<false>(io.in_l, io.in_e, io.in_blk) * 8;
   float2 v[V];
+  float4 q[STG ? NCH : 1];
+  if constexpr (STG) {
+    const int vstg = (blockIdx.x * L * n + 2 * (int)threadIdx.x) * 8;
 #pragma unroll
-  for (int m = 0; m < V / RI::R0; ++m)
+    for (int k = 0; k < NCH; ++k) q[k] = buf_load4(rin, vstg, k * L * P * 16);
+  } else {
+    const int vin = fast_off<false>(li, t, io.in_l, io.in_e, io.in_blk) * 8;
+    const int kin = fast_k<false>(io.in_l, io.in_e, io.in_blk) * 8;
 #pragma unroll
-    for (int r = 0; r < RI::R0; ++r)
-      v[m * RI::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RI::R0)) * kin);
+    for (int m = 0; m < V / RI::R0; ++m)
+#pragma unroll
+      for (int r = 0; r < RI::R0; ++r)
+        v[m * RI::R0 + r] = buf_load(rin, vin, (P * m + r * (n / RI::R0)) * kin);
+  }
   // the target values are loaded up front, in flight during the inverse transform (the
   // inverse leaves point (m, r) at position t + P m + r n/RL, known now); the target row is
   // nx long, so positions past it (HALF) read a clamped, unused address
@@ -1032,6 +1045,24 @@ k_fast_sqint(FastIO io, const float2* __restrict__ tabi, const float2* __restric
       const int x = t + P * m + r * (n / RL);
       tg[m * RL + r] = Ib[(!HALF || x < nx) ? x : 0];
     }
+  if constexpr (STG) {
+    // chunk k holds elements 2 (k L P + tid) and the next of the run: block row, block, then row
+    // and column within the block
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+      const int e = 2 * (k * L * P + (int)threadIdx.x);
+      const int blk = e >> 4;
+      float2* d = smem + (4 * (blk / (n / 4)) + ((e >> 2) & 3)) * LS + lds_pos(4 * (blk % (n / 4)) + (e & 3));
+      d[0] = make_float2(q[k].x, q[k].y);
+      asm volatile("" ::: "memory");  // two accesses, not one instruction over adjacent slots
+      d[1] = make_float2(q[k].z, q[k].w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < V / RI::R0; ++m)
+#pragma unroll
+      for (int r = 0; r < RI::R0; ++r) v[m * RI::R0 + r] = s[lds_pos(t + P * m + r * (n / RI::R0))];
+  }
   fast_fft<float2, RI, true>(v, s, t, tabi);
   double acc = 0.0;
 #pragma unroll
@@ -1673,6 +1704,10 @@ static int fast_sqint_l(fh_ctx* c, const FastIO& io, int batch, const float* I, 
   if (!rc) rc = get_stage_tw(c, RDF::n, RDF::R0, RDF::R1, RDF::R2, false, &tabf);
   if (rc) return rc;
   auto kern = k_fast_sqint<RDF, L, HALF>;
+  if constexpr (LOGN == 11 && L == 4) {
+    // rows of 2048 points in whole, unpadded 4-row groups: the input staged through LDS
+    if (io.in_blk && io.in_l == RDF::n && io.nlines % L == 0) kern = k_fast_sqint<RDF, L, HALF, true>;
+  }
   const size_t lds = (size_t)L * LdsLine<RDF::n, L>::size * sizeof(float2);
   FH_HIP(ensure_lds(kern, lds));
   hipLaunchKernelGGL(kern, dim3(gx, batch), dim3(L * RDF::P), lds, c->stream, io, tabi, tabf, I, tstride, nx, dc,
