@@ -117,3 +117,19 @@ def test_staged_x_pass_with_phase_prologue_bitwise():
     print(f"[staged x pass, Phase prologue] loss {res[2][0]:.9e}", flush=True)
     assert res[0][0] == res[2][0]
     assert np.array_equal(res[0][1], res[2][1])
+
+
+@pytest.mark.parametrize("batch", [1, 3, 5])
+def test_asprop_2048_default_y_pass_by_batch(batch):
+    """The built-in y-pass default of a one-table 2048-point plan depends on the batch (one wave a
+    column for one or two fields, the LDS-table pipelined pass up to four, the staged-tile pass
+    from five on): each, as the default picks it, against the FP64 oracle in both directions."""
+    n, ds, lam, z = 2048, (0.5, 0.5), 0.8, 3000.0
+    u0 = c64(speckle_batch(batch, n, n, ds[0], ds[1], 150.0, 60 + batch))
+    u = fo.ScalarField(to_dev(u0), ds, lam)
+    p = fo.ASProp(u, z)
+    print(f"[default y pass] batch {batch}", flush=True)
+    for d, s in ((fo.Forward, 1), (fo.Backward, -1)):
+        v = to_host(p.propagate(u, d).electric)
+        err = rel_l2(v, O.asprop(u0, ds, lam, z, 1.0, s))
+        assert err <= FIELD_TOL, (batch, s, err)
