@@ -1801,10 +1801,15 @@ static int fast_phase_grad(fh_ctx* c, int logn, const FastIO& io, int batch, con
 // 0.158 ms, 16 x 1024^2: 0.325 against 0.314 ms).  Plans padded to 2n (RS, Collins: half rows on
 // the natural side, out of place) take lanes over rows there: 4096-point forward x pass of one
 // field 0.0235 ms against 0.0316 staged and 0.0324 lanes over x, of 16 fields 0.355 against 0.467
-// and 0.633 ms; 2048-point rows tie.
+// and 0.633 ms; of 2048-point rows the forward passes tie and the inverse takes the staged form
+// (3 fields 0.0156 against 0.0169 ms).
 static const char* const kXVariants[] = {"lanes over x", "lanes over rows", "blocked side via LDS"};
 constexpr int kNumXVariants = 3;
-static int default_x_variant(int logn, bool padded) { return logn >= 11 ? (padded ? 1 : 2) : 0; }
+static int default_x_variant(int logn, bool padded, bool inv) {
+  if (logn < 11) return 0;
+  if (!padded) return 2;
+  return (logn == 11 && inv) ? 2 : 1;
+}
 
 // y-pass variants (radices, columns per workgroup) for the autotuner
 using FastConvLaunch = int (*)(fh_ctx*, const FastIO&, int, QuadMul, bool);
@@ -2128,7 +2133,7 @@ static int run_prop(fh_plan* p, float2* u, int direction, XOp xop, int only = -1
     io.in_l = p->nx; io.out_l = nxl; io.in_e = io.out_e = 1; io.in_blk = 0; io.out_blk = 1;
     io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = midf * 8;
     io.in_valid = p->nx;
-    const int xv = p->xf_var >= 0 ? p->xf_var : default_x_variant(p->lx, rs);
+    const int xv = p->xf_var >= 0 ? p->xf_var : default_x_variant(p->lx, rs, false);
     if ((rc = fast_xfwd(c, p->lx, rs, io, p->batch, xop, xv))) return rc;
   }
   if (only == 0) return FH_OK;
@@ -2156,7 +2161,7 @@ static int run_xifft_op(fh_plan* p, float2* u, XOp xop) {
   io.in_l = nxl; io.out_l = p->nx; io.in_e = io.out_e = 1; io.in_blk = 1; io.out_blk = 0;
   io.nlines = p->ny; io.in_bytes = io.in_f * 8; io.out_bytes = io.out_f * 8;
   io.out_valid = p->nx;
-  const int xv = p->xi_var >= 0 ? p->xi_var : default_x_variant(p->lx, rs);
+  const int xv = p->xi_var >= 0 ? p->xi_var : default_x_variant(p->lx, rs, true);
   return fast_xinv(p->ctx, p->lx, rs, io, p->batch, xop, xv);
 }
 
